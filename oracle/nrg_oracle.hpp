@@ -1,0 +1,913 @@
+// nrg_oracle — FP64 CPU restatement of srbhp/nrgplusplus's Wilson-chain step.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, the smoke check and the bench's cpu_baseline leg load
+// this code; the product (libnrgk.so and its host API) never links, calls or routes through it.
+//
+// Every routine restates a reference routine and cites it.  Paths are relative to the reference
+// repository root.  Semantics kept on purpose (see the Appendix-A list in the design notes):
+//  * truncation cut = sorted[K] + 1e-8 with strict '<', K+1 or more states kept
+//    (nrgcore/include/nrgcore/nrgcore.hpp:285,298,338);
+//  * the shift to E_min = 0 only when truncating (nrgcore.hpp:317-326); all_eigenvalue pre-shift;
+//  * sector order first-seen over (prev i, bath j), in-block index it + il*k_ii (nrgcore.hpp:181,241);
+//  * chi_bath[bb'] in the assembly, none in the f-dagger rotation (nrgcore.hpp:204-205,457-461);
+//  * eigenvectors in the columns of U, eigenvalues ascending (qmatrix.hpp:390-391);
+//  * T = 0 FDM with a 1e-5 ground-state window (fdmSpectrum.hpp:122-155,363);
+//  * pairwise ground-energy correction and d^k shell factor in the thermodynamics
+//    (static/fdmThermodynamics.hpp:133-135,162,240-249).
+// The file is compiled with -ffp-contract=off so a*b+c is never fused, like the reference's
+// unoptimised x86-64 build, which keeps the assembled Hamiltonian bit-comparable.
+#pragma once
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <numeric>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "lapack_dl.hpp"
+
+namespace nrg_oracle {
+
+// ---------------------------------------------------------------- dense row-major matrix
+// restates qmatrix<double> (nrgcore/include/utils/qmatrix.hpp:66-75): row-major, (i,j) at i*c+j.
+struct Mat {
+  size_t r = 0, c = 0;
+  std::vector<double> a;
+  Mat() = default;
+  Mat(size_t rr, size_t cc, double v = 0.0) : r(rr), c(cc), a(rr * cc, v) {}
+  double& operator()(size_t i, size_t j) { return a[i * c + j]; }
+  double operator()(size_t i, size_t j) const { return a[i * c + j]; }
+  size_t size() const { return a.size(); }
+};
+
+// qmatrix::cTranspose (qmatrix.hpp:280-296): an explicit transposed copy.
+inline Mat transpose(const Mat& m) {
+  Mat t(m.c, m.r);
+  for (size_t i = 0; i < m.r; i++)
+    for (size_t j = 0; j < m.c; j++) t(j, i) = m(i, j);
+  return t;
+}
+
+// qmatrix::dot (qmatrix.hpp:351-374): cblas_dgemm(RowMajor, NoTrans, NoTrans), zero result for an
+// empty dimension.
+inline Mat dot(const Mat& A, const Mat& B) {
+  if (A.c != B.r) throw std::runtime_error("dot:qmatrix have different size for dot\n");
+  Mat C(A.r, B.c);
+  if (A.r == 0 || A.c == 0 || B.c == 0) return C;
+  auto& L = Lapack::get();
+  L.dgemm(Lapack::kRowMajor, Lapack::kNoTrans, Lapack::kNoTrans, (int)A.r, (int)B.c, (int)A.c,
+          1.0, A.a.data(), (int)A.c, B.a.data(), (int)B.c, 0.0, C.a.data(), (int)B.c);
+  return C;
+}
+
+// qmatrix::diag (qmatrix.hpp:375-404): LAPACKE_dsyevd(ROW_MAJOR,'V','U') in place.  The matrix
+// becomes U with eigenvectors in its columns; the eigenvalues come back ascending.  As in the
+// reference a LAPACK failure (info > 0) is reported, not thrown; the oracle records it.
+inline int g_last_syevd_info = 0;
+inline std::vector<double> diag(Mat& A) {
+  if (A.r != A.c) throw std::runtime_error("Error: Matrix is not a square matrix! \n");
+  std::vector<double> w(A.r, 0.0);
+  if (A.r == 0) return w;
+  int info = Lapack::get().dsyevd(Lapack::kRowMajor, 'V', 'U', (int)A.r, A.a.data(), (int)A.r,
+                                  w.data());
+  g_last_syevd_info = info;
+  return w;
+}
+
+// qmatrix::krDot (qmatrix.hpp:542-561): result((r*p+v),(s*q+w)) = A(r,s)*B(v,w)*alpha.
+inline Mat kron(const Mat& A, const Mat& B, double alpha = 1.0) {
+  Mat R(A.r * B.r, A.c * B.c);
+  for (size_t r = 0; r < A.r; r++)
+    for (size_t s = 0; s < A.c; s++)
+      for (size_t v = 0; v < B.r; v++)
+        for (size_t w = 0; w < B.c; w++) R(r * B.r + v, s * B.c + w) = A(r, s) * B(v, w) * alpha;
+  return R;
+}
+
+inline Mat eye(size_t n) {
+  Mat m(n, n);
+  for (size_t i = 0; i < n; i++) m(i, i) = 1.0;
+  return m;
+}
+inline Mat add(const Mat& A, const Mat& B) {
+  Mat R(A.r, A.c);
+  for (size_t i = 0; i < A.a.size(); i++) R.a[i] = A.a[i] + B.a[i];
+  return R;
+}
+inline Mat sub(const Mat& A, const Mat& B) {
+  Mat R(A.r, A.c);
+  for (size_t i = 0; i < A.a.size(); i++) R.a[i] = A.a[i] - B.a[i];
+  return R;
+}
+inline Mat scale(const Mat& A, double x) {
+  Mat R(A.r, A.c);
+  for (size_t i = 0; i < A.a.size(); i++) R.a[i] = A.a[i] * x;
+  return R;
+}
+inline std::vector<double> diagonal(const Mat& A) {
+  std::vector<double> d(std::min(A.r, A.c));
+  for (size_t i = 0; i < d.size(); i++) d[i] = A(i, i);
+  return d;
+}
+
+// ---------------------------------------------------------------- block-sparse operator
+// restates qOperator (nrgcore/include/nrgcore/qOperator.hpp:18-158): map {i,j} -> block, set()
+// throws on a duplicate block, get() returns nullptr when absent.
+struct BlockOp {
+  std::map<std::pair<size_t, size_t>, Mat> blocks;
+  const Mat* get(size_t i, size_t j) const {
+    auto it = blocks.find({i, j});
+    return it == blocks.end() ? nullptr : &it->second;
+  }
+  Mat* get(size_t i, size_t j) {
+    auto it = blocks.find({i, j});
+    return it == blocks.end() ? nullptr : &it->second;
+  }
+  void set(Mat m, size_t i, size_t j) {
+    if (blocks.count({i, j})) throw std::runtime_error("Found operator already");
+    blocks.emplace(std::make_pair(i, j), std::move(m));
+  }
+  void clear() { blocks.clear(); }
+};
+
+// ---------------------------------------------------------------- fermion basis and models
+// restates fermionBasis (nrgcore/include/models/fermionBasis.hpp:20-442), chargeAndSpin symmetry.
+struct FermionBasis {
+  std::vector<Mat> fdag_full;              // fermionOprMat
+  std::vector<std::vector<double>> occ;    // fnParticle
+  std::vector<std::vector<int>> qn_state;  // nQuantum
+  std::vector<std::vector<int>> unique_qn; // unique_Qnumbers
+  std::vector<std::vector<size_t>> blocks; // nQBlocks
+
+  FermionBasis() = default;
+  explicit FermionBasis(size_t dof) {
+    create_fermion_basis(dof);
+    qn_charge_and_spin();
+    block_structure();
+  }
+  // createFermionBasis (fermionBasis.hpp:397-420): f_i = I^(dof-i-1) (x) f (x) sz^i.
+  void create_fermion_basis(size_t dof) {
+    Mat fd(2, 2), sz(2, 2), id2 = eye(2);
+    fd(1, 0) = 1.0;
+    sz(0, 0) = 1.0;
+    sz(1, 1) = -1.0;
+    fdag_full.clear();
+    for (size_t i = 0; i < dof; i++) {
+      Mat prev(1, 1, 1.0), nxt(1, 1, 1.0);
+      for (size_t j = 0; j + i + 1 < dof; j++) prev = kron(id2, prev);
+      for (size_t j = 0; j < i; j++) nxt = kron(sz, nxt);
+      fdag_full.push_back(kron(prev, kron(fd, nxt)));
+    }
+    occ.clear();
+    for (size_t i = 0; i < dof; i++) occ.push_back(diagonal(dot(fdag_full[i], transpose(fdag_full[i]))));
+  }
+  // create_QuantumNspinCharge (fermionBasis.hpp:122-141): [N of even orbitals, N of odd orbitals].
+  void qn_charge_and_spin() {
+    qn_state.clear();
+    for (size_t s = 0; s < occ[0].size(); s++) {
+      int up = 0, dn = 0;
+      for (size_t j = 0; j < occ.size(); j++) {
+        if (j % 2 == 0) up += static_cast<int>(occ[j][s]);
+        else dn += static_cast<int>(occ[j][s]);
+      }
+      qn_state.push_back({up, dn});
+    }
+  }
+  // create_Block_structure (fermionBasis.hpp:230-268): sectors in first-appearance order.
+  void block_structure() {
+    blocks.clear();
+    unique_qn.clear();
+    std::vector<size_t> firsts;
+    std::set<std::vector<int>> seen;
+    for (size_t s = 0; s < qn_state.size(); s++)
+      if (seen.insert(qn_state[s]).second) firsts.push_back(s);
+    for (size_t s : firsts) {
+      unique_qn.push_back(qn_state[s]);
+      std::vector<size_t> b{s};
+      for (size_t t = s + 1; t < qn_state.size(); t++)
+        if (qn_state[t] == qn_state[s]) b.push_back(t);
+      blocks.push_back(b);
+    }
+  }
+  // get_block_operators (fermionBasis.hpp:291-318): keep a block iff sum |elem| > 0.
+  std::vector<BlockOp> block_operators(const std::vector<Mat>& ops) const {
+    std::vector<BlockOp> out(ops.size());
+    for (size_t i = 0; i < blocks.size(); i++)
+      for (size_t j = 0; j < blocks.size(); j++)
+        for (size_t p = 0; p < ops.size(); p++) {
+          Mat f(blocks[i].size(), blocks[j].size());
+          double tv = 0;
+          for (size_t a = 0; a < blocks[i].size(); a++)
+            for (size_t b = 0; b < blocks[j].size(); b++) {
+              f(a, b) = ops[p](blocks[i][a], blocks[j][b]);
+              tv += std::fabs(ops[p](blocks[i][a], blocks[j][b]));
+            }
+          if (tv > 0) out[p].set(f, i, j);
+        }
+    return out;
+  }
+  // get_block_Hamiltonian (fermionBasis.hpp:329-373), with its two consistency checks.
+  BlockOp block_hamiltonian(const Mat& H) const {
+    double tv = 0;
+    for (size_t i = 0; i < H.c; i++)
+      for (size_t j = i + 1; j < H.r; j++) tv += std::fabs(H(i, j) - H(j, i));
+    if (tv > 1e-10) throw std::runtime_error("Hamiltonian is not Hermitian\n ");
+    BlockOp out;
+    for (size_t i = 0; i < blocks.size(); i++) {
+      Mat f(blocks[i].size(), blocks[i].size());
+      for (size_t a = 0; a < blocks[i].size(); a++)
+        for (size_t b = 0; b < blocks[i].size(); b++) f(a, b) = H(blocks[i][a], blocks[i][b]);
+      out.set(f, i, i);
+    }
+    double off = 0;
+    for (size_t i = 0; i < blocks.size(); i++)
+      for (size_t j = i + 1; j < blocks.size(); j++)
+        for (size_t a : blocks[i])
+          for (size_t b : blocks[j]) off += H(a, b);
+    if (std::abs(off) > 1e-10) throw std::runtime_error("Warning: Hamiltonian is not block diagonal \n");
+    return out;
+  }
+};
+
+// The duck-typed model concept of nrgcore (nrgcore.hpp:46-48,68-76,119-133).
+struct SiteModel {
+  std::vector<std::vector<int>> n_Q;
+  std::vector<std::vector<double>> eigenvalues_Q;
+  std::vector<double> chi_Q;
+  std::vector<BlockOp> f_dag_operator;
+  std::vector<BlockOp> double_occupancy;  // spinhalf only (spinhalf.hpp:114)
+};
+
+inline std::vector<double> chi_of(const std::vector<std::vector<int>>& nq) {
+  std::vector<double> chi;
+  for (auto& q : nq) chi.push_back(std::pow(-1., (double)std::accumulate(q.begin(), q.end(), 0)));
+  return chi;
+}
+
+// spinhalf (nrgcore/include/models/spinhalf.hpp:81-124): H = eps(n_up+n_dn) + h(n_up-n_dn) + U n_up n_dn.
+inline SiteModel make_spinhalf(double eps, double U, double h = 0.0) {
+  FermionBasis fb(2);
+  SiteModel m;
+  m.f_dag_operator = fb.block_operators(fb.fdag_full);
+  m.n_Q = fb.unique_qn;
+  m.chi_Q = chi_of(m.n_Q);
+  Mat nu = dot(fb.fdag_full[0], transpose(fb.fdag_full[0]));
+  Mat nd = dot(fb.fdag_full[1], transpose(fb.fdag_full[1]));
+  Mat H = add(add(scale(add(nu, nd), eps), scale(sub(nu, nd), h)), scale(dot(nu, nd), U));
+  BlockOp hb = fb.block_hamiltonian(H);
+  m.double_occupancy = fb.block_operators({dot(nu, nd)});
+  m.eigenvalues_Q.resize(m.n_Q.size());
+  for (size_t i = 0; i < m.n_Q.size(); i++) m.eigenvalues_Q[i] = diag(*hb.get(i, i));
+  return m;
+}
+
+// twoChannel (nrgcore/src/models/twoChannel.cpp:12-48): four orbitals, H = 0.
+inline SiteModel make_two_channel() {
+  FermionBasis fb(4);
+  SiteModel m;
+  m.f_dag_operator = fb.block_operators(fb.fdag_full);
+  m.n_Q = fb.unique_qn;
+  m.chi_Q = chi_of(m.n_Q);
+  Mat nu = dot(fb.fdag_full[0], transpose(fb.fdag_full[0]));
+  Mat nd = dot(fb.fdag_full[1], transpose(fb.fdag_full[1]));
+  Mat H = scale(add(nu, nd), 0.0);
+  BlockOp hb = fb.block_hamiltonian(H);
+  m.eigenvalues_Q.resize(m.n_Q.size());
+  for (size_t i = 0; i < m.n_Q.size(); i++) m.eigenvalues_Q[i] = diag(*hb.get(i, i));
+  return m;
+}
+
+// spinhalfKondo (nrgcore/include/models/spinhalfKondo.hpp:28-166): impurity spin S exchange-coupled
+// to the first Wilson site; the f-dagger blocks are rotated into the block eigenbases.  Needed only
+// to replay the reference's Kondo fixture.
+inline SiteModel make_kondo(double J, double S) {
+  size_t d = static_cast<size_t>(2. * S + 1.);
+  Mat Sz(d, d), Sp(d, d);
+  for (size_t i = 0; i < d; i++) {
+    double mm = -S + static_cast<double>(i);
+    Sz(i, i) = mm;
+    if (i != d - 1) Sp(i, i + 1) = std::sqrt(S * (S + 1.) - mm * (mm + 1));
+  }
+  Mat Sm = transpose(Sp);
+  Mat Sx = scale(add(Sp, Sm), 0.5);
+  Mat Sy = scale(sub(Sp, Sm), 0.5);
+  FermionBasis fb;
+  fb.create_fermion_basis(2);
+  Mat cu = fb.fdag_full[0], cd = fb.fdag_full[1];
+  Mat wx = scale(add(dot(cu, transpose(cd)), dot(cd, transpose(cu))), 0.5);
+  Mat wy = scale(sub(dot(cd, transpose(cu)), dot(cu, transpose(cd))), 0.5);
+  Mat wz = scale(sub(dot(cu, transpose(cu)), dot(cd, transpose(cd))), 0.5);
+  Mat wN = add(dot(cu, transpose(cu)), dot(cd, transpose(cd)));
+  Mat H = scale(sub(add(kron(Sz, wz), kron(Sx, wx)), kron(Sy, wy)), J);
+  Mat SzF = kron(Sz, eye(wx.r));
+  wz = kron(eye(Sx.r), wz);
+  wN = kron(eye(Sx.r), wN);
+  cu = kron(eye(Sx.r), cu);
+  cd = kron(eye(Sx.r), cd);
+  fb.occ.clear();
+  Mat I = eye(SzF.r);
+  fb.occ.push_back(diagonal(scale(add(I, scale(SzF, 2.)), 0.5)));
+  fb.occ.push_back(diagonal(scale(sub(I, scale(SzF, 2.)), 0.5)));
+  fb.occ.push_back(diagonal(scale(add(wN, scale(wz, 2.)), 0.5)));
+  fb.occ.push_back(diagonal(scale(sub(wN, scale(wz, 2.)), 0.5)));
+  fb.qn_charge_and_spin();
+  fb.block_structure();
+  SiteModel m;
+  m.n_Q = fb.unique_qn;
+  m.chi_Q = chi_of(m.n_Q);
+  BlockOp hb = fb.block_hamiltonian(H);
+  m.eigenvalues_Q.resize(m.n_Q.size());
+  for (size_t i = 0; i < m.n_Q.size(); i++) m.eigenvalues_Q[i] = diag(*hb.get(i, i));
+  auto fops = fb.block_operators({cu, cd});
+  m.f_dag_operator.assign(fops.size(), BlockOp());
+  for (size_t p = 0; p < fops.size(); p++)
+    for (size_t i = 0; i < m.n_Q.size(); i++)
+      for (size_t j = 0; j < m.n_Q.size(); j++)
+        if (const Mat* f = fops[p].get(i, j))
+          m.f_dag_operator[p].set(dot(transpose(*hb.get(i, i)), dot(*f, *hb.get(j, j))), i, j);
+  return m;
+}
+
+// d^dagger chi operators of a spinhalf site (examples/fdmSpectrumSiam/main.cpp:86-106):
+// f_i^dagger . chi with chi = (-1)^N, in the block structure of the spinhalf basis.
+inline std::vector<BlockOp> make_fdag_chi_spinhalf() {
+  FermionBasis fb(2);
+  auto fr = fb.fdag_full;
+  Mat nt = dot(fr[0], transpose(fr[0]));
+  for (size_t i = 1; i < fr.size(); i++) nt = add(nt, dot(fr[i], transpose(fr[i])));
+  Mat chi = nt;
+  for (size_t i = 0; i < chi.r; i++) chi(i, i) = std::pow(-1, nt(i, i));
+  for (auto& a : fr) a = dot(a, chi);
+  return fb.block_operators(fr);
+}
+
+// Wilson-chain hopping (examples/rgflowSIAM/main.cpp:5-10).
+inline double hopping(int n, double L) {
+  return 0.5 * (1.0 + 1.0 / L) * (1. - std::pow(L, -n - 1)) /
+         std::sqrt((1.0 - std::pow(L, -2. * n - 1)) * (1.0 - std::pow(L, -2. * n - 3)));
+}
+
+// ---------------------------------------------------------------- the chain (nrgcore)
+struct StepTimes {
+  double assemble = 0, diag = 0, rotate = 0, truncate = 0;
+};
+
+class Chain {
+ public:
+  Chain(const SiteModel& imp, const SiteModel& bath) : imp_(imp), bath_(bath) {
+    chi_bath = bath_.chi_Q;
+    bath_eigenvaluesQ = bath_.eigenvalues_Q;
+    nq_bath = bath_.n_Q;
+    set_parameters();
+    // nrgcore::test (nrgcore.hpp:117-134)
+    if (imp_.n_Q[0].size() != bath_.n_Q[0].size())
+      throw std::runtime_error("Number of quantum number is different on model and impurity!");
+    if (imp_.f_dag_operator.size() != bath_.f_dag_operator.size())
+      throw std::runtime_error("f_dag_operator.size is different on model and impurity!\n");
+  }
+
+  // nrgcore::set_parameters (nrgcore.hpp:336-342)
+  void set_parameters(size_t n = 1024) {
+    no_of_kept_states = n;
+    errorbarInEnergy = 1e-8;
+    nrg_iterations_cnt = 0;
+    nrg_iterations_min = 0;
+    max_kept_states = n;
+  }
+
+  // nrgcore::add_bath_site (nrgcore.hpp:65-97)
+  void add_bath_site(const std::vector<double>& t, double rescale) {
+    using clk = std::chrono::steady_clock;
+    bath_eigenvaluesQ = bath_.eigenvalues_Q;
+    nq_bath = bath_.n_Q;
+    chi_bath = bath_.chi_Q;
+    if (nrg_iterations_cnt == 0) {
+      eigenvaluesQ = imp_.eigenvalues_Q;
+      pre_sysmQ = imp_.n_Q;
+      pre_fdag = imp_.f_dag_operator;
+      discard_higher_energies();
+    }
+    auto t0 = clk::now();
+    create_next_basis();
+    create_next_hamiltonians(t, rescale);
+    auto t1 = clk::now();
+    if (keep_hamiltonians) hamiltonians_before_diag = current_hamiltonQ;
+    eigenvaluesQ.assign(current_hamiltonQ.size(), {});
+    syevd_failures = 0;
+    for (size_t i = 0; i < current_hamiltonQ.size(); i++) {
+      eigenvaluesQ[i] = diag(current_hamiltonQ[i]);
+      if (g_last_syevd_info > 0) syevd_failures++;
+    }
+    auto t2 = clk::now();
+    set_current_fdag_operator();
+    auto t3 = clk::now();
+    times.assemble = std::chrono::duration<double>(t1 - t0).count();
+    times.diag = std::chrono::duration<double>(t2 - t1).count();
+    times.rotate = std::chrono::duration<double>(t3 - t2).count();
+  }
+
+  // nrgcore::update_internal_state (nrgcore.hpp:105-109)
+  void update_internal_state() {
+    auto t0 = std::chrono::steady_clock::now();
+    nrg_iterations_cnt++;
+    discard_higher_energies();
+    pre_sysmQ = std::move(current_sysmQ);
+    current_sysmQ.clear();
+    times.truncate = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  bool checkHigherEnergyDiscarded() const { return no_of_kept_states >= max_kept_states; }
+
+  // nrgcore::create_next_basis (nrgcore.hpp:238-261)
+  void create_next_basis() {
+    coupled_nQ_index.clear();
+    current_sysmQ.clear();
+    for (size_t i = 0; i < pre_sysmQ.size(); i++)
+      for (size_t j = 0; j < nq_bath.size(); j++) {
+        auto tm = pre_sysmQ[i];
+        for (size_t x = 0; x < nq_bath[j].size(); x++) tm[x] += nq_bath[j][x];
+        bool ex = false;
+        for (size_t x = 0; x < current_sysmQ.size(); x++)
+          if (tm == current_sysmQ[x]) {
+            ex = true;
+            coupled_nQ_index[x].push_back(i * nq_bath.size() + j);
+          }
+        if (!ex) {
+          current_sysmQ.push_back(tm);
+          coupled_nQ_index.push_back({i * nq_bath.size() + j});
+        }
+      }
+  }
+
+  // nrgcore::create_next_hamiltonians (nrgcore.hpp:146-230), same loop order and same FP operation
+  // order: h += ((F_prev * F_bath) * chi) * t, mirrored into h^T.
+  void create_next_hamiltonians(const std::vector<double>& t, double rescale) {
+    if (t.size() != pre_fdag.size())
+      throw std::runtime_error("t_hopping " + std::to_string(t.size()) + " =! pre_fdag_oparator " +
+                               std::to_string(pre_fdag.size()));
+    const size_t nb = nq_bath.size();
+    current_hamiltonQ.assign(coupled_nQ_index.size(), Mat());
+    for (size_t s = 0; s < coupled_nQ_index.size(); s++) {
+      const auto& idx = coupled_nQ_index[s];
+      size_t ldim = 0;
+      for (auto k : idx) ldim += eigenvaluesQ_kept_indices[k / nb].size() * bath_eigenvaluesQ[k % nb].size();
+      Mat h(ldim, ldim);
+      size_t kdim = 0;
+      for (auto k : idx) {
+        size_t ii = k / nb, bb = k % nb;
+        size_t kii = eigenvaluesQ_kept_indices[ii].size();
+        for (size_t it : eigenvaluesQ_kept_indices[ii])
+          for (size_t il = 0; il < bath_eigenvaluesQ[bb].size(); il++)
+            h(kdim + it + il * kii, kdim + it + il * kii) =
+                eigenvaluesQ[ii][it] * rescale + bath_eigenvaluesQ[bb][il];
+        size_t kpdim = 0;
+        for (auto kp : idx) {
+          size_t iip = kp / nb, bbp = kp % nb;
+          size_t kiip = eigenvaluesQ_kept_indices[iip].size();
+          for (size_t ip = 0; ip < pre_fdag.size(); ip++) {
+            const Mat* fp = pre_fdag[ip].get(ii, iip);
+            const Mat* fb = bath_.f_dag_operator[ip].get(bbp, bb);
+            if (fp && fb) {
+              for (size_t it : eigenvaluesQ_kept_indices[ii])
+                for (size_t il = 0; il < bath_eigenvaluesQ[bb].size(); il++)
+                  for (size_t itp : eigenvaluesQ_kept_indices[iip])
+                    for (size_t ilp = 0; ilp < bath_eigenvaluesQ[bbp].size(); ilp++) {
+                      double aa = (*fp)(it, itp) * (*fb)(ilp, il) * chi_bath[bbp];
+                      h(kdim + it + il * kii, kpdim + itp + ilp * kiip) += aa * t[ip];
+                      h(kpdim + itp + ilp * kiip, kdim + it + il * kii) += aa * t[ip];
+                    }
+            }
+          }
+          kpdim += kiip * bath_eigenvaluesQ[bbp].size();
+        }
+        kdim += kii * bath_eigenvaluesQ[bb].size();
+      }
+      current_hamiltonQ[s] = std::move(h);
+    }
+  }
+
+  // nrgcore::discard_higher_energies (nrgcore.hpp:269-327)
+  void discard_higher_energies() {
+    all_eigenvalue.clear();
+    for (auto& a : eigenvaluesQ) all_eigenvalue.insert(all_eigenvalue.end(), a.begin(), a.end());
+    std::sort(all_eigenvalue.begin(), all_eigenvalue.end());
+    eigenvaluesQ_kept_indices.clear();
+    double En_max = all_eigenvalue.back();
+    double En_min = all_eigenvalue[0];
+    if (all_eigenvalue.size() <= max_kept_states) {
+      for (auto& a : eigenvaluesQ) {
+        std::vector<size_t> tm(a.size());
+        std::iota(tm.begin(), tm.end(), 0);
+        eigenvaluesQ_kept_indices.push_back(tm);
+      }
+    } else {
+      En_max = all_eigenvalue[max_kept_states];
+      for (auto& a : eigenvaluesQ) {
+        std::vector<size_t> tm;
+        for (size_t id = 0; id < a.size(); id++)
+          if (a[id] < En_max + errorbarInEnergy) tm.push_back(id);
+        eigenvaluesQ_kept_indices.push_back(tm);
+      }
+    }
+    size_t kept = 0;
+    for (auto& a : eigenvaluesQ_kept_indices) kept += a.size();
+    no_of_kept_states = kept;
+    if (no_of_kept_states <= max_kept_states) {
+      nrg_iterations_min = nrg_iterations_cnt;
+    } else {
+      relativeGroundStateEnergy.push_back(En_min);
+      for (auto& a : eigenvaluesQ)
+        for (auto& x : a) x = x - En_min;
+    }
+  }
+
+  // nrgcore::set_current_fdag_operator (nrgcore.hpp:401-477): the new site's f-dagger with the
+  // system identity, rotated U_i^T . T . U_j, full n_i x n_j blocks.
+  void set_current_fdag_operator() {
+    for (auto& a : pre_fdag) a.clear();
+    const size_t nb = nq_bath.size();
+    std::map<std::array<size_t, 3>, std::vector<std::array<size_t, 6>>> idxmap;
+    for (size_t ip = 0; ip < pre_fdag.size(); ip++)
+      for (size_t i = 0; i < current_sysmQ.size(); i++)
+        for (size_t j = 0; j < current_sysmQ.size(); j++) {
+          size_t kidx = 0;
+          for (auto k : coupled_nQ_index[i]) {
+            size_t ii = k / nb, bb = k % nb;
+            size_t kidxp = 0;
+            for (auto kp : coupled_nQ_index[j]) {
+              size_t iip = kp / nb, bbp = kp % nb;
+              if (ii == iip && bath_.f_dag_operator[ip].get(bb, bbp))
+                idxmap[{ip, i, j}].push_back({ii, bb, iip, bbp, kidx, kidxp});
+              kidxp += eigenvaluesQ_kept_indices[iip].size() * bath_eigenvaluesQ[bbp].size();
+            }
+            kidx += eigenvaluesQ_kept_indices[ii].size() * bath_eigenvaluesQ[bb].size();
+          }
+        }
+    for (auto& [key, lst] : idxmap) {
+      size_t ip = key[0], i = key[1], j = key[2];
+      Mat T(eigenvaluesQ[i].size(), eigenvaluesQ[j].size());
+      for (auto& d : lst) {
+        size_t ii = d[0], bb = d[1], bbp = d[3], kidx = d[4], kidxp = d[5];
+        const Mat* fb = bath_.f_dag_operator[ip].get(bb, bbp);
+        size_t kii = eigenvaluesQ_kept_indices[ii].size();
+        for (size_t il = 0; il < bath_eigenvaluesQ[bb].size(); il++)
+          for (size_t ilp = 0; ilp < bath_eigenvaluesQ[bbp].size(); ilp++) {
+            double rv = (*fb)(il, ilp);
+            for (size_t it : eigenvaluesQ_kept_indices[ii]) T(kidx + it + il * kii, kidxp + it + ilp * kii) += rv;
+          }
+      }
+      pre_fdag[ip].set(dot(transpose(current_hamiltonQ[i]), dot(T, current_hamiltonQ[j])), i, j);
+    }
+  }
+
+  // public state, same names and meaning as nrgcore (nrgcore.hpp:520-550)
+  std::vector<double> all_eigenvalue;
+  std::vector<double> relativeGroundStateEnergy;
+  std::vector<std::vector<size_t>> eigenvaluesQ_kept_indices;
+  std::vector<Mat> current_hamiltonQ;
+  std::vector<std::vector<int>> current_sysmQ;
+  std::vector<std::vector<int>> pre_sysmQ;
+  std::vector<std::vector<double>> eigenvaluesQ;
+  std::vector<std::vector<size_t>> coupled_nQ_index;
+  int nrg_iterations_cnt = 0;
+  int nrg_iterations_min = 0;
+  std::vector<std::vector<double>> bath_eigenvaluesQ;
+  std::vector<std::vector<int>> nq_bath;
+  std::vector<double> chi_bath;
+  std::vector<BlockOp> pre_fdag;
+  size_t no_of_kept_states = 0, max_kept_states = 0;
+  double errorbarInEnergy = 1e-8;
+  // oracle extras
+  bool keep_hamiltonians = false;
+  std::vector<Mat> hamiltonians_before_diag;
+  int syevd_failures = 0;
+  StepTimes times;
+
+  const SiteModel& bath() const { return bath_; }
+
+ private:
+  SiteModel imp_, bath_;
+};
+
+// update_system_operator (nrgcore/include/nrgcore/sysOperator.hpp:35-133): the observable with the
+// new site's identity, dense zero blocks for every (i,j) pair, rotated where the operator has
+// entries.  The dense per-pair allocation is kept (it is part of the reference's cost).
+inline void update_system_operator(const Chain& nrg, std::vector<BlockOp>& ops) {
+  const size_t nb = nrg.nq_bath.size();
+  const size_t S = nrg.current_sysmQ.size();
+  std::vector<BlockOp> upd(ops.size());
+  std::vector<std::vector<size_t>> cum(S);
+  for (size_t i = 0; i < S; i++) {
+    size_t kidx = 0;
+    for (auto k : nrg.coupled_nQ_index[i]) {
+      cum[i].push_back(kidx);
+      kidx += nrg.eigenvaluesQ_kept_indices[k / nb].size() * nrg.bath_eigenvaluesQ[k % nb].size();
+    }
+  }
+  for (size_t i = 0; i < S; i++)
+    for (size_t j = 0; j < S; j++) {
+      Mat tm(nrg.eigenvaluesQ[i].size(), nrg.eigenvaluesQ[j].size());
+      std::vector<Mat> c(ops.size(), tm);
+      std::vector<bool> has(ops.size(), false);
+      for (size_t cd = 0; cd < nrg.coupled_nQ_index[i].size(); cd++)
+        for (size_t cdp = 0; cdp < nrg.coupled_nQ_index[j].size(); cdp++) {
+          size_t kidx = cum[i][cd], k = nrg.coupled_nQ_index[i][cd];
+          size_t ii = k / nb, bb = k % nb;
+          size_t kidxp = cum[j][cdp], kp = nrg.coupled_nQ_index[j][cdp];
+          size_t iip = kp / nb, bbp = kp % nb;
+          if (bb != bbp) continue;
+          for (size_t ip = 0; ip < ops.size(); ip++) {
+            const Mat* so = ops[ip].get(ii, iip);
+            if (!so) continue;
+            has[ip] = true;
+            size_t kii = nrg.eigenvaluesQ_kept_indices[ii].size();
+            size_t kiip = nrg.eigenvaluesQ_kept_indices[iip].size();
+            for (size_t il = 0; il < nrg.bath_eigenvaluesQ[bb].size(); il++)
+              for (size_t it : nrg.eigenvaluesQ_kept_indices[ii])
+                for (size_t itp : nrg.eigenvaluesQ_kept_indices[iip])
+                  c[ip](kidx + it + kii * il, kidxp + itp + kiip * il) += (*so)(it, itp);
+          }
+        }
+      for (size_t iv = 0; iv < has.size(); iv++)
+        if (has[iv])
+          upd[iv].set(dot(transpose(nrg.current_hamiltonQ[i]), dot(c[iv], nrg.current_hamiltonQ[j])), i, j);
+    }
+  ops = upd;
+}
+
+// ---------------------------------------------------------------- forward/backward hand-off
+// What NrgData::saveCurrentData writes per saved iteration (nrgcore/include/nrgcore/nrgData.hpp:134-164),
+// kept in memory instead of an HDF5 temp file.
+struct Shell {
+  int cnt = 0;
+  std::vector<Mat> U;
+  std::vector<std::vector<int>> current_sysmQ, pre_sysmQ;
+  std::vector<std::vector<double>> eigenvaluesQ;
+  std::vector<std::vector<size_t>> coupled_nQ_index;
+  std::vector<std::vector<size_t>> kept_indices;  // of the previous iteration (as saved)
+  std::vector<double> all_eigenvalue;
+  std::vector<BlockOp> ops;  // observable saved beside the shell (saveqOperator)
+};
+
+inline Shell save_shell(const Chain& c) {
+  Shell s;
+  s.cnt = c.nrg_iterations_cnt;
+  s.U = c.current_hamiltonQ;
+  s.current_sysmQ = c.current_sysmQ;
+  s.pre_sysmQ = c.pre_sysmQ;
+  s.eigenvaluesQ = c.eigenvaluesQ;
+  s.coupled_nQ_index = c.coupled_nQ_index;
+  s.kept_indices = c.eigenvaluesQ_kept_indices;
+  s.all_eigenvalue = c.all_eigenvalue;
+  return s;
+}
+
+// ---------------------------------------------------------------- FDM spectrum (T = 0)
+// restates fdmSpectrum (nrgcore/include/dynamics/fdmSpectrum.hpp:24-364).  The shell view replaces
+// the nrgcore members that NrgData::loadCurrentData overwrites.
+struct FdmSpectrum {
+  int energyPts = 100000;
+  double maxEnergy = 10, minEnergy = 1e-10;
+  double delE;
+  double spWeightErrorBar = 1e-20;
+  double energyErrorBar = 1e-5;
+  bool lastiteration = true;
+  double energyRescale = 1;
+  double localGroundStateEnergy = 0, localPartitionFunction = 0;
+  std::vector<std::vector<double>> BoltzmannFactor;
+  std::vector<std::vector<size_t>> currentKeptIndex, previoudKeptIndex;
+  std::vector<Mat> rhoZero, reducedRho;
+  std::vector<std::vector<double>> positiveWeight, negativeWeight;
+  std::vector<double> rhoTraces, specSums;
+  size_t nops = 0;
+
+  explicit FdmSpectrum(size_t n_ops, int pts = 100000) : energyPts(pts) {
+    delE = (energyPts - 1.0) / std::log(maxEnergy / minEnergy);
+    nops = n_ops;
+    positiveWeight.assign(n_ops, std::vector<double>(energyPts, 0.0));
+    negativeWeight.assign(n_ops, std::vector<double>(energyPts, 0.0));
+  }
+
+  // calcSpectrum (fdmSpectrum.hpp:55-63)
+  void calc(const Shell& sh, const std::vector<std::vector<double>>& bathE, double scale) {
+    energyRescale = scale;
+    set_current_index(sh);
+    set_rho_zero(sh);
+    rho_dot_operators(sh);
+    set_reduced_density_matrix(sh, bathE);
+    lastiteration = false;
+  }
+  // setCurrentIndex (fdmSpectrum.hpp:289-298)
+  void set_current_index(const Shell& sh) {
+    if (currentKeptIndex.empty()) currentKeptIndex.assign(sh.current_sysmQ.size(), {});
+    else currentKeptIndex = previoudKeptIndex;
+    previoudKeptIndex = sh.kept_indices;
+  }
+  // setLocalPartitionFunction (fdmSpectrum.hpp:122-155)
+  void set_local_partition(const Shell& sh) {
+    localGroundStateEnergy = 0;
+    localPartitionFunction = 0;
+    for (auto& a : sh.eigenvaluesQ)
+      if (!a.empty()) localGroundStateEnergy = std::min(localGroundStateEnergy, *std::min_element(a.begin(), a.end()));
+    for (size_t i = 0; i < sh.current_sysmQ.size(); i++)
+      for (double e : sh.eigenvaluesQ[i])
+        if (std::fabs(e - localGroundStateEnergy) < energyErrorBar) localPartitionFunction += 1.;
+    BoltzmannFactor = sh.eigenvaluesQ;
+    for (size_t i = 0; i < sh.current_sysmQ.size(); i++)
+      for (size_t e = 0; e < sh.eigenvaluesQ[i].size(); e++)
+        BoltzmannFactor[i][e] =
+            std::fabs(sh.eigenvaluesQ[i][e] - localGroundStateEnergy) < energyErrorBar ? 1. / localPartitionFunction : 0;
+  }
+  // setRhoZero (fdmSpectrum.hpp:165-199)
+  void set_rho_zero(const Shell& sh) {
+    if (lastiteration) set_local_partition(sh);
+    double tr = 0;
+    rhoZero.clear();
+    for (size_t i = 0; i < sh.current_sysmQ.size(); i++) {
+      size_t n = sh.eigenvaluesQ[i].size();
+      Mat t(n, n);
+      if (lastiteration)
+        for (size_t e = 0; e < n; e++) t(e, e) = BoltzmannFactor[i][e];
+      if (!reducedRho.empty())
+        for (auto a : currentKeptIndex[i])
+          for (auto b : currentKeptIndex[i]) t(a, b) = reducedRho[i](a, b);
+      for (size_t e = 0; e < n; e++) tr += t(e, e);
+      rhoZero.push_back(std::move(t));
+    }
+    rhoTraces.push_back(tr);
+  }
+  // rhoDotOperators (fdmSpectrum.hpp:207-277), aOperator == nullptr form.
+  void rho_dot_operators(const Shell& sh) {
+    double specSum = 0;
+    const size_t S = sh.current_sysmQ.size();
+    for (size_t i = 0; i < S; i++)
+      for (size_t j = 0; j < S; j++) {
+        size_t ni = sh.eigenvaluesQ[i].size(), nj = sh.eigenvaluesQ[j].size();
+        for (size_t ip = 0; ip < sh.ops.size(); ip++) {
+          const Mat* B = sh.ops[ip].get(i, j);
+          if (!B) continue;
+          Mat A = transpose(*B);
+          for (auto v : currentKeptIndex[i])
+            for (auto vp : currentKeptIndex[j]) A(vp, v) = 0;
+          Mat rhoA = dot(*B, rhoZero[j]);
+          Mat ARho = dot(rhoZero[i], *B);
+          for (size_t v = 0; v < ni; v++)
+            for (size_t vp = 0; vp < nj; vp++) {
+              double aa = rhoA(v, vp) * A(vp, v);
+              double bbv = ARho(v, vp) * A(vp, v);
+              int k = int(std::log(1 + (std::fabs(energyRescale * (sh.eigenvaluesQ[i][v] - sh.eigenvaluesQ[j][vp])) /
+                                         minEnergy)) * delE);
+              if (k < energyPts && k >= 0) {
+                if (std::fabs(aa) > spWeightErrorBar) {
+                  positiveWeight[ip][k] += std::fabs(aa);
+                  specSum += std::fabs(aa);
+                }
+                if (std::fabs(bbv) > spWeightErrorBar) {
+                  negativeWeight[ip][k] += std::fabs(bbv);
+                  specSum += std::fabs(bbv);
+                }
+              }
+            }
+        }
+      }
+    specSums.push_back(specSum);
+  }
+  // setReduceDensityMatrix (fdmSpectrum.hpp:71-114)
+  void set_reduced_density_matrix(const Shell& sh, const std::vector<std::vector<double>>& bathE) {
+    const size_t nb = bathE.size();
+    reducedRho.clear();
+    for (size_t i = 0; i < sh.pre_sysmQ.size(); i++)
+      reducedRho.emplace_back(sh.kept_indices[i].size(), sh.kept_indices[i].size());
+    for (size_t i = 0; i < sh.current_sysmQ.size(); i++)
+      rhoZero[i] = dot(sh.U[i], dot(rhoZero[i], transpose(sh.U[i])));
+    for (size_t i = 0; i < sh.current_sysmQ.size(); i++) {
+      size_t kidx = 0;
+      for (auto k : sh.coupled_nQ_index[i]) {
+        size_t ii = k / nb, bb = k % nb;
+        size_t kii = sh.kept_indices[ii].size();
+        for (size_t it : sh.kept_indices[ii])
+          for (size_t itp : sh.kept_indices[ii]) {
+            double aa = 0;
+            for (size_t il = 0; il < bathE[bb].size(); il++) aa += rhoZero[i](kidx + it + kii * il, kidx + itp + kii * il);
+            reducedRho[ii](it, itp) += aa;
+          }
+        kidx += kii * bathE[bb].size();
+      }
+    }
+  }
+  // saveFinalData bin centres (fdmSpectrum.hpp:323-337)
+  std::vector<double> energy_points() const {
+    std::vector<double> e(energyPts);
+    for (int i = 0; i < energyPts; i++) e[i] = minEnergy * std::exp(i * 1. / delE);
+    return e;
+  }
+};
+
+// ---------------------------------------------------------------- thermodynamics
+// restates fdmThermodynamics (nrgcore/include/static/fdmThermodynamics.hpp:17-260).
+struct FdmThermo {
+  int nrgMaxIterations = 0;
+  size_t bathDimension = 0;
+  std::vector<double> temperatureArray;
+  std::vector<std::vector<size_t>> currentKeptIndex, previoudKeptIndex;
+  std::vector<std::vector<double>> discardedEigValues, allEigValues;
+
+  FdmThermo(int max_iter, const std::vector<std::vector<double>>& bathE, std::vector<double> T)
+      : nrgMaxIterations(max_iter), temperatureArray(std::move(T)) {
+    for (auto& b : bathE) bathDimension += b.size();
+  }
+  // regulateExp (fdmThermodynamics.hpp:240-249).  low_clamp_zero selects the variant left commented
+  // out at fdmThermodynamics.hpp:245 (return 0 below -100), kept for fixture provenance checks.
+  bool low_clamp_zero = false;
+  double regulate_exp(double x) const {
+    if (x > 100) return std::exp(100);
+    if (x < -100) return low_clamp_zero ? 0.0 : std::exp(-100);
+    return std::exp(x);
+  }
+  // calcThermodynamics (fdmThermodynamics.hpp:42-89), without the optional system operator
+  void calc(const Shell& sh, double scale) {
+    if (currentKeptIndex.empty()) currentKeptIndex.assign(sh.current_sysmQ.size(), {});
+    else currentKeptIndex = previoudKeptIndex;
+    previoudKeptIndex = sh.kept_indices;
+    std::vector<double> ev;
+    for (size_t i = 0; i < sh.eigenvaluesQ.size(); i++)
+      for (size_t j = currentKeptIndex[i].size(); j < sh.eigenvaluesQ[i].size(); j++) ev.push_back(sh.eigenvaluesQ[i][j] * scale);
+    discardedEigValues.push_back(ev);
+    ev.clear();
+    for (size_t i = 0; i < sh.eigenvaluesQ.size(); i++)
+      for (size_t j = 0; j < sh.eigenvaluesQ[i].size(); j++) ev.push_back(sh.eigenvaluesQ[i][j] * scale);
+    allEigValues.push_back(ev);
+  }
+  // saveFinalData (fdmThermodynamics.hpp:101-236): returns (entropy, specificHeat)
+  std::pair<std::vector<double>, std::vector<double>> finish() {
+    std::vector<double> gs(allEigValues.size(), 0);
+    for (size_t it = 0; it < allEigValues.size(); it++) {
+      double r = *std::min_element(allEigValues[it].begin(), allEigValues[it].end());
+      for (auto& x : discardedEigValues[it]) x = x - r;
+      gs[it] = r;
+    }
+    for (size_t it = 0; it + 1 < gs.size(); it++) gs[it] += gs[it + 1];
+    double g0 = gs[0];
+    for (auto& x : gs) x = x - g0;
+    std::vector<double> S(temperatureArray.size()), C(temperatureArray.size());
+    for (size_t itm = 0; itm < temperatureArray.size(); itm++) {
+      double kBT = temperatureArray[itm];
+      int cnt = nrgMaxIterations;
+      std::vector<double> wm(discardedEigValues.size()), Zp(discardedEigValues.size());
+      for (size_t id = 0; id < discardedEigValues.size(); id++) {
+        double prefac = std::pow((double)bathDimension, nrgMaxIterations - cnt);
+        double zi = 0;
+        for (double a : discardedEigValues[id]) zi += regulate_exp((gs[id] - a) / kBT);
+        cnt--;
+        wm[id] = zi * prefac;
+        Zp[id] = zi;
+      }
+      double lpart = std::accumulate(wm.begin(), wm.end(), 0.0);
+      for (auto& a : wm) a = a / lpart;
+      double eAv = 0, eSq = 0;
+      for (size_t id = 0; id < discardedEigValues.size(); id++)
+        for (double a : discardedEigValues[id]) {
+          double blm = regulate_exp((gs[id] - a) / kBT) / Zp[id];
+          eAv += (a - gs[id]) * wm[id] * blm;
+          eSq += (a - gs[id]) * (a - gs[id]) * wm[id] * blm;
+        }
+      S[itm] = std::log(lpart) + (eAv / kBT);
+      C[itm] = (eSq - eAv * eAv) / (kBT * kBT);
+    }
+    return {S, C};
+  }
+};
+
+// log-spaced temperature grid of the entropy drivers (examples/entropySIAM/main.cpp:68-76)
+inline std::vector<double> temperature_grid(double vMin, double vMax, int n) {
+  double del = (n - 1.0) / std::log(vMax / vMin);
+  std::vector<double> T(n);
+  for (int i = 0; i < n; i++) T[i] = vMin * std::exp(std::fabs((double)i) * 1. / del);
+  return T;
+}
+
+// log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44):
+// A(w) = (1/w) sum_k w_k exp(-(ln(w/x_k)/b - b/4)^2) / (b sqrt(pi))
+inline std::vector<double> broaden(const std::vector<double>& x, const std::vector<double>& w,
+                                   const std::vector<double>& omega, double b) {
+  std::vector<double> out(omega.size(), 0.0);
+  double pre = 1.0 / (b * std::sqrt(M_PI));
+  for (size_t o = 0; o < omega.size(); o++) {
+    double s = 0;
+    for (size_t k = 0; k < x.size(); k++) {
+      double y = -b * 0.25 + std::log(omega[o] / x[k]) / b;
+      s += w[k] * std::exp(-y * y);
+    }
+    out[o] = s / omega[o] * pre;
+  }
+  return out;
+}
+
+}  // namespace nrg_oracle
