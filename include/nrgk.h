@@ -1,0 +1,194 @@
+/* nrgk.h — C ABI of the MI355X NRG iteration engine (libnrgk.so).
+ *
+ * The reference (srbhp/nrgplusplus) has no C ABI: its boundary is the header-only C++ template
+ * API nrgcore<im_type, bath_type> (nrgcore/include/nrgcore/nrgcore.hpp:25-550) plus the free
+ * function update_system_operator (nrgcore/include/nrgcore/sysOperator.hpp:35-133) and the
+ * observables fdmSpectrum / fdmThermodynamics.  The host C++ layer (nrgplusplus_amd/include/nrgk/)
+ * keeps that template API verbatim and calls the entry points below; each entry point names the
+ * reference routine it replaces.
+ *
+ * Conventions
+ *  - plain pointers and sizes only; every entry point returns NRGK_OK (0) or a negative status and
+ *    never throws; nrgk_last_error() gives the message of the last failure on the calling thread.
+ *  - matrices are row-major doubles, as qmatrix<double> (nrgcore/include/utils/qmatrix.hpp:66-75);
+ *    eigenvectors are the COLUMNS of a block's U (qmatrix.hpp:390-391).
+ *  - one chain is driven by one host thread; chains are independent and may share a context.
+ *  - all device work of a chain runs on the chain's own HIP stream.
+ */
+#ifndef NRGK_H
+#define NRGK_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRGK_OK 0
+#define NRGK_ERR_INVALID (-1)   /* bad argument (the reference throws std::runtime_error) */
+#define NRGK_ERR_HIP (-2)       /* HIP runtime failure */
+#define NRGK_ERR_NOMEM (-3)     /* device capacity exceeded (HBM admission) */
+#define NRGK_ERR_STATE (-4)     /* call out of order / empty state */
+#define NRGK_ERR_NOCONV (-5)    /* eigensolver did not converge within its sweep budget */
+#define NRGK_ERR_UNSUPPORTED (-6)
+
+/* eigensolver / rotation precision modes */
+#define NRGK_MODE_MIXED 0   /* FP32 block Jacobi + FP64 Ogita-Aishima refinement, batched  */
+#define NRGK_MODE_PLAIN64 1 /* one launch per block: rocSOLVER dsyevd + rocBLAS dgemm, FP64 */
+
+typedef struct nrgk_ctx nrgk_ctx;
+typedef struct nrgk_chain nrgk_chain;
+typedef struct nrgk_obs nrgk_obs;
+typedef struct nrgk_fdm nrgk_fdm;
+
+/* A site model in the reference's duck-typed model concept (nrgcore.hpp:46-48,68-76,119-133):
+ * get_basis() -> qn, get_eigenvaluesQ() -> dims/energies, get_chi_Q() -> chi, f_dag_operator -> ops.
+ * Operator p has op_nblocks[p] blocks (bi,bj) with data dims[bi] x dims[bj] row-major; blocks of all
+ * operators are concatenated in order. */
+typedef struct {
+  int n_sectors;
+  int n_qn;
+  const int* qn;          /* [n_sectors * n_qn] */
+  const int* dims;        /* [n_sectors] */
+  const double* energies; /* [sum dims], ascending within a sector */
+  const double* chi;      /* [n_sectors] */
+  int n_ops;
+  const int* op_nblocks;  /* [n_ops] */
+  const int* op_bi;       /* [sum op_nblocks] */
+  const int* op_bj;       /* [sum op_nblocks] */
+  const double* op_data;  /* concatenated blocks */
+} nrgk_site_desc;
+
+/* ---------------------------------------------------------------- site models (host setup) */
+typedef struct nrgk_site nrgk_site;
+/* spinhalf(eps, U, h) (nrgcore/include/models/spinhalf.hpp:81-124) */
+int nrgk_site_spinhalf(double eps, double U, double h, nrgk_site** out);
+/* twoChannel() (nrgcore/src/models/twoChannel.cpp:12-48) */
+int nrgk_site_two_channel(nrgk_site** out);
+/* spinhalfKondo(J, S) (nrgcore/include/models/spinhalfKondo.hpp:28-166) */
+int nrgk_site_kondo(double J, double S, nrgk_site** out);
+/* f^dagger_sigma chi on a spinhalf site, chi = (-1)^N (examples/fdmSpectrumSiam/main.cpp:86-106);
+ * the operators of the returned site are the observable blocks */
+int nrgk_site_fdag_chi_spinhalf(nrgk_site** out);
+/* impurity.f_dag_operator.push_back(f[0]); push_back(f[1]) (examples/twoChannelSiam/main.cpp:24-25) */
+int nrgk_site_duplicate_ops(nrgk_site* site);
+/* a view of the site; pointers stay valid until the site is destroyed or modified */
+int nrgk_site_view(const nrgk_site* site, nrgk_site_desc* out);
+int nrgk_site_destroy(nrgk_site* site);
+
+/* ---------------------------------------------------------------- context */
+const char* nrgk_last_error(void);
+const char* nrgk_version(void);
+/* opens HIP device `device`; no reference counterpart (the reference has no device) */
+int nrgk_ctx_create(int device, nrgk_ctx** out);
+int nrgk_ctx_destroy(nrgk_ctx* ctx);
+/* bytes of device memory the context may hand to chains (0 = 90% of the free HBM) */
+int nrgk_ctx_set_capacity(nrgk_ctx* ctx, size_t bytes);
+int nrgk_ctx_used_bytes(nrgk_ctx* ctx, size_t* used, size_t* capacity);
+
+/* ---------------------------------------------------------------- chain (nrgcore) */
+/* nrgcore(im_type&, bath_type&) (nrgcore.hpp:44-53) + set_parameters(max_kept) (nrgcore.hpp:336).
+ * The model-consistency checks of nrgcore::test (nrgcore.hpp:117-134) return NRGK_ERR_INVALID. */
+int nrgk_chain_create(nrgk_ctx* ctx, const nrgk_site_desc* impurity, const nrgk_site_desc* bath,
+                      size_t max_kept, int mode, nrgk_chain** out);
+int nrgk_chain_destroy(nrgk_chain* chain);
+/* nrgcore::set_parameters (nrgcore.hpp:336-342): resets the iteration counters */
+int nrgk_chain_set_parameters(nrgk_chain* chain, size_t max_kept);
+/* replaces the bath model read on every add_bath_site (nrgcore.hpp:68-70) */
+int nrgk_chain_set_bath(nrgk_chain* chain, const nrgk_site_desc* bath);
+/* nrgcore::add_bath_site (nrgcore.hpp:65-97): basis, sector assembly, batched eigensolve.
+ * The Wilson-site f-dagger rotation (nrgcore.hpp:91, set_current_fdag_operator 401-477) is
+ * deferred to update_internal_state and restricted to kept x kept blocks, the only part the next
+ * assembly reads (nrgcore.hpp:199-205). */
+int nrgk_chain_add_bath_site(nrgk_chain* chain, const double* t_hopping, int n_hopping, double rescale);
+/* nrgcore::update_internal_state (nrgcore.hpp:105-109): truncation (discard_higher_energies,
+ * nrgcore.hpp:269-327), energy shift, f-dagger rotation */
+int nrgk_chain_update_internal_state(nrgk_chain* chain);
+/* nrgk_chain_add_bath_site + nrgk_chain_update_internal_state in one call */
+int nrgk_chain_step(nrgk_chain* chain, const double* t_hopping, int n_hopping, double rescale);
+/* nrg_iterations_cnt / nrg_iterations_min (nrgcore.hpp:536-538) */
+int nrgk_chain_counters(nrgk_chain* chain, int* cnt, int* cnt_min);
+/* overwrite nrg_iterations_cnt (drivers decrement it, examples/fdmSpectrumSiam/main.cpp:163) */
+int nrgk_chain_set_counter(nrgk_chain* chain, int cnt);
+/* checkHigherEnergyDiscarded (nrgcore.hpp:384-386) */
+int nrgk_chain_higher_energy_discarded(nrgk_chain* chain, int* out);
+/* all_eigenvalue (nrgcore.hpp:520): sorted, pre-shift.  *n receives the length; out may be NULL */
+int nrgk_chain_all_eigenvalues(nrgk_chain* chain, double* out, size_t cap, size_t* n);
+/* relativeGroundStateEnergy (nrgcore.hpp:522) */
+int nrgk_chain_relative_gs(nrgk_chain* chain, double* out, size_t cap, size_t* n);
+/* current_sysmQ (nrgcore.hpp:528) / eigenvaluesQ sizes: qn[n_sectors*n_qn], dims[n_sectors];
+ * both may be NULL to query n_sectors */
+int nrgk_chain_sectors(nrgk_chain* chain, int* n_sectors, int* qn, int* dims);
+/* eigenvaluesQ (nrgcore.hpp:532), concatenated over sectors (shifted after truncation) */
+int nrgk_chain_eigenvalues_q(nrgk_chain* chain, double* out, size_t cap, size_t* n);
+/* eigenvaluesQ_kept_indices sizes (nrgcore.hpp:524); kept indices are always a prefix */
+int nrgk_chain_kept_counts(nrgk_chain* chain, int* out, int cap, int* n_sectors);
+/* current_hamiltonQ[sector] (nrgcore.hpp:526): U after the eigensolve, n x n row-major */
+int nrgk_chain_block_u(nrgk_chain* chain, int sector, double* out, size_t cap);
+/* the assembled sector Hamiltonian before the eigensolve (diagnostic; needs keep_h) */
+int nrgk_chain_keep_hamiltonians(nrgk_chain* chain, int on);
+int nrgk_chain_block_h(nrgk_chain* chain, int sector, double* out, size_t cap);
+/* getPreWilsonSiteOperators (nrgcore.hpp:548): kept x kept blocks of f-dagger op p after
+ * update_internal_state.  bi/bj/rows/cols receive the block table; data the concatenated blocks. */
+int nrgk_chain_fdag_blocks(nrgk_chain* chain, int op, int* n_blocks, int* bi, int* bj, int* rows,
+                           int* cols, double* data, size_t cap);
+/* Overwrite the state the next add_bath_site reads (what NrgData::loadCurrentData does for the
+ * members it restores, nrgcore/include/nrgcore/nrgData.hpp:266-293): previous sectors (qn),
+ * their kept energies and the kept x kept f-dagger blocks.  Layout as nrgk_site_desc with
+ * dims = kept counts. */
+int nrgk_chain_set_state(nrgk_chain* chain, const nrgk_site_desc* state, int cnt);
+/* per-step device timings (ms): assemble, eigensolve, truncate, rotate; and eigensolver stats */
+int nrgk_chain_step_stats(nrgk_chain* chain, double* ms4, int* jacobi_sweeps, int* max_cluster);
+/* refinement statistics of the last eigensolve: Ogita-Aishima steps taken and blocks that did not
+ * meet the convergence test within the step budget */
+int nrgk_chain_refine_stats(nrgk_chain* chain, int* oa_steps, int* unconverged);
+
+/* ---------------------------------------------------------------- observables (sysOperator.hpp) */
+/* A set of system operators (e.g. d^dagger chi) defined on the impurity sectors; blocks given as
+ * in nrgk_site_desc (op_nblocks/op_bi/op_bj/op_data, dims from the impurity). */
+int nrgk_obs_create(nrgk_chain* chain, int n_ops, const int* op_nblocks, const int* op_bi,
+                    const int* op_bj, const double* op_data, nrgk_obs** out);
+int nrgk_obs_destroy(nrgk_obs* obs);
+/* update_system_operator (sysOperator.hpp:35-133): call after add_bath_site, before
+ * update_internal_state.  Produces full n_i x n_j blocks in the new eigenbasis. */
+int nrgk_obs_update(nrgk_obs* obs);
+int nrgk_obs_blocks(nrgk_obs* obs, int op, int* n_blocks, int* bi, int* bj, int* rows, int* cols,
+                    double* data, size_t cap);
+
+/* ---------------------------------------------------------------- FDM spectrum (fdmSpectrum.hpp) */
+/* Recorder of the forward pass (the state NrgData saves per iteration, nrgData.hpp:134-205), kept
+ * resident in HBM.  nrgk_fdm_record is called where the reference drivers call saveCurrentData +
+ * saveqOperator (examples/fdmSpectrumSiam/main.cpp:134-137): after add_bath_site and
+ * nrgk_obs_update, before update_internal_state. */
+int nrgk_fdm_create(nrgk_chain* chain, nrgk_obs* obs, int n_bins, double e_min, double e_max,
+                    nrgk_fdm** out);
+int nrgk_fdm_destroy(nrgk_fdm* fdm);
+int nrgk_fdm_record(nrgk_fdm* fdm);
+/* The T = 0 backward pass (fdmSpectrum::calcSpectrum, fdmSpectrum.hpp:55-63, driven as
+ * examples/fdmSpectrumSiam/main.cpp:158-180) over the recorded shells with iteration counter
+ * cnt >= first_cnt, last shell first.  A shell recorded at counter cnt uses the energy scale
+ * lambda^{-(cnt + scale_offset)/2} (the SIAM drivers: scale_offset = -1).  pos/neg: [n_ops][n_bins],
+ * weights summed over shells; traces/sums: per processed shell (rhoTrace, specSum), may be NULL. */
+int nrgk_fdm_backward(nrgk_fdm* fdm, double lambda, int scale_offset, int first_cnt, double* pos,
+                      double* neg, double* traces, double* sums, int cap_shells, int* n_shells);
+/* bin centres e_min*exp(i/delE) (fdmSpectrum.hpp:323-327) */
+int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out);
+
+/* ---------------------------------------------------------------- thermodynamics */
+/* fdmThermodynamics::calcThermodynamics + saveFinalData (static/fdmThermodynamics.hpp:42-236) over
+ * the shells recorded by an nrgk_fdm recorder (created with obs = NULL for a thermodynamics-only
+ * record), shells with cnt >= first_cnt, energy scale lambda^{-(cnt + scale_offset)/2}
+ * (examples/entropySIAM/main.cpp:83-84: scale_offset = -1; examples/entropyKondo/main.cpp:105:
+ * scale_offset = 0).  Writes entropy S[nT] and specific heat C[nT]. */
+int nrgk_thermo(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, const double* T,
+                int nT, double* S, double* C);
+
+/* ---------------------------------------------------------------- broadening */
+/* log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44): out[o] = (1/w_o) sum_k
+ * weight_k exp(-(ln(w_o/x_k)/b - b/4)^2)/(b sqrt(pi)), on the device */
+int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* weight, int n, const double* omega,
+                 int n_omega, double b, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NRGK_H */

@@ -1,0 +1,545 @@
+"""nrgplusplus_amd — MI355X-native Wilson-chain NRG step behind the nrgcore API.
+
+Python front-end over the C ABI of libnrgk.so (include/nrgk.h).  The class and method names follow
+the reference's C++ API (srbhp/nrgplusplus nrgcore/include/nrgcore/nrgcore.hpp):
+``NrgCore.set_parameters / add_bath_site / update_internal_state / all_eigenvalue ...``.
+
+There is no CPU fallback: every numerical call runs the HIP kernels of libnrgk.so, and importing
+the library fails loudly when it is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libnrgk.so")
+
+NRGK_MODE_MIXED = 0
+NRGK_MODE_PLAIN64 = 1
+
+_LIB = None
+
+
+class SiteDesc(C.Structure):
+    _fields_ = [
+        ("n_sectors", C.c_int),
+        ("n_qn", C.c_int),
+        ("qn", C.POINTER(C.c_int)),
+        ("dims", C.POINTER(C.c_int)),
+        ("energies", C.POINTER(C.c_double)),
+        ("chi", C.POINTER(C.c_double)),
+        ("n_ops", C.c_int),
+        ("op_nblocks", C.POINTER(C.c_int)),
+        ("op_bi", C.POINTER(C.c_int)),
+        ("op_bj", C.POINTER(C.c_int)),
+        ("op_data", C.POINTER(C.c_double)),
+    ]
+
+
+class NrgkError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("nrgk status %d: %s" % (status, msg))
+        self.status = status
+
+
+def build():
+    """Compile libnrgk.so in-tree for gfx950 (hipcc)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-j8", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError("libnrgk.so is not built (run nrgplusplus_amd.build() or make -C nrgplusplus_amd)")
+        L = C.CDLL(LIBPATH)
+        vp, d, i, sz = C.c_void_p, C.c_double, C.c_int, C.c_size_t
+        dp, ip, szp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_size_t)
+        pp = C.POINTER(C.c_void_p)
+        sd = C.POINTER(SiteDesc)
+        sig = {
+            "nrgk_last_error": (C.c_char_p, []),
+            "nrgk_version": (C.c_char_p, []),
+            "nrgk_site_spinhalf": (i, [d, d, d, pp]),
+            "nrgk_site_two_channel": (i, [pp]),
+            "nrgk_site_kondo": (i, [d, d, pp]),
+            "nrgk_site_fdag_chi_spinhalf": (i, [pp]),
+            "nrgk_site_duplicate_ops": (i, [vp]),
+            "nrgk_site_view": (i, [vp, sd]),
+            "nrgk_site_destroy": (i, [vp]),
+            "nrgk_ctx_create": (i, [i, pp]),
+            "nrgk_ctx_destroy": (i, [vp]),
+            "nrgk_ctx_set_capacity": (i, [vp, sz]),
+            "nrgk_ctx_used_bytes": (i, [vp, szp, szp]),
+            "nrgk_chain_create": (i, [vp, sd, sd, sz, i, pp]),
+            "nrgk_chain_destroy": (i, [vp]),
+            "nrgk_chain_set_parameters": (i, [vp, sz]),
+            "nrgk_chain_set_bath": (i, [vp, sd]),
+            "nrgk_chain_add_bath_site": (i, [vp, dp, i, d]),
+            "nrgk_chain_update_internal_state": (i, [vp]),
+            "nrgk_chain_step": (i, [vp, dp, i, d]),
+            "nrgk_chain_counters": (i, [vp, ip, ip]),
+            "nrgk_chain_set_counter": (i, [vp, i]),
+            "nrgk_chain_higher_energy_discarded": (i, [vp, ip]),
+            "nrgk_chain_all_eigenvalues": (i, [vp, dp, sz, szp]),
+            "nrgk_chain_relative_gs": (i, [vp, dp, sz, szp]),
+            "nrgk_chain_sectors": (i, [vp, ip, ip, ip]),
+            "nrgk_chain_eigenvalues_q": (i, [vp, dp, sz, szp]),
+            "nrgk_chain_kept_counts": (i, [vp, ip, i, ip]),
+            "nrgk_chain_block_u": (i, [vp, i, dp, sz]),
+            "nrgk_chain_keep_hamiltonians": (i, [vp, i]),
+            "nrgk_chain_block_h": (i, [vp, i, dp, sz]),
+            "nrgk_chain_fdag_blocks": (i, [vp, i, ip, ip, ip, ip, ip, dp, sz]),
+            "nrgk_chain_set_state": (i, [vp, sd, i]),
+            "nrgk_chain_step_stats": (i, [vp, dp, ip, ip]),
+            "nrgk_chain_refine_stats": (i, [vp, ip, ip]),
+        }
+        optional = {
+            "nrgk_obs_create": (i, [vp, i, ip, ip, ip, dp, pp]),
+            "nrgk_obs_destroy": (i, [vp]),
+            "nrgk_obs_update": (i, [vp]),
+            "nrgk_obs_blocks": (i, [vp, i, ip, ip, ip, ip, ip, dp, sz]),
+            "nrgk_fdm_create": (i, [vp, vp, i, d, d, pp]),
+            "nrgk_fdm_destroy": (i, [vp]),
+            "nrgk_fdm_record": (i, [vp]),
+            "nrgk_fdm_backward": (i, [vp, d, i, i, dp, dp, dp, dp, i, ip]),
+            "nrgk_fdm_energy_points": (i, [vp, dp]),
+            "nrgk_thermo": (i, [vp, d, i, i, dp, i, dp, dp]),
+            "nrgk_broaden": (i, [vp, dp, dp, i, dp, i, d, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        for name, (res, args) in optional.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.restype, f.argtypes = res, args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise NrgkError(rc, lib().nrgk_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+# ---------------------------------------------------------------- site models
+class Site:
+    """A site model (impurity or bath): sectors, energies, chi and f-dagger blocks.
+
+    Built by the library (spinhalf / two_channel / kondo / fdag_chi_spinhalf) or from arrays.
+    """
+
+    def __init__(self, handle=None, arrays=None):
+        self._h = handle
+        self._arrays = arrays
+        self._desc = None
+
+    @classmethod
+    def spinhalf(cls, eps, U, h=0.0):
+        p = C.c_void_p()
+        _check(lib().nrgk_site_spinhalf(eps, U, h, C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def two_channel(cls):
+        p = C.c_void_p()
+        _check(lib().nrgk_site_two_channel(C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def kondo(cls, J, S=0.5):
+        p = C.c_void_p()
+        _check(lib().nrgk_site_kondo(J, S, C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def fdag_chi_spinhalf(cls):
+        p = C.c_void_p()
+        _check(lib().nrgk_site_fdag_chi_spinhalf(C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def from_arrays(cls, qn, dims, energies, chi, ops):
+        """ops: list over operators of [(bi, bj, 2-D block)]"""
+        qn = np.ascontiguousarray(qn, np.int32).reshape(-1)
+        dims = np.ascontiguousarray(dims, np.int32)
+        energies = np.ascontiguousarray(energies, np.float64)
+        chi = np.ascontiguousarray(chi, np.float64)
+        nbl = np.array([len(o) for o in ops], np.int32)
+        bi = np.array([b[0] for o in ops for b in o], np.int32)
+        bj = np.array([b[1] for o in ops for b in o], np.int32)
+        data = np.concatenate([np.ascontiguousarray(b[2], np.float64).reshape(-1) for o in ops for b in o]) \
+            if len(bi) else np.zeros(1)
+        arrays = dict(qn=qn, dims=dims, energies=energies, chi=chi, nbl=nbl, bi=bi, bj=bj, data=data,
+                      n_qn=int(len(qn) // max(1, len(dims))))
+        return cls(None, arrays)
+
+    def duplicate_ops(self):
+        _check(lib().nrgk_site_duplicate_ops(self._h))
+        self._desc = None
+        return self
+
+    def desc(self):
+        if self._h is not None:
+            d = SiteDesc()
+            _check(lib().nrgk_site_view(self._h, C.byref(d)))
+            return d
+        a = self._arrays
+        d = SiteDesc()
+        d.n_sectors = len(a["dims"])
+        d.n_qn = a["n_qn"]
+        d.qn = _ip(a["qn"])
+        d.dims = _ip(a["dims"])
+        d.energies = _dp(a["energies"])
+        d.chi = _dp(a["chi"])
+        d.n_ops = len(a["nbl"])
+        d.op_nblocks = _ip(a["nbl"])
+        d.op_bi = _ip(a["bi"])
+        d.op_bj = _ip(a["bj"])
+        d.op_data = _dp(a["data"])
+        return d
+
+    def export(self):
+        """(qn[S, n_qn], dims[S], energies (concatenated), chi[S], ops=[[(bi, bj, block)]])"""
+        d = self.desc()
+        S, nq = d.n_sectors, d.n_qn
+        qn = np.ctypeslib.as_array(d.qn, (S * nq,)).reshape(S, nq).copy()
+        dims = np.ctypeslib.as_array(d.dims, (S,)).copy()
+        en = np.ctypeslib.as_array(d.energies, (int(dims.sum()),)).copy() if dims.sum() else np.zeros(0)
+        chi = np.ctypeslib.as_array(d.chi, (S,)).copy()
+        nbl = np.ctypeslib.as_array(d.op_nblocks, (d.n_ops,)).copy() if d.n_ops else np.zeros(0, np.int32)
+        tot = int(nbl.sum())
+        bi = np.ctypeslib.as_array(d.op_bi, (tot,)).copy() if tot else np.zeros(0, np.int32)
+        bj = np.ctypeslib.as_array(d.op_bj, (tot,)).copy() if tot else np.zeros(0, np.int32)
+        sizes = dims[bi] * dims[bj]
+        data = np.ctypeslib.as_array(d.op_data, (int(sizes.sum()),)).copy() if tot else np.zeros(0)
+        ops, b, off = [], 0, 0
+        for p in range(d.n_ops):
+            blocks = []
+            for _ in range(nbl[p]):
+                r, c = dims[bi[b]], dims[bj[b]]
+                blocks.append((int(bi[b]), int(bj[b]), data[off: off + r * c].reshape(r, c).copy()))
+                off += r * c
+                b += 1
+            ops.append(blocks)
+        return qn, dims, en, chi, ops
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nrgk_site_destroy(self._h)
+            self._h = None
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One HIP device.  Chains allocate from its capacity (HBM admission)."""
+
+    def __init__(self, device=0):
+        p = C.c_void_p()
+        _check(lib().nrgk_ctx_create(device, C.byref(p)))
+        self._h = p
+
+    def set_capacity(self, nbytes):
+        _check(lib().nrgk_ctx_set_capacity(self._h, nbytes))
+
+    def used_bytes(self):
+        u, c = C.c_size_t(), C.c_size_t()
+        _check(lib().nrgk_ctx_used_bytes(self._h, C.byref(u), C.byref(c)))
+        return u.value, c.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nrgk_ctx_destroy(self._h)
+            self._h = None
+
+
+_DEFAULT_CTX = None
+
+
+def default_context():
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+# ---------------------------------------------------------------- nrgcore
+class NrgCore:
+    """nrgcore<im_type, bath_type> (nrgcore/include/nrgcore/nrgcore.hpp:25-550) on the GPU."""
+
+    def __init__(self, impurity, bath, max_kept=1024, mode=NRGK_MODE_MIXED, ctx=None):
+        self.ctx = ctx or default_context()
+        self.impurity, self.bath = impurity, bath
+        di, db = impurity.desc(), bath.desc()
+        p = C.c_void_p()
+        _check(lib().nrgk_chain_create(self.ctx._h, C.byref(di), C.byref(db), max_kept, mode, C.byref(p)))
+        self._h = p
+        self.n_ops = di.n_ops
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nrgk_chain_destroy(self._h)
+            self._h = None
+
+    # -- the reference's methods
+    def set_parameters(self, n=1024):
+        _check(lib().nrgk_chain_set_parameters(self._h, n))
+
+    def add_bath_site(self, thopping, rescale):
+        t = np.ascontiguousarray(thopping, np.float64)
+        _check(lib().nrgk_chain_add_bath_site(self._h, _dp(t), len(t), float(rescale)))
+
+    def update_internal_state(self):
+        _check(lib().nrgk_chain_update_internal_state(self._h))
+
+    def step(self, thopping, rescale):
+        t = np.ascontiguousarray(thopping, np.float64)
+        _check(lib().nrgk_chain_step(self._h, _dp(t), len(t), float(rescale)))
+
+    def checkHigherEnergyDiscarded(self):
+        o = C.c_int()
+        _check(lib().nrgk_chain_higher_energy_discarded(self._h, C.byref(o)))
+        return bool(o.value)
+
+    @property
+    def nrg_iterations_cnt(self):
+        c, m = C.c_int(), C.c_int()
+        _check(lib().nrgk_chain_counters(self._h, C.byref(c), C.byref(m)))
+        return c.value
+
+    @nrg_iterations_cnt.setter
+    def nrg_iterations_cnt(self, v):
+        _check(lib().nrgk_chain_set_counter(self._h, int(v)))
+
+    @property
+    def nrg_iterations_min(self):
+        c, m = C.c_int(), C.c_int()
+        _check(lib().nrgk_chain_counters(self._h, C.byref(c), C.byref(m)))
+        return m.value
+
+    @property
+    def all_eigenvalue(self):
+        n = C.c_size_t()
+        _check(lib().nrgk_chain_all_eigenvalues(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        _check(lib().nrgk_chain_all_eigenvalues(self._h, _dp(out), n.value, C.byref(n)))
+        return out
+
+    @property
+    def relativeGroundStateEnergy(self):
+        n = C.c_size_t()
+        _check(lib().nrgk_chain_relative_gs(self._h, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value))
+        _check(lib().nrgk_chain_relative_gs(self._h, _dp(out), len(out), C.byref(n)))
+        return out[: n.value]
+
+    def sectors(self):
+        """(current_sysmQ as [S, n_qn], block dimensions [S])"""
+        s = C.c_int()
+        _check(lib().nrgk_chain_sectors(self._h, C.byref(s), None, None))
+        qn = np.zeros((s.value, 2), np.int32)
+        dims = np.zeros(s.value, np.int32)
+        _check(lib().nrgk_chain_sectors(self._h, C.byref(s), _ip(qn), _ip(dims)))
+        return qn, dims
+
+    @property
+    def current_sysmQ(self):
+        return self.sectors()[0]
+
+    @property
+    def eigenvaluesQ(self):
+        _, dims = self.sectors()
+        n = C.c_size_t()
+        out = np.zeros(int(dims.sum()))
+        _check(lib().nrgk_chain_eigenvalues_q(self._h, _dp(out), len(out), C.byref(n)))
+        res, off = [], 0
+        for k in dims:
+            res.append(out[off: off + k].copy())
+            off += k
+        return res
+
+    def kept_counts(self):
+        n = C.c_int()
+        _check(lib().nrgk_chain_kept_counts(self._h, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value), np.int32)
+        _check(lib().nrgk_chain_kept_counts(self._h, _ip(out), len(out), C.byref(n)))
+        return out[: n.value]
+
+    @property
+    def eigenvaluesQ_kept_indices(self):
+        return [list(range(k)) for k in self.kept_counts()]
+
+    def current_hamiltonQ(self, s):
+        """U of sector s (eigenvectors in columns)."""
+        _, dims = self.sectors()
+        n = int(dims[s])
+        out = np.zeros((n, n))
+        _check(lib().nrgk_chain_block_u(self._h, s, _dp(out), out.size))
+        return out
+
+    def keep_hamiltonians(self, on=True):
+        _check(lib().nrgk_chain_keep_hamiltonians(self._h, int(on)))
+
+    def block_h(self, s):
+        _, dims = self.sectors()
+        n = int(dims[s])
+        out = np.zeros((n, n))
+        _check(lib().nrgk_chain_block_h(self._h, s, _dp(out), out.size))
+        return out
+
+    def get_f_dag_operator(self):
+        """kept x kept blocks of each Wilson-site f-dagger: list over ops of [(i, j, block)]"""
+        res = []
+        for p in range(self.n_ops):
+            nb = C.c_int()
+            _check(lib().nrgk_chain_fdag_blocks(self._h, p, C.byref(nb), None, None, None, None, None, 0))
+            k = nb.value
+            bi, bj, r, c = (np.zeros(max(1, k), np.int32) for _ in range(4))
+            _check(lib().nrgk_chain_fdag_blocks(self._h, p, C.byref(nb), _ip(bi), _ip(bj), _ip(r), _ip(c), None, 0))
+            tot = int((r[:k] * c[:k]).sum())
+            data = np.zeros(max(1, tot))
+            _check(lib().nrgk_chain_fdag_blocks(self._h, p, C.byref(nb), _ip(bi), _ip(bj), _ip(r), _ip(c), _dp(data),
+                                                len(data)))
+            blocks, off = [], 0
+            for b in range(k):
+                blocks.append((int(bi[b]), int(bj[b]), data[off: off + r[b] * c[b]].reshape(r[b], c[b]).copy()))
+                off += r[b] * c[b]
+            res.append(blocks)
+        return res
+
+    def set_state(self, state_site, cnt):
+        d = state_site.desc()
+        _check(lib().nrgk_chain_set_state(self._h, C.byref(d), cnt))
+
+    def step_stats(self):
+        ms = np.zeros(4)
+        sw, mc = C.c_int(), C.c_int()
+        _check(lib().nrgk_chain_step_stats(self._h, _dp(ms), C.byref(sw), C.byref(mc)))
+        oa, unc = C.c_int(), C.c_int()
+        _check(lib().nrgk_chain_refine_stats(self._h, C.byref(oa), C.byref(unc)))
+        return dict(ms_assemble=ms[0], ms_eig=ms[1], ms_truncate=ms[2], ms_rotate=ms[3], jacobi_sweeps=sw.value,
+                    max_cluster=mc.value, oa_steps=oa.value, oa_unconverged=unc.value)
+
+
+def _blocks_from(getter, h, n_ops):
+    res = []
+    for p in range(n_ops):
+        nb = C.c_int()
+        _check(getter(h, p, C.byref(nb), None, None, None, None, None, 0))
+        k = nb.value
+        bi, bj, r, c = (np.zeros(max(1, k), np.int32) for _ in range(4))
+        _check(getter(h, p, C.byref(nb), _ip(bi), _ip(bj), _ip(r), _ip(c), None, 0))
+        tot = int((r[:k].astype(np.int64) * c[:k]).sum())
+        data = np.zeros(max(1, tot))
+        _check(getter(h, p, C.byref(nb), _ip(bi), _ip(bj), _ip(r), _ip(c), _dp(data), len(data)))
+        blocks, off = [], 0
+        for b in range(k):
+            blocks.append((int(bi[b]), int(bj[b]), data[off: off + r[b] * c[b]].reshape(r[b], c[b]).copy()))
+            off += r[b] * c[b]
+        res.append(blocks)
+    return res
+
+
+class SystemOperator:
+    """A set of system operators rotated along the chain: update_system_operator
+    (nrgcore/include/nrgcore/sysOperator.hpp:35-133).  ``site`` supplies the blocks in the impurity
+    sectors (e.g. ``Site.fdag_chi_spinhalf()``)."""
+
+    def __init__(self, chain, site):
+        self.chain = chain
+        _, dims, _, _, ops = site.export()
+        self.n_ops = len(ops)
+        nbl = np.array([len(o) for o in ops], np.int32)
+        bi = np.array([b[0] for o in ops for b in o], np.int32)
+        bj = np.array([b[1] for o in ops for b in o], np.int32)
+        data = np.concatenate([b[2].reshape(-1) for o in ops for b in o])
+        p = C.c_void_p()
+        _check(lib().nrgk_obs_create(chain._h, self.n_ops, _ip(nbl), _ip(bi), _ip(bj), _dp(data), C.byref(p)))
+        self._h = p
+
+    def update(self):
+        _check(lib().nrgk_obs_update(self._h))
+
+    def blocks(self):
+        return _blocks_from(lib().nrgk_obs_blocks, self._h, self.n_ops)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nrgk_obs_destroy(self._h)
+            self._h = None
+
+
+def update_system_operator(chain, op):
+    """update_system_operator(nrg_object, operators) (sysOperator.hpp:35-37)"""
+    op.update()
+
+
+class FdmRecorder:
+    """Device-resident hand-off of the forward pass (what NrgData saves per iteration) plus the T = 0
+    FDM backward pass (fdmSpectrum) and the thermodynamics (fdmThermodynamics)."""
+
+    def __init__(self, chain, op=None, n_bins=100000, e_min=1e-10, e_max=10.0):
+        self.chain, self.op = chain, op
+        self.n_bins = n_bins
+        p = C.c_void_p()
+        _check(lib().nrgk_fdm_create(chain._h, op._h if op is not None else None, n_bins, e_min, e_max, C.byref(p)))
+        self._h = p
+
+    def record(self):
+        _check(lib().nrgk_fdm_record(self._h))
+
+    def energy_points(self):
+        out = np.zeros(self.n_bins)
+        _check(lib().nrgk_fdm_energy_points(self._h, _dp(out)))
+        return out
+
+    def backward(self, lam, scale_offset=-1, first_cnt=0, max_shells=4096):
+        n_ops = self.op.n_ops
+        pos = np.zeros(n_ops * self.n_bins)
+        neg = np.zeros(n_ops * self.n_bins)
+        tr = np.zeros(max_shells)
+        sm = np.zeros(max_shells)
+        ns = C.c_int()
+        _check(lib().nrgk_fdm_backward(self._h, lam, scale_offset, first_cnt, _dp(pos), _dp(neg), _dp(tr), _dp(sm),
+                                       max_shells, C.byref(ns)))
+        return pos.reshape(n_ops, self.n_bins), neg.reshape(n_ops, self.n_bins), tr[: ns.value], sm[: ns.value]
+
+    def thermodynamics(self, lam, T, scale_offset=-1, first_cnt=0):
+        T = np.ascontiguousarray(T, np.float64)
+        S = np.zeros(len(T))
+        Cv = np.zeros(len(T))
+        _check(lib().nrgk_thermo(self._h, lam, scale_offset, first_cnt, _dp(T), len(T), _dp(S), _dp(Cv)))
+        return S, Cv
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nrgk_fdm_destroy(self._h)
+            self._h = None
+
+
+def broaden(x, w, omega, b, ctx=None):
+    """log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44), on the device."""
+    ctx = ctx or default_context()
+    x = np.ascontiguousarray(x, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    omega = np.ascontiguousarray(omega, np.float64)
+    out = np.zeros(len(omega))
+    _check(lib().nrgk_broaden(ctx._h, _dp(x), _dp(w), len(x), _dp(omega), len(omega), float(b), _dp(out)))
+    return out
+
+
+def hopping(n, lam):
+    """Wilson-chain hopping (examples/rgflowSIAM/main.cpp:5-10)."""
+    return 0.5 * (1.0 + 1.0 / lam) * (1.0 - lam ** (-n - 1)) / np.sqrt(
+        (1.0 - lam ** (-2.0 * n - 1)) * (1.0 - lam ** (-2.0 * n - 3)))

@@ -1,0 +1,433 @@
+// extern "C" entry points of libnrgk (declared in include/nrgk.h).  Every call converts exceptions
+// into a status code and keeps the message for nrgk_last_error().
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "chain.hpp"
+#include "kernels.hpp"
+#include "observables.hpp"
+
+namespace nrgk {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    f();
+    return NRGK_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return NRGK_ERR_INVALID;
+  }
+}
+}  // namespace nrgk
+
+using namespace nrgk;
+
+struct nrgk_ctx {
+  Ctx c;
+};
+struct nrgk_chain {
+  std::unique_ptr<Chain> c;
+};
+
+extern "C" {
+
+const char* nrgk_last_error(void) { return g_last_error.c_str(); }
+const char* nrgk_version(void) { return "nrgk 0.1 (gfx950)"; }
+
+int nrgk_ctx_create(int device, nrgk_ctx** out) {
+  return guard([&] {
+    NRGK_REQUIRE(out, NRGK_ERR_INVALID, "nrgk_ctx_create: null output");
+    int ndev = 0;
+    NRGK_HIP(hipGetDeviceCount(&ndev));
+    NRGK_REQUIRE(device >= 0 && device < ndev, NRGK_ERR_INVALID, "nrgk_ctx_create: no such device");
+    NRGK_HIP(hipSetDevice(device));
+    auto* h = new nrgk_ctx;
+    h->c.device = device;
+    hipDeviceProp_t prop;
+    NRGK_HIP(hipGetDeviceProperties(&prop, device));
+    h->c.n_cu = prop.multiProcessorCount;
+    int lds = (int)prop.sharedMemPerBlock;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess && optin > lds)
+      lds = optin;
+    h->c.lds_max = lds - 1024;  // room for the kernels' static LDS
+    set_kernel_lds_limits(h->c.lds_max);
+    size_t fr = 0, tot = 0;
+    NRGK_HIP(hipMemGetInfo(&fr, &tot));
+    h->c.capacity = (size_t)(0.9 * (double)fr);
+    *out = h;
+  });
+}
+int nrgk_ctx_destroy(nrgk_ctx* ctx) {
+  delete ctx;
+  return NRGK_OK;
+}
+int nrgk_ctx_set_capacity(nrgk_ctx* ctx, size_t bytes) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx, NRGK_ERR_INVALID, "null context");
+    if (bytes == 0) {
+      size_t fr = 0, tot = 0;
+      NRGK_HIP(hipMemGetInfo(&fr, &tot));
+      bytes = (size_t)(0.9 * (double)fr) + ctx->c.used;
+    }
+    ctx->c.capacity = bytes;
+  });
+}
+int nrgk_ctx_used_bytes(nrgk_ctx* ctx, size_t* used, size_t* capacity) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx, NRGK_ERR_INVALID, "null context");
+    if (used) *used = ctx->c.used;
+    if (capacity) *capacity = ctx->c.capacity;
+  });
+}
+
+int nrgk_chain_create(nrgk_ctx* ctx, const nrgk_site_desc* impurity, const nrgk_site_desc* bath, size_t max_kept,
+                      int mode, nrgk_chain** out) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx && out, NRGK_ERR_INVALID, "nrgk_chain_create: null argument");
+    auto* h = new nrgk_chain;
+    try {
+      h->c = std::make_unique<Chain>(&ctx->c, impurity, bath, max_kept, mode);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int nrgk_chain_destroy(nrgk_chain* chain) {
+  delete chain;
+  return NRGK_OK;
+}
+int nrgk_chain_set_parameters(nrgk_chain* ch, size_t max_kept) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    c.K = max_kept;
+    c.no_kept = max_kept;
+    c.errbar = 1e-8;
+    c.cnt = 0;
+    c.cnt_min = 0;
+    c.phase = Chain::FRESH;
+  });
+}
+int nrgk_chain_set_bath(nrgk_chain* ch, const nrgk_site_desc* bath) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    ch->c->set_bath(bath);
+  });
+}
+int nrgk_chain_add_bath_site(nrgk_chain* ch, const double* t, int n, double rescale) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && (t || n == 0), NRGK_ERR_INVALID, "null argument");
+    ch->c->add_bath_site(t, n, rescale);
+  });
+}
+int nrgk_chain_update_internal_state(nrgk_chain* ch) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    ch->c->update_internal_state();
+  });
+}
+int nrgk_chain_step(nrgk_chain* ch, const double* t, int n, double rescale) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && (t || n == 0), NRGK_ERR_INVALID, "null argument");
+    ch->c->add_bath_site(t, n, rescale);
+    ch->c->update_internal_state();
+  });
+}
+int nrgk_chain_counters(nrgk_chain* ch, int* cnt, int* cnt_min) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    if (cnt) *cnt = ch->c->cnt;
+    if (cnt_min) *cnt_min = ch->c->cnt_min;
+  });
+}
+int nrgk_chain_set_counter(nrgk_chain* ch, int cnt) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    ch->c->cnt = cnt;
+  });
+}
+int nrgk_chain_higher_energy_discarded(nrgk_chain* ch, int* out) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && out, NRGK_ERR_INVALID, "null argument");
+    *out = ch->c->no_kept >= ch->c->K ? 1 : 0;  // checkHigherEnergyDiscarded (nrgcore.hpp:384-386)
+  });
+}
+int nrgk_chain_all_eigenvalues(nrgk_chain* ch, double* out, size_t cap, size_t* n) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    NRGK_REQUIRE(c.phase != Chain::FRESH || !c.all_eig.empty(), NRGK_ERR_STATE, "nrgk: no eigenvalues yet");
+    if (c.phase == Chain::TRUNCATED) c.ensure_all_eig();
+    if (n) *n = c.all_eig.size();
+    if (out) {
+      NRGK_REQUIRE(cap >= c.all_eig.size(), NRGK_ERR_INVALID, "nrgk: output capacity too small");
+      std::memcpy(out, c.all_eig.data(), c.all_eig.size() * sizeof(double));
+    }
+  });
+}
+int nrgk_chain_relative_gs(nrgk_chain* ch, double* out, size_t cap, size_t* n) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    auto& v = ch->c->relgs;
+    if (n) *n = v.size();
+    if (out) {
+      NRGK_REQUIRE(cap >= v.size(), NRGK_ERR_INVALID, "nrgk: output capacity too small");
+      std::memcpy(out, v.data(), v.size() * sizeof(double));
+    }
+  });
+}
+int nrgk_chain_sectors(nrgk_chain* ch, int* n_sectors, int* qn, int* dims) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    NRGK_REQUIRE(!c.cur_qn.empty(), NRGK_ERR_STATE, "current_sysmQ.empty!");
+    if (n_sectors) *n_sectors = (int)c.cur_qn.size();
+    for (size_t s = 0; s < c.cur_qn.size(); s++) {
+      if (qn)
+        for (int q = 0; q < c.imp.nqn; q++) qn[s * c.imp.nqn + q] = c.cur_qn[s][q];
+      if (dims) dims[s] = c.cur_n[s];
+    }
+  });
+}
+int nrgk_chain_eigenvalues_q(nrgk_chain* ch, double* out, size_t cap, size_t* n) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    NRGK_REQUIRE(c.phase != Chain::FRESH, NRGK_ERR_STATE, "eigenvaluesQ.empty!");
+    if (n) *n = (size_t)c.N_tot;
+    if (out) {
+      NRGK_REQUIRE(cap >= (size_t)c.N_tot, NRGK_ERR_INVALID, "nrgk: output capacity too small");
+      const double* src = c.phase == Chain::ASSEMBLED ? c.dlam_raw.as<double>() : c.dlam_shift[c.lcur].as<double>();
+      d2h(out, src, c.N_tot, c.st);
+      NRGK_HIP(hipStreamSynchronize(c.st));
+    }
+  });
+}
+int nrgk_chain_kept_counts(nrgk_chain* ch, int* out, int cap, int* n_sectors) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    const std::vector<int>& k = c.phase == Chain::TRUNCATED ? c.kept : c.pre_kept;
+    if (n_sectors) *n_sectors = (int)k.size();
+    if (out) {
+      NRGK_REQUIRE(cap >= (int)k.size(), NRGK_ERR_INVALID, "nrgk: output capacity too small");
+      std::copy(k.begin(), k.end(), out);
+    }
+  });
+}
+int nrgk_chain_block_u(nrgk_chain* ch, int s, double* out, size_t cap) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && out, NRGK_ERR_INVALID, "null argument");
+    Chain& c = *ch->c;
+    NRGK_REQUIRE(c.phase != Chain::FRESH && s >= 0 && s < (int)c.cur_n.size(), NRGK_ERR_INVALID, "nrgk: bad sector");
+    const size_t sz = (size_t)c.cur_n[s] * c.cur_n[s];
+    NRGK_REQUIRE(cap >= sz, NRGK_ERR_INVALID, "nrgk: output capacity too small");
+    d2h(out, c.dU.as<double>() + c.cur_off[s], sz, c.st);
+    NRGK_HIP(hipStreamSynchronize(c.st));
+  });
+}
+int nrgk_chain_keep_hamiltonians(nrgk_chain* ch, int on) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    ch->c->keep_h = on != 0;
+  });
+}
+int nrgk_chain_block_h(nrgk_chain* ch, int s, double* out, size_t cap) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && out, NRGK_ERR_INVALID, "null argument");
+    Chain& c = *ch->c;
+    NRGK_REQUIRE(s >= 0 && s < (int)c.h_keep.size(), NRGK_ERR_STATE, "nrgk: no kept Hamiltonian (keep_hamiltonians)");
+    NRGK_REQUIRE(cap >= c.h_keep[s].size(), NRGK_ERR_INVALID, "nrgk: output capacity too small");
+    std::memcpy(out, c.h_keep[s].data(), c.h_keep[s].size() * sizeof(double));
+  });
+}
+int nrgk_chain_fdag_blocks(nrgk_chain* ch, int op, int* n_blocks, int* bi, int* bj, int* rows, int* cols,
+                           double* data, size_t cap) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    NRGK_REQUIRE(op >= 0 && op < (int)c.fprev.size(), NRGK_ERR_INVALID, "nrgk: bad operator index");
+    const OpMap& m = c.fprev[op];
+    if (n_blocks) *n_blocks = (int)m.size();
+    size_t need = 0;
+    int b = 0;
+    for (auto& [key, ref] : m) {
+      if (bi) bi[b] = key.first;
+      if (bj) bj[b] = key.second;
+      if (rows) rows[b] = ref.rows;
+      if (cols) cols[b] = ref.cols;
+      need += (size_t)ref.rows * ref.cols;
+      b++;
+    }
+    if (data) {
+      NRGK_REQUIRE(cap >= need, NRGK_ERR_INVALID, "nrgk: output capacity too small");
+      size_t o = 0;
+      for (auto& [key, ref] : m) {
+        d2h(data + o, c.fbuf[c.fcur].as<double>() + ref.off, (size_t)ref.rows * ref.cols, c.st);
+        o += (size_t)ref.rows * ref.cols;
+      }
+      NRGK_HIP(hipStreamSynchronize(c.st));
+    }
+  });
+}
+int nrgk_chain_set_state(nrgk_chain* ch, const nrgk_site_desc* state, int cnt) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && state, NRGK_ERR_INVALID, "null argument");
+    ch->c->set_state(state, cnt);
+  });
+}
+// ---------------------------------------------------------------- observables / FDM / thermo
+struct nrgk_obs {
+  std::unique_ptr<Obs> o;
+};
+struct nrgk_fdm {
+  std::unique_ptr<Fdm> f;
+};
+
+int nrgk_obs_create(nrgk_chain* ch, int n_ops, const int* op_nblocks, const int* op_bi, const int* op_bj,
+                    const double* op_data, nrgk_obs** out) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && out && n_ops > 0 && op_nblocks, NRGK_ERR_INVALID, "nrgk_obs_create: null argument");
+    auto* h = new nrgk_obs;
+    try {
+      h->o = std::make_unique<Obs>(ch->c.get(), n_ops, op_nblocks, op_bi, op_bj, op_data);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int nrgk_obs_destroy(nrgk_obs* obs) {
+  delete obs;
+  return NRGK_OK;
+}
+int nrgk_obs_update(nrgk_obs* obs) {
+  return guard([&] {
+    NRGK_REQUIRE(obs, NRGK_ERR_INVALID, "null observable");
+    obs->o->update();
+  });
+}
+int nrgk_obs_blocks(nrgk_obs* obs, int op, int* n_blocks, int* bi, int* bj, int* rows, int* cols, double* data,
+                    size_t cap) {
+  return guard([&] {
+    NRGK_REQUIRE(obs && op >= 0 && op < obs->o->nops, NRGK_ERR_INVALID, "nrgk: bad observable index");
+    Obs& o = *obs->o;
+    const OpMap& m = o.blocks[op];
+    if (n_blocks) *n_blocks = (int)m.size();
+    size_t need = 0;
+    int b = 0;
+    for (auto& [key, ref] : m) {
+      if (bi) bi[b] = key.first;
+      if (bj) bj[b] = key.second;
+      if (rows) rows[b] = ref.rows;
+      if (cols) cols[b] = ref.cols;
+      need += (size_t)ref.rows * ref.cols;
+      b++;
+    }
+    if (data) {
+      NRGK_REQUIRE(cap >= need, NRGK_ERR_INVALID, "nrgk: output capacity too small");
+      size_t off = 0;
+      for (auto& [key, ref] : m) {
+        d2h(data + off, o.buf[o.cur].as<double>() + ref.off, (size_t)ref.rows * ref.cols, o.ch->st);
+        off += (size_t)ref.rows * ref.cols;
+      }
+      NRGK_HIP(hipStreamSynchronize(o.ch->st));
+    }
+  });
+}
+int nrgk_fdm_create(nrgk_chain* ch, nrgk_obs* obs, int n_bins, double e_min, double e_max, nrgk_fdm** out) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && out, NRGK_ERR_INVALID, "nrgk_fdm_create: null argument");
+    auto* h = new nrgk_fdm;
+    try {
+      h->f = std::make_unique<Fdm>(ch->c.get(), obs ? obs->o.get() : nullptr, n_bins, e_min, e_max);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int nrgk_fdm_destroy(nrgk_fdm* fdm) {
+  delete fdm;
+  return NRGK_OK;
+}
+int nrgk_fdm_record(nrgk_fdm* fdm) {
+  return guard([&] {
+    NRGK_REQUIRE(fdm, NRGK_ERR_INVALID, "null recorder");
+    fdm->f->record();
+  });
+}
+int nrgk_fdm_backward(nrgk_fdm* fdm, double lambda, int scale_offset, int first_cnt, double* pos, double* neg,
+                      double* traces, double* sums, int cap_shells, int* n_shells) {
+  return guard([&] {
+    NRGK_REQUIRE(fdm && pos && neg, NRGK_ERR_INVALID, "nrgk_fdm_backward: null argument");
+    fdm->f->backward(lambda, scale_offset, first_cnt, pos, neg, traces, sums, cap_shells, n_shells);
+  });
+}
+int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out) {
+  return guard([&] {
+    NRGK_REQUIRE(fdm && out, NRGK_ERR_INVALID, "null argument");
+    const Fdm& f = *fdm->f;
+    for (int i = 0; i < f.nbins; i++) out[i] = f.e_min * std::exp(i * 1. / f.delE);
+  });
+}
+int nrgk_thermo(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S,
+                double* C) {
+  return guard([&] {
+    NRGK_REQUIRE(rec && T && S && C && nT > 0, NRGK_ERR_INVALID, "nrgk_thermo: null argument");
+    rec->f->thermo(lambda, scale_offset, first_cnt, T, nT, S, C);
+  });
+}
+int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* w, int n, const double* omega, int n_omega, double b,
+                 double* out) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx && x && w && omega && out && n >= 0 && n_omega >= 0 && b > 0, NRGK_ERR_INVALID,
+                 "nrgk_broaden: bad argument");
+    NRGK_HIP(hipSetDevice(ctx->c.device));
+    DevBuf d(&ctx->c);
+    d.ensure(((size_t)2 * n + 2 * n_omega + 4) * sizeof(double));
+    double* dx = d.as<double>();
+    double* dw = dx + n;
+    double* dom = dw + n;
+    double* dout = dom + n_omega;
+    h2d(dx, x, n, nullptr);
+    h2d(dw, w, n, nullptr);
+    h2d(dom, omega, n_omega, nullptr);
+    launch_broaden(dx, dw, n, dom, n_omega, b, dout, nullptr);
+    d2h(out, dout, n_omega, nullptr);
+    NRGK_HIP(hipStreamSynchronize(nullptr));
+  });
+}
+
+int nrgk_chain_refine_stats(nrgk_chain* ch, int* oa_steps, int* unconverged) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    if (oa_steps) *oa_steps = ch->c->oa_steps;
+    if (unconverged) *unconverged = ch->c->oa_unconverged;
+  });
+}
+int nrgk_chain_step_stats(nrgk_chain* ch, double* ms4, int* sweeps, int* max_cluster) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    if (ms4)
+      for (int i = 0; i < 4; i++) ms4[i] = c.ms[i];
+    if (sweeps) *sweeps = c.sweeps;
+    if (max_cluster) *max_cluster = c.unresolved ? -c.maxcluster : c.maxcluster;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,713 @@
+// Chain orchestration: the device-resident counterpart of nrgcore<imp,bath>
+// (nrgcore/include/nrgcore/nrgcore.hpp).  Integer bookkeeping (sectors, coupled index lists,
+// descriptor tables) stays on the host; every floating-point stage runs on the chain's stream.
+#include "chain.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+
+#include "kernels.hpp"
+
+namespace nrgk {
+
+// ------------------------------------------------------------------ DevBuf
+void DevBuf::ensure(size_t bytes) {
+  if (bytes <= cap) return;
+  release();
+  size_t want = bytes + bytes / 4 + 256;
+  if (ctx) ctx->charge(want);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    if (ctx) ctx->refund(want);
+    p = nullptr;
+    throw Error(NRGK_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+  }
+  cap = want;
+}
+void DevBuf::release() {
+  if (p) {
+    (void)hipFree(p);
+    if (ctx) ctx->refund(cap);
+  }
+  p = nullptr;
+  cap = 0;
+}
+
+template <class T>
+static T* upload(DevBuf& buf, const std::vector<T>& v, hipStream_t s) {
+  buf.ensure(std::max<size_t>(1, v.size()) * sizeof(T));
+  h2d(buf.as<T>(), v.data(), v.size(), s);
+  return buf.as<T>();
+}
+
+// ------------------------------------------------------------------ site models
+SiteData SiteData::from_desc(const nrgk_site_desc* d) {
+  NRGK_REQUIRE(d && d->n_sectors > 0 && d->n_qn > 0 && d->qn && d->dims && d->chi, NRGK_ERR_INVALID,
+               "nrgk: site description is incomplete");
+  SiteData s;
+  s.S = d->n_sectors;
+  s.nqn = d->n_qn;
+  size_t eo = 0;
+  for (int i = 0; i < s.S; i++) {
+    s.qn.emplace_back(d->qn + (size_t)i * s.nqn, d->qn + (size_t)(i + 1) * s.nqn);
+    NRGK_REQUIRE(d->dims[i] >= 0, NRGK_ERR_INVALID, "nrgk: negative sector dimension");
+    s.E.emplace_back(d->energies + eo, d->energies + eo + d->dims[i]);
+    eo += d->dims[i];
+    s.chi.push_back(d->chi[i]);
+  }
+  s.nops = d->n_ops;
+  s.ops.resize(s.nops);
+  size_t bo = 0, dof = 0;
+  for (int p = 0; p < s.nops; p++)
+    for (int b = 0; b < d->op_nblocks[p]; b++, bo++) {
+      const int bi = d->op_bi[bo], bj = d->op_bj[bo];
+      NRGK_REQUIRE(bi >= 0 && bi < s.S && bj >= 0 && bj < s.S, NRGK_ERR_INVALID, "nrgk: operator block out of range");
+      const size_t sz = (size_t)s.dim(bi) * s.dim(bj);
+      NRGK_REQUIRE(!s.ops[p].count({bi, bj}), NRGK_ERR_INVALID, "Found operator already");
+      s.ops[p][{bi, bj}] = std::vector<double>(d->op_data + dof, d->op_data + dof + sz);
+      dof += sz;
+    }
+  return s;
+}
+
+// ------------------------------------------------------------------ chain
+Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, size_t K_, int mode_)
+    : ctx(c), K(K_), mode(mode_) {
+  imp = SiteData::from_desc(imp_d);
+  bath = SiteData::from_desc(bath_d);
+  // nrgcore::test (nrgcore.hpp:117-134)
+  NRGK_REQUIRE(imp.nqn == bath.nqn, NRGK_ERR_INVALID, "Number of quantum number is different on model and impurity!");
+  NRGK_REQUIRE(imp.nops == bath.nops, NRGK_ERR_INVALID,
+               "f_dag_operator.size is different on model and impurity!\nimpurityModel->f_dag_operator.size() = " +
+                   std::to_string(imp.nops) + "bath_model->f_dag_operator.size() = " + std::to_string(bath.nops));
+  NRGK_REQUIRE(mode == NRGK_MODE_MIXED || mode == NRGK_MODE_PLAIN64, NRGK_ERR_INVALID, "nrgk: unknown mode");
+  for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
+                    &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
+                    &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act})
+    b->ctx = ctx;
+  NRGK_HIP(hipSetDevice(ctx->device));
+  NRGK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  for (auto& e : ev) NRGK_HIP(hipEventCreate(&e));
+  upload_bath();
+}
+
+Chain::~Chain() {
+  if (st) (void)hipStreamSynchronize(st);
+  plain_destroy(*this);
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (st) (void)hipStreamDestroy(st);
+}
+
+void Chain::set_bath(const nrgk_site_desc* b) {
+  SiteData nb = SiteData::from_desc(b);
+  NRGK_REQUIRE(nb.nqn == imp.nqn && nb.nops == imp.nops, NRGK_ERR_INVALID, "nrgk: bath model does not match the impurity");
+  bath = std::move(nb);
+  upload_bath();
+}
+
+void Chain::upload_bath() {
+  std::vector<double> e;
+  bath_eoff.assign(bath.S, 0);
+  for (int s = 0; s < bath.S; s++) {
+    bath_eoff[s] = (int)e.size();
+    e.insert(e.end(), bath.E[s].begin(), bath.E[s].end());
+  }
+  upload(d_bathE, e, st);
+  std::vector<double> f;
+  bathF.assign(bath.nops, OpMap());
+  for (int p = 0; p < bath.nops; p++)
+    for (auto& [key, blk] : bath.ops[p]) {
+      bathF[p][key] = BlockRef{(long long)f.size(), bath.dim(key.first), bath.dim(key.second)};
+      f.insert(f.end(), blk.begin(), blk.end());
+    }
+  upload(d_bathF, f, st);
+  NRGK_HIP(hipStreamSynchronize(st));
+}
+
+// First add_bath_site: impurity data and discard_higher_energies on it (nrgcore.hpp:72-78).
+void Chain::import_impurity() {
+  pre_qn = imp.qn;
+  std::vector<double> allE;
+  for (auto& v : imp.E) allE.insert(allE.end(), v.begin(), v.end());
+  NRGK_REQUIRE(!allE.empty(), NRGK_ERR_INVALID, "nrgk: empty impurity");
+  std::vector<double> sorted = allE;
+  std::sort(sorted.begin(), sorted.end());
+  all_eig = sorted;
+  const double emin = sorted[0];
+  pre_kept.assign(imp.S, 0);
+  std::vector<std::vector<double>> E = imp.E;
+  if (sorted.size() <= K) {
+    for (int s = 0; s < imp.S; s++) pre_kept[s] = imp.dim(s);
+  } else {
+    const double emax = sorted[K];
+    for (int s = 0; s < imp.S; s++) {
+      int k = 0;
+      for (double x : imp.E[s])
+        if (x < emax + errbar) k++;
+      pre_kept[s] = k;
+    }
+  }
+  no_kept = 0;
+  for (int k : pre_kept) no_kept += k;
+  if (no_kept <= K) {
+    cnt_min = cnt;
+  } else {
+    relgs.push_back(emin);
+    for (auto& v : E)
+      for (auto& x : v) x = x - emin;
+  }
+  std::vector<double> flat;
+  pre_eoff.assign(imp.S, 0);
+  for (int s = 0; s < imp.S; s++) {
+    pre_eoff[s] = (int)flat.size();
+    flat.insert(flat.end(), E[s].begin(), E[s].end());
+  }
+  d_eprev = upload(eprev_own, flat, st);
+  std::vector<double> f;
+  fprev.assign(imp.nops, OpMap());
+  for (int p = 0; p < imp.nops; p++)
+    for (auto& [key, blk] : imp.ops[p]) {
+      fprev[p][key] = BlockRef{(long long)f.size(), imp.dim(key.first), imp.dim(key.second)};
+      f.insert(f.end(), blk.begin(), blk.end());
+    }
+  upload(fbuf[fcur], f, st);
+  h_last_shift = flat;
+}
+
+// nrgcore::create_next_basis (nrgcore.hpp:238-261)
+void Chain::create_next_basis() {
+  cur_qn.clear();
+  coupled.clear();
+  for (size_t i = 0; i < pre_qn.size(); i++)
+    for (int j = 0; j < bath.S; j++) {
+      std::vector<int> tm = pre_qn[i];
+      for (int x = 0; x < bath.nqn; x++) tm[x] += bath.qn[j][x];
+      bool ex = false;
+      for (size_t s = 0; s < cur_qn.size(); s++)
+        if (tm == cur_qn[s]) {
+          ex = true;
+          coupled[s].push_back({(int)i, j});
+        }
+      if (!ex) {
+        cur_qn.push_back(tm);
+        coupled.push_back({{(int)i, j}});
+      }
+    }
+  const int S = (int)cur_qn.size();
+  cur_n.assign(S, 0);
+  cur_eoff.assign(S, 0);
+  cur_off.assign(S, 0);
+  sub_kidx.assign(S, {});
+  tot_elems = 0;
+  N_tot = 0;
+  nmax = 0;
+  for (int s = 0; s < S; s++) {
+    int n = 0;
+    for (auto [ii, bb] : coupled[s]) {
+      sub_kidx[s].push_back(n);
+      n += pre_kept[ii] * bath.dim(bb);
+    }
+    cur_n[s] = n;
+    cur_eoff[s] = (int)N_tot;
+    cur_off[s] = tot_elems;
+    N_tot += n;
+    tot_elems += (long long)n * n;
+    nmax = std::max(nmax, n);
+  }
+}
+
+// K1 launch (nrgcore::create_next_hamiltonians, nrgcore.hpp:146-230)
+void Chain::assemble(const std::vector<double>& t, double rescale) {
+  const int S = (int)cur_qn.size();
+  std::vector<AsmSector> secs;
+  std::vector<AsmSub> subs;
+  std::vector<AsmCon> cons;
+  int row0 = 0;
+  for (int s = 0; s < S; s++) {
+    if (cur_n[s] == 0) continue;
+    AsmSector sec{};
+    sec.h_off = cur_off[s];
+    sec.n = cur_n[s];
+    sec.row0 = row0;
+    row0 += cur_n[s];
+    sec.sub_begin = (int)subs.size();
+    const auto& cl = coupled[s];
+    for (size_t k = 0; k < cl.size(); k++) {
+      auto [ii, bb] = cl[k];
+      subs.push_back(AsmSub{sub_kidx[s][k], pre_kept[ii], bath.dim(bb), pre_eoff[ii], bath_eoff[bb]});
+    }
+    sec.sub_end = (int)subs.size();
+    sec.con_begin = (int)cons.size();
+    for (size_t k = 0; k < cl.size(); k++) {
+      auto [ii, bb] = cl[k];
+      if (pre_kept[ii] == 0 || bath.dim(bb) == 0) continue;
+      for (size_t kp = 0; kp < cl.size(); kp++) {
+        auto [iip, bbp] = cl[kp];
+        if (pre_kept[iip] == 0 || bath.dim(bbp) == 0) continue;
+        for (int p = 0; p < imp.nops; p++) {
+          auto fp = fprev[p].find({ii, iip});
+          auto fb = bathF[p].find({bbp, bb});
+          if (fp == fprev[p].end() || fb == bathF[p].end()) continue;
+          AsmCon c{};
+          c.k = (int)k;
+          c.kp = (int)kp;
+          c.fp_off = fp->second.off;
+          c.fp_ld = fp->second.cols;
+          c.fb_off = (int)fb->second.off;
+          c.fb_ld = fb->second.cols;
+          c.chi = bath.chi[bbp];
+          c.t = t[p];
+          cons.push_back(c);
+        }
+      }
+    }
+    sec.con_end = (int)cons.size();
+    secs.push_back(sec);
+  }
+  // one upload: sectors | subs | cons
+  const size_t b0 = secs.size() * sizeof(AsmSector), b1 = subs.size() * sizeof(AsmSub), b2 = cons.size() * sizeof(AsmCon);
+  const size_t o1 = (b0 + 255) & ~size_t(255), o2 = (o1 + b1 + 255) & ~size_t(255);
+  d_asm.ensure(o2 + b2 + 256);
+  char* base = d_asm.as<char>();
+  if (b0) NRGK_HIP(hipMemcpyAsync(base, secs.data(), b0, hipMemcpyHostToDevice, st));
+  if (b1) NRGK_HIP(hipMemcpyAsync(base + o1, subs.data(), b1, hipMemcpyHostToDevice, st));
+  if (b2) NRGK_HIP(hipMemcpyAsync(base + o2, cons.data(), b2, hipMemcpyHostToDevice, st));
+  dH.ensure(std::max<long long>(1, tot_elems) * sizeof(double));
+  launch_assemble((const AsmSector*)base, (int)secs.size(), row0, (const AsmSub*)(base + o1),
+                  (const AsmCon*)(base + o2), d_eprev, d_bathE.as<double>(), fbuf[fcur].as<double>(),
+                  d_bathF.as<double>(), rescale, dH.as<double>(), st);
+  // the host vectors must stay alive until the copies have been consumed
+  NRGK_HIP(hipStreamSynchronize(st));
+}
+
+void Chain::run_gemms(std::vector<GemmProb>& probs) {
+  if (mode == NRGK_MODE_PLAIN64) {
+    plain_gemms(*this, probs);
+    return;
+  }
+  std::vector<GemmProb> live;
+  for (auto& p : probs)
+    if (p.m > 0 && p.n > 0) live.push_back(p);
+  if (live.empty()) return;
+  const int tiles = prepare_gemm(live);
+  d_gemm.ensure(live.size() * sizeof(GemmProb));
+  h2d(d_gemm.as<GemmProb>(), live.data(), live.size(), st);
+  launch_gemm(d_gemm.as<GemmProb>(), (int)live.size(), tiles, st);
+  NRGK_HIP(hipStreamSynchronize(st));  // keeps the descriptor buffer reusable by the next batch
+}
+
+// tuning knobs read from the environment (defaults are the tuned values)
+static double knob(const char* name, double dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atof(v) : dflt;
+}
+
+static int choose_bw(int n, int lds_max) {
+  if (n <= 32) return std::max(1, (n + 1) / 2);
+  for (int bw = 16; bw >= 2; bw >>= 1)
+    if ((size_t)2 * bw * n * sizeof(float) <= (size_t)lds_max) return bw;
+  return 1;
+}
+
+// K2 + K6: FP32 block Jacobi, then two Ogita-Aishima refinement steps in FP64.
+void Chain::eigensolve_mixed() {
+  const int S = (int)cur_qn.size();
+  std::vector<EigBlock> blks;
+  std::vector<int> row_block, row_local;
+  std::vector<int2> tasks;
+  int nbc_max = 2;
+  size_t panel_lds = 0;
+  for (int s = 0; s < S; s++) {
+    const int n = cur_n[s];
+    if (n == 0) continue;
+    EigBlock b{};
+    b.off = cur_off[s];
+    b.n = n;
+    b.bw = choose_bw(n, ctx->lds_max);
+    NRGK_REQUIRE((size_t)2 * b.bw * n * sizeof(float) <= (size_t)ctx->lds_max, NRGK_ERR_UNSUPPORTED,
+                 "nrgk: sector block of dimension " + std::to_string(n) + " exceeds the Jacobi panel capacity");
+    int nbc = (n + b.bw - 1) / b.bw;
+    if (nbc < 2) nbc = 2;
+    if (nbc & 1) nbc++;
+    b.nbc = nbc;
+    b.e_off = cur_eoff[s];
+    const int bi = (int)blks.size();
+    for (int j = 0; j < nbc / 2; j++) tasks.push_back(make_int2(bi, j));
+    for (int r = 0; r < n; r++) {
+      row_block.push_back(bi);
+      row_local.push_back(r);
+    }
+    nbc_max = std::max(nbc_max, nbc);
+    panel_lds = std::max(panel_lds, (size_t)2 * b.bw * n * sizeof(float));
+    blks.push_back(b);
+  }
+  const int nb = (int)blks.size();
+  const int nrows = (int)row_block.size();
+  d_eig.ensure(nb * sizeof(EigBlock) + 256);
+  h2d(d_eig.as<EigBlock>(), blks.data(), nb, st);
+  const size_t o_rb = 0, o_rl = ((nrows * 4 + 255) / 256) * 256, o_tk = o_rl + ((nrows * 4 + 255) / 256) * 256;
+  d_misc.ensure(o_tk + tasks.size() * sizeof(int2) + 256);
+  char* mb = d_misc.as<char>();
+  h2d((int*)(mb + o_rb), row_block.data(), nrows, st);
+  h2d((int*)(mb + o_rl), row_local.data(), nrows, st);
+  h2d((int2*)(mb + o_tk), tasks.data(), tasks.size(), st);
+  const EigBlock* dblk = d_eig.as<EigBlock>();
+
+  const size_t bytes64 = std::max<long long>(1, tot_elems) * sizeof(double);
+  dW.ensure(std::max<long long>(1, tot_elems) * sizeof(float));
+  d_sigma_bits.ensure(std::max(1, nb) * sizeof(unsigned));
+  d_sigma.ensure(std::max(1, nb) * sizeof(float));
+  NRGK_HIP(hipMemsetAsync(d_sigma_bits.p, 0, std::max(1, nb) * sizeof(unsigned), st));
+  launch_shift_convert(dblk, (const int*)(mb + o_rb), (const int*)(mb + o_rl), nrows, dH.as<double>(),
+                       d_sigma_bits.as<unsigned>(), d_sigma.as<float>(), dW.as<float>(), st);
+  // Jacobi sweeps
+  std::vector<int> flags0(3 * std::max(1, nb), 0);
+  for (int b = 0; b < nb; b++) flags0[3 * b + 2] = 1;
+  d_flags.ensure(flags0.size() * sizeof(int));
+  h2d(d_flags.as<int>(), flags0.data(), flags0.size(), st);
+  const int max_sweeps = 20;
+  // rotate a column pair while cos > jac_tol and the rotation angle tan > jac_ang: the angle test
+  // stops pairs of very different norms at the FP32 floor, the cosine test degenerate pairs
+  const float jac_tol = knob("NRGK_JAC_TOL", 5e-7);
+  const float jac_ang = knob("NRGK_JAC_ANGLE", 1e-7);
+  std::vector<int> hflags(flags0.size());
+  sweeps = 0;
+  bool converged = false;
+  for (int sw = 0; sw < max_sweeps && nb > 0; sw++) {
+    for (int r = 0; r < nbc_max - 1; r++)
+      launch_jacobi_round(dblk, (const int2*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
+                          jac_tol, jac_ang, 3, panel_lds, st);
+    d2h(hflags.data(), d_flags.as<int>(), hflags.size(), st);
+    NRGK_HIP(hipStreamSynchronize(st));
+    sweeps = sw + 1;
+    bool any = false;
+    for (int b = 0; b < nb; b++) any |= hflags[3 * b + sw % 3] != 0;
+    if (!any) {
+      converged = true;
+      break;
+    }
+  }
+  NRGK_REQUIRE(converged || nb == 0, NRGK_ERR_NOCONV, "nrgk: FP32 Jacobi did not converge in 20 sweeps");
+  dX.ensure(bytes64);
+  dX2.ensure(bytes64);
+  dP.ensure(bytes64);
+  dS.ensure(bytes64);
+  dR.ensure(bytes64);
+  dE.ensure(bytes64);
+  dU.ensure(bytes64);
+  dlam_tmp.ensure(std::max<long long>(1, N_tot) * sizeof(double));
+  dlam_raw.ensure(std::max<long long>(1, N_tot) * sizeof(double));
+  launch_normalize_sort(dblk, nb, dW.as<float>(), d_sigma.as<float>(), dX.as<double>(), dlam_tmp.as<double>(),
+                        normalize_sort_lds(nmax), st);
+  // NRGK_DEBUG_FINITE=1: report the first stage and block that produces a non-finite value
+  const bool dbg = knob("NRGK_DEBUG_FINITE", 0) != 0;
+  auto check = [&](const char* stage, const DevBuf& buf, bool is_float, bool per_block_vec) {
+    if (!dbg) return;
+    NRGK_HIP(hipStreamSynchronize(st));
+    for (auto& b : blks) {
+      const size_t cnt = per_block_vec ? (size_t)b.n : (size_t)b.n * b.n;
+      const size_t off = per_block_vec ? (size_t)b.e_off : (size_t)b.off;
+      std::vector<double> v(cnt);
+      if (is_float) {
+        std::vector<float> f(cnt);
+        d2h(f.data(), buf.as<float>() + off, cnt, st);
+        NRGK_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < cnt; i++) v[i] = f[i];
+      } else {
+        d2h(v.data(), buf.as<double>() + off, cnt, st);
+        NRGK_HIP(hipStreamSynchronize(st));
+      }
+      for (double x : v)
+        if (!std::isfinite(x)) {
+          std::fprintf(stderr, "nrgk debug: non-finite after %s in block n=%d off=%lld\n", stage, b.n, b.off);
+          if (const char* path = std::getenv("NRGK_DEBUG_DUMP")) {
+            // raw dump: n, then S, R, E (n*n doubles each) and lam (n) of the block
+            FILE* f = std::fopen(path, "wb");
+            if (f) {
+              std::vector<double> S(b.n * b.n), R(b.n * b.n), E(b.n * b.n), lam(b.n);
+              d2h(S.data(), dS.as<double>() + b.off, S.size(), st);
+              d2h(R.data(), dR.as<double>() + b.off, R.size(), st);
+              d2h(E.data(), dE.as<double>() + b.off, E.size(), st);
+              d2h(lam.data(), dlam_tmp.as<double>() + b.e_off, lam.size(), st);
+              NRGK_HIP(hipStreamSynchronize(st));
+              const double nn = b.n;
+              std::fwrite(&nn, 8, 1, f);
+              std::fwrite(S.data(), 8, S.size(), f);
+              std::fwrite(R.data(), 8, R.size(), f);
+              std::fwrite(E.data(), 8, E.size(), f);
+              std::fwrite(lam.data(), 8, lam.size(), f);
+              std::fclose(f);
+            }
+            std::exit(3);
+          }
+          return;
+        }
+    }
+  };
+  check("shift_convert+jacobi (W)", dW, true, false);
+  check("normalize_sort (X)", dX, false, false);
+  check("normalize_sort (lam)", dlam_tmp, false, true);
+  d_stats.ensure(4 * sizeof(int));
+  NRGK_HIP(hipMemsetAsync(d_stats.p, 0, 4 * sizeof(int), st));
+  // Ogita-Aishima steps, per block until the out-of-cluster corrections are below 1e-8 (the step
+  // after which the Rayleigh quotients are exact to ~1e-16 relative) and the orthogonality defect
+  // below 1e-12.  Converged blocks leave the batch; both X buffers then hold their final vectors.
+  const int oa_max = (int)knob("NRGK_OA_ITERS", 8);
+  const double e_tol = knob("NRGK_OA_ETOL", 1e-8), d_tol = knob("NRGK_OA_DTOL", 1e-12);
+  std::vector<EigBlock> act = blks;
+  d_conv.ensure(std::max(1, nb) * 2 * sizeof(double));
+  oa_steps = 0;
+  for (int it = 0; it < oa_max && !act.empty(); it++) {
+    const int na = (int)act.size();
+    d_eig_act.ensure(na * sizeof(EigBlock));
+    h2d(d_eig_act.as<EigBlock>(), act.data(), na, st);
+    NRGK_HIP(hipMemsetAsync(d_conv.p, 0, na * 2 * sizeof(double), st));
+    std::vector<GemmProb> g1, g2, g3;
+    for (auto& b : act) {
+      const int n = b.n;
+      double* H = dH.as<double>() + b.off;
+      double* X = dX.as<double>() + b.off;
+      double* P = dP.as<double>() + b.off;
+      double* Sm = dS.as<double>() + b.off;
+      double* R = dR.as<double>() + b.off;
+      double* E = dE.as<double>() + b.off;
+      double* X2 = dX2.as<double>() + b.off;
+      g1.push_back(GemmProb{H, X, nullptr, P, n, n, n, n, n, n, n, 0, 0, 1.0, 0.0, 0, 0});
+      g1.push_back(GemmProb{X, X, nullptr, R, n, n, n, n, n, n, n, 1, 0, 1.0, 0.0, 0, 0});
+      g2.push_back(GemmProb{X, P, nullptr, Sm, n, n, n, n, n, n, n, 1, 0, 1.0, 0.0, 0, 0});
+      g3.push_back(GemmProb{X, E, X, X2, n, n, n, n, n, n, n, 0, 0, 1.0, 1.0, 0, 0});
+    }
+    run_gemms(g1);
+    run_gemms(g2);
+    check("P=HX", dP, false, false);
+    check("S=X^T P", dS, false, false);
+    check("R=X^T X", dR, false, false);
+    launch_oa_update(d_eig_act.as<EigBlock>(), na, dS.as<double>(), dR.as<double>(), dE.as<double>(),
+                     dlam_tmp.as<double>(), d_stats.as<int>(), d_conv.as<double>(), oa_update_lds(nmax), st);
+    check("oa_update (E)", dE, false, false);
+    check("oa_update (lam)", dlam_tmp, false, true);
+    std::vector<double> conv(2 * na);
+    d2h(conv.data(), d_conv.as<double>(), conv.size(), st);
+    run_gemms(g3);  // synchronises the stream
+    std::swap(dX, dX2);
+    check("X+XE", dX, false, false);
+    oa_steps = it + 1;
+    std::vector<EigBlock> next;
+    for (int a = 0; a < na; a++) {
+      if (conv[2 * a] > e_tol || conv[2 * a + 1] > d_tol) {
+        next.push_back(act[a]);
+      } else {
+        const EigBlock& b = act[a];
+        NRGK_HIP(hipMemcpyAsync(dX2.as<double>() + b.off, dX.as<double>() + b.off, (size_t)b.n * b.n * sizeof(double),
+                                hipMemcpyDeviceToDevice, st));
+      }
+    }
+    act.swap(next);
+  }
+  oa_unconverged = (int)act.size();
+  launch_final_sort(dblk, nb, dX.as<double>(), dlam_tmp.as<double>(), dU.as<double>(), dlam_raw.as<double>(),
+                    final_sort_lds(nmax), st);
+  int hs[4];
+  d2h(hs, d_stats.as<int>(), 4, st);
+  NRGK_HIP(hipStreamSynchronize(st));
+  maxcluster = hs[0];
+  unresolved = hs[1];
+}
+
+void Chain::add_bath_site(const double* t, int nt, double rescale) {
+  NRGK_REQUIRE(phase != ASSEMBLED, NRGK_ERR_STATE,
+               "nrgk: add_bath_site called twice without update_internal_state");
+  NRGK_REQUIRE(nt == (int)imp.nops, NRGK_ERR_INVALID,
+               std::string("t_hopping ") + std::to_string(nt) + " =! pre_fdag_oparator " + std::to_string(imp.nops));
+  if (cnt == 0 && phase == FRESH) import_impurity();
+  std::vector<double> tv(t, t + nt);
+  NRGK_HIP(hipEventRecord(ev[0], st));
+  create_next_basis();
+  assemble(tv, rescale);
+  NRGK_HIP(hipEventRecord(ev[1], st));
+  if (keep_h) {
+    h_keep.assign(cur_qn.size(), {});
+    for (size_t s = 0; s < cur_qn.size(); s++) {
+      h_keep[s].resize((size_t)cur_n[s] * cur_n[s]);
+      d2h(h_keep[s].data(), dH.as<double>() + cur_off[s], h_keep[s].size(), st);
+    }
+    NRGK_HIP(hipStreamSynchronize(st));
+  }
+  if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
+  else eigensolve_mixed();
+  NRGK_HIP(hipEventRecord(ev[2], st));
+  NRGK_HIP(hipStreamSynchronize(st));
+  NRGK_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
+  NRGK_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
+  phase = ASSEMBLED;
+}
+
+// nrgcore::update_internal_state (nrgcore.hpp:105-109)
+void Chain::update_internal_state() {
+  NRGK_REQUIRE(phase == ASSEMBLED, NRGK_ERR_STATE, "nrgk: update_internal_state without add_bath_site");
+  cnt++;
+  const int S = (int)cur_qn.size();
+  std::vector<int> bn(cur_n), beo(cur_eoff);
+  const size_t o1 = ((S * 4 + 255) / 256) * 256;
+  d_trunc.ensure(2 * o1 + sizeof(TruncOut) + 256);
+  char* tb = d_trunc.as<char>();
+  h2d((int*)tb, bn.data(), S, st);
+  h2d((int*)(tb + o1), beo.data(), S, st);
+  TruncOut* dout = (TruncOut*)(tb + 2 * o1);
+  d_kept.ensure(std::max(1, S) * sizeof(int));
+  const int lnext = lcur ^ 1;
+  dlam_shift[lnext].ensure(std::max<long long>(1, N_tot) * sizeof(double));
+  NRGK_HIP(hipEventRecord(ev[3], st));
+  launch_truncate((const int*)tb, (const int*)(tb + o1), S, dlam_raw.as<double>(), N_tot, (long long)K, errbar,
+                  d_kept.as<int>(), dout, dlam_shift[lnext].as<double>(), st);
+  TruncOut out{};
+  kept.assign(S, 0);
+  d2h(kept.data(), d_kept.as<int>(), S, st);
+  d2h(&out, dout, 1, st);
+  NRGK_HIP(hipStreamSynchronize(st));
+  NRGK_HIP(hipEventRecord(ev[4], st));
+  all_eig.clear();  // sorted lazily from dlam_raw
+  no_kept = (size_t)out.n_kept;
+  if (no_kept <= K) cnt_min = cnt;
+  else relgs.push_back(out.e_min);
+  rotate_fdag();
+  NRGK_HIP(hipEventRecord(ev[5], st));
+  NRGK_HIP(hipStreamSynchronize(st));
+  NRGK_HIP(hipEventElapsedTime(&ms[2], ev[3], ev[4]));
+  NRGK_HIP(hipEventElapsedTime(&ms[3], ev[4], ev[5]));
+  pre_qn = cur_qn;
+  pre_kept = kept;
+  pre_eoff = cur_eoff;
+  lcur = lnext;
+  d_eprev = dlam_shift[lcur].as<double>();
+  h_last_shift.clear();
+  phase = TRUNCATED;
+}
+
+// f-dagger of the new site in the kept basis (nrgcore::set_current_fdag_operator, nrgcore.hpp:401-477)
+void Chain::rotate_fdag() {
+  const int S = (int)cur_qn.size();
+  // (ii, bb) -> (sector, sub index)
+  std::map<std::pair<int, int>, std::pair<int, int>> where;
+  for (int s = 0; s < S; s++)
+    for (size_t k = 0; k < coupled[s].size(); k++) where[coupled[s][k]] = {s, (int)k};
+  std::vector<RotTri> tris;
+  std::vector<RotGroup> groups;
+  std::vector<RotSegment> segs;
+  std::vector<GemmProb> gp;
+  std::vector<OpMap> fnew(imp.nops);
+  long long yoff = 0, foff = 0;
+  struct Pending {
+    int p, i, j;
+  };
+  std::vector<Pending> pend;
+  for (int p = 0; p < imp.nops; p++) {
+    // triples (p, i, j) in the reference's map order: (p, i, j) lexicographic
+    std::map<std::pair<int, int>, std::vector<std::tuple<int, int, int>>> trip;  // (i,j) -> (sub_i, sub_j, fb key)
+    for (int i = 0; i < S; i++)
+      for (size_t k = 0; k < coupled[i].size(); k++) {
+        auto [ii, bb] = coupled[i][k];
+        for (auto& [key, ref] : bathF[p]) {
+          if (key.first != bb) continue;
+          const int bbp = key.second;
+          auto w = where.find({ii, bbp});
+          if (w == where.end()) continue;
+          trip[{i, w->second.first}].push_back({(int)k, w->second.second, bbp});
+        }
+      }
+    for (auto& [ij, lst] : trip) {
+      const int i = ij.first, j = ij.second;
+      const int ki = kept[i], kj = kept[j], ni = cur_n[i], nj = cur_n[j];
+      BlockRef ref{foff, ki, kj};
+      fnew[p][ij] = ref;
+      foff += (long long)ki * kj;
+      if (ki == 0 || kj == 0 || ni == 0) continue;
+      const int tri = (int)tris.size();
+      tris.push_back(RotTri{cur_off[j], nj, kj, yoff});
+      // one group per sub-block of sector i (rows without a coupling stay zero)
+      for (size_t k = 0; k < coupled[i].size(); k++) {
+        auto [ii, bb] = coupled[i][k];
+        const int kii = pre_kept[ii], dbb = bath.dim(bb);
+        if (kii * dbb == 0) continue;
+        RotGroup g{tri, sub_kidx[i][k], kii, dbb, (int)segs.size(), 0};
+        for (auto& [sk, sj, bbp] : lst) {
+          if (sk != (int)k) continue;
+          const BlockRef fb = bathF[p].at({bb, bbp});
+          segs.push_back(RotSegment{sub_kidx[j][sj], bath.dim(bbp), (int)fb.off, fb.cols});
+        }
+        g.seg_end = (int)segs.size();
+        groups.push_back(g);
+      }
+      pend.push_back({p, i, j});
+      gp.push_back(GemmProb{dU.as<double>() + cur_off[i], nullptr, nullptr, nullptr, ki, kj, ni, ni, kj, kj, 0, 1, 0,
+                            1.0, 0.0, 0, 0});
+      gp.back().B = (const double*)(intptr_t)yoff;  // patched below once Y is allocated
+      gp.back().C = (double*)(intptr_t)ref.off;
+      yoff += (long long)ni * kj;
+    }
+  }
+  const int nf = fcur ^ 1;
+  fbuf[nf].ensure(std::max<long long>(1, foff) * sizeof(double));
+  dY.ensure(std::max<long long>(1, yoff) * sizeof(double));
+  for (auto& g : gp) {
+    g.B = dY.as<double>() + (intptr_t)g.B;
+    g.C = fbuf[nf].as<double>() + (intptr_t)g.C;
+  }
+  const size_t b0 = groups.size() * sizeof(RotGroup), b1 = segs.size() * sizeof(RotSegment),
+               b2 = tris.size() * sizeof(RotTri);
+  const size_t o1 = (b0 + 255) & ~size_t(255), o2 = (o1 + b1 + 255) & ~size_t(255);
+  d_rot.ensure(o2 + b2 + 256);
+  char* rb = d_rot.as<char>();
+  if (b0) NRGK_HIP(hipMemcpyAsync(rb, groups.data(), b0, hipMemcpyHostToDevice, st));
+  if (b1) NRGK_HIP(hipMemcpyAsync(rb + o1, segs.data(), b1, hipMemcpyHostToDevice, st));
+  if (b2) NRGK_HIP(hipMemcpyAsync(rb + o2, tris.data(), b2, hipMemcpyHostToDevice, st));
+  launch_fdag_gather((const RotGroup*)rb, (int)groups.size(), (const RotSegment*)(rb + o1), (const RotTri*)(rb + o2),
+                     dU.as<double>(), d_bathF.as<double>(), dY.as<double>(), st);
+  run_gemms(gp);
+  fprev = std::move(fnew);
+  fcur = nf;
+}
+
+void Chain::ensure_all_eig() {
+  if (!all_eig.empty() || N_tot == 0 || phase == FRESH) return;
+  std::vector<double> v(N_tot);
+  d2h(v.data(), dlam_raw.as<double>(), N_tot, st);
+  NRGK_HIP(hipStreamSynchronize(st));
+  std::sort(v.begin(), v.end());
+  all_eig = std::move(v);
+}
+
+// NrgData::loadCurrentData-style state overwrite (nrgData.hpp:266-293): the previous sectors, their
+// kept energies and the kept x kept f-dagger blocks the next assembly reads.
+void Chain::set_state(const nrgk_site_desc* d, int cnt_) {
+  SiteData s = SiteData::from_desc(d);
+  NRGK_REQUIRE(s.nqn == imp.nqn && s.nops == imp.nops, NRGK_ERR_INVALID, "nrgk: state does not match the model");
+  pre_qn = s.qn;
+  pre_kept.assign(s.S, 0);
+  pre_eoff.assign(s.S, 0);
+  std::vector<double> flat;
+  for (int i = 0; i < s.S; i++) {
+    pre_kept[i] = s.dim(i);
+    pre_eoff[i] = (int)flat.size();
+    flat.insert(flat.end(), s.E[i].begin(), s.E[i].end());
+  }
+  d_eprev = upload(eprev_own, flat, st);
+  std::vector<double> f;
+  fprev.assign(s.nops, OpMap());
+  for (int p = 0; p < s.nops; p++)
+    for (auto& [key, blk] : s.ops[p]) {
+      fprev[p][key] = BlockRef{(long long)f.size(), s.dim(key.first), s.dim(key.second)};
+      f.insert(f.end(), blk.begin(), blk.end());
+    }
+  upload(fbuf[fcur], f, st);
+  NRGK_HIP(hipStreamSynchronize(st));
+  cnt = cnt_;
+  phase = TRUNCATED;
+  h_last_shift = flat;
+}
+
+}  // namespace nrgk

@@ -1,0 +1,113 @@
+// Host-side chain state of libnrgk (what nrgcore<imp,bath> holds, nrgcore.hpp:512-550), with the
+// numerical state resident on the device.
+#pragma once
+#include <map>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+
+namespace nrgk {
+
+struct BlockRef {
+  long long off;  // element offset in its arena
+  int rows, cols; // leading dimension == cols
+};
+using OpMap = std::map<std::pair<int, int>, BlockRef>;
+
+// A site model (impurity or bath) in the reference's model concept.
+struct SiteData {
+  int S = 0, nqn = 0;
+  std::vector<std::vector<int>> qn;
+  std::vector<std::vector<double>> E;
+  std::vector<double> chi;
+  int nops = 0;
+  std::vector<std::map<std::pair<int, int>, std::vector<double>>> ops;
+  int dim(int s) const { return (int)E[s].size(); }
+  static SiteData from_desc(const nrgk_site_desc* d);
+};
+
+struct Obs;
+
+struct Chain {
+  Ctx* ctx = nullptr;
+  hipStream_t st = nullptr;
+  SiteData imp, bath;
+  size_t K = 1024;
+  int mode = NRGK_MODE_MIXED;
+  int cnt = 0, cnt_min = 0;
+  size_t no_kept = 0;
+  double errbar = 1e-8;
+  bool keep_h = false;
+  enum Phase { FRESH, ASSEMBLED, TRUNCATED } phase = FRESH;
+
+  // previous (kept) basis, read by the next assembly
+  std::vector<std::vector<int>> pre_qn;
+  std::vector<int> pre_kept;
+  std::vector<int> pre_eoff;
+  const double* d_eprev = nullptr;
+  DevBuf eprev_own;
+  std::vector<OpMap> fprev;
+  DevBuf fbuf[2];
+  int fcur = 0;
+
+  // bath on the device
+  DevBuf d_bathE, d_bathF;
+  std::vector<int> bath_eoff;
+  std::vector<OpMap> bathF;
+
+  // current iteration
+  std::vector<std::vector<int>> cur_qn;
+  std::vector<std::vector<std::pair<int, int>>> coupled;  // (ii, bb) per new sector
+  std::vector<std::vector<int>> sub_kidx;                 // first row of each coupled sub-block
+  std::vector<int> cur_n, cur_eoff;
+  std::vector<long long> cur_off;
+  long long tot_elems = 0;
+  long long N_tot = 0;
+  int nmax = 0;
+  std::vector<int> kept;  // after update_internal_state
+
+  DevBuf dH, dW, dX, dX2, dP, dS, dR, dE, dU, dY, dC;
+  DevBuf dlam_tmp, dlam_raw, dlam_shift[2];
+  int lcur = 0;
+  DevBuf d_asm, d_eig, d_misc, d_rot, d_gemm;
+  DevBuf d_flags, d_sigma_bits, d_sigma, d_stats, d_kept, d_trunc, d_conv, d_eig_act;
+  int oa_steps = 0, oa_unconverged = 0;
+
+  // host mirrors
+  std::vector<double> all_eig;
+  std::vector<double> relgs;
+  std::vector<double> h_last_shift;  // shifted eigenvalues of the previous basis (host copy, lazily)
+
+  // diagnostics
+  float ms[4] = {0, 0, 0, 0};
+  int sweeps = 0, maxcluster = 0, unresolved = 0;
+  hipEvent_t ev[6];
+  std::vector<std::vector<double>> h_keep;  // assembled blocks (keep_h)
+
+  // plain-mode handles (rocSOLVER / rocBLAS), created on first use
+  void* rocblas = nullptr;
+
+  Chain(Ctx* c, const nrgk_site_desc* imp, const nrgk_site_desc* bath, size_t K, int mode);
+  ~Chain();
+  void set_bath(const nrgk_site_desc* b);
+  void upload_bath();
+  void import_impurity();
+  void create_next_basis();
+  void assemble(const std::vector<double>& t, double rescale);
+  void eigensolve_mixed();
+  void eigensolve_plain();
+  void add_bath_site(const double* t, int nt, double rescale);
+  void update_internal_state();
+  void rotate_fdag();
+  void set_state(const nrgk_site_desc* st, int cnt);
+  void ensure_all_eig();
+  void run_gemms(std::vector<GemmProb>& probs);
+};
+
+// plain FP64 mode (plain.cpp)
+void plain_gemms(Chain& c, const std::vector<GemmProb>& probs);
+void plain_destroy(Chain& c);
+
+}  // namespace nrgk

@@ -1,0 +1,203 @@
+// Shared host/device definitions of libnrgk.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nrgk.h"
+
+namespace nrgk {
+
+// ------------------------------------------------------------------ errors
+// Host code throws nrgk::Error; the C entry points catch it, keep the message for
+// nrgk_last_error() and return the status.
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define NRGK_HIP(call)                                                                     \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::nrgk::Error(NRGK_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + \
+                                            " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define NRGK_REQUIRE(cond, status, msg)            \
+  do {                                             \
+    if (!(cond)) throw ::nrgk::Error(status, msg); \
+  } while (0)
+
+void set_last_error(const std::string& m);
+
+// ------------------------------------------------------------------ device buffers
+// A device allocation that only grows; per-iteration scratch reuses it without hipMalloc in the
+// steady state.  Every allocation is charged to the owning context's capacity (HBM admission).
+struct Ctx;
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  Ctx* ctx = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(Ctx* c) : ctx(c) {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap), ctx(o.ctx) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      cap = o.cap;
+      ctx = o.ctx;
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void ensure(size_t bytes);  // grows (contents not preserved)
+  void release();
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Ctx {
+  int device = 0;
+  size_t capacity = 0;
+  size_t used = 0;
+  int lds_max = 65536;  // bytes of LDS one workgroup may allocate
+  int n_cu = 0;
+  void charge(size_t bytes) {
+    if (capacity && used + bytes > capacity)
+      throw Error(NRGK_ERR_NOMEM, "nrgk: device capacity exceeded (" + std::to_string(used + bytes) +
+                                      " > " + std::to_string(capacity) + " bytes)");
+    used += bytes;
+  }
+  void refund(size_t bytes) { used -= bytes < used ? bytes : used; }
+};
+
+template <class T>
+inline void h2d(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (n) NRGK_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+inline void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (n) NRGK_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+// ------------------------------------------------------------------ descriptors (host <-> device)
+// Sector assembly (nrgcore.hpp:146-230).  One AsmSub per coupled (previous sector ii, bath sector
+// bb) of a new sector, in coupled_nQ_index order; one AsmCon per (sub-block k, sub-block kp, op p)
+// for which both F_prev[p](ii_k, ii_kp) and F_bath[p](bb_kp, bb_k) exist, in the reference's
+// program order (k outer, kp inner, p innermost).
+struct AsmSub {
+  int kidx;   // first row of the sub-block inside the sector block
+  int kii;    // kept count of the previous sector ii
+  int dbb;    // dimension of bath sector bb
+  int e_off;  // offset of E_prev[ii] in the kept-energy array
+  int eb_off; // offset of E_bath[bb] in the bath-energy array
+};
+struct AsmCon {
+  int k, kp;
+  long long fp_off;  // F_prev[p](ii_k, ii_kp): kept x kept, row-major, leading dim fp_ld
+  int fp_ld;
+  int fb_off;        // F_bath[p](bb_kp, bb_k): dbb_kp x dbb_k, row-major, leading dim fb_ld
+  int fb_ld;
+  double chi;        // chi_bath[bb_kp]
+  double t;          // t_hopping[p]
+};
+struct AsmSector {
+  long long h_off;  // element offset of the n x n block in the FP64 arena
+  int n;
+  int row0;         // first global row of this sector (prefix of n)
+  int sub_begin, sub_end;
+  int con_begin, con_end;
+};
+
+// Batched eigensolver blocks.
+struct EigBlock {
+  long long off;   // element offset of the n x n block (same in every per-block arena)
+  int n;
+  int nbc;         // number of block columns in the Jacobi tournament (even)
+  int bw;          // block-column width
+  int e_off;       // offset of the block's eigenvalues (prefix of n)
+};
+
+// Batched GEMM problem: C = alpha * op(A) * op(B) + beta * C0, all row-major.
+struct GemmProb {
+  const double* A;
+  const double* B;
+  const double* C0;
+  double* C;
+  int m, n, k;
+  int lda, ldb, ldc, ldc0;
+  int transA, transB;
+  double alpha, beta;
+  int tile0;  // first tile index of this problem (prefix over problems)
+  int tiles_n;
+};
+
+// Operator rotation (k_rotate.hip) and truncation (k_trunc.hip) descriptors.
+struct RotGroup {   // rows [kidx, kidx + kii*dbb) of sector i, sub-block (ii, bb), for one triple
+  int triple;
+  int kidx, kii, dbb;
+  int seg_begin, seg_end;
+};
+
+struct RotSegment {  // a coupled sub-block (ii, bb') of sector j with F_bath[p](bb, bb') present
+  int kidxp, dbbp;
+  int fb_off, fb_ld;
+};
+
+struct RotTri {
+  long long uj_off;  // U_j (n_j x n_j row-major) offset in the U arena
+  int nj, kj;
+  long long y_off;   // Y (n_i x kj row-major) offset in the scratch arena
+};
+
+struct ObsSeg {      // one nonzero sub-block of c(i,j)
+  long long c_off;   // c(i,j) block (n_i x n_j row-major) offset
+  int ldc;           // n_j
+  int kidx, kidxp, kii, kiip, dbb;
+  long long o_off;   // O_prev(ii, ii') block offset
+  int ldo;           // its leading dimension (columns of the full previous block)
+};
+
+// FDM backward pass (k_fdm.hip)
+struct FdmBinItem {
+  long long b_off;    // B(i,j): n_i x n_j row-major
+  long long ra_off;   // rhoA = B rho_j : n_i x kc_j (ld kc_j)
+  long long ar_off;   // ARho = rho_i B : kc_i x n_j (ld n_j)
+  int ni, nj, kci, kcj;
+  int ei, ej;         // eigenvalue offsets of sectors i, j
+  int op;
+  int row0, rows;     // row tile
+  int diag_mode;      // 1: rho diagonal with Boltzmann factors (last shell)
+};
+struct PtItem {       // reducedRho[ii] (kii x kii) and its contributions
+  long long r_off;
+  int kii;
+  int c_begin, c_end;
+};
+struct PtContrib {    // sub-block (ii, bb) of sector i: rows kidx + it + kii*il
+  long long rho_off;  // rho_i (n_i x n_i)
+  int ni, kidx, dbb;
+};
+
+struct TruncOut {
+  double e_cut, e_min, e_max;
+  int truncated;
+  int n_kept;
+};
+
+}  // namespace nrgk

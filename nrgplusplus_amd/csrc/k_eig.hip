@@ -1,0 +1,585 @@
+// K2/K6: batched symmetric eigensolver replacing per-block LAPACKE_dsyevd
+// (nrgcore/include/utils/qmatrix.hpp:375-404; called serially per sector at nrgcore.hpp:85-89).
+//
+// Stage 1 (FP32): block one-sided (Hestenes) Jacobi on W = H + sigma I (positive definite, so
+//   W converges to V diag(lambda + sigma) and no separate V is kept).  Block columns of width bw
+//   are paired by a round-robin tournament; one workgroup owns one pair: it stages the n x 2bw
+//   panel in LDS, runs inner Jacobi sweeps over the panel's columns (16 waves = 16 column pairs in
+//   flight, dot products accumulated in FP64) and writes the panel back.  All blocks of an
+//   iteration advance one tournament round per launch.
+// Stage 2 (FP64): Ogita-Aishima refinement X <- X + X E from S = X^T H X and R = X^T X, with
+//   clusters of close eigenvalues resolved by a Rayleigh-Ritz step in FP64 (one-sided Jacobi on the
+//   cluster's k x k matrix), then eigenvalues sorted ascending with their columns, as dsyevd
+//   returns them (eigenvectors in the columns of U, qmatrix.hpp:390-391).
+#include "common.hpp"
+
+namespace nrgk {
+
+constexpr int JAC_THREADS = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p, int& q) {
+  // N players (even); round in [0, N-1); pair j in [0, N/2)
+  if (j == 0) {
+    p = 0;
+    q = 1 + (round % (N - 1));
+  } else {
+    p = 1 + (j + round) % (N - 1);
+    q = 1 + (N - 1 - j + round) % (N - 1);
+  }
+}
+
+// One tournament round for all blocks.  flags[3*b + s%3] is set when block b rotated during sweep s;
+// a block whose previous sweep did not rotate has converged and its tasks return at once.
+__global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __restrict__ blocks,
+                                                              const int2* __restrict__ tasks, float* __restrict__ W,
+                                                              int sweep, int round, int* __restrict__ flags,
+                                                              float tol, float ang_tol, int max_inner) {
+  extern __shared__ float panel[];
+  __shared__ int s_rot_any, s_rot_sweep;
+  const int2 task = tasks[blockIdx.x];
+  const int b = task.x, j = task.y;
+  const EigBlock blk = blocks[b];
+  const int N = blk.nbc;
+  if (round >= N - 1) return;
+  // slot (sweep+1)%3 is cleared every sweep, also for converged blocks, so that the host never reads
+  // a flag left over from three sweeps earlier
+  if (round == 0 && j == 0 && threadIdx.x == 0) flags[3 * b + (sweep + 1) % 3] = 0;
+  if (flags[3 * b + (sweep + 2) % 3] == 0) return;
+  int p, q;
+  tournament_pair(N, round, j, p, q);
+  const int n = blk.n, bw = blk.bw;
+  const int c0p = p * bw, c0q = q * bw;
+  const int wp = c0p < n ? min(bw, n - c0p) : 0;
+  const int wq = c0q < n ? min(bw, n - c0q) : 0;
+  const int w = wp + wq;
+  if (w == 0) return;
+  float* Wb = W + blk.off;
+  // stage the panel: local column i -> global column
+  for (int i = 0; i < w; i++) {
+    const int gc = i < wp ? c0p + i : c0q + (i - wp);
+    const float* src = Wb + (long long)gc * n;
+    for (int r = threadIdx.x; r < n; r += JAC_THREADS) panel[i * n + r] = src[r];
+  }
+  if (threadIdx.x == 0) s_rot_any = 0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wpad = (w + 1) & ~1;  // even player count (a dummy if w is odd)
+  int rotated_task = 0;
+  for (int inner = 0; inner < max_inner; inner++) {
+    if (threadIdx.x == 0) s_rot_sweep = 0;
+    __syncthreads();
+    for (int step = 0; step < wpad - 1; step++) {
+      for (int pj = wave; pj < wpad / 2; pj += JAC_THREADS / 64) {
+        int a, c;
+        tournament_pair(wpad, step, pj, a, c);
+        if (a >= w || c >= w) continue;
+        float* x = panel + a * n;
+        float* y = panel + c * n;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int r = lane; r < n; r += 64) {
+          const double xv = x[r], yv = y[r];
+          al += xv * xv;
+          be += yv * yv;
+          ga += xv * yv;
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        if (ga != 0.0 && fabs(ga) > (double)tol * sqrt(al * be)) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
+          if (fabs(t) <= (double)ang_tol) continue;
+          const double cs = 1.0 / sqrt(1.0 + t * t);
+          const float c = (float)cs, s = (float)(cs * t);
+          for (int r = lane; r < n; r += 64) {
+            const float xv = x[r], yv = y[r];
+            x[r] = c * xv - s * yv;
+            y[r] = s * xv + c * yv;
+          }
+          if (lane == 0) {
+            s_rot_sweep = 1;
+            s_rot_any = 1;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const int any = s_rot_sweep;
+    __syncthreads();
+    if (!any) break;
+  }
+  rotated_task = s_rot_any;
+  for (int i = 0; i < w; i++) {
+    const int gc = i < wp ? c0p + i : c0q + (i - wp);
+    float* dst = Wb + (long long)gc * n;
+    for (int r = threadIdx.x; r < n; r += JAC_THREADS) dst[r] = panel[i * n + r];
+  }
+  if (rotated_task && threadIdx.x == 0) flags[3 * b + sweep % 3] = 1;
+}
+
+// Column norms of the converged W give lambda + sigma; columns are normalised, sorted by lambda
+// and written as FP64 X (row-major, eigenvectors in columns).
+__global__ void __launch_bounds__(1024) k_normalize_sort(const EigBlock* __restrict__ blocks, const float* __restrict__ W,
+                                                         const float* __restrict__ sigma, double* __restrict__ X,
+                                                         double* __restrict__ lam) {
+  extern __shared__ double sh[];
+  const EigBlock blk = blocks[blockIdx.x];
+  const int n = blk.n;
+  if (n == 0) return;
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  double* key = sh;                 // [np2]
+  int* idx = (int*)(sh + np2);      // [np2]
+  double* nrm = sh + np2 + (np2 + 1) / 2 + 1;  // [n]
+  const float* Wb = W + blk.off;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = wave; c < n; c += 16) {
+    double s = 0.0;
+    for (int r = lane; r < n; r += 64) {
+      const double v = Wb[(long long)c * n + r];
+      s += v * v;
+    }
+    s = wave_sum(s);
+    if (lane == 0) nrm[c] = sqrt(s);
+  }
+  __syncthreads();
+  const double sg = sigma[blockIdx.x];
+  for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+    key[i] = i < n ? nrm[i] - sg : 1e300;
+    idx[i] = i;
+  }
+  __syncthreads();
+  // bitonic sort ascending
+  for (int k = 2; k <= np2; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+        const int l = i ^ jj;
+        if (l > i) {
+          const bool up = ((i & k) == 0);
+          const double a = key[i], bb = key[l];
+          if ((a > bb) == up) {
+            key[i] = bb;
+            key[l] = a;
+            const int t = idx[i];
+            idx[i] = idx[l];
+            idx[l] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  double* Xb = X + blk.off;
+  for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) {
+    const int r = (int)(e / n), c = (int)(e % n);
+    const int src = idx[c];
+    Xb[e] = (double)Wb[(long long)src * n + r] / nrm[src];
+  }
+  for (int c = threadIdx.x; c < n; c += blockDim.x) lam[blk.e_off + c] = key[c];
+}
+
+// ---------------------------------------------------------------- Ogita-Aishima update
+// One workgroup per block.  Inputs S = X^T H X, Rp = X^T X.  Output E (X <- X + X E) and lam.
+//   lt_i = S_ii / Rp_ii; E_ii = (1 - Rp_ii)/2;  E_ij = (S_ij - lt_j Rp_ij)/(lt_j - lt_i)
+// for eigenvalues farther apart than delta = max(32 max_{i!=j}(|S_ij| + |lt_j Rp_ij|) + 64 eps max|lt|,
+// 1e5 sqrt(n) eps max|lt|).
+// Clusters (runs of sorted lt with gaps <= delta) get a Rayleigh-Ritz step: M = 1.5 I - 0.5 Rp_JJ,
+// (M S_JJ M) Q = Q mu, E_JJ = M Q - I and the out-of-cluster rows of E(:,J) are mapped through M Q.
+constexpr int RR_MAX = 64;
+
+__device__ void rr_jacobi(double* Wm, int k, double* mu, int* perm, double* nrm) {
+  // one-sided Jacobi on the SPD k x k matrix Wm (column-major, in LDS); on return the columns are
+  // orthogonal: W = Q diag(mu + shift)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ int s_rot;
+  const int kp = (k + 1) & ~1;
+  for (int sweep = 0; sweep < 30; sweep++) {
+    if (threadIdx.x == 0) s_rot = 0;
+    __syncthreads();
+    for (int step = 0; step < kp - 1; step++) {
+      for (int pj = wave; pj < kp / 2; pj += blockDim.x / 64) {
+        int a, c;
+        tournament_pair(kp, step, pj, a, c);
+        if (a >= k || c >= k) continue;
+        double* x = Wm + a * k;
+        double* y = Wm + c * k;
+        double al = 0, be = 0, ga = 0;
+        for (int r = lane; r < k; r += 64) {
+          al += x[r] * x[r];
+          be += y[r] * y[r];
+          ga += x[r] * y[r];
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+          for (int r = lane; r < k; r += 64) {
+            const double xv = x[r], yv = y[r];
+            x[r] = cs * xv - sn * yv;
+            y[r] = sn * xv + cs * yv;
+          }
+          if (lane == 0) s_rot = 1;
+        }
+      }
+      __syncthreads();
+    }
+    const int any = s_rot;
+    __syncthreads();
+    if (!any) break;
+  }
+  for (int c = wave; c < k; c += blockDim.x / 64) {
+    double s = 0;
+    for (int r = lane; r < k; r += 64) s += Wm[c * k + r] * Wm[c * k + r];
+    s = wave_sum(s);
+    if (lane == 0) nrm[c] = sqrt(s);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < k; c++) perm[c] = c;
+    for (int a = 1; a < k; a++) {  // insertion sort by norm (k small)
+      int v = perm[a];
+      int bpos = a - 1;
+      while (bpos >= 0 && nrm[perm[bpos]] > nrm[v]) {
+        perm[bpos + 1] = perm[bpos];
+        bpos--;
+      }
+      perm[bpos + 1] = v;
+    }
+    for (int c = 0; c < k; c++) mu[c] = nrm[perm[c]];
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__ blocks, const double* __restrict__ S,
+                                                    const double* __restrict__ Rp, double* __restrict__ E,
+                                                    double* __restrict__ lam, int* __restrict__ stats,
+                                                    double* __restrict__ conv) {
+  extern __shared__ double sh[];
+  const EigBlock blk = blocks[blockIdx.x];
+  const int n = blk.n;
+  if (n == 0) return;
+  const double* Sb = S + blk.off;
+  const double* Rb = Rp + blk.off;
+  double* Eb = E + blk.off;
+  // S and R come from separate GEMM tiles and are symmetric only up to rounding; the update uses
+  // their symmetric parts, so that E + E^T = -(R - I) holds to rounding and the step preserves
+  // orthogonality even where a small gap magnifies the GEMM noise
+  auto Ss = [&](int i, int j) { return 0.5 * (Sb[(long long)i * n + j] + Sb[(long long)j * n + i]); };
+  auto Rs = [&](int i, int j) { return 0.5 * (Rb[(long long)i * n + j] + Rb[(long long)j * n + i]); };
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  double* lt = sh;                     // [n]
+  double* key = lt + n;                // [np2]
+  int* idx = (int*)(key + np2);        // [np2]
+  int* cid = idx + np2;                // [n] cluster id (index of its first sorted member)
+  int* cstart = cid + n;               // [n+1] cluster boundaries in sorted order
+  double* red = (double*)(((uintptr_t)(cstart + n + 2) + 7) & ~(uintptr_t)7);  // [32]
+  double* rrW = red + 32;              // [RR_MAX*RR_MAX]
+  double* rrM = rrW + RR_MAX * RR_MAX; // [RR_MAX*RR_MAX] M, then G = M Q
+  double* rrMu = rrM + RR_MAX * RR_MAX;  // [RR_MAX]
+  double* rrN = rrMu + RR_MAX;         // [RR_MAX]
+  int* rrP = (int*)(rrN + RR_MAX);     // [RR_MAX]
+  __shared__ double s_delta;
+  __shared__ int s_ncl;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) lt[i] = Sb[(long long)i * n + i] / Rb[(long long)i * n + i];
+  __syncthreads();
+  // delta
+  double mx = 0.0, ml = 0.0;
+  for (long long e = tid; e < (long long)n * n; e += blockDim.x) {
+    const int i = (int)(e / n), jj = (int)(e % n);
+    if (i != jj) mx = fmax(mx, fabs(Ss(i, jj)) + fabs(lt[jj] * Rs(i, jj)));
+  }
+  for (int i = tid; i < n; i += blockDim.x) ml = fmax(ml, fabs(lt[i]));
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    ml = fmax(ml, __shfl_xor(ml, o, 64));
+  }
+  if ((tid & 63) == 0) {
+    red[tid >> 6] = mx;
+    red[16 + (tid >> 6)] = ml;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, bq = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      a = fmax(a, red[w]);
+      bq = fmax(bq, red[16 + w]);
+    }
+    // Below the rounding noise of S_ij (about sqrt(n) eps max|lt|) the OA quotient is noise over
+    // gap; pairs closer than 1e5 times that noise are handled as clusters, which caps the
+    // out-of-cluster corrections made of pure rounding at about 1e-5 and keeps the iteration at
+    // its fixed point once converged.
+    const double floor_ = 1e5 * sqrt((double)n) * 2.220446049250313e-16 * bq;
+    s_delta = fmax(32.0 * a + 64.0 * 2.220446049250313e-16 * bq, floor_);
+  }
+  // sort lt
+  for (int i = tid; i < np2; i += blockDim.x) {
+    key[i] = i < n ? lt[i] : 1e300;
+    idx[i] = i;
+  }
+  __syncthreads();
+  for (int k = 2; k <= np2; k <<= 1)
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int i = tid; i < np2; i += blockDim.x) {
+        const int l = i ^ jj;
+        if (l > i) {
+          const bool up = ((i & k) == 0);
+          const double a = key[i], bb = key[l];
+          if ((a > bb) == up) {
+            key[i] = bb;
+            key[l] = a;
+            const int t = idx[i];
+            idx[i] = idx[l];
+            idx[l] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  const double delta = s_delta;
+  if (tid == 0) {
+    int ncl = 0;
+    cstart[0] = 0;
+    for (int s = 1; s < n; s++)
+      if (key[s] - key[s - 1] > delta) cstart[++ncl] = s;
+    cstart[++ncl] = n;
+    s_ncl = ncl;
+  }
+  __syncthreads();
+  const int ncl = s_ncl;
+  for (int c = tid; c < ncl; c += blockDim.x)
+    for (int s = cstart[c]; s < cstart[c + 1]; s++) cid[idx[s]] = c;
+  __syncthreads();
+  // standard OA entries; conv[0] = largest out-of-cluster correction, conv[1] = largest
+  // orthogonality defect |R_ij - delta_ij| (the host stops iterating once both are tiny)
+  double emax = 0.0, dmax = 0.0;
+  for (long long e = tid; e < (long long)n * n; e += blockDim.x) {
+    const int i = (int)(e / n), jj = (int)(e % n);
+    double v;
+    if (i == jj) {
+      v = 0.5 * (1.0 - Rb[e]);
+      dmax = fmax(dmax, fabs(1.0 - Rb[e]));
+    } else {
+      dmax = fmax(dmax, fabs(Rs(i, jj)));
+      if (cid[i] == cid[jj]) v = 0.0;
+      else {
+        v = (Ss(i, jj) - lt[jj] * Rs(i, jj)) / (lt[jj] - lt[i]);
+        emax = fmax(emax, fabs(v));
+      }
+    }
+    Eb[e] = v;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    emax = fmax(emax, __shfl_xor(emax, o, 64));
+    dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+  }
+  if ((tid & 63) == 0) {
+    atomicMax((unsigned long long*)&conv[2 * blockIdx.x], (unsigned long long)__double_as_longlong(emax));
+    atomicMax((unsigned long long*)&conv[2 * blockIdx.x + 1], (unsigned long long)__double_as_longlong(dmax));
+  }
+  for (int i = tid; i < n; i += blockDim.x) lam[blk.e_off + i] = lt[i];
+  __syncthreads();
+  // Rayleigh-Ritz on clusters
+  int maxk = 1;
+  for (int c = 0; c < ncl; c++) {
+    const int s0 = cstart[c], k = cstart[c + 1] - s0;
+    if (k < 2) continue;
+    maxk = max(maxk, k);
+    if (k > RR_MAX) {
+      if (tid == 0) atomicOr(&stats[1], 1);  // unresolved cluster: left to the OA formula
+      continue;
+    }
+    // M = 1.5 I - 0.5 Rp_JJ;  T = M S_JJ M  (row-major k x k in rrW after two products)
+    for (int e = tid; e < k * k; e += blockDim.x) {
+      const int a = e / k, bq = e % k;
+      const int ia = idx[s0 + a], ib = idx[s0 + bq];
+      rrM[e] = (a == bq ? 1.5 : 0.0) - 0.5 * Rs(ia, ib);
+    }
+    __syncthreads();
+    for (int e = tid; e < k * k; e += blockDim.x) {  // rrW = S_JJ M
+      const int a = e / k, bq = e % k;
+      double s = 0;
+      for (int l = 0; l < k; l++) s += Ss(idx[s0 + a], idx[s0 + l]) * rrM[l * k + bq];
+      rrW[e] = s;
+    }
+    __syncthreads();
+    // T = M (S M): store into E's cluster block temporarily? use rrN/rrMu as scratch is too small;
+    // compute column by column into registers then write back to rrW after a barrier.
+    double tv[RR_MAX * RR_MAX / 1024 + 1];
+    int cnt = 0;
+    for (int e = tid; e < k * k; e += blockDim.x) {
+      const int a = e / k, bq = e % k;
+      double s = 0;
+      for (int l = 0; l < k; l++) s += rrM[a * k + l] * rrW[l * k + bq];
+      tv[cnt++] = s;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int e = tid; e < k * k; e += blockDim.x) rrW[e] = tv[cnt++];
+    __syncthreads();
+    // shift by the cluster centre and make it positive definite: T' = T - c I + sig I
+    if (tid == 0) {
+      double cen = 0;
+      for (int a = 0; a < k; a++) cen += rrW[a * k + a];
+      cen /= k;
+      double rad = 0;
+      for (int a = 0; a < k; a++) {
+        double s = 0;
+        for (int bq = 0; bq < k; bq++) s += fabs(rrW[a * k + bq] - (a == bq ? cen : 0.0));
+        rad = fmax(rad, s);
+      }
+      // an exactly degenerate cluster has rad == 0: any positive shift then gives Q = I exactly;
+      // the floor keeps the squared column norms far from underflow
+      const double sig = rad > 1e-30 * fmax(1.0, fabs(cen)) ? 1.05 * rad : 1.0;
+      // (T_aa - cen) is exact for a tight cluster; adding sig afterwards keeps the shift even when it
+      // is far below one ulp of cen (T_aa + (sig - cen) would round it away)
+      for (int a = 0; a < k; a++) rrW[a * k + a] = (rrW[a * k + a] - cen) + sig;
+      red[0] = cen;
+      red[1] = sig;
+    }
+    __syncthreads();
+    rr_jacobi(rrW, k, rrMu, rrP, rrN);  // symmetric: column-major == row-major
+    const double cen = red[0], sig = red[1];
+    // Q(:,c) = W(:,perm[c]) / nrm ;  G = M Q  -> rrW reused after computing into registers
+    cnt = 0;
+    for (int e = tid; e < k * k; e += blockDim.x) {
+      const int a = e / k, cc = e % k;
+      const int src = rrP[cc];
+      double s = 0;
+      for (int l = 0; l < k; l++) s += rrM[a * k + l] * (rrW[src * k + l] / rrN[src]);
+      tv[cnt++] = s;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int e = tid; e < k * k; e += blockDim.x) rrM[e] = tv[cnt++];  // rrM := G
+    __syncthreads();
+    // out-of-cluster rows: E(i, J) <- E(i, J) G.  One thread owns a whole row i, so every old value
+    // of the row is read before any of them is overwritten.
+    for (int i = tid; i < n; i += blockDim.x) {
+      if (cid[i] == c) continue;
+      double old[RR_MAX];
+      for (int l = 0; l < k; l++) old[l] = Eb[(long long)i * n + idx[s0 + l]];
+      for (int cc = 0; cc < k; cc++) {
+        double s = 0;
+        for (int l = 0; l < k; l++) s += old[l] * rrM[l * k + cc];
+        Eb[(long long)i * n + idx[s0 + cc]] = s;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < k * k; e += blockDim.x) {
+      const int a = e / k, cc = e % k;
+      Eb[(long long)idx[s0 + a] * n + idx[s0 + cc]] = rrM[e] - (a == cc ? 1.0 : 0.0);
+    }
+    for (int cc = tid; cc < k; cc += blockDim.x) lam[blk.e_off + idx[s0 + cc]] = rrMu[cc] - sig + cen;
+    __syncthreads();
+  }
+  if (tid == 0) atomicMax(&stats[0], maxk);
+}
+
+// Final ordering: U(:, c) = X(:, perm[c]) with eigenvalues ascending (dsyevd order).
+__global__ void __launch_bounds__(1024) k_final_sort(const EigBlock* __restrict__ blocks, const double* __restrict__ X,
+                                                     const double* __restrict__ lam_in, double* __restrict__ U,
+                                                     double* __restrict__ lam_out) {
+  extern __shared__ double sh[];
+  const EigBlock blk = blocks[blockIdx.x];
+  const int n = blk.n;
+  if (n == 0) return;
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  double* key = sh;
+  int* idx = (int*)(sh + np2);
+  for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+    key[i] = i < n ? lam_in[blk.e_off + i] : 1e300;
+    idx[i] = i;
+  }
+  __syncthreads();
+  for (int k = 2; k <= np2; k <<= 1)
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+        const int l = i ^ jj;
+        if (l > i) {
+          const bool up = ((i & k) == 0);
+          const double a = key[i], bb = key[l];
+          if ((a > bb) == up) {
+            key[i] = bb;
+            key[l] = a;
+            const int t = idx[i];
+            idx[i] = idx[l];
+            idx[l] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  const double* Xb = X + blk.off;
+  double* Ub = U + blk.off;
+  for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) {
+    const int r = (int)(e / n), c = (int)(e % n);
+    Ub[e] = Xb[(long long)r * n + idx[c]];
+  }
+  for (int c = threadIdx.x; c < n; c += blockDim.x) lam_out[blk.e_off + c] = key[c];
+}
+
+size_t normalize_sort_lds(int nmax) {
+  int np2 = 1;
+  while (np2 < nmax) np2 <<= 1;
+  return (size_t)(np2 + (np2 + 1) / 2 + 1 + nmax) * sizeof(double) + 64;
+}
+size_t oa_update_lds(int nmax) {
+  int np2 = 1;
+  while (np2 < nmax) np2 <<= 1;
+  size_t b = (size_t)nmax * 8 + np2 * 8 + np2 * 4 + nmax * 4 + (nmax + 2) * 4;
+  b = (b + 7) & ~size_t(7);
+  b += 32 * 8 + 2 * RR_MAX * RR_MAX * 8 + 2 * RR_MAX * 8 + RR_MAX * 4 + 64;
+  return b;
+}
+size_t final_sort_lds(int nmax) {
+  int np2 = 1;
+  while (np2 < nmax) np2 <<= 1;
+  return (size_t)np2 * 12 + 64;
+}
+
+void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks, float* W, int sweep, int round,
+                         int* flags, float tol, float ang_tol, int max_inner, size_t lds, hipStream_t s) {
+  if (n_tasks <= 0) return;
+  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), lds, s, blocks, tasks, W, sweep, round, flags,
+                     tol, ang_tol, max_inner);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
+                           size_t lds, hipStream_t s) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, W, sigma, X, lam);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_oa_update(const EigBlock* blocks, int nb, const double* S, const double* Rp, double* E, double* lam,
+                      int* stats, double* conv, size_t lds, hipStream_t s) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_oa_update, dim3(nb), dim3(1024), lds, s, blocks, S, Rp, E, lam, stats, conv);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
+                       double* lam_out, size_t lds, hipStream_t s) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_final_sort, dim3(nb), dim3(1024), lds, s, blocks, X, lam_in, U, lam_out);
+  NRGK_HIP(hipGetLastError());
+}
+void set_kernel_lds_limits(int bytes) {
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_jacobi_round, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_normalize_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_update, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_final_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+}
+
+}  // namespace nrgk

@@ -1,0 +1,196 @@
+// K9-K12: FDM spectral accumulation, reduced density matrix, thermodynamics and broadening.
+//
+// k_fdm_bin (fdmSpectrum::rhoDotOperators, nrgcore/include/dynamics/fdmSpectrum.hpp:207-277):
+//   w+(v,v') = (B rho_j)(v,v') * B(v,v'),  w-(v,v') = (rho_i B)(v,v') * B(v,v'),
+//   both zero where v and v' are kept states of this shell (the A = B^T kept x kept zeroing, :226-230);
+//   bin = int(ln(1 + |scale (E_i[v] - E_j[v'])| / e_min) * delE) (:245-251); |w| > 1e-20 is added.
+//   One workgroup per (operator block, row tile); the histogram is privatised in LDS and merged
+//   into HBM with FP64 atomics once per workgroup.
+// k_partial_trace (fdmSpectrum::setReduceDensityMatrix, :87-113).
+// k_thermo (fdmThermodynamics::saveFinalData, nrgcore/include/static/fdmThermodynamics.hpp:154-226).
+// k_broaden (examples/fdmSpectrumSiam/spec.py:27-44).
+#include "common.hpp"
+
+namespace nrgk {
+
+constexpr int BIN_THREADS = 256;
+
+__global__ void __launch_bounds__(BIN_THREADS) k_fdm_bin(const FdmBinItem* __restrict__ items, const double* __restrict__ B,
+                                                         const double* __restrict__ RA, const double* __restrict__ AR,
+                                                         const double* __restrict__ E, const double* __restrict__ bf,
+                                                         double scale, double e_min, double delE, int nbins,
+                                                         double* __restrict__ pos, double* __restrict__ neg,
+                                                         double* __restrict__ spec_sum) {
+  extern __shared__ double hist[];  // [2*nbins] when nbins is small enough, else unused
+  const FdmBinItem it = items[blockIdx.x];
+  const bool lds = nbins > 0 && (size_t)nbins * 2 * sizeof(double) <= 65536;
+  if (lds)
+    for (int b = threadIdx.x; b < 2 * nbins; b += blockDim.x) hist[b] = 0.0;
+  __syncthreads();
+  double* gp = pos + (size_t)it.op * nbins;
+  double* gn = neg + (size_t)it.op * nbins;
+  double ssum = 0.0;
+  const long long tot = (long long)it.rows * it.nj;
+  for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int v = it.row0 + (int)(e / it.nj), vp = (int)(e % it.nj);
+    const double b = B[it.b_off + (long long)v * it.nj + vp];
+    if (b == 0.0) continue;
+    double ra, ar;
+    if (it.diag_mode) {
+      ra = b * bf[it.ej + vp];
+      ar = bf[it.ei + v] * b;
+    } else {
+      if (v < it.kci && vp < it.kcj) continue;  // kept x kept excluded
+      ra = vp < it.kcj ? RA[it.ra_off + (long long)v * it.kcj + vp] : 0.0;
+      ar = v < it.kci ? AR[it.ar_off + (long long)v * it.nj + vp] : 0.0;
+    }
+    const double aa = ra * b, bbv = ar * b;
+    const int k = (int)(log(1.0 + (fabs(scale * (E[it.ei + v] - E[it.ej + vp])) / e_min)) * delE);
+    if (k < nbins && k >= 0) {
+      if (fabs(aa) > 1e-20) {
+        if (lds) atomicAdd(&hist[k], fabs(aa));
+        else atomicAdd(&gp[k], fabs(aa));
+        ssum += fabs(aa);
+      }
+      if (fabs(bbv) > 1e-20) {
+        if (lds) atomicAdd(&hist[nbins + k], fabs(bbv));
+        else atomicAdd(&gn[k], fabs(bbv));
+        ssum += fabs(bbv);
+      }
+    }
+  }
+  __syncthreads();
+  if (lds)
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+      if (hist[b] != 0.0) atomicAdd(&gp[b], hist[b]);
+      if (hist[nbins + b] != 0.0) atomicAdd(&gn[b], hist[nbins + b]);
+    }
+  for (int o = 32; o > 0; o >>= 1) ssum += __shfl_xor(ssum, o, 64);
+  if ((threadIdx.x & 63) == 0 && ssum != 0.0) atomicAdd(spec_sum, ssum);
+}
+
+// reducedRho[ii](it,itp) += sum_il rho_i[kidx + it + kii*il, kidx + itp + kii*il], contributions taken
+// in the reference's order (sector i ascending, coupled sub-block order).  One workgroup per ii.
+__global__ void __launch_bounds__(256) k_partial_trace(const PtItem* __restrict__ items, const PtContrib* __restrict__ cons,
+                                                       const double* __restrict__ rho, double* __restrict__ red) {
+  const PtItem it = items[blockIdx.x];
+  const int k = it.kii;
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    const int a = e / k, b = e % k;
+    double r = 0.0;
+    for (int c = it.c_begin; c < it.c_end; c++) {
+      const PtContrib pc = cons[c];
+      double aa = 0.0;
+      for (int il = 0; il < pc.dbb; il++)
+        aa += rho[pc.rho_off + (long long)(pc.kidx + a + k * il) * pc.ni + (pc.kidx + b + k * il)];
+      r += aa;
+    }
+    red[it.r_off + e] = r;
+  }
+}
+
+// Thermodynamics: for each temperature, per shell m: Z'_m = sum_{discarded} exp_reg((gs_m - E)/T),
+// and the Boltzmann-weighted first and second moments of (E - gs_m).  One workgroup per T.
+__device__ __forceinline__ double regulate_exp(double x) {
+  if (x > 100) return exp(100.0);
+  if (x < -100) return exp(-100.0);
+  return exp(x);
+}
+__global__ void __launch_bounds__(256) k_thermo(const double* __restrict__ E, const int* __restrict__ shell_off,
+                                                const double* __restrict__ gs, int n_shells, const double* __restrict__ T,
+                                                double* __restrict__ z, double* __restrict__ m1, double* __restrict__ m2) {
+  const int t = blockIdx.x;
+  const double kT = T[t];
+  __shared__ double red[3][4];
+  for (int m = 0; m < n_shells; m++) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int e = shell_off[m] + threadIdx.x; e < shell_off[m + 1]; e += blockDim.x) {
+      const double w = regulate_exp((gs[m] - E[e]) / kT);
+      const double d = E[e] - gs[m];
+      s0 += w;
+      s1 += d * w;
+      s2 += d * d * w;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      s0 += __shfl_xor(s0, o, 64);
+      s1 += __shfl_xor(s1, o, 64);
+      s2 += __shfl_xor(s2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = s0;
+      red[1][threadIdx.x >> 6] = s1;
+      red[2][threadIdx.x >> 6] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      z[(size_t)t * n_shells + m] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+      m1[(size_t)t * n_shells + m] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+      m2[(size_t)t * n_shells + m] = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_broaden(const double* __restrict__ x, const double* __restrict__ w, int n,
+                                                 const double* __restrict__ omega, int no, double b,
+                                                 double* __restrict__ out) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= no) return;
+  const double om = omega[o];
+  double s = 0.0;
+  for (int k = 0; k < n; k++) {
+    if (w[k] == 0.0) continue;
+    const double y = -b * 0.25 + log(om / x[k]) / b;
+    s += w[k] * exp(-y * y);
+  }
+  out[o] = s / om / (b * sqrt(M_PI));
+}
+
+// dense transpose of square blocks (plain FP64 mode: rocSOLVER returns column-major eigenvectors)
+__global__ void __launch_bounds__(256) k_transpose(const double* __restrict__ in, double* __restrict__ out, int n) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  for (int j = ty; j < 32; j += 8)
+    if (by + j < n && bx + tx < n) tile[j][tx] = in[(long long)(by + j) * n + bx + tx];
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8)
+    if (bx + j < n && by + tx < n) out[(long long)(bx + j) * n + by + tx] = tile[tx][j];
+}
+
+void launch_fdm_bin(const void* items, int n_items, const double* B, const double* RA, const double* AR,
+                    const double* E, const double* bf, double scale, double e_min, double delE, int nbins, double* pos,
+                    double* neg, double* spec_sum, hipStream_t s) {
+  if (n_items <= 0) return;
+  const size_t lds = (size_t)nbins * 2 * sizeof(double) <= 65536 ? (size_t)nbins * 2 * sizeof(double) : 0;
+  hipLaunchKernelGGL(k_fdm_bin, dim3(n_items), dim3(BIN_THREADS), lds, s, (const FdmBinItem*)items, B, RA, AR, E, bf,
+                     scale, e_min, delE, nbins, pos, neg, spec_sum);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_partial_trace(const void* items, int n_items, const void* cons, const double* rho, double* red,
+                          hipStream_t s) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(k_partial_trace, dim3(n_items), dim3(256), 0, s, (const PtItem*)items, (const PtContrib*)cons, rho,
+                     red);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_thermo(const double* E, const int* shell_off, const double* gs, int n_shells, const double* T, int nT,
+                   double* z, double* m1, double* m2, hipStream_t s) {
+  if (nT <= 0 || n_shells <= 0) return;
+  hipLaunchKernelGGL(k_thermo, dim3(nT), dim3(256), 0, s, E, shell_off, gs, n_shells, T, z, m1, m2);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,
+                    hipStream_t s) {
+  if (no <= 0) return;
+  hipLaunchKernelGGL(k_broaden, dim3((no + 255) / 256), dim3(256), 0, s, x, w, n, omega, no, b, out);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_transpose(const double* in, double* out, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, s, in, out, n);
+  NRGK_HIP(hipGetLastError());
+}
+size_t fdm_bin_item_size() { return sizeof(FdmBinItem); }
+
+}  // namespace nrgk

@@ -1,0 +1,111 @@
+// K7: truncation (restates nrgcore::discard_higher_energies, nrgcore/include/nrgcore/nrgcore.hpp:269-327).
+//
+// The reference sorts all N_tot eigenvalues and keeps, per sector, every state with
+// E < sorted[K] + 1e-8 (strict, K 0-based), then shifts all energies by the minimum if and only if
+// it truncated.  Each block's eigenvalues are already ascending, so sorted[K] is found without a
+// global sort: a bisection over the order-preserving 64-bit key of an FP64 value, where each step
+// counts the elements <= the probe with one binary search per block.  The result is the exact
+// element value, and the kept counts are lower_bound(sorted[K] + 1e-8) per block; both are
+// bit-identical to the reference for identical eigenvalues.
+#include "common.hpp"
+
+namespace nrgk {
+
+
+__device__ __forceinline__ unsigned long long okey(double x) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double okey_inv(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+// number of elements <= v (upper_bound) and < v (lower_bound) in an ascending list
+__device__ __forceinline__ int count_le(const double* a, int n, double v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int m = (lo + hi) >> 1;
+    if (a[m] <= v) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+__device__ __forceinline__ int count_lt(const double* a, int n, double v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int m = (lo + hi) >> 1;
+    if (a[m] < v) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+template <class T, class Op>
+__device__ T block_reduce(T v, T* sh, Op op) {
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T r = sh[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = op(r, sh[w]);
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, const int* __restrict__ beoff, int S,
+                                                   const double* __restrict__ lam, long long N_tot, long long K,
+                                                   double errbar, int* __restrict__ kept, TruncOut* __restrict__ out,
+                                                   double* __restrict__ lam_shift) {
+  __shared__ double shd[16];
+  __shared__ long long shl[16];
+  const int tid = threadIdx.x;
+  double mn = 1e308, mx = -1e308;
+  for (int b = tid; b < S; b += blockDim.x)
+    if (bn[b] > 0) {
+      mn = fmin(mn, lam[beoff[b]]);
+      mx = fmax(mx, lam[beoff[b] + bn[b] - 1]);
+    }
+  mn = block_reduce(mn, shd, [](double a, double c) { return fmin(a, c); });
+  mx = block_reduce(mx, shd, [](double a, double c) { return fmax(a, c); });
+  const bool trunc = N_tot > K;
+  double e_cut = mx;
+  if (trunc) {
+    unsigned long long lo = okey(mn), hi = okey(mx);
+    while (lo < hi) {
+      const unsigned long long mid = lo + ((hi - lo) >> 1);
+      const double v = okey_inv(mid);
+      long long c = 0;
+      for (int b = tid; b < S; b += blockDim.x) c += count_le(lam + beoff[b], bn[b], v);
+      c = block_reduce(c, shl, [](long long a, long long d) { return a + d; });
+      if (c >= K + 1) hi = mid;
+      else lo = mid + 1;
+    }
+    e_cut = okey_inv(lo);
+  }
+  const double x = e_cut + errbar;
+  long long nk = 0;
+  for (int b = tid; b < S; b += blockDim.x) {
+    const int k = trunc ? count_lt(lam + beoff[b], bn[b], x) : bn[b];
+    kept[b] = k;
+    nk += k;
+  }
+  nk = block_reduce(nk, shl, [](long long a, long long d) { return a + d; });
+  // shift every eigenvalue (not only the kept ones) when truncating (nrgcore.hpp:320-325); the
+  // per-block eigenvalue ranges are contiguous prefix offsets covering [0, N_tot)
+  for (long long i = tid; i < N_tot; i += blockDim.x) lam_shift[i] = trunc ? lam[i] - mn : lam[i];
+  if (tid == 0) {
+    out->e_cut = e_cut;
+    out->e_min = mn;
+    out->e_max = mx;
+    out->truncated = trunc ? 1 : 0;
+    out->n_kept = (int)nk;
+  }
+}
+
+void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
+                     double errbar, int* kept, TruncOut* out, double* lam_shift, hipStream_t s) {
+  hipLaunchKernelGGL(k_truncate, dim3(1), dim3(1024), 0, s, bn, beoff, S, lam, N_tot, K, errbar, kept, out, lam_shift);
+  NRGK_HIP(hipGetLastError());
+}
+
+}  // namespace nrgk

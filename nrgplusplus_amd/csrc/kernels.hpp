@@ -1,0 +1,48 @@
+// Launchers of the device kernels (one translation unit per kernel family).
+#pragma once
+#include "common.hpp"
+
+namespace nrgk {
+
+// k_assemble.hip
+void launch_assemble(const AsmSector* sectors, int n_sectors, int n_rows, const AsmSub* subs, const AsmCon* cons,
+                     const double* e_prev, const double* e_bath, const double* f_prev, const double* f_bath,
+                     double rescale, double* H, hipStream_t s);
+void launch_shift_convert(const EigBlock* blocks, const int* row_block, const int* row_local, int n_rows,
+                          const double* H, unsigned int* sigma_bits, float* sigma_out, float* W, hipStream_t s);
+// k_eig.hip
+void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks, float* W, int sweep, int round,
+                         int* flags, float tol, float ang_tol, int max_inner, size_t lds, hipStream_t s);
+void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
+                           size_t lds, hipStream_t s);
+void launch_oa_update(const EigBlock* blocks, int nb, const double* S, const double* Rp, double* E, double* lam,
+                      int* stats, double* conv, size_t lds, hipStream_t s);
+void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
+                       double* lam_out, size_t lds, hipStream_t s);
+size_t normalize_sort_lds(int nmax);
+size_t oa_update_lds(int nmax);
+size_t final_sort_lds(int nmax);
+void set_kernel_lds_limits(int bytes);
+// k_gemm.hip
+int prepare_gemm(std::vector<GemmProb>& probs);
+void launch_gemm(const GemmProb* d_probs, int n_probs, int tiles, hipStream_t s);
+// k_trunc.hip
+void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
+                     double errbar, int* kept, TruncOut* out, double* lam_shift, hipStream_t s);
+// k_rotate.hip
+void launch_fdag_gather(const RotGroup* g, int ng, const RotSegment* s, const RotTri* t, const double* U,
+                        const double* fbath, double* Y, hipStream_t st);
+void launch_obs_scatter(const ObsSeg* s, int ns, const double* Oprev, double* C, hipStream_t st);
+// k_fdm.hip
+void launch_fdm_bin(const void* items, int n_items, const double* B, const double* RA, const double* AR,
+                    const double* E, const double* bf, double scale, double e_min, double delE, int nbins, double* pos,
+                    double* neg, double* spec_sum, hipStream_t s);
+void launch_partial_trace(const void* items, int n_items, const void* cons, const double* rho, double* red,
+                          hipStream_t s);
+void launch_thermo(const double* E, const int* shell_off, const double* gs, int n_shells, const double* T, int nT,
+                   double* z, double* m1, double* m2, hipStream_t s);
+void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,
+                    hipStream_t s);
+void launch_transpose(const double* in, double* out, int n, hipStream_t s);
+
+}  // namespace nrgk

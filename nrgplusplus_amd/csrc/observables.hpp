@@ -1,0 +1,48 @@
+// Observables, FDM spectrum and thermodynamics on the device.
+#pragma once
+#include <memory>
+
+#include "chain.hpp"
+
+namespace nrgk {
+
+// System operators rotated along the chain (update_system_operator, sysOperator.hpp:35-133).
+struct Obs {
+  Chain* ch = nullptr;
+  int nops = 0;
+  std::vector<OpMap> blocks;  // full n_i x n_j blocks in the current basis
+  DevBuf buf[2];
+  int cur = 0;
+  DevBuf dseg, dC, dY;
+  Obs(Chain* c, int n_ops, const int* nblocks, const int* bi, const int* bj, const double* data);
+  void update();
+};
+
+// One saved iteration (what NrgData::saveCurrentData + saveqOperator write, nrgData.hpp:134-205).
+struct Shell {
+  int cnt = 0;
+  std::vector<int> n, eoff;
+  std::vector<long long> uoff;
+  std::vector<std::vector<std::pair<int, int>>> coupled;
+  std::vector<std::vector<int>> sub_kidx;
+  std::vector<int> pre_kept;  // eigenvaluesQ_kept_indices sizes at save time (previous iteration)
+  std::vector<double> E;      // eigenvalues (pre-shift), concatenated
+  DevBuf dU, dE, dO;
+  std::vector<OpMap> ops;
+};
+
+struct Fdm {
+  Chain* ch = nullptr;
+  Obs* obs = nullptr;
+  int nbins = 0;
+  double e_min = 1e-10, e_max = 10, delE = 0;
+  std::vector<std::unique_ptr<Shell>> shells;
+  DevBuf dpos, dneg, dsum, dbf, dred[2], dRA, dAR, dT1, drho, ditems, dpt, dE, dth;
+  Fdm(Chain* c, Obs* o, int nb, double emin, double emax);
+  void record();
+  void backward(double lambda, int scale_offset, int first_cnt, double* pos, double* neg, double* traces,
+                double* sums, int cap, int* n_shells);
+  void thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S, double* C);
+};
+
+}  // namespace nrgk

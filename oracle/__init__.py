@@ -89,6 +89,7 @@ def lib():
             "orc_run_thermo": (i, [C.POINTER(OrcDriver), i, dp, i, dp, dp, i]),
             "orc_run_fdm": (i, [C.POINTER(OrcDriver), i, dp, dp, dp, dp, ip]),
             "orc_broaden": (None, [dp, dp, i, dp, i, d, dp]),
+            "orc_time_forward": (i, [C.POINTER(OrcDriver), i, i, dp]),
             "orc_hopping": (d, [i, d]),
         }
         for name, (res, args) in sig.items():
@@ -306,6 +307,17 @@ def run_fdm(drv, nbins):
     _check(lib().orc_run_fdm(C.byref(drv), nbins, _dp(pos), _dp(neg), _dp(tr), _dp(sums), C.byref(ns)))
     n = ns.value
     return pos.reshape(nops, nbins), neg.reshape(nops, nbins), tr[:n], sums[:n]
+
+
+def time_forward(drv, n_skip, n_time):
+    """Seconds spent in loop sites [n_skip, n_skip + n_time) of the faithful FDM forward pass."""
+    s = C.c_double(0)
+    _check(lib().orc_time_forward(C.byref(drv), n_skip, n_time, C.byref(s)))
+    return s.value
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
 
 
 def broaden(x, w, omega, b):

@@ -330,6 +330,38 @@ int orc_run_fdm(const OrcDriver* d, int nbins, double* pos, double* neg, double*
   });
 }
 
+// CPU baseline: the FDM-driver forward pass in the reference's form (add_bath_site,
+// update_system_operator of f^dagger chi, shell save, update_internal_state), timed over loop sites
+// [n_skip, n_skip + n_time) with std::chrono like the reference's timer (nrgcore.hpp:66,90-93).
+int orc_time_forward(const OrcDriver* d, int n_skip, int n_time, double* seconds) {
+  return guard([&] {
+    SiteModel imp = driver_imp(*d), bath = driver_bath(*d);
+    Chain c(imp, bath);
+    c.set_parameters((size_t)d->K);
+    size_t nop = imp.f_dag_operator.size();
+    auto ops = make_fdag_chi_spinhalf();
+    std::vector<Shell> shells;
+    c.add_bath_site(std::vector<double>(nop, d->first_V), 1.0);
+    update_system_operator(c, ops);
+    c.update_internal_state();
+    double tot = 0;
+    for (int in = 0; in < n_skip + n_time; in++) {
+      auto t0 = std::chrono::steady_clock::now();
+      double rescale = in > 0 ? std::sqrt(d->Lambda) : 1.0;
+      c.add_bath_site(std::vector<double>(nop, hopping(in, d->Lambda)), rescale);
+      update_system_operator(c, ops);
+      if (c.checkHigherEnergyDiscarded()) {
+        shells.push_back(save_shell(c));
+        shells.back().ops = ops;
+      }
+      c.update_internal_state();
+      auto t1 = std::chrono::steady_clock::now();
+      if (in >= n_skip) tot += std::chrono::duration<double>(t1 - t0).count();
+    }
+    *seconds = tot;
+  });
+}
+
 void orc_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out) {
   auto r = broaden(std::vector<double>(x, x + n), std::vector<double>(w, w + n), std::vector<double>(omega, omega + no), b);
   std::memcpy(out, r.data(), r.size() * sizeof(double));

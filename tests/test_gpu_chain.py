@@ -1,0 +1,234 @@
+"""GPU parity: libnrgk (HIP, through its C ABI) against the FP64 oracle and the reference's outputs.
+
+Tolerances (stated per result):
+  * sector labels, block dimensions, kept counts: bit-exact;
+  * assembled sector Hamiltonians fed the oracle's state (lockstep): bit-exact;
+  * eigenvalues of each step vs LAPACK on the same Hamiltonian: <= 1e-10 (iteration energy units);
+  * eigenvectors: ||U^T U - I||_max and ||H U - U diag(E)||_max <= 1e-10;
+  * chain eigenvalues vs the oracle chain / the reference's files: <= 1e-8 absolute;
+  * thermodynamics vs the reference's files: as for the oracle (where the exp clamp does not dominate);
+  * FDM weights: total weight to 1e-9 relative, broadened spectrum to 1e-6 of its maximum.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _state_from_oracle(nk, o):
+    """The previous-basis state an assembly reads, exported from the oracle chain (after
+    update_internal_state): sectors, kept energies, kept x kept f-dagger blocks."""
+    qn = o.pre_qn()
+    kept = o.kept_counts()
+    eq = o.eigenvalues_q()
+    energies = np.concatenate([eq[i][:kept[i]] for i in range(len(kept))]) if kept.sum() else np.zeros(0)
+    ops = []
+    for blocks in o.fdag():
+        ops.append([(i, j, np.ascontiguousarray(m[:kept[i], :kept[j]])) for i, j, m in blocks])
+    return nk.Site.from_arrays(qn, kept, energies, np.ones(len(kept)), ops)
+
+
+def _models(nk, oracle, kind, eps, U):
+    if kind == "siam":
+        return (nk.Site.spinhalf(eps, U), nk.Site.spinhalf(0, 0),
+                oracle.Model("spinhalf", eps=eps, U=U), oracle.Model("spinhalf"), 2)
+    return (nk.Site.spinhalf(eps, U).duplicate_ops(), nk.Site.two_channel(),
+            oracle.Model("spinhalf", eps=eps, U=U, dup_ops=True), oracle.Model("two_channel"), 4)
+
+
+def _check_eigen_step(g, tol=1e-10):
+    qn, dims = g.sectors()
+    eq = g.eigenvaluesQ
+    worst = 0.0
+    for s, n in enumerate(dims):
+        if n == 0:
+            continue
+        H = g.block_h(s)
+        w = np.linalg.eigvalsh(H)
+        assert np.abs(w - eq[s]).max() <= tol, (s, n, np.abs(w - eq[s]).max())
+        U = g.current_hamiltonQ(s)
+        assert np.abs(U.T @ U - np.eye(n)).max() <= tol
+        assert np.abs(H @ U - U * eq[s]).max() <= tol
+        worst = max(worst, np.abs(w - eq[s]).max())
+    return worst
+
+
+@pytest.mark.parametrize("kind,K,nit", [("siam", 200, 10), ("two_channel", 200, 3)])
+def test_lockstep_assembly_bit_exact_and_eigen_accuracy(nk, oracle_mod, kind, K, nit):
+    imp, bath, oimp, obath, nops = _models(nk, oracle_mod, kind, -0.2, 0.4)
+    g = nk.NrgCore(imp, bath, K)
+    g.keep_hamiltonians(True)
+    o = oracle_mod.Chain(oimp, obath, K, keep_h=True)
+    L = 3.0 if kind == "two_channel" else 2.0
+    V = np.sqrt(2 * 0.01 / np.pi)
+    ts = [V] + [oracle_mod.hopping(n, L) for n in range(nit)]
+    rs = [1.0] + [1.0 if n == 0 else np.sqrt(L) for n in range(nit)]
+    for it, (t, r) in enumerate(zip(ts, rs)):
+        if it > 0:
+            g.set_state(_state_from_oracle(nk, o), o.cnt)
+        g.add_bath_site([t] * nops, r)
+        o.add_bath_site([t] * nops, r)
+        qn, dims = g.sectors()
+        assert np.array_equal(qn, o.cur_qn()) and np.array_equal(dims, o.sector_dims())
+        for s, n in enumerate(dims):
+            if n:
+                hg, ho = g.block_h(s), o.block_h(s, n)
+                assert np.array_equal(hg, ho), (it, s, np.abs(hg - ho).max())
+        _check_eigen_step(g)
+        g.update_internal_state()
+        o.update_internal_state()
+        assert np.array_equal(g.kept_counts(), o.kept_counts()), it
+
+
+def test_free_running_chain_matches_oracle(nk, oracle_mod):
+    """config 1 form (SIAM U=0.5, eps=-0.25, Gamma=0.02, Lambda=2.5) at K=256, 14 sites."""
+    K, L = 256, 2.5
+    V = np.sqrt(2 * 0.02 / np.pi)
+    g = nk.NrgCore(nk.Site.spinhalf(-0.25, 0.5), nk.Site.spinhalf(0, 0), K)
+    g.keep_hamiltonians(True)
+    o = oracle_mod.Chain(oracle_mod.Model("spinhalf", eps=-0.25, U=0.5), oracle_mod.Model("spinhalf"), K)
+    ts = [V] + [oracle_mod.hopping(n, L) for n in range(14)]
+    rs = [1.0] + [1.0 if n == 0 else np.sqrt(L) for n in range(14)]
+    for it, (t, r) in enumerate(zip(ts, rs)):
+        g.add_bath_site([t, t], r)
+        o.add_bath_site([t, t], r)
+        _check_eigen_step(g)
+        g.update_internal_state()
+        o.update_internal_state()
+        assert np.array_equal(g.kept_counts(), o.kept_counts()), it
+        a, b = g.all_eigenvalue, o.all_eigenvalue()
+        assert len(a) == len(b) and np.abs(a - b).max() < 1e-8, (it, np.abs(a - b).max())
+        assert np.array_equal(g.relativeGroundStateEnergy.shape, o.relative_gs().shape)
+        # Wilson-site f-dagger: kept x kept blocks are gauge-dependent, their Frobenius norms are not
+        fg, fo = g.get_f_dag_operator(), o.fdag()
+        kept = o.kept_counts()
+        for bg, bo in zip(fg, fo):
+            assert [(i, j) for i, j, _ in bg] == [(i, j) for i, j, _ in bo]
+            for (_, _, mg), (i, j, mo) in zip(bg, bo):
+                ng, no = np.linalg.norm(mg), np.linalg.norm(mo[:kept[i], :kept[j]])
+                assert abs(ng - no) <= 1e-9 * max(1.0, no)
+
+
+@pytest.mark.parametrize("key,kondo", [("free_chain_L2_K1024", False), ("kondo_J025_L2_K1024", True)])
+def test_reference_rgflow_outputs_on_gpu(nk, oracle_mod, golden, key, kondo):
+    gd = golden(key)
+    imp = nk.Site.kondo(0.25, 0.5) if kondo else nk.Site.spinhalf(0, 0)
+    g = nk.NrgCore(imp, nk.Site.spinhalf(0, 0), 1024)
+    for n in range(41):
+        t = oracle_mod.hopping(n, 2.0)
+        g.add_bath_site([t, t], np.sqrt(2.0))
+        g.update_internal_state()
+        ref = gd["Eigenvalues%d" % n]
+        a = g.all_eigenvalue
+        assert len(a) == len(ref), n
+        assert np.abs(a - ref).max() < 1e-8, (n, np.abs(a - ref).max())
+
+
+def test_reference_thermodynamics_on_gpu(nk, golden):
+    """examples/entropySIAM driver (recovered parameters of resultSIAMU0.200000.h5)."""
+    gd = golden("siam_U02_L25_K512")
+    T = gd["temperatureArray"]
+    L = 2.5
+    g = nk.NrgCore(nk.Site.spinhalf(-0.1, 0.2), nk.Site.spinhalf(0, 0), 1024)
+    rec = nk.FdmRecorder(g, None)
+    V = np.sqrt(2 * 0.01 / np.pi)
+    g.add_bath_site([V, V], 1.0)
+    g.update_internal_state()
+    min_it = 0
+    for n in range(41):
+        t = nk.hopping(n, L)
+        g.add_bath_site([t, t], 1.0 if n == 0 else np.sqrt(L))
+        if g.checkHigherEnergyDiscarded():
+            rec.record()
+        else:
+            min_it += 1
+        g.update_internal_state()
+    S, Cv = rec.thermodynamics(L, T, scale_offset=-1, first_cnt=min_it + 1)
+    m = T >= 3e-7
+    assert np.abs(S - gd["entropy"])[m].max() < 1e-8
+    rel = np.abs(Cv - gd["specificHeat"])[m] / np.maximum(np.abs(gd["specificHeat"][m]), 1e-3)
+    assert rel.max() < 2e-6
+
+
+def _gpu_fdm(nk, eps, U, G, L, K, nsites, nbins):
+    g = nk.NrgCore(nk.Site.spinhalf(eps, U), nk.Site.spinhalf(0, 0), K)
+    op = nk.SystemOperator(g, nk.Site.fdag_chi_spinhalf())
+    rec = nk.FdmRecorder(g, op, nbins, 1e-10, 10.0)
+    V = np.sqrt(2 * G / np.pi)
+    g.add_bath_site([V, V], 1.0)
+    op.update()
+    g.update_internal_state()
+    min_it = 0
+    for n in range(nsites):
+        t = nk.hopping(n, L)
+        g.add_bath_site([t, t], 1.0 if n == 0 else np.sqrt(L))
+        op.update()
+        if g.checkHigherEnergyDiscarded():
+            rec.record()
+        else:
+            min_it += 1
+        g.update_internal_state()
+    return rec.backward(L, scale_offset=-1, first_cnt=min_it + 1), rec.energy_points()
+
+
+def test_fdm_spectrum_matches_oracle(nk, oracle_mod):
+    eps, U, G, L, K, ns, nb = -0.15, 0.3, 0.01, 2.0, 128, 18, 400
+    (pos, neg, tr, sums), x = _gpu_fdm(nk, eps, U, G, L, K, ns, nb)
+    drv = oracle_mod.driver(imp_kind=0, eps=eps, U=U, bath_kind=0, Lambda=L, first_V=np.sqrt(2 * G / np.pi),
+                            nsites=ns, K=K)
+    opos, oneg, otr, osums = oracle_mod.run_fdm(drv, nb)
+    assert len(tr) == len(otr)
+    np.testing.assert_allclose(tr, 1.0, atol=1e-10)  # rhoTrace per shell
+    np.testing.assert_allclose(sums, osums, rtol=1e-8)
+    tot, otot = pos.sum() + neg.sum(), opos.sum() + oneg.sum()
+    assert abs(tot - otot) <= 1e-9 * otot
+    # sum rule per spin: <{d, d^dagger}> = 1 minus the weight outside the bins
+    for p in range(2):
+        assert 0.9 < pos[p].sum() + neg[p].sum() <= 1.0 + 1e-9
+    # broadened spectra (gauge- and bin-edge-robust)
+    om = np.logspace(-8, 0, 200)
+    for p in range(2):
+        for a, b in ((pos[p], opos[p]), (neg[p], oneg[p])):
+            ag, ao = nk.broaden(x, a, om, 0.6), oracle_mod.broaden(x, b, om, 0.6)
+            assert np.abs(ag - ao).max() <= 1e-6 * max(1e-12, np.abs(ao).max())
+
+
+def test_broadening_matches_oracle(nk, oracle_mod):
+    rng = np.random.default_rng(1)
+    x = np.sort(10 ** rng.uniform(-9, 1, 300))
+    w = rng.uniform(0, 1, 300)
+    om = np.logspace(-10, 1, 500)
+    a = nk.broaden(x, w, om, 0.6)
+    b = oracle_mod.broaden(x, w, om, 0.6)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_plain_fp64_mode_matches_oracle(nk, oracle_mod):
+    K, L = 128, 2.5
+    V = np.sqrt(2 * 0.02 / np.pi)
+    g = nk.NrgCore(nk.Site.spinhalf(-0.25, 0.5), nk.Site.spinhalf(0, 0), K, mode=nk.NRGK_MODE_PLAIN64)
+    o = oracle_mod.Chain(oracle_mod.Model("spinhalf", eps=-0.25, U=0.5), oracle_mod.Model("spinhalf"), K)
+    ts = [V] + [oracle_mod.hopping(n, L) for n in range(10)]
+    rs = [1.0] + [1.0 if n == 0 else np.sqrt(L) for n in range(10)]
+    for it, (t, r) in enumerate(zip(ts, rs)):
+        g.add_bath_site([t, t], r)
+        o.add_bath_site([t, t], r)
+        g.update_internal_state()
+        o.update_internal_state()
+        assert np.array_equal(g.kept_counts(), o.kept_counts()), it
+        assert np.abs(g.all_eigenvalue - o.all_eigenvalue()).max() < 1e-8
+
+
+def test_error_paths(nk):
+    g = nk.NrgCore(nk.Site.spinhalf(-0.25, 0.5), nk.Site.spinhalf(0, 0), 64)
+    with pytest.raises(nk.NrgkError) as e:
+        g.add_bath_site([0.1, 0.1, 0.1, 0.1], 1.0)  # 4 hoppings for 2 operators (nrgcore.hpp:148-152)
+    assert "t_hopping" in str(e.value)
+    with pytest.raises(nk.NrgkError):
+        g.update_internal_state()  # before add_bath_site
+    g.add_bath_site([0.1, 0.1], 1.0)
+    with pytest.raises(nk.NrgkError):
+        g.add_bath_site([0.1, 0.1], 1.0)  # twice without update
+    g.update_internal_state()
+    with pytest.raises(nk.NrgkError):  # nrgcore::test: operator counts differ (nrgcore.hpp:123-133)
+        nk.NrgCore(nk.Site.spinhalf(0, 0).duplicate_ops(), nk.Site.spinhalf(0, 0), 64)

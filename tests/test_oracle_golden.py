@@ -1,0 +1,155 @@
+"""The FP64 oracle against what pins it: the reference's own HDF5 outputs (converted to
+tests/golden/*.npz by tests/golden/make_fixtures.py), closed forms and brute-force exact
+diagonalisation.  CPU only."""
+import numpy as np
+import pytest
+
+LOG_TGRID = dict(vmin=1e-10, vmax=2.0, n=200)
+
+
+def _rgflow(oracle, **kw):
+    return oracle.run_rgflow(oracle.driver(**kw))
+
+
+def test_free_chain_first_iteration_closed_form(oracle_mod, golden):
+    # examples/freeModel/main.cpp: spinhalf(0,0) impurity coupled by t0 = hopping(0, 2) to the first
+    # site: single-particle levels +-t0 per spin -> {+-2t0, +-t0 (x4), 0 (x6)}
+    t0 = oracle_mod.hopping(0, 2.0)
+    exact = np.sort([-2 * t0, 2 * t0] + [-t0] * 4 + [t0] * 4 + [0.0] * 6)
+    eig = _rgflow(oracle_mod, imp_kind=0, bath_kind=0, Lambda=2.0, first_V=0.0, nsites=1, K=1024)[0]
+    assert np.abs(eig - exact).max() < 1e-14
+    ref = golden("free_chain_L2_K1024")["Eigenvalues0"]
+    assert np.abs(ref - exact).max() < 1e-12
+
+
+@pytest.mark.parametrize("key,imp_kind", [("free_chain_L2_K1024", 0), ("kondo_J025_L2_K1024", 1)])
+def test_rgflow_matches_reference_outputs(oracle_mod, golden, key, imp_kind):
+    """All 41 iterations: identical state counts (so identical truncation decisions and kept counts)
+    and eigenvalues equal to 1e-8 absolute (different LAPACK builds accumulate rounding along the
+    chain; the measured difference is ~5e-10)."""
+    g = golden(key)
+    eigs = _rgflow(oracle_mod, imp_kind=imp_kind, J=0.25, S=0.5, bath_kind=0, Lambda=2.0, first_V=0.0, nsites=41,
+                   K=1024)
+    for n in range(41):
+        ref = g["Eigenvalues%d" % n]
+        assert len(ref) == len(eigs[n]), n
+        assert np.abs(ref - eigs[n]).max() < 1e-8, n
+
+
+def _thermo_case(oracle, key):
+    if key == "free_chain_L2_K1024":
+        return dict(drv=dict(imp_kind=0, bath_kind=0, Lambda=2.0, first_V=0.0, nsites=41, K=1024), min_iter0=1)
+    if key == "kondo_J025_L2_K1024":
+        return dict(drv=dict(imp_kind=1, J=0.25, S=0.5, bath_kind=0, Lambda=2.0, first_V=0.0, nsites=41, K=1024),
+                    min_iter0=1)
+    if key == "siam_U02_L25_K512":
+        # recovered parameters: eps = -U/2, K = 1024, V = sqrt(2 Gamma / pi) (no A_Lambda factor)
+        return dict(drv=dict(imp_kind=0, eps=-0.1, U=0.2, bath_kind=0, Lambda=2.5,
+                             first_V=np.sqrt(2 * 0.01 / np.pi), nsites=41, K=1024), min_iter0=0)
+    if key == "siam_zero_L25_K512":
+        # recovered: the free-chain driver form (no V site, rescale sqrt(Lambda) from site 0), K = 1024
+        return dict(drv=dict(imp_kind=0, bath_kind=0, Lambda=2.5, first_V=0.0, nsites=41, K=1024), min_iter0=0)
+    raise KeyError(key)
+
+
+@pytest.mark.parametrize("key", ["free_chain_L2_K1024", "kondo_J025_L2_K1024", "siam_U02_L25_K512",
+                                 "siam_zero_L25_K512"])
+def test_thermodynamics_matches_reference_outputs(oracle_mod, golden, key):
+    """Entropy and specific heat on the reference's 200-point grid.  Below T ~ 3e-7 both codes are
+    dominated by the exp(-100) clamp of regulateExp (fdmThermodynamics.hpp:240-249) times the
+    d^k shell prefactors; there the shipped files disagree with this code version, so the
+    comparison is made where the clamp does not dominate."""
+    g = golden(key)
+    T = g["temperatureArray"]
+    assert np.abs(T - oracle_mod.temperature_grid(**LOG_TGRID)).max() < 1e-15
+    case = _thermo_case(oracle_mod, key)
+    S, Cv = oracle_mod.run_thermo(oracle_mod.driver(**case["drv"]), T, case["min_iter0"])
+    m = T >= 3e-7
+    assert np.abs(S - g["entropy"])[m].max() < 1e-8
+    rel = np.abs(Cv - g["specificHeat"])[m] / np.maximum(np.abs(g["specificHeat"][m]), 1e-3)
+    assert rel.max() < 2e-6
+
+
+# ---------------------------------------------------------------- brute-force exact diagonalisation
+def _jw_ops(n_orb):
+    fd = np.array([[0.0, 0.0], [1.0, 0.0]])
+    sz = np.diag([1.0, -1.0])
+    ops = []
+    for i in range(n_orb):
+        m = np.eye(1)
+        for j in range(n_orb):
+            if j < i:
+                m = np.kron(m, sz)
+            elif j == i:
+                m = np.kron(m, fd)
+            else:
+                m = np.kron(m, np.eye(2))
+        ops.append(m)
+    return ops
+
+
+def _ed_siam(eps, U, V, ts, rescales):
+    """Spectra of H_0 = eps n_d + U n_up n_dn + V sum_s (d+ c0 + h.c.), then
+    H_{k+1} = r_k H_k + t_k sum_s (c_k+ c_{k+1} + h.c.), by dense diagonalisation of the full
+    Jordan-Wigner Fock space (a textbook fermion chain, independent of the reference's code)."""
+    n_sites = len(ts) + 1
+    n_orb = 2 * (n_sites + 1)
+    c = _jw_ops(n_orb)  # creation operators
+    a = [x.T for x in c]
+    nd = [c[s] @ a[s] for s in (0, 1)]
+    H = eps * (nd[0] + nd[1]) + U * (nd[0] @ nd[1])
+    for s in (0, 1):
+        hop = c[s] @ a[2 + s]
+        H = H + V * (hop + hop.T)
+    specs = []
+    active = [0, 1, 2, 3]
+    specs.append(np.sort(np.linalg.eigvalsh(H)))  # all orbitals present; extra sites idle (x4 each)
+    for k, (t, r) in enumerate(zip(ts, rescales)):
+        H = r * H
+        for s in (0, 1):
+            hop = c[2 + 2 * k + s] @ a[2 + 2 * (k + 1) + s]
+            H = H + t * (hop + hop.T)
+        specs.append(np.sort(np.linalg.eigvalsh(H)))
+    return specs, n_orb
+
+
+def test_siam_chain_equals_exact_diagonalisation(oracle_mod):
+    """Untruncated oracle iterations 0..2 (16, 64, 256 states) against the exact spectrum of the
+    Jordan-Wigner chain.  The idle orbitals of the ED space multiply each level by 4 per idle site."""
+    eps, U, V, L = -0.25, 0.5, 0.3, 2.5
+    ts = [oracle_mod.hopping(0, L), oracle_mod.hopping(1, L)]
+    rs = [1.0, np.sqrt(L)]
+    specs, n_orb = _ed_siam(eps, U, V, ts, rs)
+    imp = oracle_mod.Model("spinhalf", eps=eps, U=U)
+    bath = oracle_mod.Model("spinhalf")
+    ch = oracle_mod.Chain(imp, bath, 100000)
+    ch.add_bath_site([V, V], 1.0)
+    ch.update_internal_state()
+    got = [ch.all_eigenvalue()]
+    for t, r in zip(ts, rs):
+        ch.add_bath_site([t, t], r)
+        ch.update_internal_state()
+        got.append(ch.all_eigenvalue())
+    # the ED spectrum carries a 4-fold copy per idle (not yet coupled, zero-energy) site
+    for k, g in enumerate(got):
+        idle = (n_orb // 2) - (k + 2)
+        expect = np.repeat(g, 4 ** idle) if idle > 0 else g
+        np.testing.assert_allclose(np.sort(expect), specs[k], atol=1e-12)
+
+
+def test_truncation_keeps_K_plus_one_and_shifts_only_when_truncating(oracle_mod):
+    imp = oracle_mod.Model("spinhalf", eps=-0.25, U=0.5)
+    bath = oracle_mod.Model("spinhalf")
+    ch = oracle_mod.Chain(imp, bath, 40)
+    ch.add_bath_site([0.2, 0.2], 1.0)
+    ch.update_internal_state()            # 16 states <= 40: no truncation, no shift
+    assert ch.kept_counts().sum() == 16 and len(ch.relative_gs()) == 0
+    ch.add_bath_site([oracle_mod.hopping(0, 2.5)] * 2, 1.0)
+    ch.update_internal_state()            # 64 states > 40: keep E < sorted[40] + 1e-8
+    allE = ch.all_eigenvalue()
+    kept = ch.kept_counts().sum()
+    assert kept >= 41
+    assert kept == np.sum(allE < allE[40] + 1e-8)
+    assert len(ch.relative_gs()) == 1 and ch.relative_gs()[0] == allE[0]
+    # the shifted spectrum starts at 0
+    assert min(e.min() for e in ch.eigenvalues_q() if len(e)) == 0.0
