@@ -399,6 +399,8 @@ void Chain::eigensolve_mixed() {
   dR.ensure(bytes64);
   dE.ensure(bytes64);
   dU.ensure(bytes64);
+  dQ.ensure(bytes64);
+  d_rrv.ensure(std::max<long long>(1, N_tot) * 3 * sizeof(double));
   dlam_tmp.ensure(std::max<long long>(1, N_tot) * sizeof(double));
   dlam_raw.ensure(std::max<long long>(1, N_tot) * sizeof(double));
   launch_normalize_sort(dblk, nb, dW.as<float>(), d_sigma.as<float>(), dX.as<double>(), dlam_tmp.as<double>(),
@@ -486,8 +488,11 @@ void Chain::eigensolve_mixed() {
     check("P=HX", dP, false, false);
     check("S=X^T P", dS, false, false);
     check("R=X^T X", dR, false, false);
+    // clusters wider than the LDS Rayleigh-Ritz work in P (free after S = X^T P), X2 (written only
+    // by the X + X E product that follows) and Q
     launch_oa_update(d_eig_act.as<EigBlock>(), na, dS.as<double>(), dR.as<double>(), dE.as<double>(),
-                     dlam_tmp.as<double>(), d_stats.as<int>(), d_conv.as<double>(), oa_update_lds(nmax), st);
+                     dlam_tmp.as<double>(), d_stats.as<int>(), d_conv.as<double>(), dP.as<double>(),
+                     dX2.as<double>(), dQ.as<double>(), d_rrv.as<double>(), oa_update_lds(nmax), st);
     check("oa_update (E)", dE, false, false);
     check("oa_update (lam)", dlam_tmp, false, true);
     std::vector<double> conv(2 * na);

@@ -68,7 +68,7 @@ struct Chain {
   int nmax = 0;
   std::vector<int> kept;  // after update_internal_state
 
-  DevBuf dH, dW, dX, dX2, dP, dS, dR, dE, dU, dY, dC;
+  DevBuf dH, dW, dX, dX2, dP, dS, dR, dE, dU, dY, dC, dQ, d_rrv;
   DevBuf dlam_tmp, dlam_raw, dlam_shift[2];
   int lcur = 0;
   DevBuf d_asm, d_eig, d_misc, d_rot, d_gemm;

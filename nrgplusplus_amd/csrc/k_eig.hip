@@ -244,26 +244,126 @@ __device__ void rr_jacobi(double* Wm, int k, double* mu, int* perm, double* nrm)
     if (lane == 0) nrm[c] = sqrt(s);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int c = 0; c < k; c++) perm[c] = c;
-    for (int a = 1; a < k; a++) {  // insertion sort by norm (k small)
-      int v = perm[a];
-      int bpos = a - 1;
-      while (bpos >= 0 && nrm[perm[bpos]] > nrm[v]) {
-        perm[bpos + 1] = perm[bpos];
-        bpos--;
-      }
-      perm[bpos + 1] = v;
+  // ascending order of the norms by rank (ties by column index)
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    const double v = nrm[c];
+    int r = 0;
+    for (int c2 = 0; c2 < k; c2++) {
+      const double u = nrm[c2];
+      r += (u < v) || (u == v && c2 < c);
     }
-    for (int c = 0; c < k; c++) mu[c] = nrm[perm[c]];
+    perm[r] = c;
+    mu[r] = v;
   }
+  __syncthreads();
+}
+
+// Block-wide reductions over 1024 threads through red[0..15] (LDS); every thread gets the result.
+__device__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+  __syncthreads();
+  return s;
+}
+__device__ double block_max(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) s = fmax(s, red[w]);
+  __syncthreads();
+  return s;
+}
+
+// Rayleigh-Ritz step for a cluster wider than RR_MAX, with its k x k matrices in global scratch
+// (three n x n regions of the block, free while the update runs) instead of LDS.  Same algebra as
+// the LDS path: M = 1.5 I - 0.5 Rp_JJ, T = M S_JJ M, one-sided Jacobi on T - cen I + sig I,
+// G = M Q, E(i,J) <- E(i,J) G outside the cluster, E_JJ = G - I.  J = members (block indices).
+__device__ void rr_wide_cluster(const double* Sb, const double* Rb, double* Eb, int n, int k, const int* J,
+                                const int* cid, int c, double* gW, double* gM, double* gT, double* rv, double* lam,
+                                double* red) {
+  auto Ss = [&](int i, int j) { return 0.5 * (Sb[(long long)i * n + j] + Sb[(long long)j * n + i]); };
+  auto Rs = [&](int i, int j) { return 0.5 * (Rb[(long long)i * n + j] + Rb[(long long)j * n + i]); };
+  const int tid = threadIdx.x;
+  const long long kk = (long long)k * k;
+  for (long long e = tid; e < kk; e += blockDim.x) {
+    const int a = (int)(e / k), b = (int)(e % k);
+    gM[e] = (a == b ? 1.5 : 0.0) - 0.5 * Rs(J[a], J[b]);
+  }
+  __syncthreads();
+  for (long long e = tid; e < kk; e += blockDim.x) {  // gT = S_JJ M
+    const int a = (int)(e / k), b = (int)(e % k);
+    double s = 0.0;
+    for (int l = 0; l < k; l++) s += Ss(J[a], J[l]) * gM[(long long)l * k + b];
+    gT[e] = s;
+  }
+  __syncthreads();
+  for (long long e = tid; e < kk; e += blockDim.x) {  // gW = M (S M)
+    const int a = (int)(e / k), b = (int)(e % k);
+    double s = 0.0;
+    for (int l = 0; l < k; l++) s += gM[(long long)a * k + l] * gT[(long long)l * k + b];
+    gW[e] = s;
+  }
+  __syncthreads();
+  double tr = 0.0;
+  for (int a = tid; a < k; a += blockDim.x) tr += gW[(long long)a * k + a];
+  const double cen = block_sum(tr, red) / k;
+  double rmax = 0.0;
+  for (int a = tid; a < k; a += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < k; b++) s += fabs(gW[(long long)a * k + b] - (a == b ? cen : 0.0));
+    rmax = fmax(rmax, s);
+  }
+  const double rad = block_max(rmax, red);
+  const double sig = rad > 1e-30 * fmax(1.0, fabs(cen)) ? 1.05 * rad : 1.0;
+  for (int a = tid; a < k; a += blockDim.x) gW[(long long)a * k + a] = (gW[(long long)a * k + a] - cen) + sig;
+  __syncthreads();
+  double* mu = rv;
+  double* nrm = rv + k;
+  int* perm = (int*)(rv + 2 * k);
+  rr_jacobi(gW, k, mu, perm, nrm);
+  for (long long e = tid; e < kk; e += blockDim.x) {  // gT := G = M Q
+    const int a = (int)(e / k), cc = (int)(e % k);
+    const int src = perm[cc];
+    const double inv = 1.0 / nrm[src];
+    double s = 0.0;
+    for (int l = 0; l < k; l++) s += gM[(long long)a * k + l] * (gW[(long long)src * k + l] * inv);
+    gT[e] = s;
+  }
+  __syncthreads();
+  // old E(:, J) of the rows outside the cluster -> gW (n x k), then E(i, J) = old(i, :) G
+  for (long long e = tid; e < (long long)n * k; e += blockDim.x) {
+    const int i = (int)(e / k), l = (int)(e % k);
+    if (cid[i] != c) gW[e] = Eb[(long long)i * n + J[l]];
+  }
+  __syncthreads();
+  for (long long e = tid; e < (long long)n * k; e += blockDim.x) {
+    const int i = (int)(e / k), cc = (int)(e % k);
+    if (cid[i] == c) continue;
+    double s = 0.0;
+    for (int l = 0; l < k; l++) s += gW[(long long)i * k + l] * gT[(long long)l * k + cc];
+    Eb[(long long)i * n + J[cc]] = s;
+  }
+  for (long long e = tid; e < kk; e += blockDim.x) {
+    const int a = (int)(e / k), cc = (int)(e % k);
+    Eb[(long long)J[a] * n + J[cc]] = gT[e] - (a == cc ? 1.0 : 0.0);
+  }
+  for (int cc = tid; cc < k; cc += blockDim.x) lam[J[cc]] = mu[cc] - sig + cen;
   __syncthreads();
 }
 
 __global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__ blocks, const double* __restrict__ S,
                                                     const double* __restrict__ Rp, double* __restrict__ E,
                                                     double* __restrict__ lam, int* __restrict__ stats,
-                                                    double* __restrict__ conv) {
+                                                    double* __restrict__ conv, double* __restrict__ scrW,
+                                                    double* __restrict__ scrG, double* __restrict__ scrT,
+                                                    double* __restrict__ rrv) {
   extern __shared__ double sh[];
   const EigBlock blk = blocks[blockIdx.x];
   const int n = blk.n;
@@ -397,7 +497,9 @@ __global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__
     if (k < 2) continue;
     maxk = max(maxk, k);
     if (k > RR_MAX) {
-      if (tid == 0) atomicOr(&stats[1], 1);  // unresolved cluster: left to the OA formula
+      if (tid == 0) atomicOr(&stats[1], 1);  // counted: a wide cluster takes the global-memory path
+      rr_wide_cluster(Sb, Rb, Eb, n, k, idx + s0, cid, c, scrW + blk.off, scrG + blk.off, scrT + blk.off,
+                      rrv + 3 * (long long)blk.e_off, lam + blk.e_off, red);
       continue;
     }
     // M = 1.5 I - 0.5 Rp_JJ;  T = M S_JJ M  (row-major k x k in rrW after two products)
@@ -564,9 +666,11 @@ void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const
   NRGK_HIP(hipGetLastError());
 }
 void launch_oa_update(const EigBlock* blocks, int nb, const double* S, const double* Rp, double* E, double* lam,
-                      int* stats, double* conv, size_t lds, hipStream_t s) {
+                      int* stats, double* conv, double* scrW, double* scrG, double* scrT, double* rrv, size_t lds,
+                      hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_oa_update, dim3(nb), dim3(1024), lds, s, blocks, S, Rp, E, lam, stats, conv);
+  hipLaunchKernelGGL(k_oa_update, dim3(nb), dim3(1024), lds, s, blocks, S, Rp, E, lam, stats, conv, scrW, scrG, scrT,
+                     rrv);
   NRGK_HIP(hipGetLastError());
 }
 void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,

@@ -16,7 +16,8 @@ void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks,
 void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
                            size_t lds, hipStream_t s);
 void launch_oa_update(const EigBlock* blocks, int nb, const double* S, const double* Rp, double* E, double* lam,
-                      int* stats, double* conv, size_t lds, hipStream_t s);
+                      int* stats, double* conv, double* scrW, double* scrG, double* scrT, double* rrv, size_t lds,
+                      hipStream_t s);
 void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
                        double* lam_out, size_t lds, hipStream_t s);
 size_t normalize_sort_lds(int nmax);

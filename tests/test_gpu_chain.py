@@ -182,9 +182,11 @@ def test_fdm_spectrum_matches_oracle(nk, oracle_mod):
     np.testing.assert_allclose(sums, osums, rtol=1e-8)
     tot, otot = pos.sum() + neg.sum(), opos.sum() + oneg.sum()
     assert abs(tot - otot) <= 1e-9 * otot
-    # sum rule per spin: <{d, d^dagger}> = 1 minus the weight outside the bins
+    # Per spin the weight is near the <{d, d+}> = 1 sum rule but not on it: the reference bins
+    # |(B rho)_vv' A_v'v| element by element (fdmSpectrum.hpp:237-258), so contributions of the
+    # off-diagonal reduced density matrix add with either sign, a few per cent over one here.
     for p in range(2):
-        assert 0.9 < pos[p].sum() + neg[p].sum() <= 1.0 + 1e-9
+        assert 0.9 < pos[p].sum() + neg[p].sum() < 1.1
     # broadened spectra (gauge- and bin-edge-robust)
     om = np.logspace(-8, 0, 200)
     for p in range(2):

@@ -137,6 +137,61 @@ def test_siam_chain_equals_exact_diagonalisation(oracle_mod):
         np.testing.assert_allclose(np.sort(expect), specs[k], atol=1e-12)
 
 
+def _ed_siam_final(eps, U, V, ts, rescales):
+    """Final Hamiltonian of the chain of _ed_siam and the impurity creation operators d+_up, d+_dn."""
+    n_orb = 2 * (len(ts) + 2)
+    c = _jw_ops(n_orb)
+    a = [x.T for x in c]
+    nd = [c[s] @ a[s] for s in (0, 1)]
+    H = eps * (nd[0] + nd[1]) + U * (nd[0] @ nd[1])
+    for s in (0, 1):
+        hop = c[s] @ a[2 + s]
+        H = H + V * (hop + hop.T)
+    for k, (t, r) in enumerate(zip(ts, rescales)):
+        H = r * H
+        for s in (0, 1):
+            hop = c[2 + 2 * k + s] @ a[2 + 2 * (k + 1) + s]
+            H = H + t * (hop + hop.T)
+    return H, c[0], c[1]
+
+
+def test_fdm_last_shell_equals_exact_diagonalisation(oracle_mod):
+    """With K equal to the 256 states of iteration 1, iteration 2 is the first (and last) one that
+    checkHigherEnergyDiscarded (nrgcore.hpp:384-386) reports, yet nothing has been discarded: it
+    stores one FDM shell over the full, exact eigenbasis (1024 states), whose density matrix is the
+    ground-state projector (fdmSpectrum.hpp:122-155, 165-199).
+    Its binned weights (fdmSpectrum.hpp:207-277) are then the Lehmann weights of the exact chain:
+    |<n|d+|g>|^2 at E_n - E_g (positive) and |<n|d|g>|^2 at E_g - E_n (negative), averaged over the
+    degenerate ground states, in bins int(ln(1 + |s dE| / 1e-10) delE), s = 1/Lambda; and each
+    spin's total weight is <d d+ + d+ d> = 1."""
+    eps, U, G, L, nb = -0.2, 0.5, 0.02, 2.0, 400
+    V = np.sqrt(2 * G / np.pi)
+    drv = oracle_mod.driver(imp_kind=0, eps=eps, U=U, bath_kind=0, Lambda=L, first_V=V, nsites=3, K=256)
+    pos, neg, tr, sums = oracle_mod.run_fdm(drv, nb)
+    assert len(tr) == 1
+    np.testing.assert_allclose(tr, 1.0, atol=1e-13)
+    ts = [oracle_mod.hopping(n, L) for n in range(3)]
+    H, cu, cd = _ed_siam_final(eps, U, V, ts, [1.0, np.sqrt(L), np.sqrt(L)])
+    w, Q = np.linalg.eigh(H)
+    gs = np.nonzero(np.abs(w - w[0]) < 1e-5)[0]
+    scale = 1.0 / L
+    dele = (nb - 1) / np.log(10 / 1e-10)
+    for p, cdag in enumerate((cu, cd)):
+        M = Q.T @ cdag @ Q  # <n|d+|m>
+        ep, en = np.zeros(nb), np.zeros(nb)
+        for g in gs:
+            for n in range(len(w)):
+                x = np.log(1 + abs(scale * (w[n] - w[g])) / 1e-10) * dele
+                assert abs(x - round(x)) > 1e-6 or round(x) == 0  # no weight on a bin edge
+                k = int(x)
+                if k < nb:
+                    ep[k] += M[n, g] ** 2 / len(gs)
+                    en[k] += M[g, n] ** 2 / len(gs)
+        np.testing.assert_allclose(pos[p].sum() + neg[p].sum(), 1.0, atol=1e-12)
+        np.testing.assert_allclose(pos[p], ep, atol=1e-12)
+        np.testing.assert_allclose(neg[p], en, atol=1e-12)
+
+
 def test_truncation_keeps_K_plus_one_and_shifts_only_when_truncating(oracle_mod):
     imp = oracle_mod.Model("spinhalf", eps=-0.25, U=0.5)
     bath = oracle_mod.Model("spinhalf")
@@ -153,3 +208,15 @@ def test_truncation_keeps_K_plus_one_and_shifts_only_when_truncating(oracle_mod)
     assert len(ch.relative_gs()) == 1 and ch.relative_gs()[0] == allE[0]
     # the shifted spectrum starts at 0
     assert min(e.min() for e in ch.eigenvalues_q() if len(e)) == 0.0
+
+
+def test_fdm_reduced_density_matrices_have_unit_trace(oracle_mod):
+    """The backward sweep hands each shell the partial trace of the previous shell's density matrix
+    over the bath site (fdmSpectrum.hpp:71-114) plus, at the last shell, the ground-state projector:
+    every shell's rhoZero then has trace 1 (fdmSpectrum.hpp:191-196 prints it)."""
+    drv = oracle_mod.driver(imp_kind=0, eps=-0.15, U=0.3, bath_kind=0, Lambda=2.0,
+                            first_V=np.sqrt(2 * 0.01 / np.pi), nsites=14, K=96)
+    pos, neg, tr, sums = oracle_mod.run_fdm(drv, 200)
+    assert len(tr) >= 8
+    np.testing.assert_allclose(tr, 1.0, atol=1e-12)
+    assert np.all(pos >= 0) and np.all(neg >= 0)
