@@ -87,7 +87,8 @@ Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, 
   NRGK_REQUIRE(mode == NRGK_MODE_MIXED || mode == NRGK_MODE_PLAIN64, NRGK_ERR_INVALID, "nrgk: unknown mode");
   for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
                     &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
-                    &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act})
+                    &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act,
+                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax})
     b->ctx = ctx;
   NRGK_HIP(hipSetDevice(ctx->device));
   NRGK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -455,19 +456,42 @@ void Chain::eigensolve_mixed() {
   check("normalize_sort (lam)", dlam_tmp, false, true);
   d_stats.ensure(4 * sizeof(int));
   NRGK_HIP(hipMemsetAsync(d_stats.p, 0, 4 * sizeof(int), st));
-  // Ogita-Aishima steps, per block until the out-of-cluster corrections are below 1e-8 (the step
-  // after which the Rayleigh quotients are exact to ~1e-16 relative) and the orthogonality defect
-  // below 1e-12.  Converged blocks leave the batch; both X buffers then hold their final vectors.
-  const int oa_max = (int)knob("NRGK_OA_ITERS", 8);
-  const double e_tol = knob("NRGK_OA_ETOL", 1e-8), d_tol = knob("NRGK_OA_DTOL", 1e-12);
+  // Ogita-Aishima steps, per block until the out-of-cluster residuals |S_ij - lt_j Rp_ij| are within
+  // 64 times the rounding noise of S (sqrt(n) eps max|lt|: no further step can lower them) and the
+  // orthogonality defect is below 1e-12.  Converged blocks leave the batch; both X buffers then hold
+  // their final vectors.
+  const int oa_max = (int)knob("NRGK_OA_ITERS", 12);
+  const double e_tol = knob("NRGK_OA_ETOL", 64.0), d_tol = knob("NRGK_OA_DTOL", 1e-12);
+  const int dbg_oa = (int)knob("NRGK_DEBUG_OA", 0);
   std::vector<EigBlock> act = blks;
-  d_conv.ensure(std::max(1, nb) * 2 * sizeof(double));
+  d_conv.ensure(std::max(1, nb) * 3 * sizeof(double));
   oa_steps = 0;
+  d_cid.ensure(std::max<long long>(1, N_tot) * sizeof(int));
+  d_sidx.ensure(std::max<long long>(1, N_tot) * sizeof(int));
+  d_offmax.ensure(std::max(1, nb) * sizeof(double));
   for (int it = 0; it < oa_max && !act.empty(); it++) {
     const int na = (int)act.size();
-    d_eig_act.ensure(na * sizeof(EigBlock));
-    h2d(d_eig_act.as<EigBlock>(), act.data(), na, st);
-    NRGK_HIP(hipMemsetAsync(d_conv.p, 0, na * 2 * sizeof(double), st));
+    // 32 x 32 tiles of the active blocks: all of them, and the (i <= j) half for the symmetrisation
+    std::vector<int4> tiles_all, tiles_sym;
+    for (int a = 0; a < na; a++) {
+      const int T = (act[a].n + 31) / 32;
+      for (int ti = 0; ti < T; ti++)
+        for (int tj = 0; tj < T; tj++) {
+          tiles_all.push_back(int4{a, ti, tj, 0});
+          if (ti <= tj) tiles_sym.push_back(int4{a, ti, tj, 0});
+        }
+    }
+    const size_t blk_bytes = (na * sizeof(EigBlock) + 15) & ~size_t(15);  // int4 tiles start 16-byte aligned
+    const size_t meta = blk_bytes + (tiles_all.size() + tiles_sym.size()) * sizeof(int4);
+    d_eig_act.ensure(meta);
+    EigBlock* m_blk = d_eig_act.as<EigBlock>();
+    int4* m_all = (int4*)((char*)d_eig_act.p + blk_bytes);
+    int4* m_sym = m_all + tiles_all.size();
+    h2d(m_blk, act.data(), na, st);
+    h2d(m_all, tiles_all.data(), tiles_all.size(), st);
+    h2d(m_sym, tiles_sym.data(), tiles_sym.size(), st);
+    NRGK_HIP(hipMemsetAsync(d_conv.p, 0, na * 3 * sizeof(double), st));
+    NRGK_HIP(hipMemsetAsync(d_offmax.p, 0, na * sizeof(double), st));
     std::vector<GemmProb> g1, g2, g3;
     for (auto& b : act) {
       const int n = b.n;
@@ -490,12 +514,13 @@ void Chain::eigensolve_mixed() {
     check("R=X^T X", dR, false, false);
     // clusters wider than the LDS Rayleigh-Ritz work in P (free after S = X^T P), X2 (written only
     // by the X + X E product that follows) and Q
-    launch_oa_update(d_eig_act.as<EigBlock>(), na, dS.as<double>(), dR.as<double>(), dE.as<double>(),
-                     dlam_tmp.as<double>(), d_stats.as<int>(), d_conv.as<double>(), dP.as<double>(),
-                     dX2.as<double>(), dQ.as<double>(), d_rrv.as<double>(), oa_update_lds(nmax), st);
+    launch_oa(m_blk, na, m_sym, (int)tiles_sym.size(), m_all, (int)tiles_all.size(), dS.as<double>(), dR.as<double>(),
+              dE.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_sidx.as<int>(), d_offmax.as<double>(),
+              d_stats.as<int>(), d_conv.as<double>(), dP.as<double>(), dX2.as<double>(), dQ.as<double>(),
+              d_rrv.as<double>(), nmax, e_tol, d_tol, st);
     check("oa_update (E)", dE, false, false);
     check("oa_update (lam)", dlam_tmp, false, true);
-    std::vector<double> conv(2 * na);
+    std::vector<double> conv(3 * na);
     d2h(conv.data(), d_conv.as<double>(), conv.size(), st);
     run_gemms(g3);  // synchronises the stream
     std::swap(dX, dX2);
@@ -503,7 +528,12 @@ void Chain::eigensolve_mixed() {
     oa_steps = it + 1;
     std::vector<EigBlock> next;
     for (int a = 0; a < na; a++) {
-      if (conv[2 * a] > e_tol || conv[2 * a + 1] > d_tol) {
+      // NRGK_DEBUG_OA=n: per step, the residual (in noise units) and orthogonality defect of every
+      // active block at least n wide
+      if (dbg_oa > 0 && act[a].n >= dbg_oa)
+        std::fprintf(stderr, "nrgk oa step %d block off=%lld n=%d residual/noise=%.3g orth=%.3g\n", it, act[a].off,
+                     act[a].n, conv[3 * a] / conv[3 * a + 2], conv[3 * a + 1]);
+      if (conv[3 * a] > e_tol * conv[3 * a + 2] || conv[3 * a + 1] > d_tol) {
         next.push_back(act[a]);
       } else {
         const EigBlock& b = act[a];

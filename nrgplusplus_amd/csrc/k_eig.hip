@@ -186,10 +186,10 @@ __global__ void __launch_bounds__(1024) k_normalize_sort(const EigBlock* __restr
 }
 
 // ---------------------------------------------------------------- Ogita-Aishima update
-// One workgroup per block.  Inputs S = X^T H X, Rp = X^T X.  Output E (X <- X + X E) and lam.
+// Inputs S = X^T H X, Rp = X^T X.  Output E (X <- X + X E) and lam.
 //   lt_i = S_ii / Rp_ii; E_ii = (1 - Rp_ii)/2;  E_ij = (S_ij - lt_j Rp_ij)/(lt_j - lt_i)
-// for eigenvalues farther apart than delta = max(32 max_{i!=j}(|S_ij| + |lt_j Rp_ij|) + 64 eps max|lt|,
-// 1e5 sqrt(n) eps max|lt|).
+// for eigenvalues farther apart than delta = max(32 max_{i!=j}|rho_ij| + 64 eps max|lt|,
+// 1e5 sqrt(n) eps max|lt|), rho_ij = S_ij - (lt_i + lt_j)/2 Rp_ij.
 // Clusters (runs of sorted lt with gaps <= delta) get a Rayleigh-Ritz step: M = 1.5 I - 0.5 Rp_JJ,
 // (M S_JJ M) Q = Q mu, E_JJ = M Q - I and the out-of-cluster rows of E(:,J) are mapped through M Q.
 constexpr int RR_MAX = 64;
@@ -287,7 +287,7 @@ __device__ double block_max(double v, double* red) {
 // G = M Q, E(i,J) <- E(i,J) G outside the cluster, E_JJ = G - I.  J = members (block indices).
 __device__ void rr_wide_cluster(const double* Sb, const double* Rb, double* Eb, int n, int k, const int* J,
                                 const int* cid, int c, double* gW, double* gM, double* gT, double* rv, double* lam,
-                                double* red) {
+                                double* red, bool map_rows) {
   auto Ss = [&](int i, int j) { return 0.5 * (Sb[(long long)i * n + j] + Sb[(long long)j * n + i]); };
   auto Rs = [&](int i, int j) { return 0.5 * (Rb[(long long)i * n + j] + Rb[(long long)j * n + i]); };
   const int tid = threadIdx.x;
@@ -338,12 +338,12 @@ __device__ void rr_wide_cluster(const double* Sb, const double* Rb, double* Eb, 
   }
   __syncthreads();
   // old E(:, J) of the rows outside the cluster -> gW (n x k), then E(i, J) = old(i, :) G
-  for (long long e = tid; e < (long long)n * k; e += blockDim.x) {
+  for (long long e = tid; e < (long long)n * k && map_rows; e += blockDim.x) {
     const int i = (int)(e / k), l = (int)(e % k);
     if (cid[i] != c) gW[e] = Eb[(long long)i * n + J[l]];
   }
   __syncthreads();
-  for (long long e = tid; e < (long long)n * k; e += blockDim.x) {
+  for (long long e = tid; e < (long long)n * k && map_rows; e += blockDim.x) {
     const int i = (int)(e / k), cc = (int)(e % k);
     if (cid[i] == c) continue;
     double s = 0.0;
@@ -358,77 +358,155 @@ __device__ void rr_wide_cluster(const double* Sb, const double* Rb, double* Eb, 
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__ blocks, const double* __restrict__ S,
-                                                    const double* __restrict__ Rp, double* __restrict__ E,
-                                                    double* __restrict__ lam, int* __restrict__ stats,
-                                                    double* __restrict__ conv, double* __restrict__ scrW,
-                                                    double* __restrict__ scrG, double* __restrict__ scrT,
-                                                    double* __restrict__ rrv) {
+// The OA step as four launches over the active blocks:
+//   k_oa_sym      32x32 tile pairs (i,j)/(j,i): S and Rp replaced by their symmetric parts (they come
+//                 from separate GEMM tiles and are symmetric only up to rounding; with symmetric
+//                 parts E + E^T = -(Rp - I) holds to rounding, so the step keeps orthogonality even
+//                 where a small gap magnifies GEMM noise); per block the largest off-diagonal
+//                 rotation residual |rho_ij| = |S_ij - (lt_i + lt_j)/2 Rp_ij| (the orthogonality
+//                 defect, which needs no gap, is left out) and the largest |Rp_ij - delta_ij|
+//   k_oa_prep     one workgroup per block: lt, delta, ascending sort, cluster ids, and the rounding
+//                 noise level sqrt(n) eps max|lt| of the entries of S
+//   k_oa_entries  32x32 tiles: E_ij (coalesced), largest out-of-cluster residual |rho_ij|
+// conv[3b..3b+2] = (largest residual, largest orthogonality defect, noise level) of active block b.
+//   k_oa_rr       one workgroup per block: Rayleigh-Ritz of every cluster of two or more
+constexpr int OT = 32;
+
+// A block takes its last step when its out-of-cluster residuals are within e_tol times the rounding
+// noise of S and its orthogonality defect is at most d_tol.  Its out-of-cluster corrections are then
+// rounding noise over a gap, which could only spoil orthogonality: the last step applies the cluster
+// rotations and the normalisation alone.
+__device__ __forceinline__ bool oa_converged(const double* conv, int b, double e_tol, double d_tol) {
+  return conv[3 * b] <= e_tol * conv[3 * b + 2] && conv[3 * b + 1] <= d_tol;
+}
+
+__global__ void __launch_bounds__(256) k_oa_mask(const EigBlock* __restrict__ blocks, const int4* __restrict__ tiles,
+                                                 const int* __restrict__ cid_g, const double* __restrict__ conv,
+                                                 double e_tol, double d_tol, double* __restrict__ E) {
+  const int4 t = tiles[blockIdx.x];
+  if (!oa_converged(conv, t.x, e_tol, d_tol)) return;
+  const EigBlock blk = blocks[t.x];
+  const int n = blk.n;
+  double* Eb = E + blk.off;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int j = t.z * OT + tx;
+  if (j >= n) return;
+  const int cj = cid_g[blk.e_off + j];
+  for (int rr = ty; rr < OT; rr += 8) {
+    const int i = t.y * OT + rr;
+    if (i >= n) break;
+    if (cid_g[blk.e_off + i] != cj) Eb[(long long)i * n + j] = 0.0;
+  }
+}
+
+
+__global__ void __launch_bounds__(256) k_oa_sym(const EigBlock* __restrict__ blocks, const int4* __restrict__ tiles,
+                                                double* __restrict__ S, double* __restrict__ Rp,
+                                                double* __restrict__ offmax, double* __restrict__ conv) {
+  __shared__ double tS[OT][OT + 1], tR[OT][OT + 1];
+  __shared__ double red[8];
+  const int4 t = tiles[blockIdx.x];
+  const EigBlock blk = blocks[t.x];
+  const int n = blk.n;
+  double* Sb = S + blk.off;
+  double* Rb = Rp + blk.off;
+  const int r0 = t.y * OT, c0 = t.z * OT;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // the mirror tile (rows c0.., columns r0..)
+  for (int rr = ty; rr < OT; rr += 8) {
+    const int gr = c0 + rr, gc = r0 + tx;
+    if (gr < n && gc < n) {
+      tS[rr][tx] = Sb[(long long)gr * n + gc];
+      tR[rr][tx] = Rb[(long long)gr * n + gc];
+    }
+  }
+  double sv[4], rv[4];
+  const int j = c0 + tx;
+  double ltj = 0.0;
+  if (j < n) ltj = Sb[(long long)j * n + j] / Rb[(long long)j * n + j];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = r0 + ty + 8 * q;
+    if (i < n && j < n) {
+      sv[q] = Sb[(long long)i * n + j];
+      rv[q] = Rb[(long long)i * n + j];
+    }
+  }
+  __syncthreads();  // every own element read before any mirror element is written
+  double mx = 0.0, dm = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int rr = ty + 8 * q, i = r0 + rr;
+    if (i >= n || j >= n) continue;
+    const double s = 0.5 * (sv[q] + tS[tx][rr]);
+    const double r = 0.5 * (rv[q] + tR[tx][rr]);
+    if (i != j) {
+      const double lti = Sb[(long long)i * n + i] / Rb[(long long)i * n + i];
+      mx = fmax(mx, fabs(s - 0.5 * (lti + ltj) * r));
+      dm = fmax(dm, fabs(r));
+    } else {
+      dm = fmax(dm, fabs(1.0 - r));
+    }
+    Sb[(long long)i * n + j] = s;
+    Sb[(long long)j * n + i] = s;
+    Rb[(long long)i * n + j] = r;
+    Rb[(long long)j * n + i] = r;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    dm = fmax(dm, __shfl_xor(dm, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = mx;
+    red[4 + (threadIdx.x >> 6)] = dm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    dm = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+    // non-negative doubles order like their bit patterns
+    atomicMax((unsigned long long*)&offmax[t.x], (unsigned long long)__double_as_longlong(mx));
+    atomicMax((unsigned long long*)&conv[3 * t.x + 1], (unsigned long long)__double_as_longlong(dm));
+  }
+}
+
+// lt_i = S_ii / Rp_ii; delta = max(32 max|rho| + 64 eps max|lt|, 1e5 sqrt(n) eps max|lt|), so that no
+// out-of-cluster rotation exceeds 1/32; below the
+// rounding noise of S_ij (about sqrt(n) eps max|lt|) the OA quotient is noise over gap, so pairs
+// closer than 1e5 times that noise are handled as clusters, which caps the out-of-cluster corrections
+// made of pure rounding at about 1e-5 and keeps the iteration at its fixed point once converged.
+// Clusters are the runs of sorted lt with gaps <= delta; cid = cluster number, sidx = sorted order.
+__global__ void __launch_bounds__(1024) k_oa_prep(const EigBlock* __restrict__ blocks, const double* __restrict__ S,
+                                                  const double* __restrict__ Rp, const double* __restrict__ offmax,
+                                                  double* __restrict__ lam, int* __restrict__ cid_g,
+                                                  int* __restrict__ sidx_g, int* __restrict__ stats,
+                                                  double* __restrict__ conv) {
   extern __shared__ double sh[];
   const EigBlock blk = blocks[blockIdx.x];
   const int n = blk.n;
   if (n == 0) return;
-  const double* Sb = S + blk.off;
-  const double* Rb = Rp + blk.off;
-  double* Eb = E + blk.off;
-  // S and R come from separate GEMM tiles and are symmetric only up to rounding; the update uses
-  // their symmetric parts, so that E + E^T = -(R - I) holds to rounding and the step preserves
-  // orthogonality even where a small gap magnifies the GEMM noise
-  auto Ss = [&](int i, int j) { return 0.5 * (Sb[(long long)i * n + j] + Sb[(long long)j * n + i]); };
-  auto Rs = [&](int i, int j) { return 0.5 * (Rb[(long long)i * n + j] + Rb[(long long)j * n + i]); };
   int np2 = 1;
   while (np2 < n) np2 <<= 1;
-  double* lt = sh;                     // [n]
-  double* key = lt + n;                // [np2]
-  int* idx = (int*)(key + np2);        // [np2]
-  int* cid = idx + np2;                // [n] cluster id (index of its first sorted member)
-  int* cstart = cid + n;               // [n+1] cluster boundaries in sorted order
-  double* red = (double*)(((uintptr_t)(cstart + n + 2) + 7) & ~(uintptr_t)7);  // [32]
-  double* rrW = red + 32;              // [RR_MAX*RR_MAX]
-  double* rrM = rrW + RR_MAX * RR_MAX; // [RR_MAX*RR_MAX] M, then G = M Q
-  double* rrMu = rrM + RR_MAX * RR_MAX;  // [RR_MAX]
-  double* rrN = rrMu + RR_MAX;         // [RR_MAX]
-  int* rrP = (int*)(rrN + RR_MAX);     // [RR_MAX]
-  __shared__ double s_delta;
-  __shared__ int s_ncl;
+  double* key = sh;              // [np2]
+  int* idx = (int*)(key + np2);  // [np2]
+  int* head = idx + np2;         // [n] 1 where a cluster starts (sorted position)
+  __shared__ double red[16];
   const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += blockDim.x) lt[i] = Sb[(long long)i * n + i] / Rb[(long long)i * n + i];
-  __syncthreads();
-  // delta
-  double mx = 0.0, ml = 0.0;
-  for (long long e = tid; e < (long long)n * n; e += blockDim.x) {
-    const int i = (int)(e / n), jj = (int)(e % n);
-    if (i != jj) mx = fmax(mx, fabs(Ss(i, jj)) + fabs(lt[jj] * Rs(i, jj)));
-  }
-  for (int i = tid; i < n; i += blockDim.x) ml = fmax(ml, fabs(lt[i]));
-  for (int o = 32; o > 0; o >>= 1) {
-    mx = fmax(mx, __shfl_xor(mx, o, 64));
-    ml = fmax(ml, __shfl_xor(ml, o, 64));
-  }
-  if ((tid & 63) == 0) {
-    red[tid >> 6] = mx;
-    red[16 + (tid >> 6)] = ml;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double a = 0, bq = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
-      a = fmax(a, red[w]);
-      bq = fmax(bq, red[16 + w]);
-    }
-    // Below the rounding noise of S_ij (about sqrt(n) eps max|lt|) the OA quotient is noise over
-    // gap; pairs closer than 1e5 times that noise are handled as clusters, which caps the
-    // out-of-cluster corrections made of pure rounding at about 1e-5 and keeps the iteration at
-    // its fixed point once converged.
-    const double floor_ = 1e5 * sqrt((double)n) * 2.220446049250313e-16 * bq;
-    s_delta = fmax(32.0 * a + 64.0 * 2.220446049250313e-16 * bq, floor_);
-  }
-  // sort lt
+  const double* Sb = S + blk.off;
+  const double* Rb = Rp + blk.off;
+  double ml = 0.0;
   for (int i = tid; i < np2; i += blockDim.x) {
-    key[i] = i < n ? lt[i] : 1e300;
+    double v = 1e300;
+    if (i < n) {
+      v = Sb[(long long)i * n + i] / Rb[(long long)i * n + i];
+      lam[blk.e_off + i] = v;
+      ml = fmax(ml, fabs(v));
+    }
+    key[i] = v;
     idx[i] = i;
   }
-  __syncthreads();
+  const double bq = block_max(ml, red);
   for (int k = 2; k <= np2; k <<= 1)
     for (int jj = k >> 1; jj > 0; jj >>= 1) {
       for (int i = tid; i < np2; i += blockDim.x) {
@@ -447,66 +525,134 @@ __global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__
       }
       __syncthreads();
     }
-  const double delta = s_delta;
-  if (tid == 0) {
-    int ncl = 0;
-    cstart[0] = 0;
-    for (int s = 1; s < n; s++)
-      if (key[s] - key[s - 1] > delta) cstart[++ncl] = s;
-    cstart[++ncl] = n;
-    s_ncl = ncl;
-  }
+  const double eps = 2.220446049250313e-16;
+  const double noise = sqrt((double)n) * eps * bq;
+  const double delta = fmax(32.0 * offmax[blockIdx.x] + 64.0 * eps * bq, 1e5 * noise);
+  if (tid == 0) conv[3 * blockIdx.x + 2] = noise;
+  for (int s = tid; s < n; s += blockDim.x) head[s] = (s == 0) || (key[s] - key[s - 1] > delta);
   __syncthreads();
-  const int ncl = s_ncl;
-  for (int c = tid; c < ncl; c += blockDim.x)
-    for (int s = cstart[c]; s < cstart[c + 1]; s++) cid[idx[s]] = c;
-  __syncthreads();
-  // standard OA entries; conv[0] = largest out-of-cluster correction, conv[1] = largest
-  // orthogonality defect |R_ij - delta_ij| (the host stops iterating once both are tiny)
-  double emax = 0.0, dmax = 0.0;
-  for (long long e = tid; e < (long long)n * n; e += blockDim.x) {
-    const int i = (int)(e / n), jj = (int)(e % n);
-    double v;
-    if (i == jj) {
-      v = 0.5 * (1.0 - Rb[e]);
-      dmax = fmax(dmax, fabs(1.0 - Rb[e]));
-    } else {
-      dmax = fmax(dmax, fabs(Rs(i, jj)));
-      if (cid[i] == cid[jj]) v = 0.0;
-      else {
-        v = (Ss(i, jj) - lt[jj] * Rs(i, jj)) / (lt[jj] - lt[i]);
-        emax = fmax(emax, fabs(v));
+  if (tid == 0) {  // cluster numbers (prefix count) and the widest cluster
+    int c = -1, maxk = 1, run = 0;
+    for (int s = 0; s < n; s++) {
+      if (head[s]) {
+        c++;
+        run = 0;
       }
+      run++;
+      maxk = max(maxk, run);
+      cid_g[blk.e_off + idx[s]] = c;
+      sidx_g[blk.e_off + s] = idx[s];
     }
-    Eb[e] = v;
+    atomicMax(&stats[0], maxk);
+    if (maxk > RR_MAX) atomicOr(&stats[1], 1);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    emax = fmax(emax, __shfl_xor(emax, o, 64));
-    dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+}
+
+__global__ void __launch_bounds__(256) k_oa_entries(const EigBlock* __restrict__ blocks, const int4* __restrict__ tiles,
+                                                    const double* __restrict__ S, const double* __restrict__ Rp,
+                                                    const double* __restrict__ lam, const int* __restrict__ cid_g,
+                                                    double* __restrict__ E, double* __restrict__ conv) {
+  __shared__ double red[4];
+  const int4 t = tiles[blockIdx.x];
+  const EigBlock blk = blocks[t.x];
+  const int n = blk.n;
+  const double* Sb = S + blk.off;
+  const double* Rb = Rp + blk.off;
+  double* Eb = E + blk.off;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int j = t.z * OT + tx;
+  double em = 0.0;
+  if (j < n) {
+    const double ltj = lam[blk.e_off + j];
+    const int cj = cid_g[blk.e_off + j];
+    for (int rr = ty; rr < OT; rr += 8) {
+      const int i = t.y * OT + rr;
+      if (i >= n) break;
+      const long long e = (long long)i * n + j;
+      double v;
+      if (i == j) {
+        v = 0.5 * (1.0 - Rb[e]);
+      } else if (cid_g[blk.e_off + i] == cj) {
+        v = 0.0;
+      } else {
+        // E_ij = (S_ij - lt_j R_ij)/(lt_j - lt_i), split into its rotation rho_ij/(lt_j - lt_i),
+        // rho_ij = S_ij - (lt_i + lt_j)/2 R_ij, and its normalisation part -R_ij/2: the split keeps
+        // E + E^T = -(R - I) exact in floating point and avoids a quotient of two small numbers
+        const double lti = lam[blk.e_off + i];
+        const double rho = Sb[e] - 0.5 * (lti + ltj) * Rb[e];
+        v = rho / (ltj - lti) - 0.5 * Rb[e];
+        em = fmax(em, fabs(rho));
+      }
+      Eb[e] = v;
+    }
   }
-  if ((tid & 63) == 0) {
-    atomicMax((unsigned long long*)&conv[2 * blockIdx.x], (unsigned long long)__double_as_longlong(emax));
-    atomicMax((unsigned long long*)&conv[2 * blockIdx.x + 1], (unsigned long long)__double_as_longlong(dmax));
-  }
-  for (int i = tid; i < n; i += blockDim.x) lam[blk.e_off + i] = lt[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) em = fmax(em, __shfl_xor(em, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = em;
   __syncthreads();
-  // Rayleigh-Ritz on clusters
-  int maxk = 1;
+  if (threadIdx.x == 0) {
+    em = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax((unsigned long long*)&conv[3 * t.x], (unsigned long long)__double_as_longlong(em));
+  }
+}
+
+// Rayleigh-Ritz on every cluster of two or more, in sorted order: M = 1.5 I - 0.5 Rp_JJ,
+// (M S_JJ M) Q = Q mu, E_JJ = M Q - I and the out-of-cluster rows of E(:,J) mapped through M Q.
+__global__ void __launch_bounds__(1024) k_oa_rr(const EigBlock* __restrict__ blocks, const double* __restrict__ S,
+                                                const double* __restrict__ Rp, double* __restrict__ E,
+                                                double* __restrict__ lam, const int* __restrict__ cid_g,
+                                                const int* __restrict__ sidx_g, double* __restrict__ scrW,
+                                                double* __restrict__ scrG, double* __restrict__ scrT,
+                                                double* __restrict__ rrv, const double* __restrict__ conv,
+                                                double e_tol, double d_tol) {
+  extern __shared__ double sh[];
+  const EigBlock blk = blocks[blockIdx.x];
+  const int n = blk.n;
+  if (n < 2) return;
+  const bool last = oa_converged(conv, blockIdx.x, e_tol, d_tol);
+  const double* Sb = S + blk.off;
+  const double* Rb = Rp + blk.off;
+  double* Eb = E + blk.off;
+  const int* cidb = cid_g + blk.e_off;
+  const int* idx = sidx_g + blk.e_off;
+  double* rrW = sh;                      // [RR_MAX*RR_MAX]
+  double* rrM = rrW + RR_MAX * RR_MAX;   // [RR_MAX*RR_MAX] M, then G = M Q
+  double* rrMu = rrM + RR_MAX * RR_MAX;  // [RR_MAX]
+  double* rrN = rrMu + RR_MAX;           // [RR_MAX]
+  double* red = rrN + RR_MAX;            // [32]
+  int* rrP = (int*)(red + 32);           // [RR_MAX]
+  int* cstart = rrP + RR_MAX;            // [n + 1] cluster starts of width >= 2 (sorted positions)
+  __shared__ int s_nc;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int nc = 0, s0 = 0;
+    for (int s = 1; s <= n; s++)
+      if (s == n || cidb[idx[s]] != cidb[idx[s - 1]]) {
+        if (s - s0 >= 2) {
+          cstart[2 * nc] = s0;
+          cstart[2 * nc + 1] = s;
+          nc++;
+        }
+        s0 = s;
+      }
+    s_nc = nc;
+  }
+  __syncthreads();
+  const int ncl = s_nc;
+  auto Ss = [&](int i, int j) { return Sb[(long long)i * n + j]; };
+  auto Rs = [&](int i, int j) { return Rb[(long long)i * n + j]; };
   for (int c = 0; c < ncl; c++) {
-    const int s0 = cstart[c], k = cstart[c + 1] - s0;
-    if (k < 2) continue;
-    maxk = max(maxk, k);
+    const int s0 = cstart[2 * c], k = cstart[2 * c + 1] - s0;
+    const int cl = cidb[idx[s0]];
     if (k > RR_MAX) {
-      if (tid == 0) atomicOr(&stats[1], 1);  // counted: a wide cluster takes the global-memory path
-      rr_wide_cluster(Sb, Rb, Eb, n, k, idx + s0, cid, c, scrW + blk.off, scrG + blk.off, scrT + blk.off,
-                      rrv + 3 * (long long)blk.e_off, lam + blk.e_off, red);
+      rr_wide_cluster(Sb, Rb, Eb, n, k, idx + s0, cidb, cl, scrW + blk.off, scrG + blk.off, scrT + blk.off,
+                      rrv + 3 * (long long)blk.e_off, lam + blk.e_off, red, !last);
       continue;
     }
     // M = 1.5 I - 0.5 Rp_JJ;  T = M S_JJ M  (row-major k x k in rrW after two products)
     for (int e = tid; e < k * k; e += blockDim.x) {
       const int a = e / k, bq = e % k;
-      const int ia = idx[s0 + a], ib = idx[s0 + bq];
-      rrM[e] = (a == bq ? 1.5 : 0.0) - 0.5 * Rs(ia, ib);
+      rrM[e] = (a == bq ? 1.5 : 0.0) - 0.5 * Rs(idx[s0 + a], idx[s0 + bq]);
     }
     __syncthreads();
     for (int e = tid; e < k * k; e += blockDim.x) {  // rrW = S_JJ M
@@ -516,8 +662,7 @@ __global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__
       rrW[e] = s;
     }
     __syncthreads();
-    // T = M (S M): store into E's cluster block temporarily? use rrN/rrMu as scratch is too small;
-    // compute column by column into registers then write back to rrW after a barrier.
+    // T = M (S M), held in registers until every thread has read rrW
     double tv[RR_MAX * RR_MAX / 1024 + 1];
     int cnt = 0;
     for (int e = tid; e < k * k; e += blockDim.x) {
@@ -553,7 +698,7 @@ __global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__
     __syncthreads();
     rr_jacobi(rrW, k, rrMu, rrP, rrN);  // symmetric: column-major == row-major
     const double cen = red[0], sig = red[1];
-    // Q(:,c) = W(:,perm[c]) / nrm ;  G = M Q  -> rrW reused after computing into registers
+    // Q(:,c) = W(:,perm[c]) / nrm ;  G = M Q
     cnt = 0;
     for (int e = tid; e < k * k; e += blockDim.x) {
       const int a = e / k, cc = e % k;
@@ -567,9 +712,10 @@ __global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__
     for (int e = tid; e < k * k; e += blockDim.x) rrM[e] = tv[cnt++];  // rrM := G
     __syncthreads();
     // out-of-cluster rows: E(i, J) <- E(i, J) G.  One thread owns a whole row i, so every old value
-    // of the row is read before any of them is overwritten.
-    for (int i = tid; i < n; i += blockDim.x) {
-      if (cid[i] == c) continue;
+    // of the row is read before any of them is overwritten.  (On a block's last step k_oa_mask
+    // clears these entries instead.)
+    for (int i = tid; i < n && !last; i += blockDim.x) {
+      if (cidb[i] == cl) continue;
       double old[RR_MAX];
       for (int l = 0; l < k; l++) old[l] = Eb[(long long)i * n + idx[s0 + l]];
       for (int cc = 0; cc < k; cc++) {
@@ -586,7 +732,6 @@ __global__ void __launch_bounds__(1024) k_oa_update(const EigBlock* __restrict__
     for (int cc = tid; cc < k; cc += blockDim.x) lam[blk.e_off + idx[s0 + cc]] = rrMu[cc] - sig + cen;
     __syncthreads();
   }
-  if (tid == 0) atomicMax(&stats[0], maxk);
 }
 
 // Final ordering: U(:, c) = X(:, perm[c]) with eigenvalues ascending (dsyevd order).
@@ -638,13 +783,13 @@ size_t normalize_sort_lds(int nmax) {
   while (np2 < nmax) np2 <<= 1;
   return (size_t)(np2 + (np2 + 1) / 2 + 1 + nmax) * sizeof(double) + 64;
 }
-size_t oa_update_lds(int nmax) {
+size_t oa_prep_lds(int nmax) {
   int np2 = 1;
   while (np2 < nmax) np2 <<= 1;
-  size_t b = (size_t)nmax * 8 + np2 * 8 + np2 * 4 + nmax * 4 + (nmax + 2) * 4;
-  b = (b + 7) & ~size_t(7);
-  b += 32 * 8 + 2 * RR_MAX * RR_MAX * 8 + 2 * RR_MAX * 8 + RR_MAX * 4 + 64;
-  return b;
+  return (size_t)np2 * 12 + (size_t)nmax * 4 + 64;
+}
+size_t oa_rr_lds(int nmax) {
+  return (size_t)(2 * RR_MAX * RR_MAX + 2 * RR_MAX + 32) * 8 + (size_t)RR_MAX * 4 + (size_t)(nmax + 2) * 4 + 64;
 }
 size_t final_sort_lds(int nmax) {
   int np2 = 1;
@@ -665,12 +810,22 @@ void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const
   hipLaunchKernelGGL(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, W, sigma, X, lam);
   NRGK_HIP(hipGetLastError());
 }
-void launch_oa_update(const EigBlock* blocks, int nb, const double* S, const double* Rp, double* E, double* lam,
-                      int* stats, double* conv, double* scrW, double* scrG, double* scrT, double* rrv, size_t lds,
-                      hipStream_t s) {
+void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym, const int4* all_tiles, int n_all,
+               double* S, double* Rp, double* E, double* lam, int* cid, int* sidx, double* offmax, int* stats,
+               double* conv, double* scrW, double* scrG, double* scrT, double* rrv, int nmax, double e_tol,
+               double d_tol, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_oa_update, dim3(nb), dim3(1024), lds, s, blocks, S, Rp, E, lam, stats, conv, scrW, scrG, scrT,
-                     rrv);
+  hipLaunchKernelGGL(k_oa_sym, dim3(n_sym), dim3(256), 0, s, blocks, sym_tiles, S, Rp, offmax, conv);
+  NRGK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_oa_prep, dim3(nb), dim3(1024), oa_prep_lds(nmax), s, blocks, S, Rp, offmax, lam, cid, sidx,
+                     stats, conv);
+  NRGK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_oa_entries, dim3(n_all), dim3(256), 0, s, blocks, all_tiles, S, Rp, lam, cid, E, conv);
+  NRGK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_oa_rr, dim3(nb), dim3(1024), oa_rr_lds(nmax), s, blocks, S, Rp, E, lam, cid, sidx, scrW, scrG,
+                     scrT, rrv, conv, e_tol, d_tol);
+  NRGK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_oa_mask, dim3(n_all), dim3(256), 0, s, blocks, all_tiles, cid, conv, e_tol, d_tol, E);
   NRGK_HIP(hipGetLastError());
 }
 void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
@@ -682,7 +837,8 @@ void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const do
 void set_kernel_lds_limits(int bytes) {
   NRGK_HIP(hipFuncSetAttribute((const void*)k_jacobi_round, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_normalize_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-  NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_update, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_prep, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_rr, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_final_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
 }
 

@@ -15,13 +15,13 @@ void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks,
                          int* flags, float tol, float ang_tol, int max_inner, size_t lds, hipStream_t s);
 void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
                            size_t lds, hipStream_t s);
-void launch_oa_update(const EigBlock* blocks, int nb, const double* S, const double* Rp, double* E, double* lam,
-                      int* stats, double* conv, double* scrW, double* scrG, double* scrT, double* rrv, size_t lds,
-                      hipStream_t s);
+void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym, const int4* all_tiles, int n_all,
+               double* S, double* Rp, double* E, double* lam, int* cid, int* sidx, double* offmax, int* stats,
+               double* conv, double* scrW, double* scrG, double* scrT, double* rrv, int nmax, double e_tol,
+               double d_tol, hipStream_t s);
 void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
                        double* lam_out, size_t lds, hipStream_t s);
 size_t normalize_sort_lds(int nmax);
-size_t oa_update_lds(int nmax);
 size_t final_sort_lds(int nmax);
 void set_kernel_lds_limits(int bytes);
 // k_gemm.hip
