@@ -307,12 +307,9 @@ static double knob(const char* name, double dflt) {
   return v ? std::atof(v) : dflt;
 }
 
-static int choose_bw(int n, int lds_max) {
-  if (n <= 32) return std::max(1, (n + 1) / 2);
-  for (int bw = 16; bw >= 2; bw >>= 1)
-    if ((size_t)2 * bw * n * sizeof(float) <= (size_t)lds_max) return bw;
-  return 1;
-}
+// block-column width of the Jacobi tournament: 16 (a task's Gram matrix is 32 x 32), or half the
+// block for blocks of at most 32
+static int choose_bw(int n) { return n <= 32 ? std::max(1, (n + 1) / 2) : 16; }
 
 // K2 + K6: FP32 block Jacobi, then two Ogita-Aishima refinement steps in FP64.
 void Chain::eigensolve_mixed() {
@@ -321,16 +318,13 @@ void Chain::eigensolve_mixed() {
   std::vector<int> row_block, row_local;
   std::vector<int2> tasks;
   int nbc_max = 2;
-  size_t panel_lds = 0;
   for (int s = 0; s < S; s++) {
     const int n = cur_n[s];
     if (n == 0) continue;
     EigBlock b{};
     b.off = cur_off[s];
     b.n = n;
-    b.bw = choose_bw(n, ctx->lds_max);
-    NRGK_REQUIRE((size_t)2 * b.bw * n * sizeof(float) <= (size_t)ctx->lds_max, NRGK_ERR_UNSUPPORTED,
-                 "nrgk: sector block of dimension " + std::to_string(n) + " exceeds the Jacobi panel capacity");
+    b.bw = choose_bw(n);
     int nbc = (n + b.bw - 1) / b.bw;
     if (nbc < 2) nbc = 2;
     if (nbc & 1) nbc++;
@@ -343,7 +337,6 @@ void Chain::eigensolve_mixed() {
       row_local.push_back(r);
     }
     nbc_max = std::max(nbc_max, nbc);
-    panel_lds = std::max(panel_lds, (size_t)2 * b.bw * n * sizeof(float));
     blks.push_back(b);
   }
   const int nb = (int)blks.size();
@@ -381,7 +374,7 @@ void Chain::eigensolve_mixed() {
   for (int sw = 0; sw < max_sweeps && nb > 0; sw++) {
     for (int r = 0; r < nbc_max - 1; r++)
       launch_jacobi_round(dblk, (const int2*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
-                          jac_tol, jac_ang, 3, panel_lds, st);
+                          jac_tol, jac_ang, 3, st);
     d2h(hflags.data(), d_flags.as<int>(), hflags.size(), st);
     NRGK_HIP(hipStreamSynchronize(st));
     sweeps = sw + 1;

@@ -36,11 +36,24 @@ __device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p,
 
 // One tournament round for all blocks.  flags[3*b + s%3] is set when block b rotated during sweep s;
 // a block whose previous sweep did not rotate has converged and its tasks return at once.
+//
+// A task owns the pair (p, q) of block columns, w <= 2 JAC_BW columns of W (column-major, length n).
+// Instead of rotating the columns pair by pair, it forms their Gram matrix G = P^T P in FP64 (P read
+// once, staged through LDS in row chunks), runs the cyclic Jacobi rotations on G (two-sided, the
+// same rotations a one-sided sweep of P would make) while accumulating V, and finally writes
+// P <- P V in one pass.  The rounding of P to FP32 then happens once per round, not per rotation.
+constexpr int JAC_BW = 16;
+constexpr int JAC_W = 2 * JAC_BW;
+constexpr int JAC_CHUNK = 256;  // rows per LDS chunk of the Gram pass
+constexpr int GP = JAC_W + 1;   // padded row of G and V
+
 __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __restrict__ blocks,
                                                               const int2* __restrict__ tasks, float* __restrict__ W,
                                                               int sweep, int round, int* __restrict__ flags,
                                                               float tol, float ang_tol, int max_inner) {
-  extern __shared__ float panel[];
+  __shared__ float chunk[JAC_W * JAC_CHUNK];
+  __shared__ double G[JAC_W * GP], V[JAC_W * GP];
+  __shared__ double rc[JAC_W / 2], rs[JAC_W / 2];
   __shared__ int s_rot_any, s_rot_sweep;
   const int2 task = tasks[blockIdx.x];
   const int b = task.x, j = task.y;
@@ -60,53 +73,112 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
   const int w = wp + wq;
   if (w == 0) return;
   float* Wb = W + blk.off;
-  // stage the panel: local column i -> global column
-  for (int i = 0; i < w; i++) {
-    const int gc = i < wp ? c0p + i : c0q + (i - wp);
-    const float* src = Wb + (long long)gc * n;
-    for (int r = threadIdx.x; r < n; r += JAC_THREADS) panel[i * n + r] = src[r];
+  const int tid = threadIdx.x;
+  auto gcol = [&](int i) { return i < wp ? c0p + i : c0q + (i - wp); };
+  // ---- G = P^T P: thread = (4x4 tile of G, row phase g); the 16 phases of a tile are 16 lanes
+  const int tile = tid >> 4, g = tid & 15;
+  const int ta = (tile >> 3) * 4, tc = (tile & 7) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) acc[x][y] = 0.0;
+  for (int r0 = 0; r0 < n; r0 += JAC_CHUNK) {
+    const int rows = min(JAC_CHUNK, n - r0);
+    for (int e = tid; e < JAC_W * JAC_CHUNK; e += JAC_THREADS) {
+      const int i = e / JAC_CHUNK, r = e % JAC_CHUNK;
+      chunk[e] = (i < w && r < rows) ? Wb[(long long)gcol(i) * n + r0 + r] : 0.0f;
+    }
+    __syncthreads();
+    for (int r = g; r < rows; r += 16) {
+      double xa[4], xc[4];
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        xa[x] = chunk[(ta + x) * JAC_CHUNK + r];
+        xc[x] = chunk[(tc + x) * JAC_CHUNK + r];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] += xa[x] * xc[y];
+    }
+    __syncthreads();
   }
-  if (threadIdx.x == 0) s_rot_any = 0;
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      double v = acc[x][y];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+      acc[x][y] = v;
+    }
+  if (g == 0) {
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) G[(ta + x) * GP + tc + y] = acc[x][y];
+  }
+  for (int e = tid; e < JAC_W * JAC_W; e += JAC_THREADS) V[(e / JAC_W) * GP + e % JAC_W] = (e / JAC_W == e % JAC_W);
+  if (tid == 0) s_rot_any = 0;
   __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wpad = (w + 1) & ~1;  // even player count (a dummy if w is odd)
-  int rotated_task = 0;
+  // ---- cyclic Jacobi on G with the tournament ordering over the w columns
+  const int wpad = (w + 1) & ~1;
   for (int inner = 0; inner < max_inner; inner++) {
-    if (threadIdx.x == 0) s_rot_sweep = 0;
+    if (tid == 0) s_rot_sweep = 0;
     __syncthreads();
     for (int step = 0; step < wpad - 1; step++) {
-      for (int pj = wave; pj < wpad / 2; pj += JAC_THREADS / 64) {
+      if (tid < wpad / 2) {
         int a, c;
-        tournament_pair(wpad, step, pj, a, c);
-        if (a >= w || c >= w) continue;
-        float* x = panel + a * n;
-        float* y = panel + c * n;
-        double al = 0.0, be = 0.0, ga = 0.0;
-        for (int r = lane; r < n; r += 64) {
-          const double xv = x[r], yv = y[r];
-          al += xv * xv;
-          be += yv * yv;
-          ga += xv * yv;
+        tournament_pair(wpad, step, tid, a, c);
+        double cs = 1.0, sn = 0.0;
+        if (a < w && c < w) {
+          const double al = G[a * GP + a], be = G[c * GP + c], ga = G[a * GP + c];
+          if (ga != 0.0 && fabs(ga) > (double)tol * sqrt(al * be)) {
+            const double zeta = (be - al) / (2.0 * ga);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
+            if (fabs(t) > (double)ang_tol) {
+              cs = 1.0 / sqrt(1.0 + t * t);
+              sn = cs * t;
+              s_rot_sweep = 1;
+              s_rot_any = 1;
+            }
+          }
         }
-        al = wave_sum(al);
-        be = wave_sum(be);
-        ga = wave_sum(ga);
-        if (ga != 0.0 && fabs(ga) > (double)tol * sqrt(al * be)) {
-          const double zeta = (be - al) / (2.0 * ga);
-          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
-          if (fabs(t) <= (double)ang_tol) continue;
-          const double cs = 1.0 / sqrt(1.0 + t * t);
-          const float c = (float)cs, s = (float)(cs * t);
-          for (int r = lane; r < n; r += 64) {
-            const float xv = x[r], yv = y[r];
-            x[r] = c * xv - s * yv;
-            y[r] = s * xv + c * yv;
+        rc[tid] = cs;
+        rs[tid] = sn;
+      }
+      __syncthreads();
+      // rows of G (G <- J^T G) and columns of V (V <- V J); pairs with sn == 0 are left alone
+      {
+        const int pj = (tid >> 5) & 15, k = tid & 31;
+        if (pj < wpad / 2 && rs[pj] != 0.0) {
+          int a, c;
+          tournament_pair(wpad, step, pj, a, c);
+          const double cs = rc[pj], sn = rs[pj];
+          if (tid < 512) {
+            const double ga = G[a * GP + k], gc = G[c * GP + k];
+            G[a * GP + k] = cs * ga - sn * gc;
+            G[c * GP + k] = sn * ga + cs * gc;
+          } else {
+            const double va = V[k * GP + a], vc = V[k * GP + c];
+            V[k * GP + a] = cs * va - sn * vc;
+            V[k * GP + c] = sn * va + cs * vc;
           }
-          if (lane == 0) {
-            s_rot_sweep = 1;
-            s_rot_any = 1;
-          }
+        }
+      }
+      __syncthreads();
+      // columns of G (G <- G J)
+      if (tid < 512) {
+        const int pj = tid >> 5, k = tid & 31;
+        if (pj < wpad / 2 && rs[pj] != 0.0) {
+          int a, c;
+          tournament_pair(wpad, step, pj, a, c);
+          const double cs = rc[pj], sn = rs[pj];
+          const double ga = G[k * GP + a], gc = G[k * GP + c];
+          G[k * GP + a] = cs * ga - sn * gc;
+          G[k * GP + c] = sn * ga + cs * gc;
         }
       }
       __syncthreads();
@@ -115,13 +187,20 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     __syncthreads();
     if (!any) break;
   }
-  rotated_task = s_rot_any;
-  for (int i = 0; i < w; i++) {
-    const int gc = i < wp ? c0p + i : c0q + (i - wp);
-    float* dst = Wb + (long long)gc * n;
-    for (int r = threadIdx.x; r < n; r += JAC_THREADS) dst[r] = panel[i * n + r];
+  if (!s_rot_any) return;  // P unchanged: nothing to write
+  // ---- P <- P V, one row per thread (every value of the row is read before it is overwritten)
+  for (int r = tid; r < n; r += JAC_THREADS) {
+    float x[JAC_W];
+#pragma unroll
+    for (int i = 0; i < JAC_W; i++) x[i] = i < w ? Wb[(long long)gcol(i) * n + r] : 0.0f;
+    for (int c = 0; c < w; c++) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < JAC_W; i++) s += (double)x[i] * V[i * GP + c];
+      Wb[(long long)gcol(c) * n + r] = (float)s;
+    }
   }
-  if (rotated_task && threadIdx.x == 0) flags[3 * b + sweep % 3] = 1;
+  if (tid == 0) flags[3 * b + sweep % 3] = 1;
 }
 
 // Column norms of the converged W give lambda + sigma; columns are normalised, sorted by lambda
@@ -798,9 +877,9 @@ size_t final_sort_lds(int nmax) {
 }
 
 void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks, float* W, int sweep, int round,
-                         int* flags, float tol, float ang_tol, int max_inner, size_t lds, hipStream_t s) {
+                         int* flags, float tol, float ang_tol, int max_inner, hipStream_t s) {
   if (n_tasks <= 0) return;
-  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), lds, s, blocks, tasks, W, sweep, round, flags,
+  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), 0, s, blocks, tasks, W, sweep, round, flags,
                      tol, ang_tol, max_inner);
   NRGK_HIP(hipGetLastError());
 }
@@ -835,7 +914,6 @@ void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const do
   NRGK_HIP(hipGetLastError());
 }
 void set_kernel_lds_limits(int bytes) {
-  NRGK_HIP(hipFuncSetAttribute((const void*)k_jacobi_round, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_normalize_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_prep, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_rr, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
