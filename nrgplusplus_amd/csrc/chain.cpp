@@ -368,13 +368,14 @@ void Chain::eigensolve_mixed() {
   // stops pairs of very different norms at the FP32 floor, the cosine test degenerate pairs
   const float jac_tol = knob("NRGK_JAC_TOL", 5e-7);
   const float jac_ang = knob("NRGK_JAC_ANGLE", 1e-7);
+  const int jac_inner = (int)knob("NRGK_JAC_INNER", 1);  // Jacobi sweeps over a task's Gram matrix
   std::vector<int> hflags(flags0.size());
   sweeps = 0;
   bool converged = false;
   for (int sw = 0; sw < max_sweeps && nb > 0; sw++) {
     for (int r = 0; r < nbc_max - 1; r++)
       launch_jacobi_round(dblk, (const int2*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
-                          jac_tol, jac_ang, 3, st);
+                          jac_tol, jac_ang, jac_inner, st);
     d2h(hflags.data(), d_flags.as<int>(), hflags.size(), st);
     NRGK_HIP(hipStreamSynchronize(st));
     sweeps = sw + 1;
@@ -449,12 +450,13 @@ void Chain::eigensolve_mixed() {
   check("normalize_sort (lam)", dlam_tmp, false, true);
   d_stats.ensure(4 * sizeof(int));
   NRGK_HIP(hipMemsetAsync(d_stats.p, 0, 4 * sizeof(int), st));
-  // Ogita-Aishima steps, per block until the out-of-cluster residuals |S_ij - lt_j Rp_ij| are within
-  // 64 times the rounding noise of S (sqrt(n) eps max|lt|: no further step can lower them) and the
-  // orthogonality defect is below 1e-12.  Converged blocks leave the batch; both X buffers then hold
-  // their final vectors.
+  // Ogita-Aishima steps, per block until the out-of-cluster rotation residuals |rho_ij| are at the
+  // rounding noise of S (sqrt(n) eps max|lt|: no further step can lower them) and the orthogonality
+  // defect is below 1e-12.  The eigenvectors then carry errors of noise/gap, as a backward-stable
+  // FP64 solver's do; the next iteration's Hamiltonian is built from them, so a looser stop would
+  // show up there.  Converged blocks leave the batch; both X buffers then hold their final vectors.
   const int oa_max = (int)knob("NRGK_OA_ITERS", 12);
-  const double e_tol = knob("NRGK_OA_ETOL", 64.0), d_tol = knob("NRGK_OA_DTOL", 1e-12);
+  const double e_tol = knob("NRGK_OA_ETOL", 1.0), d_tol = knob("NRGK_OA_DTOL", 1e-12);
   const int dbg_oa = (int)knob("NRGK_DEBUG_OA", 0);
   std::vector<EigBlock> act = blks;
   d_conv.ensure(std::max(1, nb) * 3 * sizeof(double));

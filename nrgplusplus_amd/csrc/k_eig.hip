@@ -52,8 +52,10 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
                                                               int sweep, int round, int* __restrict__ flags,
                                                               float tol, float ang_tol, int max_inner) {
   __shared__ float chunk[JAC_W * JAC_CHUNK];
-  __shared__ double G[JAC_W * GP], V[JAC_W * GP];
-  __shared__ double rc[JAC_W / 2], rs[JAC_W / 2];
+  __shared__ double G[JAC_W * GP], V[JAC_W * GP], G2[JAC_W * GP], V2[JAC_W * GP];
+  __shared__ double pc[JAC_W], pd[JAC_W];
+  __shared__ int pp[JAC_W];
+  __shared__ float Vf[JAC_W * GP];
   __shared__ int s_rot_any, s_rot_sweep;
   const int2 task = tasks[blockIdx.x];
   const int b = task.x, j = task.y;
@@ -122,8 +124,22 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
   for (int e = tid; e < JAC_W * JAC_W; e += JAC_THREADS) V[(e / JAC_W) * GP + e % JAC_W] = (e / JAC_W == e % JAC_W);
   if (tid == 0) s_rot_any = 0;
   __syncthreads();
-  // ---- cyclic Jacobi on G with the tournament ordering over the w columns
+  // ---- cyclic Jacobi on G with the tournament ordering over the w columns.  Per step every column
+  // x moves as P'(:,x) = cx P(:,x) + dx P(:,px) (px its partner, cx = cos, dx = -+sin); then
+  // G'(x,y) = cx cy G(x,y) + cx dy G(x,py) + dx cy G(px,y) + dx dy G(px,py) and V'(k,y) = cy V(k,y) +
+  // dy V(k,py), one element of each per thread, from the old into the other buffer.
   const int wpad = (w + 1) & ~1;
+  double* Gc = G;
+  double* Gn = G2;
+  double* Vc = V;
+  double* Vn = V2;
+  const double tol2 = (double)tol * (double)tol;
+  const int ex = tid >> 5, ey = tid & 31;
+  if (tid < JAC_W && tid >= wpad) {  // columns beyond the tournament never move
+    pc[tid] = 1.0;
+    pd[tid] = 0.0;
+    pp[tid] = tid;
+  }
   for (int inner = 0; inner < max_inner; inner++) {
     if (tid == 0) s_rot_sweep = 0;
     __syncthreads();
@@ -133,71 +149,65 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
         tournament_pair(wpad, step, tid, a, c);
         double cs = 1.0, sn = 0.0;
         if (a < w && c < w) {
-          const double al = G[a * GP + a], be = G[c * GP + c], ga = G[a * GP + c];
-          if (ga != 0.0 && fabs(ga) > (double)tol * sqrt(al * be)) {
-            const double zeta = (be - al) / (2.0 * ga);
-            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double al = Gc[a * GP + a], be = Gc[c * GP + c], ga = Gc[a * GP + c];
+          if (ga != 0.0 && ga * ga > tol2 * (al * be)) {
+            // the angle to FP32 accuracy; cos and sin then normalised in FP64 (cs^2 + sn^2 = 1 to
+            // rounding), so the accumulated V stays orthogonal
+            const float zeta = (float)(be - al) / (float)(2.0 * ga);
+            const float tf = (zeta >= 0.0f ? 1.0f : -1.0f) / (fabsf(zeta) + sqrtf(1.0f + zeta * zeta));
             // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
-            if (fabs(t) > (double)ang_tol) {
-              cs = 1.0 / sqrt(1.0 + t * t);
-              sn = cs * t;
+            if (fabsf(tf) > ang_tol) {
+              const double t = (double)tf, x = 1.0 + t * t;
+              double y = (double)(1.0f / sqrtf((float)x));
+              y = y * (1.5 - 0.5 * x * y * y);
+              y = y * (1.5 - 0.5 * x * y * y);
+              cs = y;
+              sn = y * t;
               s_rot_sweep = 1;
               s_rot_any = 1;
             }
           }
         }
-        rc[tid] = cs;
-        rs[tid] = sn;
+        pc[a] = cs;
+        pd[a] = -sn;
+        pp[a] = c;
+        pc[c] = cs;
+        pd[c] = sn;
+        pp[c] = a;
       }
       __syncthreads();
-      // rows of G (G <- J^T G) and columns of V (V <- V J); pairs with sn == 0 are left alone
       {
-        const int pj = (tid >> 5) & 15, k = tid & 31;
-        if (pj < wpad / 2 && rs[pj] != 0.0) {
-          int a, c;
-          tournament_pair(wpad, step, pj, a, c);
-          const double cs = rc[pj], sn = rs[pj];
-          if (tid < 512) {
-            const double ga = G[a * GP + k], gc = G[c * GP + k];
-            G[a * GP + k] = cs * ga - sn * gc;
-            G[c * GP + k] = sn * ga + cs * gc;
-          } else {
-            const double va = V[k * GP + a], vc = V[k * GP + c];
-            V[k * GP + a] = cs * va - sn * vc;
-            V[k * GP + c] = sn * va + cs * vc;
-          }
-        }
+        const double cx = pc[ex], dx = pd[ex], cy = pc[ey], dy = pd[ey];
+        const int qx = pp[ex], qy = pp[ey];
+        Gn[ex * GP + ey] = cx * (cy * Gc[ex * GP + ey] + dy * Gc[ex * GP + qy]) +
+                           dx * (cy * Gc[qx * GP + ey] + dy * Gc[qx * GP + qy]);
+        Vn[ex * GP + ey] = cy * Vc[ex * GP + ey] + dy * Vc[ex * GP + qy];
       }
       __syncthreads();
-      // columns of G (G <- G J)
-      if (tid < 512) {
-        const int pj = tid >> 5, k = tid & 31;
-        if (pj < wpad / 2 && rs[pj] != 0.0) {
-          int a, c;
-          tournament_pair(wpad, step, pj, a, c);
-          const double cs = rc[pj], sn = rs[pj];
-          const double ga = G[k * GP + a], gc = G[k * GP + c];
-          G[k * GP + a] = cs * ga - sn * gc;
-          G[k * GP + c] = sn * ga + cs * gc;
-        }
-      }
-      __syncthreads();
+      double* t = Gc;
+      Gc = Gn;
+      Gn = t;
+      t = Vc;
+      Vc = Vn;
+      Vn = t;
     }
     const int any = s_rot_sweep;
     __syncthreads();
     if (!any) break;
   }
   if (!s_rot_any) return;  // P unchanged: nothing to write
-  // ---- P <- P V, one row per thread (every value of the row is read before it is overwritten)
+  // ---- P <- P V in FP32, one row per thread (every value of the row is read before it is overwritten)
+  for (int e = tid; e < JAC_W * JAC_W; e += JAC_THREADS) Vf[(e / JAC_W) * GP + e % JAC_W] = (float)Vc[(e / JAC_W) * GP + e % JAC_W];
+  __syncthreads();
   for (int r = tid; r < n; r += JAC_THREADS) {
     float x[JAC_W];
 #pragma unroll
     for (int i = 0; i < JAC_W; i++) x[i] = i < w ? Wb[(long long)gcol(i) * n + r] : 0.0f;
     for (int c = 0; c < w; c++) {
-      double s = 0.0;
+      float s = 0.0f;
 #pragma unroll
-      for (int i = 0; i < JAC_W; i++) s += (double)x[i] * V[i * GP + c];
-      Wb[(long long)gcol(c) * n + r] = (float)s;
+      for (int i = 0; i < JAC_W; i++) s = fmaf(x[i], Vf[i * GP + c], s);
+      Wb[(long long)gcol(c) * n + r] = s;
     }
   }
   if (tid == 0) flags[3 * b + sweep % 3] = 1;
