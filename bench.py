@@ -102,6 +102,29 @@ def cpu_baseline(n_skip, budget_s=20.0):
             "seconds": secs}
 
 
+PEAK_F64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0    # MI355X HBM3E
+
+
+def roofline(kstats):
+    """The dominant kernel family of the timed region against its roofline.  The eigensolver
+    families are compute kernels (FP64 Gram and FP32 update in the Jacobi rounds, FP64 GEMMs) and are
+    priced against the FP64 peak; oa_update, which only streams S, R and E, against HBM."""
+    if not kstats:
+        return None
+    fam, k = max(kstats.items(), key=lambda kv: kv[1]["ms"])
+    sec = k["ms"] * 1e-3
+    if fam == "oa_update":
+        ach = k["bytes"] / sec / 1e9
+        return {"kernel": fam, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                "per_launch": {"bytes": k["bytes"] / k["launches"], "us": 1e3 * k["ms"] / k["launches"]}}
+    ach = k["flops"] / sec / 1e12
+    return {"kernel": fam, "bound": "mfma", "achieved": ach, "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_F64_TFLOPS, "traffic": None,
+            "per_launch": {"flops": k["flops"] / k["launches"], "us": 1e3 * k["ms"] / k["launches"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +132,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=0, help="0: the timed iterations")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding of the NRG step is a design "
@@ -124,11 +147,18 @@ def main():
     s = Solve(nk.NRGK_MODE_MIXED)
     for _ in range(W):
         s.step()
-    nk.lib()  # all calls return synchronised
+    # every library call returns with the chain's stream drained, so the host clock around the K
+    # calls is the device time of the steps; the kernel-family timers run on that same stream
+    s.g.profile(True)
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(K):
+        ta = time.perf_counter()
         s.step()
+        step_ms.append((time.perf_counter() - ta) * 1e3)
     t1 = time.perf_counter()
+    kstats = s.g.kernel_stats()
+    s.g.profile(False)
     ms_per_step = (t1 - t0) * 1e3 / K
     value = K / (t1 - t0)
     # finish the chain untimed, checking one iteration's eigenvalues against LAPACK
@@ -148,7 +178,7 @@ def main():
         p = Solve(nk.NRGK_MODE_PLAIN64, with_fdm=False)
         for _ in range(W):
             p.step()
-        n = max(1, min(args.plain_steps, total - W))
+        n = max(1, min(args.plain_steps or K, total - W))  # default: the same iterations as the timed region
         q0 = time.perf_counter()
         for _ in range(n):
             p.step()
@@ -180,7 +210,13 @@ def main():
         "plain_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "spectra": spectra,
-        "roofline": None,
+        "step_ms_p50": float(np.percentile(step_ms, 50)),
+        "step_ms_p99": float(np.percentile(step_ms, 99)),
+        "kernels": {k: dict(ms_per_step=v["ms"] / K, launches_per_step=v["launches"] / K,
+                            tflops=(v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else None,
+                            gbs=(v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None)
+                    for k, v in kstats.items()},
+        "roofline": roofline(kstats),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

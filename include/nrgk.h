@@ -141,6 +141,20 @@ int nrgk_chain_step_stats(nrgk_chain* chain, double* ms4, int* jacobi_sweeps, in
 /* refinement statistics of the last eigensolve: Ogita-Aishima steps taken and blocks that did not
  * meet the convergence test within the step budget */
 int nrgk_chain_refine_stats(nrgk_chain* chain, int* oa_steps, int* unconverged);
+/* Per-kernel-family device timing, the counterpart of the reference's RAII wall-clock timer
+ * (nrgcore/include/utils/timer.hpp:22-79) at kernel granularity.  While enabled (enabling clears the
+ * totals), each run of consecutive launches of one family is bracketed by HIP events on the chain's
+ * stream, and its device time is accumulated with the launch count and the algorithmic flops and
+ * bytes of those launches.  Families: "jacobi_round", "oa_update", "gemm_f64". */
+typedef struct {
+  char family[32];
+  double ms;
+  long long launches;
+  double flops;
+  double bytes;
+} nrgk_kernel_stat;
+int nrgk_chain_profile(nrgk_chain* chain, int enable);
+int nrgk_chain_kernel_stats(nrgk_chain* chain, nrgk_kernel_stat* out, int cap, int* n_out);
 
 /* ---------------------------------------------------------------- observables (sysOperator.hpp) */
 /* A set of system operators (e.g. d^dagger chi) defined on the impurity sectors; blocks given as

@@ -21,6 +21,12 @@ NRGK_MODE_PLAIN64 = 1
 _LIB = None
 
 
+class KernelStat(C.Structure):
+    """nrgk_kernel_stat: device time, launches and algorithmic flops/bytes of one kernel family."""
+    _fields_ = [("family", C.c_char * 32), ("ms", C.c_double), ("launches", C.c_longlong),
+                ("flops", C.c_double), ("bytes", C.c_double)]
+
+
 class SiteDesc(C.Structure):
     _fields_ = [
         ("n_sectors", C.c_int),
@@ -95,6 +101,8 @@ def lib():
             "nrgk_chain_set_state": (i, [vp, sd, i]),
             "nrgk_chain_step_stats": (i, [vp, dp, ip, ip]),
             "nrgk_chain_refine_stats": (i, [vp, ip, ip]),
+            "nrgk_chain_profile": (i, [vp, i]),
+            "nrgk_chain_kernel_stats": (i, [vp, vp, i, ip]),
         }
         optional = {
             "nrgk_obs_create": (i, [vp, i, ip, ip, ip, dp, pp]),
@@ -430,6 +438,19 @@ class NrgCore:
         _check(lib().nrgk_chain_refine_stats(self._h, C.byref(oa), C.byref(unc)))
         return dict(ms_assemble=ms[0], ms_eig=ms[1], ms_truncate=ms[2], ms_rotate=ms[3], jacobi_sweeps=sw.value,
                     max_cluster=mc.value, oa_steps=oa.value, oa_unconverged=unc.value)
+
+    def profile(self, enable=True):
+        """Start (clearing the totals) or stop per-kernel-family device timing."""
+        _check(lib().nrgk_chain_profile(self._h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        """{family: dict(ms, launches, flops, bytes)} accumulated while profiling was on."""
+        n = C.c_int()
+        _check(lib().nrgk_chain_kernel_stats(self._h, None, 0, C.byref(n)))
+        arr = (KernelStat * max(1, n.value))()
+        _check(lib().nrgk_chain_kernel_stats(self._h, arr, n.value, C.byref(n)))
+        return {arr[k].family.decode(): dict(ms=arr[k].ms, launches=arr[k].launches, flops=arr[k].flops,
+                                             bytes=arr[k].bytes) for k in range(n.value)}
 
 
 def _blocks_from(getter, h, n_ops):

@@ -419,6 +419,31 @@ int nrgk_chain_refine_stats(nrgk_chain* ch, int* oa_steps, int* unconverged) {
     if (unconverged) *unconverged = ch->c->oa_unconverged;
   });
 }
+int nrgk_chain_profile(nrgk_chain* ch, int enable) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    ch->c->prof_on = enable != 0;
+    if (enable) ch->c->kstats.clear();
+  });
+}
+int nrgk_chain_kernel_stats(nrgk_chain* ch, nrgk_kernel_stat* out, int cap, int* n_out) {
+  return guard([&] {
+    NRGK_REQUIRE(ch && n_out && (out || cap == 0), NRGK_ERR_INVALID, "null argument");
+    int i = 0;
+    for (auto& [name, k] : ch->c->kstats) {
+      if (i < cap) {
+        std::memset(out[i].family, 0, sizeof(out[i].family));
+        std::strncpy(out[i].family, name.c_str(), sizeof(out[i].family) - 1);
+        out[i].ms = k.ms;
+        out[i].launches = k.launches;
+        out[i].flops = k.flops;
+        out[i].bytes = k.bytes;
+      }
+      i++;
+    }
+    *n_out = i;
+  });
+}
 int nrgk_chain_step_stats(nrgk_chain* ch, double* ms4, int* sweeps, int* max_cluster) {
   return guard([&] {
     NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");

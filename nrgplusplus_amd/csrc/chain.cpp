@@ -93,13 +93,31 @@ Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, 
   NRGK_HIP(hipSetDevice(ctx->device));
   NRGK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   for (auto& e : ev) NRGK_HIP(hipEventCreate(&e));
+  for (auto& e : pev) NRGK_HIP(hipEventCreate(&e));
   upload_bath();
+}
+
+void Chain::prof_begin() {
+  if (prof_on) NRGK_HIP(hipEventRecord(pev[0], st));
+}
+void Chain::prof_end(const char* family, long long launches, double flops, double bytes) {
+  if (!prof_on) return;
+  NRGK_HIP(hipEventRecord(pev[1], st));
+  NRGK_HIP(hipEventSynchronize(pev[1]));
+  float t = 0;
+  NRGK_HIP(hipEventElapsedTime(&t, pev[0], pev[1]));
+  KStat& k = kstats[family];
+  k.ms += t;
+  k.launches += launches;
+  k.flops += flops;
+  k.bytes += bytes;
 }
 
 Chain::~Chain() {
   if (st) (void)hipStreamSynchronize(st);
   plain_destroy(*this);
   for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : pev) (void)hipEventDestroy(e);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -297,7 +315,15 @@ void Chain::run_gemms(std::vector<GemmProb>& probs) {
   const int tiles = prepare_gemm(live);
   d_gemm.ensure(live.size() * sizeof(GemmProb));
   h2d(d_gemm.as<GemmProb>(), live.data(), live.size(), st);
+  double fl = 0, by = 0;
+  if (prof_on)
+    for (auto& p : live) {
+      fl += 2.0 * p.m * p.n * p.k;
+      by += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (p.C0 ? 2.0 : 1.0) * p.m * p.n);
+    }
+  prof_begin();
   launch_gemm(d_gemm.as<GemmProb>(), (int)live.size(), tiles, st);
+  prof_end("gemm_f64", 1, fl, by);
   NRGK_HIP(hipStreamSynchronize(st));  // keeps the descriptor buffer reusable by the next batch
 }
 
@@ -373,9 +399,20 @@ void Chain::eigensolve_mixed() {
   sweeps = 0;
   bool converged = false;
   for (int sw = 0; sw < max_sweeps && nb > 0; sw++) {
+    double fl = 0, by = 0;
+    if (prof_on)  // blocks still rotating: per round, Gram (2 n w^2) and update (2 n w^2) of every pair
+      for (int b = 0; b < nb; b++)
+        if (sw == 0 || hflags[3 * b + (sw - 1) % 3]) {
+          const double n = blks[b].n, w = std::min(blks[b].n, 2 * blks[b].bw);
+          const double rounds = blks[b].nbc - 1, pairs = std::ceil(n / w);
+          fl += rounds * pairs * 4.0 * n * w * w;
+          by += rounds * pairs * 3.0 * n * w * 4.0;  // read twice, write once (FP32)
+        }
+    prof_begin();
     for (int r = 0; r < nbc_max - 1; r++)
       launch_jacobi_round(dblk, (const int2*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
                           jac_tol, jac_ang, jac_inner, st);
+    prof_end("jacobi_round", nbc_max - 1, fl, by);
     d2h(hflags.data(), d_flags.as<int>(), hflags.size(), st);
     NRGK_HIP(hipStreamSynchronize(st));
     sweeps = sw + 1;
@@ -509,10 +546,15 @@ void Chain::eigensolve_mixed() {
     check("R=X^T X", dR, false, false);
     // clusters wider than the LDS Rayleigh-Ritz work in P (free after S = X^T P), X2 (written only
     // by the X + X E product that follows) and Q
+    double oa_bytes = 0;  // symmetrise S, Rp (read and write both), E from S and Rp, masking pass
+    if (prof_on)
+      for (auto& b : act) oa_bytes += 8.0 * 9.0 * (double)b.n * b.n;
+    prof_begin();
     launch_oa(m_blk, na, m_sym, (int)tiles_sym.size(), m_all, (int)tiles_all.size(), dS.as<double>(), dR.as<double>(),
               dE.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_sidx.as<int>(), d_offmax.as<double>(),
               d_stats.as<int>(), d_conv.as<double>(), dP.as<double>(), dX2.as<double>(), dQ.as<double>(),
               d_rrv.as<double>(), nmax, e_tol, d_tol, st);
+    prof_end("oa_update", 5, 0.0, oa_bytes);
     check("oa_update (E)", dE, false, false);
     check("oa_update (lam)", dlam_tmp, false, true);
     std::vector<double> conv(3 * na);

@@ -2,6 +2,7 @@
 // numerical state resident on the device.
 #pragma once
 #include <map>
+#include <string>
 #include <memory>
 #include <utility>
 #include <vector>
@@ -88,6 +89,18 @@ struct Chain {
 
   // plain-mode handles (rocSOLVER / rocBLAS), created on first use
   void* rocblas = nullptr;
+
+  // per-kernel-family timing (HIP events on the chain's stream around runs of consecutive launches
+  // of one family) with the algorithmic flops and bytes of those launches; off unless enabled
+  struct KStat {
+    double ms = 0, flops = 0, bytes = 0;
+    long long launches = 0;
+  };
+  bool prof_on = false;
+  hipEvent_t pev[2];
+  std::map<std::string, KStat> kstats;
+  void prof_begin();
+  void prof_end(const char* family, long long launches, double flops, double bytes);
 
   Chain(Ctx* c, const nrgk_site_desc* imp, const nrgk_site_desc* bath, size_t K, int mode);
   ~Chain();
