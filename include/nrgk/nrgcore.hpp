@@ -1,0 +1,374 @@
+// C++ host API of libnrgk with the names and semantics of the reference's header-only nrgcore
+// (nrgcore/include/nrgcore/nrgcore.hpp, sysOperator.hpp, dynamics/fdmSpectrum.hpp,
+// static/fdmThermodynamics.hpp), layered on the C ABI of include/nrgk.h.  The numerical state of a
+// chain lives on the GPU; the public data members that reference callers read (all_eigenvalue,
+// eigenvaluesQ, ...) are host mirrors refreshed after every call that changes them.
+//
+// Differences a caller of the reference sees, all stated:
+//   * model types are nrgk::site (spinhalf, twoChannel, spinhalfKondo, or any basis given as
+//     arrays); nrgcore<im_type, bath_type> takes any type with a desc() returning nrgk_site_desc;
+//   * current_hamiltonQ is a method returning the eigenvector block of a sector (device copy);
+//   * fdmSpectrum records shells during the forward pass (record(), where the reference drivers call
+//     NrgData::saveCurrentData) and runs the whole backward pass in one call (backward());
+//   * errors are std::runtime_error carrying nrgk_last_error(), as the reference throws
+//     std::runtime_error (nrgcore.hpp:117-152).
+#pragma once
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../nrgk.h"
+
+namespace nrgk {
+
+inline void check(int rc) {
+  if (rc != NRGK_OK) throw std::runtime_error(std::string("nrgk: ") + nrgk_last_error());
+}
+
+// Dense row-major block: the layout of the reference's qmatrix<> (qmatrix.hpp:66-75).
+struct block {
+  size_t rows = 0, cols = 0;
+  std::vector<double> data;
+  double& at(size_t i, size_t j) { return data[i * cols + j]; }
+  double at(size_t i, size_t j) const { return data[i * cols + j]; }
+  size_t getrow() const { return rows; }
+  size_t getcol() const { return cols; }
+};
+
+// Block-sparse operator {(i, j) -> block}: the reference's qOperator (qOperator.hpp:18-158).
+struct q_operator {
+  std::map<std::pair<size_t, size_t>, block> blocks;
+  std::optional<const block*> get(size_t i, size_t j) const {
+    auto it = blocks.find({i, j});
+    if (it == blocks.end()) return std::nullopt;
+    return &it->second;
+  }
+  void set(size_t i, size_t j, block b) {
+    if (!blocks.emplace(std::make_pair(i, j), std::move(b)).second) throw std::runtime_error("Found operator already");
+  }
+};
+
+// Wilson-chain hopping t_n (examples/rgflowSIAM/main.cpp:5-10).
+inline double hopping(size_t n, double lambda) {
+  const double x = static_cast<double>(n);
+  return 0.5 * (1.0 + 1.0 / lambda) * (1.0 - std::pow(lambda, -x - 1.0)) /
+         std::sqrt((1.0 - std::pow(lambda, -2.0 * x - 1.0)) * (1.0 - std::pow(lambda, -2.0 * x - 3.0)));
+}
+
+// A site model in the reference's model concept (nrgcore.hpp:46-48,68-76).
+class site {
+ public:
+  static site spinhalf(double eps, double U, double h = 0.0) {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_spinhalf(eps, U, h, &s));
+    return site(s);
+  }
+  static site twoChannel() {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_two_channel(&s));
+    return site(s);
+  }
+  static site spinhalfKondo(double J, double S = 0.5) {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_kondo(J, S, &s));
+    return site(s);
+  }
+  // f^dagger_sigma chi on the spinhalf impurity (examples/fdmSpectrumSiam/main.cpp:40-60)
+  static site fdagChiSpinhalf() {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_fdag_chi_spinhalf(&s));
+    return site(s);
+  }
+  // the impurity f^dagger list doubled for two channels (examples/twoChannelSiam/main.cpp:24-25)
+  site& duplicate_ops() {
+    check(nrgk_site_duplicate_ops(h_.get()));
+    return *this;
+  }
+  nrgk_site_desc desc() const {
+    nrgk_site_desc d;
+    check(nrgk_site_view(h_.get(), &d));
+    return d;
+  }
+  std::vector<std::vector<int>> get_basis() const {
+    const nrgk_site_desc d = desc();
+    std::vector<std::vector<int>> q(d.n_sectors);
+    for (int s = 0; s < d.n_sectors; s++) q[s].assign(d.qn + s * d.n_qn, d.qn + (s + 1) * d.n_qn);
+    return q;
+  }
+  std::vector<std::vector<double>> get_eigenvaluesQ() const {
+    const nrgk_site_desc d = desc();
+    std::vector<std::vector<double>> e(d.n_sectors);
+    size_t o = 0;
+    for (int s = 0; s < d.n_sectors; s++) {
+      e[s].assign(d.energies + o, d.energies + o + d.dims[s]);
+      o += d.dims[s];
+    }
+    return e;
+  }
+  std::vector<double> get_chi_Q() const {
+    const nrgk_site_desc d = desc();
+    return std::vector<double>(d.chi, d.chi + d.n_sectors);
+  }
+  size_t n_ops() const { return static_cast<size_t>(desc().n_ops); }
+
+ private:
+  explicit site(nrgk_site* s) : h_(s, [](nrgk_site* p) { nrgk_site_destroy(p); }) {}
+  std::shared_ptr<nrgk_site> h_;
+};
+
+// A device context (GPU ordinal + HBM admission capacity).
+class context {
+ public:
+  explicit context(int device = 0) {
+    nrgk_ctx* c = nullptr;
+    check(nrgk_ctx_create(device, &c));
+    h_.reset(c, [](nrgk_ctx* p) { nrgk_ctx_destroy(p); });
+  }
+  static context& default_context() {
+    static context c(0);
+    return c;
+  }
+  void set_capacity(size_t bytes) { check(nrgk_ctx_set_capacity(h_.get(), bytes)); }
+  nrgk_ctx* get() const { return h_.get(); }
+
+ private:
+  std::shared_ptr<nrgk_ctx> h_;
+};
+
+// nrgcore<im_type, bath_type> (nrgcore.hpp:25-550).  The models must outlive the chain only as
+// long as it is being constructed: their descriptions are copied (the reference keeps raw pointers
+// and re-reads the bath on every add_bath_site, nrgcore.hpp:68-70; set_bath() gives that here).
+template <class im_type = site, class bath_type = site>
+class nrgcore {
+ public:
+  nrgcore(im_type& impurity, bath_type& bath, context& ctx = context::default_context(),
+          int mode = NRGK_MODE_MIXED, size_t max_kept = 1024) {
+    const nrgk_site_desc di = impurity.desc(), db = bath.desc();
+    nrgk_chain* c = nullptr;
+    check(nrgk_chain_create(ctx.get(), &di, &db, max_kept, mode, &c));
+    h_.reset(c, [](nrgk_chain* p) { nrgk_chain_destroy(p); });
+    n_ops_ = static_cast<size_t>(di.n_ops);
+    nqn_ = di.n_qn;
+  }
+  // set_parameters (nrgcore.hpp:336-339)
+  void set_parameters(size_t n = 1024) { check(nrgk_chain_set_parameters(h_.get(), n)); }
+  void set_bath(bath_type& bath) {
+    const nrgk_site_desc db = bath.desc();
+    check(nrgk_chain_set_bath(h_.get(), &db));
+  }
+  // add_bath_site (nrgcore.hpp:65-97): assembly, batched eigensolve, f^dagger rotation
+  void add_bath_site(const std::vector<double>& thopping, double rescale) {
+    check(nrgk_chain_add_bath_site(h_.get(), thopping.data(), static_cast<int>(thopping.size()), rescale));
+    refresh();
+  }
+  // update_internal_state (nrgcore.hpp:105-109): truncation, shift, counters
+  void update_internal_state() {
+    check(nrgk_chain_update_internal_state(h_.get()));
+    refresh();
+  }
+  std::vector<std::vector<int>> get_basis_nQ() {
+    if (current_sysmQ.empty()) throw std::runtime_error("current_sysmQ.empty!");
+    return current_sysmQ;
+  }
+  std::vector<std::vector<double>> get_eigenvaluesQ() {
+    if (eigenvaluesQ.empty()) throw std::runtime_error("eigenvaluesQ.empty!");
+    return eigenvaluesQ;
+  }
+  // the rotated Wilson-site f^dagger of the current iteration (nrgcore.hpp:377), kept x kept
+  std::vector<q_operator> get_f_dag_operator() {
+    std::vector<q_operator> out(n_ops_);
+    for (size_t p = 0; p < n_ops_; p++) {
+      int nb = 0;
+      check(nrgk_chain_fdag_blocks(h_.get(), (int)p, &nb, nullptr, nullptr, nullptr, nullptr, nullptr, 0));
+      std::vector<int> bi(nb), bj(nb), r(nb), c(nb);
+      check(nrgk_chain_fdag_blocks(h_.get(), (int)p, &nb, bi.data(), bj.data(), r.data(), c.data(), nullptr, 0));
+      size_t tot = 0;
+      for (int b = 0; b < nb; b++) tot += (size_t)r[b] * c[b];
+      std::vector<double> data(tot);
+      check(nrgk_chain_fdag_blocks(h_.get(), (int)p, &nb, bi.data(), bj.data(), r.data(), c.data(), data.data(),
+                                   data.size()));
+      size_t o = 0;
+      for (int b = 0; b < nb; b++) {
+        block blk{(size_t)r[b], (size_t)c[b], {}};
+        blk.data.assign(data.begin() + o, data.begin() + o + (size_t)r[b] * c[b]);
+        o += blk.data.size();
+        out[p].set((size_t)bi[b], (size_t)bj[b], std::move(blk));
+      }
+    }
+    return out;
+  }
+  // checkHigherEnergyDiscarded (nrgcore.hpp:384-386)
+  bool checkHigherEnergyDiscarded() {
+    int v = 0;
+    check(nrgk_chain_higher_energy_discarded(h_.get(), &v));
+    return v != 0;
+  }
+  // U block of a sector (eigenvectors in columns, dsyevd's layout, qmatrix.hpp:390-391)
+  block current_hamiltonQ(size_t sector) {
+    const size_t n = eigenvaluesQ.at(sector).size();
+    block b{n, n, std::vector<double>(n * n)};
+    check(nrgk_chain_block_u(h_.get(), (int)sector, b.data.data(), b.data.size()));
+    return b;
+  }
+  // the reference drivers decrement nrg_iterations_cnt during the backward pass
+  void set_nrg_iterations_cnt(size_t c) {
+    check(nrgk_chain_set_counter(h_.get(), (int)c));
+    refresh_counters();
+  }
+  nrgk_chain* handle() const { return h_.get(); }
+
+  // public data of the reference (nrgcore.hpp:520-542), host mirrors
+  std::vector<double> all_eigenvalue;
+  std::vector<double> relativeGroundStateEnergy;
+  std::vector<std::vector<size_t>> eigenvaluesQ_kept_indices;
+  std::vector<std::vector<int>> current_sysmQ;
+  std::vector<std::vector<double>> eigenvaluesQ;
+  size_t nrg_iterations_cnt = 0;
+  size_t nrg_iterations_min = 0;
+
+ private:
+  void refresh_counters() {
+    int c = 0, m = 0;
+    check(nrgk_chain_counters(h_.get(), &c, &m));
+    nrg_iterations_cnt = (size_t)c;
+    nrg_iterations_min = (size_t)m;
+  }
+  void refresh() {
+    refresh_counters();
+    size_t n = 0;
+    check(nrgk_chain_all_eigenvalues(h_.get(), nullptr, 0, &n));
+    all_eigenvalue.resize(n);
+    check(nrgk_chain_all_eigenvalues(h_.get(), all_eigenvalue.data(), n, &n));
+    check(nrgk_chain_relative_gs(h_.get(), nullptr, 0, &n));
+    relativeGroundStateEnergy.resize(n);
+    check(nrgk_chain_relative_gs(h_.get(), relativeGroundStateEnergy.data(), n, &n));
+    int S = 0;
+    check(nrgk_chain_sectors(h_.get(), &S, nullptr, nullptr));
+    std::vector<int> qn((size_t)S * nqn_), dims(S);
+    check(nrgk_chain_sectors(h_.get(), &S, qn.data(), dims.data()));
+    current_sysmQ.assign(S, std::vector<int>(nqn_));
+    for (int s = 0; s < S; s++)
+      for (int q = 0; q < nqn_; q++) current_sysmQ[s][q] = qn[(size_t)s * nqn_ + q];
+    size_t total = 0;
+    check(nrgk_chain_eigenvalues_q(h_.get(), nullptr, 0, &total));
+    std::vector<double> e(total);
+    check(nrgk_chain_eigenvalues_q(h_.get(), e.data(), e.size(), &total));
+    eigenvaluesQ.assign(S, {});
+    size_t o = 0;
+    for (int s = 0; s < S; s++) {
+      eigenvaluesQ[s].assign(e.begin() + o, e.begin() + o + dims[s]);
+      o += dims[s];
+    }
+    int Sk = 0;
+    check(nrgk_chain_kept_counts(h_.get(), nullptr, 0, &Sk));
+    std::vector<int> kept(Sk);
+    check(nrgk_chain_kept_counts(h_.get(), kept.data(), Sk, &Sk));
+    eigenvaluesQ_kept_indices.assign(Sk, {});
+    for (int s = 0; s < Sk; s++)
+      for (int k = 0; k < kept[s]; k++) eigenvaluesQ_kept_indices[s].push_back((size_t)k);
+  }
+  std::shared_ptr<nrgk_chain> h_;
+  size_t n_ops_ = 0;
+  int nqn_ = 2;
+};
+
+// System operators rotated along the chain: the std::vector<qOperator> argument of the reference's
+// update_system_operator (sysOperator.hpp:35-133), resident on the device.
+class system_operator {
+ public:
+  template <class chain_type>
+  system_operator(chain_type& chain, const site& op) {
+    const nrgk_site_desc d = op.desc();
+    nrgk_obs* o = nullptr;
+    check(nrgk_obs_create(chain.handle(), d.n_ops, d.op_nblocks, d.op_bi, d.op_bj, d.op_data, &o));
+    h_.reset(o, [](nrgk_obs* p) { nrgk_obs_destroy(p); });
+    n_ops_ = (size_t)d.n_ops;
+  }
+  nrgk_obs* handle() const { return h_.get(); }
+  size_t size() const { return n_ops_; }
+
+ private:
+  std::shared_ptr<nrgk_obs> h_;
+  size_t n_ops_ = 0;
+};
+
+// update_system_operator(nrg, ops) (sysOperator.hpp:35-37): after add_bath_site, before
+// update_internal_state; the operators move to the new eigenbasis in place.
+template <class chain_type>
+void update_system_operator(chain_type* /*nrg*/, system_operator& ops) {
+  check(nrgk_obs_update(ops.handle()));
+}
+
+// fdmSpectrum (dynamics/fdmSpectrum.hpp:24-364), T = 0, in the record-forward / backward-once form.
+class fdmSpectrum {
+ public:
+  template <class chain_type>
+  fdmSpectrum(chain_type& chain, system_operator& ops, int n_bins = 100000, double e_min = 1e-10,
+              double e_max = 10.0)
+      : n_bins_(n_bins), n_ops_(ops.size()) {
+    nrgk_fdm* f = nullptr;
+    check(nrgk_fdm_create(chain.handle(), ops.handle(), n_bins, e_min, e_max, &f));
+    h_.reset(f, [](nrgk_fdm* p) { nrgk_fdm_destroy(p); });
+  }
+  // where the reference drivers save the iteration (examples/fdmSpectrumSiam/main.cpp:90-93)
+  void record() { check(nrgk_fdm_record(h_.get())); }
+  // the backward loop of calcSpectrum(Lambda^{-(in-1)/2}) calls (examples/fdmSpectrumSiam/main.cpp:110-122)
+  void backward(double lambda, int scale_offset = -1, int first_cnt = 0) {
+    positiveWeight.assign(n_ops_, std::vector<double>(n_bins_, 0.0));
+    negativeWeight.assign(n_ops_, std::vector<double>(n_bins_, 0.0));
+    std::vector<double> pos(n_ops_ * n_bins_), neg(n_ops_ * n_bins_), tr(4096), sums(4096);
+    int ns = 0;
+    check(nrgk_fdm_backward(h_.get(), lambda, scale_offset, first_cnt, pos.data(), neg.data(), tr.data(), sums.data(),
+                            (int)tr.size(), &ns));
+    for (size_t p = 0; p < n_ops_; p++) {
+      positiveWeight[p].assign(pos.begin() + p * n_bins_, pos.begin() + (p + 1) * n_bins_);
+      negativeWeight[p].assign(neg.begin() + p * n_bins_, neg.begin() + (p + 1) * n_bins_);
+    }
+    rhoTraces.assign(tr.begin(), tr.begin() + ns);
+  }
+  std::vector<double> energyPoints() const {
+    std::vector<double> e(n_bins_);
+    check(nrgk_fdm_energy_points(h_.get(), e.data()));
+    return e;
+  }
+  nrgk_fdm* handle() const { return h_.get(); }
+
+  std::vector<std::vector<double>> positiveWeight, negativeWeight;
+  std::vector<double> rhoTraces;
+
+ private:
+  std::shared_ptr<nrgk_fdm> h_;
+  size_t n_bins_, n_ops_;
+};
+
+// fdmThermodynamics (static/fdmThermodynamics.hpp:17-260): shells recorded during the forward pass,
+// entropy and specific heat on a temperature grid.
+class fdmThermodynamics {
+ public:
+  template <class chain_type>
+  explicit fdmThermodynamics(chain_type& chain) {
+    nrgk_fdm* f = nullptr;
+    check(nrgk_fdm_create(chain.handle(), nullptr, 1, 1e-10, 10.0, &f));
+    h_.reset(f, [](nrgk_fdm* p) { nrgk_fdm_destroy(p); });
+  }
+  void record() { check(nrgk_fdm_record(h_.get())); }
+  void calcThermodynamics(const std::vector<double>& T, double lambda, int scale_offset = -1, int first_cnt = 0) {
+    temperatureArray = T;
+    entropy.assign(T.size(), 0.0);
+    specificHeat.assign(T.size(), 0.0);
+    check(nrgk_thermo(h_.get(), lambda, scale_offset, first_cnt, T.data(), (int)T.size(), entropy.data(),
+                      specificHeat.data()));
+  }
+  std::vector<double> temperatureArray, entropy, specificHeat;
+
+ private:
+  std::shared_ptr<nrgk_fdm> h_;
+};
+
+}  // namespace nrgk

@@ -1,0 +1,48 @@
+// The RG flow of examples/rgflowSIAM/main.cpp, written against the nrgcore C++ API of
+// include/nrgk/nrgcore.hpp: spinhalf impurity, spinhalf bath, first site {V, V} at rescale 1, then
+// {t_n, t_n} at rescale 1 for n = 0 and sqrt(Lambda) after.  Prints one line per iteration: the iteration number, the
+// state count and all_eigenvalue (first n_print values and the largest).
+//
+// usage: nrgk_rgflow_siam eps U Gamma Lambda K n_sites [n_print]
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../include/nrgk/nrgcore.hpp"
+
+int main(int argc, char** argv) {
+  if (argc < 7) {
+    std::fprintf(stderr, "usage: %s eps U Gamma Lambda K n_sites [n_print]\n", argv[0]);
+    return 2;
+  }
+  const double eps = std::atof(argv[1]), U = std::atof(argv[2]), Gamma = std::atof(argv[3]);
+  const double Lambda = std::atof(argv[4]);
+  const size_t K = (size_t)std::atol(argv[5]);
+  const int n_sites = std::atoi(argv[6]);
+  const size_t n_print = argc > 7 ? (size_t)std::atol(argv[7]) : 8;
+  try {
+    nrgk::site impurity = nrgk::site::spinhalf(eps, U);
+    nrgk::site bath = nrgk::site::spinhalf(0.0, 0.0);
+    nrgk::nrgcore<> chain(impurity, bath);
+    chain.set_parameters(K);
+    const double V = std::sqrt(2.0 * Gamma / M_PI);
+    chain.add_bath_site({V, V}, 1.0);
+    chain.update_internal_state();
+    for (int n = 0; n <= n_sites; n++) {
+      if (n > 0) {
+        const double t = nrgk::hopping((size_t)(n - 1), Lambda);
+        chain.add_bath_site({t, t}, n > 1 ? std::sqrt(Lambda) : 1.0);  // examples/rgflowSIAM/main.cpp:33-42
+        chain.update_internal_state();
+      }
+      const auto& e = chain.all_eigenvalue;
+      std::printf("%d %zu", n, e.size());
+      for (size_t i = 0; i < e.size() && i < n_print; i++) std::printf(" %.17g", e[i]);
+      std::printf(" %.17g\n", e.empty() ? 0.0 : e.back());
+    }
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "%s\n", ex.what());
+    return 1;
+  }
+  return 0;
+}

@@ -9,6 +9,8 @@ Tolerances (stated per result):
   * thermodynamics vs the reference's files: as for the oracle (where the exp clamp does not dominate);
   * FDM weights: total weight to 1e-9 relative, broadened spectrum to 1e-6 of its maximum.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -234,3 +236,29 @@ def test_error_paths(nk):
     g.update_internal_state()
     with pytest.raises(nk.NrgkError):  # nrgcore::test: operator counts differ (nrgcore.hpp:123-133)
         nk.NrgCore(nk.Site.spinhalf(0, 0).duplicate_ops(), nk.Site.spinhalf(0, 0), 64)
+
+
+@pytest.mark.gpu
+def test_cpp_nrgcore_api_rgflow_matches_oracle(oracle_mod):
+    """The C++ driver written against include/nrgk/nrgcore.hpp (examples/rgflowSIAM form) runs the
+    chain on the GPU; every iteration's all_eigenvalue equals the oracle's (same state counts, values
+    to 1e-9)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "nrgplusplus_amd", "nrgk_rgflow_siam")
+    if not os.path.exists(exe):
+        pytest.fail("nrgk_rgflow_siam is not built (run __graft_entry__.build())")
+    eps, U, G, L, K, ns = -0.25, 0.5, 0.02, 2.5, 512, 20
+    r = subprocess.run([exe, str(eps), str(U), str(G), str(L), str(K), str(ns), "100000"], capture_output=True,
+                       text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    lines = [list(map(float, ln.split())) for ln in r.stdout.strip().splitlines()]
+    assert len(lines) == ns + 1
+    ref = oracle_mod.run_rgflow(oracle_mod.driver(imp_kind=0, eps=eps, U=U, bath_kind=0, Lambda=L,
+                                                  first_V=np.sqrt(2 * G / np.pi), nsites=ns, K=K))
+    for n in range(ns):
+        got = lines[n + 1]
+        assert int(got[0]) == n + 1 and int(got[1]) == len(ref[n]), n
+        vals = np.array(got[2:-1])
+        assert len(vals) == len(ref[n])
+        assert np.abs(vals - ref[n]).max() < 1e-9, (n, np.abs(vals - ref[n]).max())

@@ -104,3 +104,13 @@ def test_duplicate_ops(nk):
     for p in (0, 1):
         for (i, j, m), (i2, j2, m2) in zip(ops[p], ops[p + 2]):
             assert (i, j) == (i2, j2) and np.array_equal(m, m2)
+
+
+def test_cpp_nrgcore_mirror_compiles():
+    """The C++ host API (include/nrgk/nrgcore.hpp, the reference's nrgcore method names) and its
+    RG-flow driver compile as plain host C++ against the C ABI."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "nrgplusplus_amd", "cpp", "rgflow_siam.cpp")
+    r = subprocess.run(["g++", "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
