@@ -23,6 +23,22 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// tournament_pair without integer division: j < N/2 and round < N-1 keep both sums below 2(N-1)
+__device__ __forceinline__ void tournament_pair_fast(int N, int round, int j, int& p, int& q) {
+  const int m = N - 1;
+  if (j == 0) {
+    p = 0;
+    q = 1 + round;
+  } else {
+    int x = j + round;
+    x -= x >= m ? m : 0;
+    int y = m - j + round;
+    y -= y >= m ? m : 0;
+    p = 1 + x;
+    q = 1 + y;
+  }
+}
+
 __device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p, int& q) {
   // N players (even); round in [0, N-1); pair j in [0, N/2)
   if (j == 0) {
@@ -44,19 +60,20 @@ __device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p,
 // P <- P V in one pass.  The rounding of P to FP32 then happens once per round, not per rotation.
 constexpr int JAC_BW = 16;
 constexpr int JAC_W = 2 * JAC_BW;
-constexpr int JAC_CHUNK = 256;  // rows per LDS chunk of the Gram pass
+constexpr int JAC_CHUNK = 768;  // rows per LDS chunk of the Gram pass (96 KB)
 constexpr int GP = JAC_W + 1;   // padded row of G and V
 
 __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __restrict__ blocks,
                                                               const int2* __restrict__ tasks, float* __restrict__ W,
                                                               int sweep, int round, int* __restrict__ flags,
                                                               float tol, float ang_tol, int max_inner) {
-  __shared__ float chunk[JAC_W * JAC_CHUNK];
-  __shared__ double G[JAC_W * GP], V[JAC_W * GP], G2[JAC_W * GP], V2[JAC_W * GP];
-  __shared__ double pc[JAC_W], pd[JAC_W];
-  __shared__ int pp[JAC_W];
+  extern __shared__ float4 chunk4[];  // [JAC_W * JAC_CHUNK] floats, column-major, 16-byte aligned
+  float* chunk = reinterpret_cast<float*>(chunk4);
+  __shared__ double G[JAC_W * GP], G2[JAC_W * GP];
+  __shared__ double pc[2][JAC_W], pd[2][JAC_W];
+  __shared__ int pp[2][JAC_W];
   __shared__ float Vf[JAC_W * GP];
-  __shared__ int s_rot_any, s_rot_sweep;
+  __shared__ int s_rot_any;
   const int2 task = tasks[blockIdx.x];
   const int b = task.x, j = task.y;
   const EigBlock blk = blocks[b];
@@ -77,7 +94,9 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
   float* Wb = W + blk.off;
   const int tid = threadIdx.x;
   auto gcol = [&](int i) { return i < wp ? c0p + i : c0q + (i - wp); };
-  // ---- G = P^T P: thread = (4x4 tile of G, row phase g); the 16 phases of a tile are 16 lanes
+  // ---- G = P^T P: thread = (4x4 tile of G, row phase g); the 16 phases of a tile are 16 lanes.  P is
+  // staged in LDS column-major, JAC_CHUNK rows at a time (all of it when n <= JAC_CHUNK, which then
+  // also serves the update); a phase takes 4 consecutive rows per 128-bit LDS read.
   const int tile = tid >> 4, g = tid & 15;
   const int ta = (tile >> 3) * 4, tc = (tile & 7) * 4;
   double acc[4][4];
@@ -85,26 +104,49 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
   for (int x = 0; x < 4; x++)
 #pragma unroll
     for (int y = 0; y < 4; y++) acc[x][y] = 0.0;
+  const bool resident = n <= JAC_CHUNK;
   for (int r0 = 0; r0 < n; r0 += JAC_CHUNK) {
     const int rows = min(JAC_CHUNK, n - r0);
-    for (int e = tid; e < JAC_W * JAC_CHUNK; e += JAC_THREADS) {
-      const int i = e / JAC_CHUNK, r = e % JAC_CHUNK;
-      chunk[e] = (i < w && r < rows) ? Wb[(long long)gcol(i) * n + r0 + r] : 0.0f;
+    const int rows4 = (rows + 3) & ~3;
+    // thread = (column tid/32, rows tid%32 + 32k): one column pointer per thread, 128-byte runs,
+    // eight loads in flight before their LDS stores; rows are filled up to a multiple of 4
+    {
+      const int i = tid >> 5, l = tid & 31;
+      const float* src = Wb + (long long)gcol(i < w ? i : 0) * n + r0;
+      float* dstc = chunk + i * JAC_CHUNK;
+      for (int rb = l; rb < rows4; rb += 256) {
+        float t[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const int r = rb + 32 * k;
+          t[k] = (i < w && r < rows) ? src[r] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          if (rb + 32 * k < rows4) dstc[rb + 32 * k] = t[k];
+      }
     }
     __syncthreads();
-    for (int r = g; r < rows; r += 16) {
-      double xa[4], xc[4];
+    for (int r = 4 * g; r < rows4; r += 64) {
+      float4 xa[4], xc[4];
 #pragma unroll
       for (int x = 0; x < 4; x++) {
-        xa[x] = chunk[(ta + x) * JAC_CHUNK + r];
-        xc[x] = chunk[(tc + x) * JAC_CHUNK + r];
+        xa[x] = *(const float4*)&chunk[(ta + x) * JAC_CHUNK + r];
+        xc[x] = *(const float4*)&chunk[(tc + x) * JAC_CHUNK + r];
       }
 #pragma unroll
       for (int x = 0; x < 4; x++)
 #pragma unroll
-        for (int y = 0; y < 4; y++) acc[x][y] += xa[x] * xc[y];
+        for (int y = 0; y < 4; y++) {
+          double t = acc[x][y];
+          t += (double)xa[x].x * (double)xc[y].x;
+          t += (double)xa[x].y * (double)xc[y].y;
+          t += (double)xa[x].z * (double)xc[y].z;
+          t += (double)xa[x].w * (double)xc[y].w;
+          acc[x][y] = t;
+        }
     }
-    __syncthreads();
+    if (!resident) __syncthreads();
   }
 #pragma unroll
   for (int x = 0; x < 4; x++)
@@ -121,84 +163,141 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
 #pragma unroll
       for (int y = 0; y < 4; y++) G[(ta + x) * GP + tc + y] = acc[x][y];
   }
-  for (int e = tid; e < JAC_W * JAC_W; e += JAC_THREADS) V[(e / JAC_W) * GP + e % JAC_W] = (e / JAC_W == e % JAC_W);
   if (tid == 0) s_rot_any = 0;
   __syncthreads();
-  // ---- cyclic Jacobi on G with the tournament ordering over the w columns.  Per step every column
-  // x moves as P'(:,x) = cx P(:,x) + dx P(:,px) (px its partner, cx = cos, dx = -+sin); then
-  // G'(x,y) = cx cy G(x,y) + cx dy G(x,py) + dx cy G(px,y) + dx dy G(px,py) and V'(k,y) = cy V(k,y) +
-  // dy V(k,py), one element of each per thread, from the old into the other buffer.
-  const int wpad = (w + 1) & ~1;
-  double* Gc = G;
-  double* Gn = G2;
-  double* Vc = V;
-  double* Vn = V2;
-  const double tol2 = (double)tol * (double)tol;
-  const int ex = tid >> 5, ey = tid & 31;
-  if (tid < JAC_W && tid >= wpad) {  // columns beyond the tournament never move
-    pc[tid] = 1.0;
-    pd[tid] = 0.0;
-    pp[tid] = tid;
-  }
-  for (int inner = 0; inner < max_inner; inner++) {
-    if (tid == 0) s_rot_sweep = 0;
-    __syncthreads();
-    for (int step = 0; step < wpad - 1; step++) {
-      if (tid < wpad / 2) {
-        int a, c;
-        tournament_pair(wpad, step, tid, a, c);
-        double cs = 1.0, sn = 0.0;
-        if (a < w && c < w) {
-          const double al = Gc[a * GP + a], be = Gc[c * GP + c], ga = Gc[a * GP + c];
-          if (ga != 0.0 && ga * ga > tol2 * (al * be)) {
-            // the angle to FP32 accuracy; cos and sin then normalised in FP64 (cs^2 + sn^2 = 1 to
-            // rounding), so the accumulated V stays orthogonal
-            const float zeta = (float)(be - al) / (float)(2.0 * ga);
-            const float tf = (zeta >= 0.0f ? 1.0f : -1.0f) / (fabsf(zeta) + sqrtf(1.0f + zeta * zeta));
-            // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
-            if (fabsf(tf) > ang_tol) {
-              const double t = (double)tf, x = 1.0 + t * t;
-              double y = (double)(1.0f / sqrtf((float)x));
-              y = y * (1.5 - 0.5 * x * y * y);
-              y = y * (1.5 - 0.5 * x * y * y);
-              cs = y;
-              sn = y * t;
-              s_rot_sweep = 1;
-              s_rot_any = 1;
-            }
-          }
-        }
-        pc[a] = cs;
-        pd[a] = -sn;
-        pp[a] = c;
-        pc[c] = cs;
-        pd[c] = sn;
-        pp[c] = a;
-      }
-      __syncthreads();
-      {
-        const double cx = pc[ex], dx = pd[ex], cy = pc[ey], dy = pd[ey];
-        const int qx = pp[ex], qy = pp[ey];
-        Gn[ex * GP + ey] = cx * (cy * Gc[ex * GP + ey] + dy * Gc[ex * GP + qy]) +
-                           dx * (cy * Gc[qx * GP + ey] + dy * Gc[qx * GP + qy]);
-        Vn[ex * GP + ey] = cy * Vc[ex * GP + ey] + dy * Vc[ex * GP + qy];
-      }
-      __syncthreads();
-      double* t = Gc;
-      Gc = Gn;
-      Gn = t;
-      t = Vc;
-      Vc = Vn;
-      Vn = t;
+  {
+    // a task whose columns are already orthogonal to the tolerance has nothing to rotate
+    const int x = tid >> 5, y = tid & 31;
+    bool need = false;
+    if (x < y && y < w) {
+      const double ga = G[x * GP + y];
+      need = ga * ga > ((double)tol * (double)tol) * (G[x * GP + x] * G[y * GP + y]);
     }
-    const int any = s_rot_sweep;
-    __syncthreads();
-    if (!any) break;
+    if (!__syncthreads_or(need)) return;
   }
-  if (!s_rot_any) return;  // P unchanged: nothing to write
-  // ---- P <- P V in FP32, one row per thread (every value of the row is read before it is overwritten)
-  for (int e = tid; e < JAC_W * JAC_W; e += JAC_THREADS) Vf[(e / JAC_W) * GP + e % JAC_W] = (float)Vc[(e / JAC_W) * GP + e % JAC_W];
+  // ---- cyclic Jacobi on G with the tournament ordering over the w columns.  Per step every column
+  // x moves as P'(:,x) = cx P(:,x) + dx P(:,px) (px its partner, cx = cos, dx = -+sin), so
+  // G' = J^T G J and V' = V J.  Thread (x, y) keeps G(x, y) and V(x, y) in registers; a wave holds
+  // two whole rows, so the column step R(x, y) = cy G(x, y) + dy G(x, py) and V'(x, y) = cy V(x, y) +
+  // dy V(x, py) take a shuffle, and only the row step G'(x, y) = cx R(x, y) + dx R(px, y) goes through
+  // LDS.  The first wpad/2 threads form, from R, the three entries of G' that the next step's rotation
+  // of their pair needs and compute that rotation in the same pass.
+  const int wpad = (w + 1) & ~1;
+  const int nsteps = max_inner * (wpad - 1);
+  const double tol2 = (double)tol * (double)tol;
+  const int ex = tid >> 5, ey = tid & 31, lane = tid & 63;
+  double gv = G[ex * GP + ey];
+  double vv = (ex == ey) ? 1.0 : 0.0;
+  int pb = 0;  // parameter buffer of the current step
+  if (tid < JAC_W && tid >= wpad) {  // columns beyond the tournament never move
+    pc[0][tid] = pc[1][tid] = 1.0;
+    pd[0][tid] = pd[1][tid] = 0.0;
+    pp[0][tid] = pp[1][tid] = tid;
+  }
+  // rotation of pair (a, c) from its Gram entries; writes the parameters of both players
+  auto rotation = [&](int buf, int a, int c, double al, double be, double ga) {
+    double cs = 1.0, sn = 0.0;
+    if (a < w && c < w && ga != 0.0 && ga * ga > tol2 * (al * be)) {
+      // the angle to FP32 accuracy; cos and sin then normalised in FP64 (cs^2 + sn^2 = 1 to
+      // rounding), so the accumulated V stays orthogonal
+      const float zeta = (float)(be - al) * __builtin_amdgcn_rcpf((float)(2.0 * ga));
+      const float tf = (zeta >= 0.0f ? 1.0f : -1.0f) *
+                       __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_amdgcn_sqrtf(1.0f + zeta * zeta));
+      // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
+      if (fabsf(tf) > ang_tol) {
+        const double t = (double)tf, x = 1.0 + t * t;
+        double y = (double)__builtin_amdgcn_rsqf((float)x);
+        y = y * (1.5 - 0.5 * x * y * y);
+        y = y * (1.5 - 0.5 * x * y * y);
+        cs = y;
+        sn = y * t;
+        s_rot_any = 1;
+      }
+    }
+    pc[buf][a] = cs;
+    pd[buf][a] = -sn;
+    pp[buf][a] = c;
+    pc[buf][c] = cs;
+    pd[buf][c] = sn;
+    pp[buf][c] = a;
+  };
+  if (tid < wpad / 2 && nsteps > 0) {
+    int a, c;
+    tournament_pair_fast(wpad, 0, tid, a, c);
+    const bool in = a < w && c < w;
+    rotation(0, a, c, in ? G[a * GP + a] : 1.0, in ? G[c * GP + c] : 1.0, in ? G[a * GP + c] : 0.0);
+  }
   __syncthreads();
+  double* R = G2;  // the column-rotated G of the current step
+  for (int st = 0; st < nsteps; st++) {
+    const double cy = pc[pb][ey], dy = pd[pb][ey];
+    const int qy = pp[pb][ey];
+    const int src = (lane & 32) + qy;
+    const double gq = __shfl(gv, src, 64), vq = __shfl(vv, src, 64);
+    const double r = cy * gv + dy * gq;
+    vv = cy * vv + dy * vq;
+    R[ex * GP + ey] = r;
+    __syncthreads();
+    const double cx = pc[pb][ex], dx = pd[pb][ex];
+    const int qx = pp[pb][ex];
+    gv = cx * r + dx * R[qx * GP + ey];
+    if (tid < wpad / 2 && st + 1 < nsteps) {
+      int a, c;
+      int sn1 = st + 1;
+      sn1 -= sn1 >= wpad - 1 ? wpad - 1 : 0;
+      tournament_pair_fast(wpad, sn1, tid, a, c);
+      double al = 1.0, be = 1.0, ga = 0.0;
+      if (a < w && c < w) {
+        // G'(x, y) of this step for x, y in {a, c}
+        auto gnew = [&](int x, int y) { return pc[pb][x] * R[x * GP + y] + pd[pb][x] * R[pp[pb][x] * GP + y]; };
+        al = gnew(a, a);
+        be = gnew(c, c);
+        ga = gnew(a, c);
+      }
+      rotation(pb ^ 1, a, c, al, be, ga);
+    }
+    __syncthreads();
+    pb ^= 1;
+  }
+  Vf[ex * GP + ey] = (float)vv;
+  __syncthreads();
+  if (!s_rot_any) return;  // P unchanged: nothing to write
+  // ---- P <- P V in FP32.  Resident panel: thread (column c, row-quad group) keeps V(:, c) in registers
+  // and reads four rows per 128-bit LDS load.  Otherwise one row per thread from global memory (every
+  // value of the row is read before it is overwritten).
+  if (resident) {
+    // thread (column c = tid%32, row quads rq, rq+32, ...): FMAs from the LDS panel (each 128-bit read
+    // serves the 32 columns of a quad), results written in place (a quad's rows are read by this wave
+    // only), then copied to global memory column by column
+    const int c = tid & 31, rq = tid >> 5;
+    float vc[JAC_W];
+#pragma unroll
+    for (int i = 0; i < JAC_W; i++) vc[i] = Vf[i * GP + c];
+    for (int r = 4 * rq; r < n; r += 128) {
+      float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int i = 0; i < JAC_W; i++) {
+        const float4 x = *(const float4*)&chunk[i * JAC_CHUNK + r];
+        o.x = fmaf(x.x, vc[i], o.x);
+        o.y = fmaf(x.y, vc[i], o.y);
+        o.z = fmaf(x.z, vc[i], o.z);
+        o.w = fmaf(x.w, vc[i], o.w);
+      }
+      __builtin_amdgcn_wave_barrier();  // every lane of the wave has read the quad
+      *(float4*)&chunk[c * JAC_CHUNK + r] = o;
+    }
+    __syncthreads();
+    {
+      const int i = tid >> 5, l = tid & 31;
+      if (i < w) {
+        float* dstc = Wb + (long long)gcol(i) * n;
+        const float* srcc = chunk + i * JAC_CHUNK;
+        for (int r = l; r < n; r += 32) dstc[r] = srcc[r];
+      }
+    }
+    if (tid == 0) flags[3 * b + sweep % 3] = 1;
+    return;
+  }
   for (int r = tid; r < n; r += JAC_THREADS) {
     float x[JAC_W];
 #pragma unroll
@@ -889,7 +988,8 @@ size_t final_sort_lds(int nmax) {
 void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks, float* W, int sweep, int round,
                          int* flags, float tol, float ang_tol, int max_inner, hipStream_t s) {
   if (n_tasks <= 0) return;
-  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), 0, s, blocks, tasks, W, sweep, round, flags,
+  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CHUNK * sizeof(float), s, blocks,
+                     tasks, W, sweep, round, flags,
                      tol, ang_tol, max_inner);
   NRGK_HIP(hipGetLastError());
 }
@@ -924,6 +1024,8 @@ void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const do
   NRGK_HIP(hipGetLastError());
 }
 void set_kernel_lds_limits(int bytes) {
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_jacobi_round, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                JAC_W * JAC_CHUNK * sizeof(float)));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_normalize_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_prep, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_rr, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
