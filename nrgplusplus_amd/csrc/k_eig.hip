@@ -175,29 +175,64 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     }
     if (!__syncthreads_or(need)) return;
   }
-  // ---- cyclic Jacobi on G with the tournament ordering over the w columns.  Per step every column
-  // x moves as P'(:,x) = cx P(:,x) + dx P(:,px) (px its partner, cx = cos, dx = -+sin), so
-  // G' = J^T G J and V' = V J.  Thread (x, y) keeps G(x, y) and V(x, y) in registers; a wave holds
-  // two whole rows, so the column step R(x, y) = cy G(x, y) + dy G(x, py) and V'(x, y) = cy V(x, y) +
-  // dy V(x, py) take a shuffle, and only the row step G'(x, y) = cx R(x, y) + dx R(px, y) goes through
-  // LDS.  The first wpad/2 threads form, from R, the three entries of G' that the next step's rotation
-  // of their pair needs and compute that rotation in the same pass.
-  const int wpad = (w + 1) & ~1;
-  const int nsteps = max_inner * (wpad - 1);
+  // ---- cyclic Jacobi on G.  Per step every column x moves as P'(:,x) = cx P(:,x) + dx P(:,px) (px its
+  // partner, cx = cos, dx = -+sin), so G' = J^T G J and V' = V J.  Thread (x, y) keeps G(x, y) and
+  // V(x, y) in registers; a wave holds two whole rows, so the column step R(x, y) = cy G(x, y) +
+  // dy G(x, py) and V'(x, y) = cy V(x, y) + dy V(x, py) take a shuffle, and only the row step
+  // G'(x, y) = cx R(x, y) + dx R(px, y) goes through LDS.  In the same pass the first 32 threads
+  // (one per column) form, from R, the Gram entries of the next step's pair of their column and
+  // compute its rotation.
+  // Step schedule (local columns: block p = 0..wp-1, block q = wp..w-1).  Each pair of columns is
+  // rotated once per outer sweep: the pairs inside p and inside q in round 0 (a circle tournament of
+  // bwp = bw rounded up to even players on each block at once, bwp - 1 steps), the bw x bw cross
+  // pairs in every round (step k pairs p-column i with q-column (i + k) mod bw).
+  const int bwp = bw + (bw & 1);
+  const int n_intra = round == 0 ? bwp - 1 : 0;
+  const int steps_per = n_intra + bw;
+  const int nsteps = max_inner * steps_per;
+  // partner of local column x at schedule step st, or -1 when x idles
+  auto partner = [&](int st, int x) -> int {
+    while (st >= steps_per) st -= steps_per;
+    if (x >= w) return -1;
+    const bool inp = x < wp;
+    const int y = inp ? x : x - wp;
+    const int wb = inp ? wp : wq;
+    if (st < n_intra) {  // circle tournament on bwp players: 0 meets 1 + st, else 1 + (2 st + 1 - y) mod m
+      const int m = bwp - 1;
+      int pq;
+      if (y == 0) {
+        pq = 1 + st;
+      } else if (y == 1 + st) {
+        pq = 0;
+      } else {
+        int t = 2 * st + 1 - y;
+        t += t < 0 ? m : 0;
+        t -= t >= m ? m : 0;
+        t -= t >= m ? m : 0;
+        pq = 1 + t;
+      }
+      return pq < wb ? (inp ? pq : wp + pq) : -1;
+    }
+    const int k = st - n_intra;
+    if (inp) {
+      int j = y + k;
+      j -= j >= bw ? bw : 0;
+      return j < wq ? wp + j : -1;
+    }
+    int i = y - k;
+    i += i < 0 ? bw : 0;
+    return i < wp ? i : -1;
+  };
   const double tol2 = (double)tol * (double)tol;
   const int ex = tid >> 5, ey = tid & 31, lane = tid & 63;
   double gv = G[ex * GP + ey];
   double vv = (ex == ey) ? 1.0 : 0.0;
   int pb = 0;  // parameter buffer of the current step
-  if (tid < JAC_W && tid >= wpad) {  // columns beyond the tournament never move
-    pc[0][tid] = pc[1][tid] = 1.0;
-    pd[0][tid] = pd[1][tid] = 0.0;
-    pp[0][tid] = pp[1][tid] = tid;
-  }
-  // rotation of pair (a, c) from its Gram entries; writes the parameters of both players
-  auto rotation = [&](int buf, int a, int c, double al, double be, double ga) {
+  // Parameters of column x for a step: x and its partner y each compute the rotation of (min, max)
+  // from the same three Gram entries (so they agree exactly) and write their own entry.
+  auto params = [&](int buf, int x, int y, double al, double be, double ga) {
     double cs = 1.0, sn = 0.0;
-    if (a < w && c < w && ga != 0.0 && ga * ga > tol2 * (al * be)) {
+    if (y >= 0 && ga != 0.0 && ga * ga > tol2 * (al * be)) {
       // the angle to FP32 accuracy; cos and sin then normalised in FP64 (cs^2 + sn^2 = 1 to
       // rounding), so the accumulated V stays orthogonal
       const float zeta = (float)(be - al) * __builtin_amdgcn_rcpf((float)(2.0 * ga));
@@ -205,27 +240,24 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
                        __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_amdgcn_sqrtf(1.0f + zeta * zeta));
       // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
       if (fabsf(tf) > ang_tol) {
-        const double t = (double)tf, x = 1.0 + t * t;
-        double y = (double)__builtin_amdgcn_rsqf((float)x);
-        y = y * (1.5 - 0.5 * x * y * y);
-        y = y * (1.5 - 0.5 * x * y * y);
-        cs = y;
-        sn = y * t;
+        const double t = (double)tf, xx = 1.0 + t * t;
+        double z = (double)__builtin_amdgcn_rsqf((float)xx);
+        z = z * (1.5 - 0.5 * xx * z * z);
+        z = z * (1.5 - 0.5 * xx * z * z);
+        cs = z;
+        sn = z * t;
         s_rot_any = 1;
       }
     }
-    pc[buf][a] = cs;
-    pd[buf][a] = -sn;
-    pp[buf][a] = c;
-    pc[buf][c] = cs;
-    pd[buf][c] = sn;
-    pp[buf][c] = a;
+    const bool first = y < 0 || x < y;
+    pc[buf][x] = cs;
+    pd[buf][x] = first ? -sn : sn;
+    pp[buf][x] = y < 0 ? x : y;
   };
-  if (tid < wpad / 2 && nsteps > 0) {
-    int a, c;
-    tournament_pair_fast(wpad, 0, tid, a, c);
-    const bool in = a < w && c < w;
-    rotation(0, a, c, in ? G[a * GP + a] : 1.0, in ? G[c * GP + c] : 1.0, in ? G[a * GP + c] : 0.0);
+  if (tid < JAC_W) {
+    const int y = nsteps > 0 ? partner(0, tid) : -1;
+    const int a = y < 0 ? tid : min(tid, y), c = y < 0 ? tid : max(tid, y);
+    params(0, tid, y, G[a * GP + a], G[c * GP + c], G[a * GP + c]);
   }
   __syncthreads();
   double* R = G2;  // the column-rotated G of the current step
@@ -241,20 +273,18 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     const double cx = pc[pb][ex], dx = pd[pb][ex];
     const int qx = pp[pb][ex];
     gv = cx * r + dx * R[qx * GP + ey];
-    if (tid < wpad / 2 && st + 1 < nsteps) {
-      int a, c;
-      int sn1 = st + 1;
-      sn1 -= sn1 >= wpad - 1 ? wpad - 1 : 0;
-      tournament_pair_fast(wpad, sn1, tid, a, c);
+    if (tid < JAC_W && st + 1 < nsteps) {
+      // G'(x, y) of this step for the next step's pair, from R
+      const int y = partner(st + 1, tid);
       double al = 1.0, be = 1.0, ga = 0.0;
-      if (a < w && c < w) {
-        // G'(x, y) of this step for x, y in {a, c}
-        auto gnew = [&](int x, int y) { return pc[pb][x] * R[x * GP + y] + pd[pb][x] * R[pp[pb][x] * GP + y]; };
+      if (y >= 0) {
+        const int a = min(tid, y), c = max(tid, y);
+        auto gnew = [&](int u, int v) { return pc[pb][u] * R[u * GP + v] + pd[pb][u] * R[pp[pb][u] * GP + v]; };
         al = gnew(a, a);
         be = gnew(c, c);
         ga = gnew(a, c);
       }
-      rotation(pb ^ 1, a, c, al, be, ga);
+      params(pb ^ 1, tid, y, al, be, ga);
     }
     __syncthreads();
     pb ^= 1;
