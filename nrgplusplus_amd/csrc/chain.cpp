@@ -318,8 +318,10 @@ void Chain::run_gemms(std::vector<GemmProb>& probs) {
   double fl = 0, by = 0;
   if (prof_on)
     for (auto& p : live) {
-      fl += 2.0 * p.m * p.n * p.k;
-      by += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (p.C0 ? 2.0 : 1.0) * p.m * p.n);
+      // a symmetric product computes (and counts) the n (n + 1) / 2 entries on and above the diagonal
+      const double outs = p.sym ? 0.5 * (double)p.m * (p.m + 1) : (double)p.m * p.n;
+      fl += 2.0 * outs * p.k;
+      by += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (p.C0 ? 2.0 : 1.0) * outs);
     }
   prof_begin();
   launch_gemm(d_gemm.as<GemmProb>(), (int)live.size(), tiles, st);
@@ -379,9 +381,9 @@ void Chain::eigensolve_mixed() {
 
   const size_t bytes64 = std::max<long long>(1, tot_elems) * sizeof(double);
   dW.ensure(std::max<long long>(1, tot_elems) * sizeof(float));
-  d_sigma_bits.ensure(std::max(1, nb) * sizeof(unsigned));
+  d_sigma_bits.ensure(std::max(1, nb) * 2 * sizeof(unsigned));
   d_sigma.ensure(std::max(1, nb) * sizeof(float));
-  NRGK_HIP(hipMemsetAsync(d_sigma_bits.p, 0, std::max(1, nb) * sizeof(unsigned), st));
+  NRGK_HIP(hipMemsetAsync(d_sigma_bits.p, 0, std::max(1, nb) * 2 * sizeof(unsigned), st));
   launch_shift_convert(dblk, (const int*)(mb + o_rb), (const int*)(mb + o_rl), nrows, dH.as<double>(),
                        d_sigma_bits.as<unsigned>(), d_sigma.as<float>(), dW.as<float>(), st);
   // Jacobi sweeps
@@ -535,8 +537,8 @@ void Chain::eigensolve_mixed() {
       double* E = dE.as<double>() + b.off;
       double* X2 = dX2.as<double>() + b.off;
       g1.push_back(GemmProb{H, X, nullptr, P, n, n, n, n, n, n, n, 0, 0, 1.0, 0.0, 0, 0});
-      g1.push_back(GemmProb{X, X, nullptr, R, n, n, n, n, n, n, n, 1, 0, 1.0, 0.0, 0, 0});
-      g2.push_back(GemmProb{X, P, nullptr, Sm, n, n, n, n, n, n, n, 1, 0, 1.0, 0.0, 0, 0});
+      g1.push_back(GemmProb{X, X, nullptr, R, n, n, n, n, n, n, n, 1, 0, 1.0, 0.0, 0, 0, 1});
+      g2.push_back(GemmProb{X, P, nullptr, Sm, n, n, n, n, n, n, n, 1, 0, 1.0, 0.0, 0, 0, 1});
       g3.push_back(GemmProb{X, E, X, X2, n, n, n, n, n, n, n, 0, 0, 1.0, 1.0, 0, 0});
     }
     run_gemms(g1);

@@ -145,6 +145,7 @@ struct GemmProb {
   double alpha, beta;
   int tile0;  // first tile index of this problem (prefix over problems)
   int tiles_n;
+  int sym = 0;  // symmetric result: only the tiles on and above the diagonal are computed
 };
 
 // Operator rotation (k_rotate.hip) and truncation (k_trunc.hip) descriptors.

@@ -88,8 +88,18 @@ __global__ void __launch_bounds__(256) k_assemble(const AsmSector* __restrict__ 
   }
 }
 
-// FP32 working copy for the Jacobi sweeps: W = fp32(H) + sigma*I with sigma a Gershgorin bound
-// (B = H + sigma I is positive semi-definite).  Pass 1: per-row |.|-sums -> per-block max.
+// FP32 working copy for the Jacobi sweeps: W = fp32(H) + sigma I with sigma from Gershgorin's bounds.
+// Pass 1 per row: the absolute row sum and R_i - H_ii (R_i = off-diagonal absolute sum), reduced per
+// block to rad = max row sum and low = max_i (R_i - H_ii) = -(Gershgorin lower bound).  The second
+// is signed; it is kept in an order-preserving unsigned encoding.
+__device__ __forceinline__ unsigned ordered_bits(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered_bits(unsigned o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
 __global__ void __launch_bounds__(256) k_gershgorin(const EigBlock* __restrict__ blocks, const int* __restrict__ row_block,
                                                     const int* __restrict__ row_local, const double* __restrict__ H,
                                                     unsigned int* __restrict__ sigma_bits) {
@@ -108,8 +118,13 @@ __global__ void __launch_bounds__(256) k_gershgorin(const EigBlock* __restrict__
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    float f = (float)red[0];
-    atomicMax(&sigma_bits[b], __float_as_uint(f));
+    const double d = row[r];
+    atomicMax(&sigma_bits[2 * b], __float_as_uint((float)red[0]));
+    // R_i - H_ii, rounded up so that the float bound is never below the exact one
+    const double low = (red[0] - fabs(d)) - d;
+    float lf = (float)low;
+    if ((double)lf < low) lf = nextafterf(lf, INFINITY);
+    atomicMax(&sigma_bits[2 * b + 1], ordered_bits(lf));
   }
 }
 
@@ -121,9 +136,12 @@ __global__ void __launch_bounds__(256) k_to_fp32_shift(const EigBlock* __restric
   const int b = row_block[grow];
   const int r = row_local[grow];
   const EigBlock blk = blocks[b];
-  // 1.05 * radius + tiny keeps B strictly positive definite
-  const float rad = __uint_as_float(sigma_bits[b]);
-  const float sig = rad > 1e-20f ? 1.05f * rad : 1.0f;
+  // sigma = max(0, -lower bound) + 0.05 rad: W = H + sigma I is positive definite with its smallest
+  // eigenvalue at least 0.05 rad, and as small a shift as the bound allows (a large shift compresses
+  // the relative gaps of the singular values that one-sided Jacobi converges on)
+  const float rad = __uint_as_float(sigma_bits[2 * b]);
+  const float low = from_ordered_bits(sigma_bits[2 * b + 1]);
+  const float sig = rad > 1e-20f ? fmaxf(low, 0.0f) + 0.05f * rad : 1.0f;
   const double* row = H + blk.off + (long long)r * blk.n;
   float* wrow = W + blk.off + (long long)r * blk.n;  // symmetric: row-major == column-major
   for (int c = threadIdx.x; c < blk.n; c += blockDim.x) {

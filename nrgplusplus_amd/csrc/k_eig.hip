@@ -577,10 +577,10 @@ __device__ void rr_wide_cluster(const double* Sb, const double* Rb, double* Eb, 
 }
 
 // The OA step as four launches over the active blocks:
-//   k_oa_sym      32x32 tile pairs (i,j)/(j,i): S and Rp replaced by their symmetric parts (they come
-//                 from separate GEMM tiles and are symmetric only up to rounding; with symmetric
-//                 parts E + E^T = -(Rp - I) holds to rounding, so the step keeps orthogonality even
-//                 where a small gap magnifies GEMM noise); per block the largest off-diagonal
+//   k_oa_sym      32x32 tile pairs (i,j)/(j,i): S and Rp, computed on and above the diagonal only,
+//                 completed by mirroring (exactly symmetric S and Rp keep E + E^T = -(Rp - I) exact,
+//                 so the step keeps orthogonality even where a small gap magnifies GEMM noise); per
+//                 block the largest off-diagonal
 //                 rotation residual |rho_ij| = |S_ij - (lt_i + lt_j)/2 Rp_ij| (the orthogonality
 //                 defect, which needs no gap, is left out) and the largest |Rp_ij - delta_ij|
 //   k_oa_prep     one workgroup per block: lt, delta, ascending sort, cluster ids, and the rounding
@@ -656,8 +656,9 @@ __global__ void __launch_bounds__(256) k_oa_sym(const EigBlock* __restrict__ blo
   for (int q = 0; q < 4; q++) {
     const int rr = ty + 8 * q, i = r0 + rr;
     if (i >= n || j >= n) continue;
-    const double s = 0.5 * (sv[q] + tS[tx][rr]);
-    const double r = 0.5 * (rv[q] + tR[tx][rr]);
+    // S and Rp are computed on and above the diagonal only: take the upper element for both
+    const double s = i <= j ? sv[q] : tS[tx][rr];
+    const double r = i <= j ? rv[q] : tR[tx][rr];
     if (i != j) {
       const double lti = Sb[(long long)i * n + i] / Rb[(long long)i * n + i];
       mx = fmax(mx, fabs(s - 0.5 * (lti + ltj) * r));

@@ -5,6 +5,8 @@
 //
 // Every 64x64 output tile of every problem is one workgroup; tiles of all problems form a single
 // grid (prefix sums in GemmProb::tile0), so one launch covers all uneven blocks of an iteration.
+// A problem marked sym (a product known to be symmetric) gets only its upper tiles; the caller
+// reads the upper triangle.
 // The 64x16 / 16x64 operand slabs are staged in LDS (double-buffered) and each of the 256 threads
 // keeps a 4x4 FP64 accumulator tile in registers.
 #include "common.hpp"
@@ -57,8 +59,22 @@ __global__ void __launch_bounds__(256) k_gemm64(const GemmProb* __restrict__ pro
   const int pi = find_prob(probs, n_probs, blockIdx.x);
   const GemmProb P = probs[pi];
   const int t = blockIdx.x - P.tile0;
-  const int m0 = (t / P.tiles_n) * GT;
-  const int n0 = (t % P.tiles_n) * GT;
+  int tm, tn;
+  if (P.sym) {  // upper tiles only (tm <= tn), row by row
+    const int T = P.tiles_n;
+    int rem = t;
+    tm = 0;
+    while (rem >= T - tm) {
+      rem -= T - tm;
+      tm++;
+    }
+    tn = tm + rem;
+  } else {
+    tm = t / P.tiles_n;
+    tn = t % P.tiles_n;
+  }
+  const int m0 = tm * GT;
+  const int n0 = tn * GT;
   __shared__ double As[2][GK][GT + 1];
   __shared__ double Bs[2][GK][GT + 1];
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
@@ -115,7 +131,8 @@ int prepare_gemm(std::vector<GemmProb>& probs) {
     p.tiles_n = (p.n + GT - 1) / GT;
     const int tm = (p.m + GT - 1) / GT;
     p.tile0 = tiles;
-    tiles += (p.m > 0 && p.n > 0) ? tm * p.tiles_n : 0;
+    NRGK_REQUIRE(!p.sym || p.m == p.n, NRGK_ERR_INVALID, "nrgk: symmetric product must be square");
+    tiles += (p.m > 0 && p.n > 0) ? (p.sym ? tm * (tm + 1) / 2 : tm * p.tiles_n) : 0;
   }
   return tiles;
 }
