@@ -88,7 +88,7 @@ Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, 
   for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
                     &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
                     &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act,
-                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax})
+                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv})
     b->ctx = ctx;
   NRGK_HIP(hipSetDevice(ctx->device));
   NRGK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -308,9 +308,18 @@ void Chain::run_gemms(std::vector<GemmProb>& probs) {
     plain_gemms(*this, probs);
     return;
   }
-  std::vector<GemmProb> live;
+  // products of blocks of 1024 and wider are plain library GEMMs (rocBLAS dgemm, one call each, on
+  // the chain's stream); the many smaller ones share one launch of the batched kernel
+  std::vector<GemmProb> live, large;
   for (auto& p : probs)
-    if (p.m > 0 && p.n > 0) live.push_back(p);
+    if (p.m > 0 && p.n > 0) (p.m >= 1024 && p.n >= 1024 && p.k >= 1024 ? large : live).push_back(p);
+  if (!large.empty()) {
+    double fl = 0;
+    for (auto& p : large) fl += 2.0 * p.m * p.n * p.k;
+    prof_begin();
+    plain_gemms(*this, large);  // synchronises the stream
+    prof_end("gemm_rocblas", (long long)large.size(), fl, 0.0);
+  }
   if (live.empty()) return;
   const int tiles = prepare_gemm(live);
   d_gemm.ensure(live.size() * sizeof(GemmProb));
@@ -345,7 +354,7 @@ void Chain::eigensolve_mixed() {
   std::vector<EigBlock> blks;
   std::vector<int> row_block, row_local;
   std::vector<int2> tasks;
-  int nbc_max = 2;
+  int nbc_max = 2, n_bc = 0;
   for (int s = 0; s < S; s++) {
     const int n = cur_n[s];
     if (n == 0) continue;
@@ -358,6 +367,8 @@ void Chain::eigensolve_mixed() {
     if (nbc & 1) nbc++;
     b.nbc = nbc;
     b.e_off = cur_eoff[s];
+    b.bc_off = n_bc;
+    n_bc += nbc;
     const int bi = (int)blks.size();
     for (int j = 0; j < nbc / 2; j++) tasks.push_back(make_int2(bi, j));
     for (int r = 0; r < n; r++) {
@@ -392,6 +403,10 @@ void Chain::eigensolve_mixed() {
   d_flags.ensure(flags0.size() * sizeof(int));
   h2d(d_flags.as<int>(), flags0.data(), flags0.size(), st);
   const int max_sweeps = 20;
+  // Gram blocks of the full-width block columns, carried between rounds (invalid until first formed)
+  d_dgram.ensure(std::max(1, n_bc) * 256 * sizeof(double));
+  d_dgv.ensure(std::max(1, n_bc) * sizeof(int));
+  NRGK_HIP(hipMemsetAsync(d_dgv.p, 0, std::max(1, n_bc) * sizeof(int), st));
   // rotate a column pair while cos > jac_tol and the rotation angle tan > jac_ang: the angle test
   // stops pairs of very different norms at the FP32 floor, the cosine test degenerate pairs
   const float jac_tol = knob("NRGK_JAC_TOL", 5e-7);
@@ -410,11 +425,16 @@ void Chain::eigensolve_mixed() {
           fl += rounds * pairs * 4.0 * n * w * w;
           by += rounds * pairs * 3.0 * n * w * 4.0;  // read twice, write once (FP32)
         }
+    // rounds this sweep: as many as the widest block that rotated in the previous sweep needs (the
+    // tasks of converged blocks return at once; round 0 always runs, it clears their next flag)
+    int nbc_act = 2;
+    for (int b = 0; b < nb; b++)
+      if (sw == 0 || hflags[3 * b + (sw - 1) % 3]) nbc_act = std::max(nbc_act, blks[b].nbc);
     prof_begin();
-    for (int r = 0; r < nbc_max - 1; r++)
+    for (int r = 0; r < nbc_act - 1; r++)
       launch_jacobi_round(dblk, (const int2*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
-                          jac_tol, jac_ang, jac_inner, st);
-    prof_end("jacobi_round", nbc_max - 1, fl, by);
+                          d_dgram.as<double>(), d_dgv.as<int>(), jac_tol, jac_ang, jac_inner, st);
+    prof_end("jacobi_round", nbc_act - 1, fl, by);
     d2h(hflags.data(), d_flags.as<int>(), hflags.size(), st);
     NRGK_HIP(hipStreamSynchronize(st));
     sweeps = sw + 1;

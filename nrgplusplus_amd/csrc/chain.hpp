@@ -73,7 +73,7 @@ struct Chain {
   DevBuf dlam_tmp, dlam_raw, dlam_shift[2];
   int lcur = 0;
   DevBuf d_asm, d_eig, d_misc, d_rot, d_gemm;
-  DevBuf d_flags, d_sigma_bits, d_sigma, d_stats, d_kept, d_trunc, d_conv, d_eig_act, d_cid, d_sidx, d_offmax;
+  DevBuf d_flags, d_sigma_bits, d_sigma, d_stats, d_kept, d_trunc, d_conv, d_eig_act, d_cid, d_sidx, d_offmax, d_dgram, d_dgv;
   int oa_steps = 0, oa_unconverged = 0;
 
   // host mirrors

@@ -131,6 +131,7 @@ struct EigBlock {
   int nbc;         // number of block columns in the Jacobi tournament (even)
   int bw;          // block-column width
   int e_off;       // offset of the block's eigenvalues (prefix of n)
+  int bc_off;      // offset of the block's block columns (prefix of nbc)
 };
 
 // Batched GEMM problem: C = alpha * op(A) * op(B) + beta * C0, all row-major.

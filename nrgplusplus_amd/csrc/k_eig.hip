@@ -66,6 +66,7 @@ constexpr int GP = JAC_W + 1;   // padded row of G and V
 __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __restrict__ blocks,
                                                               const int2* __restrict__ tasks, float* __restrict__ W,
                                                               int sweep, int round, int* __restrict__ flags,
+                                                              double* __restrict__ dgram, int* __restrict__ dgv,
                                                               float tol, float ang_tol, int max_inner) {
   extern __shared__ float4 chunk4[];  // [JAC_W * JAC_CHUNK] floats, column-major, 16-byte aligned
   float* chunk = reinterpret_cast<float*>(chunk4);
@@ -105,6 +106,17 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
 #pragma unroll
     for (int y = 0; y < 4; y++) acc[x][y] = 0.0;
   const bool resident = n <= JAC_CHUNK;
+  // Gram blocks of the two block columns (16 x 16, FP64), kept from their last round: when both are
+  // valid, only the cross block is formed from P.  They are the Gram blocks of the exact rotated
+  // columns, which the FP32 panel holds to rounding.
+  double* dgp = dgram + ((long long)blk.bc_off + p) * 256;
+  double* dgq = dgram + ((long long)blk.bc_off + q) * 256;
+  // a full-width block column keeps its Gram block current in every round that touches it, whatever
+  // the width of its partner, so a stored block is never stale
+  const bool keepP = wp == JAC_BW, keepQ = wq == JAC_BW;
+  const bool fast = resident && keepP && keepQ && dgv[blk.bc_off + p] && dgv[blk.bc_off + q];
+  const int ft = tid >> 6, fg = tid & 63;
+  const int fa = (ft >> 2) * 4, fc = JAC_BW + (ft & 3) * 4;
   for (int r0 = 0; r0 < n; r0 += JAC_CHUNK) {
     const int rows = min(JAC_CHUNK, n - r0);
     const int rows4 = (rows + 3) & ~3;
@@ -127,6 +139,29 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
       }
     }
     __syncthreads();
+    if (fast) {
+      // cross block G(p, q) only: thread = (4x4 tile of the 16x16 cross block, phase of 64 lanes)
+      for (int r = 4 * fg; r < rows4; r += 256) {
+        float4 xa[4], xc[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+          xa[x] = *(const float4*)&chunk[(fa + x) * JAC_CHUNK + r];
+          xc[x] = *(const float4*)&chunk[(fc + x) * JAC_CHUNK + r];
+        }
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+          for (int y = 0; y < 4; y++) {
+            double t = acc[x][y];
+            t += (double)xa[x].x * (double)xc[y].x;
+            t += (double)xa[x].y * (double)xc[y].y;
+            t += (double)xa[x].z * (double)xc[y].z;
+            t += (double)xa[x].w * (double)xc[y].w;
+            acc[x][y] = t;
+          }
+      }
+      break;  // resident: the one chunk is the whole panel
+    }
     for (int r = 4 * g; r < rows4; r += 64) {
       float4 xa[4], xc[4];
 #pragma unroll
@@ -148,23 +183,55 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     }
     if (!resident) __syncthreads();
   }
-#pragma unroll
-  for (int x = 0; x < 4; x++)
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-      double v = acc[x][y];
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-      acc[x][y] = v;
-    }
-  if (g == 0) {
+  if (fast) {
 #pragma unroll
     for (int x = 0; x < 4; x++)
 #pragma unroll
-      for (int y = 0; y < 4; y++) G[(ta + x) * GP + tc + y] = acc[x][y];
+      for (int y = 0; y < 4; y++) acc[x][y] = wave_sum(acc[x][y]);
+    if (fg == 0) {
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+          G[(fa + x) * GP + fc + y] = acc[x][y];
+          G[(fc + y) * GP + fa + x] = acc[x][y];
+        }
+    }
+    for (int e = tid; e < 256; e += JAC_THREADS) {
+      const int a = e >> 4, c = e & 15;
+      G[a * GP + c] = dgp[e];
+      G[(JAC_BW + a) * GP + JAC_BW + c] = dgq[e];
+    }
+  } else {
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        double v = acc[x][y];
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+        acc[x][y] = v;
+      }
+    if (g == 0) {
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) G[(ta + x) * GP + tc + y] = acc[x][y];
+    }
   }
   if (tid == 0) s_rot_any = 0;
   __syncthreads();
+  if (!fast) {  // keep the freshly formed diagonal blocks
+    for (int e = tid; e < 256; e += JAC_THREADS) {
+      const int a = e >> 4, c = e & 15;
+      if (keepP) dgp[e] = G[a * GP + c];
+      if (keepQ) dgq[e] = G[(wp + a) * GP + wp + c];
+    }
+    if (tid == 0) {
+      if (keepP) dgv[blk.bc_off + p] = 1;
+      if (keepQ) dgv[blk.bc_off + q] = 1;
+    }
+  }
   {
     // a task whose columns are already orthogonal to the tolerance has nothing to rotate
     const int x = tid >> 5, y = tid & 31;
@@ -290,6 +357,10 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     pb ^= 1;
   }
   Vf[ex * GP + ey] = (float)vv;
+  if (s_rot_any) {  // the diagonal blocks of G' = V^T G V for the next round
+    if (keepP && ex < wp && ey < wp) dgp[ex * JAC_BW + ey] = gv;
+    if (keepQ && ex >= wp && ey >= wp && ex < w && ey < w) dgq[(ex - wp) * JAC_BW + (ey - wp)] = gv;
+  }
   __syncthreads();
   if (!s_rot_any) return;  // P unchanged: nothing to write
   // ---- P <- P V in FP32.  Resident panel: thread (column c, row-quad group) keeps V(:, c) in registers
@@ -1017,10 +1088,10 @@ size_t final_sort_lds(int nmax) {
 }
 
 void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks, float* W, int sweep, int round,
-                         int* flags, float tol, float ang_tol, int max_inner, hipStream_t s) {
+                         int* flags, double* dgram, int* dgv, float tol, float ang_tol, int max_inner, hipStream_t s) {
   if (n_tasks <= 0) return;
   hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CHUNK * sizeof(float), s, blocks,
-                     tasks, W, sweep, round, flags,
+                     tasks, W, sweep, round, flags, dgram, dgv,
                      tol, ang_tol, max_inner);
   NRGK_HIP(hipGetLastError());
 }
