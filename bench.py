@@ -30,6 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "NRG iterations/s at <=1e-10 eig err, batched-mixed vs plain GPU; p50/p99 solve ms"
 CFG = dict(eps=-0.15, U=0.3, Gamma=0.01, Lambda=2.0, K=2048, nsites=60, nbins=2000)
 
 
@@ -106,6 +107,24 @@ PEAK_F64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0    # MI355X HBM3E
 
 
+FAMILY_KERNEL = {"jacobi_round": "k_jacobi_round", "gemm_f64": "k_gemm64"}
+
+
+def measured_traffic(family):
+    """HBM bytes per dispatch of the family's kernel from the committed counter summary
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_siam_fdm_K2048.md")
+    name = FAMILY_KERNEL.get(family)
+    if not name or not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\|\s*%s\s*\|.*\|\s*([0-9.eE+-]+)\s*\|\s*$" % name, line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
 def roofline(kstats):
     """The dominant kernel family of the timed region against its roofline.  The eigensolver
     families are compute kernels (FP64 Gram and FP32 update in the Jacobi rounds, FP64 GEMMs) and are
@@ -121,7 +140,8 @@ def roofline(kstats):
                 "per_launch": {"bytes": k["bytes"] / k["launches"], "us": 1e3 * k["ms"] / k["launches"]}}
     ach = k["flops"] / sec / 1e12
     return {"kernel": fam, "bound": "mfma", "achieved": ach, "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / PEAK_F64_TFLOPS, "traffic": None,
+            "frac": ach / PEAK_F64_TFLOPS, "traffic": measured_traffic(fam),
+            "algorithmic_bytes_per_launch": k["bytes"] / k["launches"],
             "per_launch": {"flops": k["flops"] / k["launches"], "us": 1e3 * k["ms"] / k["launches"]}}
 
 
@@ -133,6 +153,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=0, help="0: the timed iterations")
+    ap.add_argument("--stop-after-timed", action="store_true",
+                    help="end after the timed iterations (for counter runs): no eigenvalue check, FDM pass, "
+                         "plain or CPU sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding of the NRG step is a design "
@@ -161,6 +184,10 @@ def main():
     s.g.profile(False)
     ms_per_step = (t1 - t0) * 1e3 / K
     value = K / (t1 - t0)
+    if args.stop_after_timed:
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "iterations/s", "steps": K, "warmup": W,
+                          "ms_per_step": ms_per_step, "kernels": kstats, "partial": "stop-after-timed"}))
+        return
     # finish the chain untimed, checking one iteration's eigenvalues against LAPACK
     eig_err = None
     while s.i < total:
@@ -191,7 +218,7 @@ def main():
         cpu = cpu_baseline(W - 1 if W > 0 else 0)
 
     line = {
-        "metric": "NRG iterations/s at <=1e-10 eig err, batched-mixed vs plain GPU; p50/p99 solve ms",
+        "metric": METRIC,
         "value": value,
         "unit": "iterations/s",
         "n_gpus": 1,
