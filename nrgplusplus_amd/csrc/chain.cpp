@@ -201,17 +201,18 @@ void Chain::import_impurity() {
 void Chain::create_next_basis() {
   cur_qn.clear();
   coupled.clear();
+  // sectors in first-seen order over (i, j); the index map replaces the reference's linear search
+  std::map<std::vector<int>, int> index;
+  std::vector<int> tm;
   for (size_t i = 0; i < pre_qn.size(); i++)
     for (int j = 0; j < bath.S; j++) {
-      std::vector<int> tm = pre_qn[i];
+      tm = pre_qn[i];
       for (int x = 0; x < bath.nqn; x++) tm[x] += bath.qn[j][x];
-      bool ex = false;
-      for (size_t s = 0; s < cur_qn.size(); s++)
-        if (tm == cur_qn[s]) {
-          ex = true;
-          coupled[s].push_back({(int)i, j});
-        }
-      if (!ex) {
+      auto it = index.find(tm);
+      if (it != index.end()) {
+        coupled[it->second].push_back({(int)i, j});
+      } else {
+        index.emplace(tm, (int)cur_qn.size());
         cur_qn.push_back(tm);
         coupled.push_back({{(int)i, j}});
       }

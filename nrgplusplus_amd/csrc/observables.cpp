@@ -118,11 +118,23 @@ Fdm::Fdm(Chain* c, Obs* o, int nb, double emin, double emax) : ch(c), obs(o), nb
     b->ctx = ch->ctx;
 }
 
+void* Fdm::arena_alloc(size_t bytes) {
+  bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+  if (chunks.empty() || chunk_used + bytes > chunks.back()->cap) {
+    auto b = std::make_unique<DevBuf>(ch->ctx);
+    b->ensure(std::max<size_t>(bytes, size_t(512) << 20));
+    chunks.push_back(std::move(b));
+    chunk_used = 0;
+  }
+  void* p = static_cast<char*>(chunks.back()->p) + chunk_used;
+  chunk_used += bytes;
+  return p;
+}
+
 void Fdm::record() {
   Chain& c = *ch;
   NRGK_REQUIRE(c.phase == Chain::ASSEMBLED, NRGK_ERR_STATE, "nrgk: record after add_bath_site, before update_internal_state");
   auto sh = std::make_unique<Shell>();
-  for (DevBuf* b : {&sh->dU, &sh->dE, &sh->dO}) b->ctx = c.ctx;
   sh->cnt = c.cnt;
   sh->n = c.cur_n;
   sh->eoff = c.cur_eoff;
@@ -133,14 +145,14 @@ void Fdm::record() {
   sh->E.resize(c.N_tot);
   d2h(sh->E.data(), c.dlam_raw.as<double>(), c.N_tot, c.st);
   if (obs) {
-    sh->dU.ensure(std::max<long long>(1, c.tot_elems) * sizeof(double));
+    sh->dU.p = arena_alloc(std::max<long long>(1, c.tot_elems) * sizeof(double));
     NRGK_HIP(hipMemcpyAsync(sh->dU.p, c.dU.p, c.tot_elems * sizeof(double), hipMemcpyDeviceToDevice, c.st));
-    sh->dE.ensure(std::max<long long>(1, c.N_tot) * sizeof(double));
+    sh->dE.p = arena_alloc(std::max<long long>(1, c.N_tot) * sizeof(double));
     NRGK_HIP(hipMemcpyAsync(sh->dE.p, c.dlam_raw.p, c.N_tot * sizeof(double), hipMemcpyDeviceToDevice, c.st));
     long long tot = 0;
     for (auto& m : obs->blocks)
       for (auto& [k, r] : m) tot = std::max(tot, r.off + (long long)r.rows * r.cols);
-    sh->dO.ensure(std::max<long long>(1, tot) * sizeof(double));
+    sh->dO.p = arena_alloc(std::max<long long>(1, tot) * sizeof(double));
     NRGK_HIP(hipMemcpyAsync(sh->dO.p, obs->buf[obs->cur].p, tot * sizeof(double), hipMemcpyDeviceToDevice, c.st));
     sh->ops = obs->blocks;
   }

@@ -27,7 +27,14 @@ struct Shell {
   std::vector<std::vector<int>> sub_kidx;
   std::vector<int> pre_kept;  // eigenvaluesQ_kept_indices sizes at save time (previous iteration)
   std::vector<double> E;      // eigenvalues (pre-shift), concatenated
-  DevBuf dU, dE, dO;
+  // U blocks, eigenvalues and observable blocks of the shell: sub-allocations of the recorder's arena
+  struct Ref {
+    void* p = nullptr;
+    template <class T>
+    T* as() const {
+      return static_cast<T*>(p);
+    }
+  } dU, dE, dO;
   std::vector<OpMap> ops;
 };
 
@@ -37,6 +44,10 @@ struct Fdm {
   int nbins = 0;
   double e_min = 1e-10, e_max = 10, delE = 0;
   std::vector<std::unique_ptr<Shell>> shells;
+  // shells live in large chunks, filled in order: a record allocates nothing once a chunk has room
+  std::vector<std::unique_ptr<DevBuf>> chunks;
+  size_t chunk_used = 0;
+  void* arena_alloc(size_t bytes);
   DevBuf dpos, dneg, dsum, dbf, dred[2], dRA, dAR, dT1, drho, ditems, dpt, dE, dth;
   Fdm(Chain* c, Obs* o, int nb, double emin, double emax);
   void record();
