@@ -593,14 +593,11 @@ void Chain::eigensolve_mixed() {
       if (dbg_oa > 0 && act[a].n >= dbg_oa)
         std::fprintf(stderr, "nrgk oa step %d block off=%lld n=%d residual/noise=%.3g orth=%.3g\n", it, act[a].off,
                      act[a].n, conv[3 * a] / conv[3 * a + 2], conv[3 * a + 1]);
-      if (conv[3 * a] > e_tol * conv[3 * a + 2] || conv[3 * a + 1] > d_tol) {
-        next.push_back(act[a]);
-      } else {
-        const EigBlock& b = act[a];
-        NRGK_HIP(hipMemcpyAsync(dX2.as<double>() + b.off, dX.as<double>() + b.off, (size_t)b.n * b.n * sizeof(double),
-                                hipMemcpyDeviceToDevice, st));
-      }
+      if (conv[3 * a] > e_tol * conv[3 * a + 2] || conv[3 * a + 1] > d_tol) next.push_back(act[a]);
     }
+    // the converged blocks' vectors into both iterate buffers (the device applies the same test)
+    if (next.size() < act.size())
+      launch_copy_converged(m_blk, na, d_conv.as<double>(), e_tol, d_tol, dX.as<double>(), dX2.as<double>(), st);
     act.swap(next);
   }
   oa_unconverged = (int)act.size();

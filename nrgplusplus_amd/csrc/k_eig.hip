@@ -1024,6 +1024,25 @@ __global__ void __launch_bounds__(1024) k_oa_rr(const EigBlock* __restrict__ blo
   }
 }
 
+// Blocks that took their last refinement step leave the batch with their vectors in X; this copies
+// them to the other iterate buffer as well (one launch for all of them: grid = blocks x 8).
+__global__ void __launch_bounds__(256) k_copy_converged(const EigBlock* __restrict__ blocks,
+                                                        const double* __restrict__ conv, double e_tol, double d_tol,
+                                                        const double* __restrict__ src, double* __restrict__ dst) {
+  const int a = blockIdx.x >> 3, part = blockIdx.x & 7;
+  if (!oa_converged(conv, a, e_tol, d_tol)) return;
+  const EigBlock blk = blocks[a];
+  const long long nn = (long long)blk.n * blk.n;
+  for (long long e = part * 256LL + threadIdx.x; e < nn; e += 8 * 256LL) dst[blk.off + e] = src[blk.off + e];
+}
+
+void launch_copy_converged(const EigBlock* blocks, int nb, const double* conv, double e_tol, double d_tol,
+                           const double* src, double* dst, hipStream_t s) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_copy_converged, dim3(nb * 8), dim3(256), 0, s, blocks, conv, e_tol, d_tol, src, dst);
+  NRGK_HIP(hipGetLastError());
+}
+
 // Final ordering: U(:, c) = X(:, perm[c]) with eigenvalues ascending (dsyevd order).
 __global__ void __launch_bounds__(1024) k_final_sort(const EigBlock* __restrict__ blocks, const double* __restrict__ X,
                                                      const double* __restrict__ lam_in, double* __restrict__ U,

@@ -19,6 +19,8 @@ void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym,
                double* S, double* Rp, double* E, double* lam, int* cid, int* sidx, double* offmax, int* stats,
                double* conv, double* scrW, double* scrG, double* scrT, double* rrv, int nmax, double e_tol,
                double d_tol, hipStream_t s);
+void launch_copy_converged(const EigBlock* blocks, int nb, const double* conv, double e_tol, double d_tol,
+                           const double* src, double* dst, hipStream_t s);
 void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
                        double* lam_out, size_t lds, hipStream_t s);
 size_t normalize_sort_lds(int nmax);
