@@ -16,6 +16,8 @@
 namespace nrgk {
 
 constexpr int JAC_THREADS = 1024;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -53,26 +55,32 @@ __device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p,
 // One tournament round for all blocks.  flags[3*b + s%3] is set when block b rotated during sweep s;
 // a block whose previous sweep did not rotate has converged and its tasks return at once.
 //
-// A task owns the pair (p, q) of block columns, w <= 2 JAC_BW columns of W (column-major, length n).
-// Instead of rotating the columns pair by pair, it forms their Gram matrix G = P^T P in FP64 (P read
-// once, staged through LDS in row chunks), runs the cyclic Jacobi rotations on G (two-sided, the
-// same rotations a one-sided sweep of P would make) while accumulating V, and finally writes
-// P <- P V in one pass.  The rounding of P to FP32 then happens once per round, not per rotation.
+// A task owns the pair (p, q) of block columns of W (column-major, length n).  Its local columns
+// are p's at 0..wp-1 and q's at bw..bw+wq-1 (bw <= 16; the columns between exist only beside the
+// last, partial block column and are zero).  Instead of rotating the columns pair by pair, it forms
+// their Gram matrix G = P^T P in FP64 on the matrix cores (P read once, staged through LDS in row
+// chunks), runs the cyclic Jacobi rotations on G (two-sided, the same rotations a one-sided sweep of
+// P would make) while accumulating V, and finally writes P <- P V in one pass, also on the matrix
+// cores.  The rounding of P to FP32 then happens once per round, not per rotation.
 constexpr int JAC_BW = 16;
 constexpr int JAC_W = 2 * JAC_BW;
-constexpr int JAC_CHUNK = 768;  // rows per LDS chunk of the Gram pass (96 KB)
-constexpr int GP = JAC_W + 1;   // padded row of G and V
+constexpr int JAC_CHUNK = 768;         // rows per LDS chunk of the Gram pass
+constexpr int JAC_CS = JAC_CHUNK + 4;  // column stride of the chunk: 16 columns x 4 rows fall in distinct banks
+constexpr int GP = JAC_W + 1;          // padded row of G and V
+
+// sum of the four partial 16 x 16 blocks at P, P + 256, P + 512, P + 768
+__device__ __forceinline__ double sum4(const double* P, int t) { return (P[t] + P[256 + t]) + (P[512 + t] + P[768 + t]); }
 
 __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __restrict__ blocks,
                                                               const int2* __restrict__ tasks, float* __restrict__ W,
                                                               int sweep, int round, int* __restrict__ flags,
                                                               double* __restrict__ dgram, int* __restrict__ dgv,
                                                               float tol, float ang_tol, int max_inner) {
-  extern __shared__ float4 chunk4[];  // [JAC_W * JAC_CHUNK] floats, column-major, 16-byte aligned
+  extern __shared__ float4 chunk4[];  // [JAC_W * JAC_CS] floats, column-major, 16-byte aligned
   float* chunk = reinterpret_cast<float*>(chunk4);
   __shared__ double G[JAC_W * GP], G2[JAC_W * GP];
-  __shared__ double pc[2][JAC_W], pd[2][JAC_W];
-  __shared__ int pp[2][JAC_W];
+  __shared__ double Vd[JAC_W * GP], Gpart[JAC_W * GP];
+  __shared__ int s_slots[32 * 16];
   __shared__ float Vf[JAC_W * GP];
   __shared__ int s_rot_any;
   const int2 task = tasks[blockIdx.x];
@@ -90,142 +98,113 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
   const int c0p = p * bw, c0q = q * bw;
   const int wp = c0p < n ? min(bw, n - c0p) : 0;
   const int wq = c0q < n ? min(bw, n - c0q) : 0;
-  const int w = wp + wq;
-  if (w == 0) return;
+  if (wp + wq == 0) return;
+  // beside the empty block column that pads the tournament to an even count there is nothing to
+  // pair after round 0 (which rotates the pairs inside each block column)
+  if (round > 0 && (wp == 0 || wq == 0)) return;
+  const int wl = bw + wq;  // local columns in use: [0, wp) and [bw, wl)
   float* Wb = W + blk.off;
   const int tid = threadIdx.x;
-  auto gcol = [&](int i) { return i < wp ? c0p + i : c0q + (i - wp); };
-  // ---- G = P^T P: thread = (4x4 tile of G, row phase g); the 16 phases of a tile are 16 lanes.  P is
-  // staged in LDS column-major, JAC_CHUNK rows at a time (all of it when n <= JAC_CHUNK, which then
-  // also serves the update); a phase takes 4 consecutive rows per 128-bit LDS read.
-  const int tile = tid >> 4, g = tid & 15;
-  const int ta = (tile >> 3) * 4, tc = (tile & 7) * 4;
-  double acc[4][4];
-#pragma unroll
-  for (int x = 0; x < 4; x++)
-#pragma unroll
-    for (int y = 0; y < 4; y++) acc[x][y] = 0.0;
+  auto valid = [&](int i) { return i < wp || (i >= bw && i < wl); };
+  auto gcol = [&](int i) { return i < bw ? c0p + i : c0q + (i - bw); };
   const bool resident = n <= JAC_CHUNK;
-  // Gram blocks of the two block columns (16 x 16, FP64), kept from their last round: when both are
-  // valid, only the cross block is formed from P.  They are the Gram blocks of the exact rotated
-  // columns, which the FP32 panel holds to rounding.
+  // Gram blocks of the two block columns (16 x 16 slots, FP64), kept from their last round: when both
+  // are valid, only the cross block is formed from P.  They are the Gram blocks of the exact rotated
+  // columns, which the FP32 panel holds to rounding.  A block column keeps its block current in every
+  // round that touches it, so a stored block is never stale.
   double* dgp = dgram + ((long long)blk.bc_off + p) * 256;
   double* dgq = dgram + ((long long)blk.bc_off + q) * 256;
-  // a full-width block column keeps its Gram block current in every round that touches it, whatever
-  // the width of its partner, so a stored block is never stale
-  const bool keepP = wp == JAC_BW, keepQ = wq == JAC_BW;
+  const bool keepP = wp > 0, keepQ = wq > 0;
   const bool fast = resident && keepP && keepQ && dgv[blk.bc_off + p] && dgv[blk.bc_off + q];
-  const int ft = tid >> 6, fg = tid & 63;
-  const int fa = (ft >> 2) * 4, fc = JAC_BW + (ft & 3) * 4;
+  // the carried blocks are fetched now, so that their latency overlaps the panel load (entries
+  // outside a partial block column's width are zero)
+  double pre_p = 0.0, pre_q = 0.0;
+  if (fast && tid < 256) {
+    const int a = tid >> 4, c = tid & 15;
+    if (a < wp && c < wp) pre_p = dgp[tid];
+    if (a < wq && c < wq) pre_q = dgq[tid];
+  }
+  // ---- Gram matrix on the FP64 matrix cores (16 x 16 x 4 steps).  Job of wave v: the p x q block
+  // (fast: waves 0-3), or the p x p, p x q and q x q blocks (waves 0-3, 4-7, 8-11); the four waves of
+  // a job take every fourth group of 4 rows, two accumulators in flight.  Lane l supplies column
+  // l%16 of its left block as A and of its right block as B, at row 4 kb + l/16; entry (i, j) of the
+  // product lands in lane 16 (i%4) + j, register i/4.  The partial blocks meet in G2, Vd and Gpart,
+  // all free until the Jacobi steps.
+  const int wv = tid >> 6, lane = tid & 63, lx = lane & 15;
+  const int job = fast ? (wv < 4 ? 1 : -1) : (wv < 12 ? wv >> 2 : -1), kq = wv & 3;
+  const float* ca = chunk + (job == 2 ? bw + lx : lx) * JAC_CS + (lane >> 4);
+  const float* cb = chunk + (job == 0 ? lx : bw + lx) * JAC_CS + (lane >> 4);
+  // a 16-wide tile of p reaches into q when bw < 16: those lanes supply zeros
+  const bool am = job == 2 || lx < bw, bm = job != 0 || lx < bw;
+  dbl4 m0 = {0.0, 0.0, 0.0, 0.0}, m1 = {0.0, 0.0, 0.0, 0.0};
   for (int r0 = 0; r0 < n; r0 += JAC_CHUNK) {
     const int rows = min(JAC_CHUNK, n - r0);
     const int rows4 = (rows + 3) & ~3;
-    // thread = (column tid/32, rows tid%32 + 32k): one column pointer per thread, 128-byte runs,
-    // eight loads in flight before their LDS stores; rows are filled up to a multiple of 4
+    // thread = (column tid/32, rows tid%32 + 32k): one column pointer per thread, 128-byte runs, all
+    // of a chunk's loads in flight before their LDS stores; rows are filled up to a multiple of 4
     {
       const int i = tid >> 5, l = tid & 31;
-      const float* src = Wb + (long long)gcol(i < w ? i : 0) * n + r0;
-      float* dstc = chunk + i * JAC_CHUNK;
-      for (int rb = l; rb < rows4; rb += 256) {
-        float t[8];
+      const bool ok = valid(i);
+      const float* src = Wb + (long long)gcol(ok ? i : 0) * n + r0 + l;
+      float* dst = chunk + i * JAC_CS + l;
+      const int lim = ok ? rows - l : 0, lim4 = rows4 - l;
+      float t[JAC_CHUNK / 32];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const int r = rb + 32 * k;
-          t[k] = (i < w && r < rows) ? src[r] : 0.0f;
-        }
+      for (int k = 0; k < JAC_CHUNK / 32; k++) t[k] = 32 * k < lim ? src[32 * k] : 0.0f;
 #pragma unroll
-        for (int k = 0; k < 8; k++)
-          if (rb + 32 * k < rows4) dstc[rb + 32 * k] = t[k];
+      for (int k = 0; k < JAC_CHUNK / 32; k++)
+        if (32 * k < lim4) dst[32 * k] = t[k];
+    }
+    if (fast && tid < 256) {  // (resident: this is the only chunk)
+      const int a = tid >> 4, c = tid & 15;
+      if (a < bw && c < bw) {
+        G[a * GP + c] = pre_p;
+        G[(bw + a) * GP + bw + c] = pre_q;
       }
     }
     __syncthreads();
-    if (fast) {
-      // cross block G(p, q) only: thread = (4x4 tile of the 16x16 cross block, phase of 64 lanes)
-      for (int r = 4 * fg; r < rows4; r += 256) {
-        float4 xa[4], xc[4];
-#pragma unroll
-        for (int x = 0; x < 4; x++) {
-          xa[x] = *(const float4*)&chunk[(fa + x) * JAC_CHUNK + r];
-          xc[x] = *(const float4*)&chunk[(fc + x) * JAC_CHUNK + r];
-        }
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-          for (int y = 0; y < 4; y++) {
-            double t = acc[x][y];
-            t += (double)xa[x].x * (double)xc[y].x;
-            t += (double)xa[x].y * (double)xc[y].y;
-            t += (double)xa[x].z * (double)xc[y].z;
-            t += (double)xa[x].w * (double)xc[y].w;
-            acc[x][y] = t;
-          }
+    if (job >= 0) {
+      const int nkb = rows4 >> 2;
+      int kb = kq;
+      for (; kb + 4 < nkb; kb += 8) {
+        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
+                                                  0, 0, 0);
+        m1 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb + 16] : 0.0,
+                                                  bm ? (double)cb[4 * kb + 16] : 0.0, m1, 0, 0, 0);
       }
-      break;  // resident: the one chunk is the whole panel
+      if (kb < nkb)
+        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
+                                                  0, 0, 0);
     }
-    for (int r = 4 * g; r < rows4; r += 64) {
-      float4 xa[4], xc[4];
-#pragma unroll
-      for (int x = 0; x < 4; x++) {
-        xa[x] = *(const float4*)&chunk[(ta + x) * JAC_CHUNK + r];
-        xc[x] = *(const float4*)&chunk[(tc + x) * JAC_CHUNK + r];
-      }
-#pragma unroll
-      for (int x = 0; x < 4; x++)
-#pragma unroll
-        for (int y = 0; y < 4; y++) {
-          double t = acc[x][y];
-          t += (double)xa[x].x * (double)xc[y].x;
-          t += (double)xa[x].y * (double)xc[y].y;
-          t += (double)xa[x].z * (double)xc[y].z;
-          t += (double)xa[x].w * (double)xc[y].w;
-          acc[x][y] = t;
-        }
-    }
-    if (!resident) __syncthreads();
+    if (!resident) __syncthreads();  // the next chunk overwrites this one
   }
-  if (fast) {
+  if (job >= 0) {
+    double* part = (job == 0 ? G2 : job == 1 ? Vd : Gpart) + kq * 256 + (lane >> 4) * 16 + lx;
 #pragma unroll
-    for (int x = 0; x < 4; x++)
-#pragma unroll
-      for (int y = 0; y < 4; y++) acc[x][y] = wave_sum(acc[x][y]);
-    if (fg == 0) {
-#pragma unroll
-      for (int x = 0; x < 4; x++)
-#pragma unroll
-        for (int y = 0; y < 4; y++) {
-          G[(fa + x) * GP + fc + y] = acc[x][y];
-          G[(fc + y) * GP + fa + x] = acc[x][y];
-        }
-    }
-    for (int e = tid; e < 256; e += JAC_THREADS) {
-      const int a = e >> 4, c = e & 15;
-      G[a * GP + c] = dgp[e];
-      G[(JAC_BW + a) * GP + JAC_BW + c] = dgq[e];
-    }
-  } else {
-#pragma unroll
-    for (int x = 0; x < 4; x++)
-#pragma unroll
-      for (int y = 0; y < 4; y++) {
-        double v = acc[x][y];
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-        acc[x][y] = v;
+    for (int r = 0; r < 4; r++) part[64 * r] = m0[r] + m1[r];
+  }
+  __syncthreads();
+  if (tid < 256) {
+    const int a = tid >> 4, c = tid & 15;
+    if (a < bw && c < bw) {
+      const double gx = sum4(Vd, tid);
+      G[a * GP + bw + c] = gx;
+      G[(bw + c) * GP + a] = gx;
+      if (!fast) {
+        G[a * GP + c] = sum4(G2, tid);
+        G[(bw + a) * GP + bw + c] = sum4(Gpart, tid);
       }
-    if (g == 0) {
-#pragma unroll
-      for (int x = 0; x < 4; x++)
-#pragma unroll
-        for (int y = 0; y < 4; y++) G[(ta + x) * GP + tc + y] = acc[x][y];
     }
   }
   if (tid == 0) s_rot_any = 0;
   __syncthreads();
   if (!fast) {  // keep the freshly formed diagonal blocks
-    for (int e = tid; e < 256; e += JAC_THREADS) {
-      const int a = e >> 4, c = e & 15;
-      if (keepP) dgp[e] = G[a * GP + c];
-      if (keepQ) dgq[e] = G[(wp + a) * GP + wp + c];
+    if (tid < 256) {
+      const int a = tid >> 4, c = tid & 15;
+      if (a < bw && c < bw) {
+        if (keepP) dgp[tid] = G[a * GP + c];
+        if (keepQ) dgq[tid] = G[(bw + a) * GP + bw + c];
+      }
     }
     if (tid == 0) {
       if (keepP) dgv[blk.bc_off + p] = 1;
@@ -233,177 +212,200 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     }
   }
   {
-    // a task whose columns are already orthogonal to the tolerance has nothing to rotate
+    // a task whose columns are already orthogonal to the tolerance has nothing to rotate (the zero
+    // columns between p and q have zero entries)
     const int x = tid >> 5, y = tid & 31;
     bool need = false;
-    if (x < y && y < w) {
+    if (x < y && y < wl) {
       const double ga = G[x * GP + y];
       need = ga * ga > ((double)tol * (double)tol) * (G[x * GP + x] * G[y * GP + y]);
     }
     if (!__syncthreads_or(need)) return;
   }
-  // ---- cyclic Jacobi on G.  Per step every column x moves as P'(:,x) = cx P(:,x) + dx P(:,px) (px its
-  // partner, cx = cos, dx = -+sin), so G' = J^T G J and V' = V J.  Thread (x, y) keeps G(x, y) and
-  // V(x, y) in registers; a wave holds two whole rows, so the column step R(x, y) = cy G(x, y) +
-  // dy G(x, py) and V'(x, y) = cy V(x, y) + dy V(x, py) take a shuffle, and only the row step
-  // G'(x, y) = cx R(x, y) + dx R(px, y) goes through LDS.  In the same pass the first 32 threads
-  // (one per column) form, from R, the Gram entries of the next step's pair of their column and
-  // compute its rotation.
-  // Step schedule (local columns: block p = 0..wp-1, block q = wp..w-1).  Each pair of columns is
-  // rotated once per outer sweep: the pairs inside p and inside q in round 0 (a circle tournament of
-  // bwp = bw rounded up to even players on each block at once, bwp - 1 steps), the bw x bw cross
-  // pairs in every round (step k pairs p-column i with q-column (i + k) mod bw).
+  // ---- cyclic Jacobi on G.  A step rotates up to 16 disjoint column pairs ("slots"); column x of slot
+  // (u, v), u < v, moves as P'(:,u) = c P(:,u) - s P(:,v), P'(:,v) = s P(:,u) + c P(:,v), so
+  // G' = J^T G J and V' = V J.  Every thread that needs a slot's rotation computes it from the same
+  // three entries of the current G (so all copies agree exactly).  One barrier per step.
+  // Step schedule.  Each pair of columns is rotated once per outer sweep: the pairs inside p and
+  // inside q in round 0 (a circle tournament of bwp = bw rounded up to even players on each block
+  // at once, bwp - 1 steps, slots 0..bwp/2-1 on p and bwp/2..bwp-1 on q), the bw x bw cross pairs
+  // in every round (step k, slot i: p-column i with q-column (i + k) mod bw).  A column without a
+  // partner in a step sits in a slot with -1.
   const int bwp = bw + (bw & 1);
   const int n_intra = round == 0 ? bwp - 1 : 0;
   const int steps_per = n_intra + bw;
   const int nsteps = max_inner * steps_per;
-  // partner of local column x at schedule step st, or -1 when x idles
-  auto partner = [&](int st, int x) -> int {
+  auto slot = [&](int st, int sl, int& u, int& v) {
     while (st >= steps_per) st -= steps_per;
-    if (x >= w) return -1;
-    const bool inp = x < wp;
-    const int y = inp ? x : x - wp;
-    const int wb = inp ? wp : wq;
-    if (st < n_intra) {  // circle tournament on bwp players: 0 meets 1 + st, else 1 + (2 st + 1 - y) mod m
-      const int m = bwp - 1;
-      int pq;
-      if (y == 0) {
-        pq = 1 + st;
-      } else if (y == 1 + st) {
-        pq = 0;
-      } else {
-        int t = 2 * st + 1 - y;
-        t += t < 0 ? m : 0;
-        t -= t >= m ? m : 0;
-        t -= t >= m ? m : 0;
-        pq = 1 + t;
-      }
-      return pq < wb ? (inp ? pq : wp + pq) : -1;
+    if (st >= n_intra) {  // cross step (uniform branch)
+      int jj = sl + st - n_intra;
+      jj -= jj >= bw ? bw : 0;
+      u = sl < wp ? sl : -1;
+      v = sl < bw && jj < wq ? bw + jj : -1;
+      return;
     }
-    const int k = st - n_intra;
-    if (inp) {
-      int j = y + k;
-      j -= j >= bw ? bw : 0;
-      return j < wq ? wp + j : -1;
+    u = v = -1;
+    const int half = bwp >> 1, M = bwp - 1;
+    const bool sq = sl >= half;
+    const int m = sq ? sl - half : sl;
+    if (m >= half) return;
+    int a = 0, c = 1 + st;
+    if (m > 0) {
+      int x = st + m;
+      x -= x >= M ? M : 0;
+      int y = st - m;
+      y += y < 0 ? M : 0;
+      a = 1 + x;
+      c = 1 + y;
     }
-    int i = y - k;
-    i += i < 0 ? bw : 0;
-    return i < wp ? i : -1;
+    if (a > c) {
+      const int t = a;
+      a = c;
+      c = t;
+    }
+    const int wb = sq ? wq : wp, base = sq ? bw : 0;
+    u = a < wb ? base + a : -1;
+    v = c < wb ? base + c : -1;
   };
   const double tol2 = (double)tol * (double)tol;
-  const int ex = tid >> 5, ey = tid & 31, lane = tid & 63;
-  double gv = G[ex * GP + ey];
-  double vv = (ex == ey) ? 1.0 : 0.0;
-  int pb = 0;  // parameter buffer of the current step
-  // Parameters of column x for a step: x and its partner y each compute the rotation of (min, max)
-  // from the same three Gram entries (so they agree exactly) and write their own entry.
-  auto params = [&](int buf, int x, int y, double al, double be, double ga) {
-    double cs = 1.0, sn = 0.0;
-    if (y >= 0 && ga != 0.0 && ga * ga > tol2 * (al * be)) {
-      // the angle to FP32 accuracy; cos and sin then normalised in FP64 (cs^2 + sn^2 = 1 to
-      // rounding), so the accumulated V stays orthogonal
-      const float zeta = (float)(be - al) * __builtin_amdgcn_rcpf((float)(2.0 * ga));
-      const float tf = (zeta >= 0.0f ? 1.0f : -1.0f) *
-                       __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_amdgcn_sqrtf(1.0f + zeta * zeta));
-      // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
-      if (fabsf(tf) > ang_tol) {
-        const double t = (double)tf, xx = 1.0 + t * t;
-        double z = (double)__builtin_amdgcn_rsqf((float)xx);
-        z = z * (1.5 - 0.5 * xx * z * z);
-        z = z * (1.5 - 0.5 * xx * z * z);
-        cs = z;
-        sn = z * t;
-        s_rot_any = 1;
-      }
-    }
-    const bool first = y < 0 || x < y;
-    pc[buf][x] = cs;
-    pd[buf][x] = first ? -sn : sn;
-    pp[buf][x] = y < 0 ? x : y;
+  // rotation of slot (u, v) from the current G (identity for a lone column or a pair below tolerance).
+  // Branch-free, so that a thread's two rotations and its block reads overlap: the entries are read
+  // at clamped indices and the result is selected at the end (a zero or tiny ga gives inf/NaN on
+  // the discarded side only).
+  auto rot = [&](const double* Gc, int u, int v, double& cs, double& sn) {
+    const bool pair = u >= 0 && v >= 0;
+    const int uu = pair ? u : 0, vv = pair ? v : 0;
+    const double al = Gc[uu * GP + uu], be = Gc[vv * GP + vv], ga = Gc[uu * GP + vv];
+    // the angle to FP32 accuracy; cos and sin then normalised in FP64 (cs^2 + sn^2 = 1 to rounding,
+    // one Newton step on the FP32 reciprocal square root), so the accumulated V stays orthogonal
+    const float zeta = (float)(be - al) * __builtin_amdgcn_rcpf((float)(2.0 * ga));
+    const float tf = __builtin_copysignf(1.0f, zeta) *
+                     __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_amdgcn_sqrtf(1.0f + zeta * zeta));
+    const double t = (double)tf, xx = 1.0 + t * t;
+    double z = (double)__builtin_amdgcn_rsqf((float)xx);
+    z = z * (1.5 - 0.5 * xx * z * z);
+    // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
+    const bool go = pair & (ga != 0.0) & (ga * ga > tol2 * (al * be)) & (fabsf(tf) > ang_tol);
+    cs = go ? z : 1.0;
+    sn = go ? z * t : 0.0;
   };
-  if (tid < JAC_W) {
-    const int y = nsteps > 0 ? partner(0, tid) : -1;
-    const int a = y < 0 ? tid : min(tid, y), c = y < 0 ? tid : max(tid, y);
-    params(0, tid, y, G[a * GP + a], G[c * GP + c], G[a * GP + c]);
+  // slot table of the schedule (one period: steps_per x 16), packed (u + 1) | (v + 1) << 8
+  for (int e = tid; e < steps_per * 16; e += JAC_THREADS) {
+    int u, v;
+    slot(e >> 4, e & 15, u, v);
+    s_slots[e] = (u + 1) | ((v + 1) << 8);
+  }
+  for (int e = tid; e < JAC_W * JAC_W; e += JAC_THREADS) {
+    const int x = e >> 5, y = e & 31;
+    Vd[x * GP + y] = x == y ? 1.0 : 0.0;
   }
   __syncthreads();
-  double* R = G2;  // the column-rotated G of the current step
-  for (int st = 0; st < nsteps; st++) {
-    const double cy = pc[pb][ey], dy = pd[pb][ey];
-    const int qy = pp[pb][ey];
-    const int src = (lane & 32) + qy;
-    const double gq = __shfl(gv, src, 64), vq = __shfl(vv, src, 64);
-    const double r = cy * gv + dy * gq;
-    vv = cy * vv + dy * vq;
-    R[ex * GP + ey] = r;
-    __syncthreads();
-    const double cx = pc[pb][ex], dx = pd[pb][ex];
-    const int qx = pp[pb][ex];
-    gv = cx * r + dx * R[qx * GP + ey];
-    if (tid < JAC_W && st + 1 < nsteps) {
-      // G'(x, y) of this step for the next step's pair, from R
-      const int y = partner(st + 1, tid);
-      double al = 1.0, be = 1.0, ga = 0.0;
-      if (y >= 0) {
-        const int a = min(tid, y), c = max(tid, y);
-        auto gnew = [&](int u, int v) { return pc[pb][u] * R[u * GP + v] + pd[pb][u] * R[pp[pb][u] * GP + v]; };
-        al = gnew(a, a);
-        be = gnew(c, c);
-        ga = gnew(a, c);
+  // thread (a, c) < 256 rotates the entries (rows of slot a) x (columns of slot c) of G into the
+  // other buffer, and the same entries of V in place (V' = V J needs only slot c's rotation); the
+  // slots partition the columns, so every entry is written once.  Threads 256 and up only keep the
+  // barrier.
+  double* Gc = G;
+  double* Gn = G2;
+  const int sa_i = tid >> 4, sc_i = tid & 15;
+  for (int st = 0, row = 0; st < nsteps; st++) {
+    if (tid < 256) {
+      const int pa = s_slots[row * 16 + sa_i], pcs = s_slots[row * 16 + sc_i];
+      const int ua = (pa & 0xff) - 1, va = (pa >> 8) - 1, uc = (pcs & 0xff) - 1, vc = (pcs >> 8) - 1;
+      if ((ua >= 0 || va >= 0) && (uc >= 0 || vc >= 0)) {
+        double ca, sa, cc, sc;
+        rot(Gc, ua, va, ca, sa);
+        rot(Gc, uc, vc, cc, sc);
+        const int ia = max(ua, 0), ja = max(va, 0), ic = max(uc, 0), jc = max(vc, 0);
+        const double g00 = Gc[ia * GP + ic], g01 = Gc[ia * GP + jc], g10 = Gc[ja * GP + ic], g11 = Gc[ja * GP + jc];
+        const double v00 = Vd[ia * GP + ic], v01 = Vd[ia * GP + jc], v10 = Vd[ja * GP + ic], v11 = Vd[ja * GP + jc];
+        if (sa != 0.0 || sc != 0.0) s_rot_any = 1;
+        const bool a0 = ua >= 0, a1 = va >= 0, c0 = uc >= 0, c1 = vc >= 0;
+        const double b00 = a0 && c0 ? g00 : 0.0, b01 = a0 && c1 ? g01 : 0.0;
+        const double b10 = a1 && c0 ? g10 : 0.0, b11 = a1 && c1 ? g11 : 0.0;
+        // columns, then rows
+        const double r00 = cc * b00 - sc * b01, r01 = sc * b00 + cc * b01;
+        const double r10 = cc * b10 - sc * b11, r11 = sc * b10 + cc * b11;
+        if (a0) {
+          if (c0) {
+            Gn[ia * GP + ic] = ca * r00 - sa * r10;
+            Vd[ia * GP + ic] = cc * v00 - sc * v01;
+          }
+          if (c1) {
+            Gn[ia * GP + jc] = ca * r01 - sa * r11;
+            Vd[ia * GP + jc] = sc * v00 + cc * v01;
+          }
+        }
+        if (a1) {
+          if (c0) {
+            Gn[ja * GP + ic] = sa * r00 + ca * r10;
+            Vd[ja * GP + ic] = cc * v10 - sc * v11;
+          }
+          if (c1) {
+            Gn[ja * GP + jc] = sa * r01 + ca * r11;
+            Vd[ja * GP + jc] = sc * v10 + cc * v11;
+          }
+        }
       }
-      params(pb ^ 1, tid, y, al, be, ga);
     }
     __syncthreads();
-    pb ^= 1;
+    double* t = Gc;
+    Gc = Gn;
+    Gn = t;
+    row = row + 1 == steps_per ? 0 : row + 1;
   }
-  Vf[ex * GP + ey] = (float)vv;
+  const int ex = tid >> 5, ey = tid & 31;
+  Vf[ex * GP + ey] = (float)Vd[ex * GP + ey];
   if (s_rot_any) {  // the diagonal blocks of G' = V^T G V for the next round
+    const double gv = Gc[ex * GP + ey];
     if (keepP && ex < wp && ey < wp) dgp[ex * JAC_BW + ey] = gv;
-    if (keepQ && ex >= wp && ey >= wp && ex < w && ey < w) dgq[(ex - wp) * JAC_BW + (ey - wp)] = gv;
+    if (keepQ && ex >= bw && ey >= bw && ex < wl && ey < wl) dgq[(ex - bw) * JAC_BW + (ey - bw)] = gv;
   }
   __syncthreads();
   if (!s_rot_any) return;  // P unchanged: nothing to write
-  // ---- P <- P V in FP32.  Resident panel: thread (column c, row-quad group) keeps V(:, c) in registers
-  // and reads four rows per 128-bit LDS load.  Otherwise one row per thread from global memory (every
-  // value of the row is read before it is overwritten).
+  // ---- P <- P V in FP32
   if (resident) {
-    // thread (column c = tid%32, row quads rq, rq+32, ...): FMAs from the LDS panel (each 128-bit read
-    // serves the 32 columns of a quad), results written in place (a quad's rows are read by this wave
-    // only), then copied to global memory column by column
-    const int c = tid & 31, rq = tid >> 5;
-    float vc[JAC_W];
+    // (P V)^T = V^T P^T on the FP32 matrix cores in 16-row tiles (wave v takes tiles v, v + 16, ...):
+    // per tile and half h of the new columns, 8 steps of 16 x 16 x 4 over the 32 old columns.  Lane l
+    // keeps V(4 ks + l/16, 16 h + l%16) (A) in registers and reads P(row0 + l%16, 4 ks + l/16) (B);
+    // entry (i, j) = (P V)(row0 + j, 16 h + i) lands in lane 16 (i/4) + j, register i%4 and goes
+    // straight to W (16 consecutive rows of 4 columns per store).  Tile rows past n read the
+    // column's padding and are not stored.
+    const int li = lane & 15, lk = lane >> 4;
+    float va[2][8];
 #pragma unroll
-    for (int i = 0; i < JAC_W; i++) vc[i] = Vf[i * GP + c];
-    for (int r = 4 * rq; r < n; r += 128) {
-      float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int h = 0; h < 2; h++)
 #pragma unroll
-      for (int i = 0; i < JAC_W; i++) {
-        const float4 x = *(const float4*)&chunk[i * JAC_CHUNK + r];
-        o.x = fmaf(x.x, vc[i], o.x);
-        o.y = fmaf(x.y, vc[i], o.y);
-        o.z = fmaf(x.z, vc[i], o.z);
-        o.w = fmaf(x.w, vc[i], o.w);
-      }
-      __builtin_amdgcn_wave_barrier();  // every lane of the wave has read the quad
-      *(float4*)&chunk[c * JAC_CHUNK + r] = o;
-    }
-    __syncthreads();
-    {
-      const int i = tid >> 5, l = tid & 31;
-      if (i < w) {
-        float* dstc = Wb + (long long)gcol(i) * n;
-        const float* srcc = chunk + i * JAC_CHUNK;
-        for (int r = l; r < n; r += 32) dstc[r] = srcc[r];
+      for (int ks = 0; ks < 8; ks++) va[h][ks] = Vf[(4 * ks + lk) * GP + 16 * h + li];
+    const int ntile = (n + 15) >> 4;
+    for (int t = wv; t < ntile; t += JAC_THREADS / 64) {
+      const int row = 16 * t + li;
+      float pb[8];
+#pragma unroll
+      for (int ks = 0; ks < 8; ks++) pb[ks] = chunk[(4 * ks + lk) * JAC_CS + row];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        flt4 m = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 8; ks++) m = __builtin_amdgcn_mfma_f32_16x16x4f32(va[h][ks], pb[ks], m, 0, 0, 0);
+        if (row < n)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int col = 16 * h + 4 * lk + r;
+            if (valid(col)) Wb[(long long)gcol(col) * n + row] = m[r];
+          }
       }
     }
     if (tid == 0) flags[3 * b + sweep % 3] = 1;
     return;
   }
+  // larger panels: one row per thread from global memory (every value of the row is read before it
+  // is overwritten)
   for (int r = tid; r < n; r += JAC_THREADS) {
     float x[JAC_W];
 #pragma unroll
-    for (int i = 0; i < JAC_W; i++) x[i] = i < w ? Wb[(long long)gcol(i) * n + r] : 0.0f;
-    for (int c = 0; c < w; c++) {
+    for (int i = 0; i < JAC_W; i++) x[i] = valid(i) ? Wb[(long long)gcol(i) * n + r] : 0.0f;
+    for (int c = 0; c < JAC_W; c++) {
+      if (!valid(c)) continue;
       float s = 0.0f;
 #pragma unroll
       for (int i = 0; i < JAC_W; i++) s = fmaf(x[i], Vf[i * GP + c], s);
@@ -1109,7 +1111,7 @@ size_t final_sort_lds(int nmax) {
 void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks, float* W, int sweep, int round,
                          int* flags, double* dgram, int* dgv, float tol, float ang_tol, int max_inner, hipStream_t s) {
   if (n_tasks <= 0) return;
-  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CHUNK * sizeof(float), s, blocks,
+  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CS * sizeof(float), s, blocks,
                      tasks, W, sweep, round, flags, dgram, dgv,
                      tol, ang_tol, max_inner);
   NRGK_HIP(hipGetLastError());
@@ -1146,7 +1148,7 @@ void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const do
 }
 void set_kernel_lds_limits(int bytes) {
   NRGK_HIP(hipFuncSetAttribute((const void*)k_jacobi_round, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                JAC_W * JAC_CHUNK * sizeof(float)));
+                                JAC_W * JAC_CS * sizeof(float)));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_normalize_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_prep, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_oa_rr, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
