@@ -384,11 +384,16 @@ void Chain::eigensolve_mixed() {
   d_eig.ensure(nb * sizeof(EigBlock) + 256);
   h2d(d_eig.as<EigBlock>(), blks.data(), nb, st);
   const size_t o_rb = 0, o_rl = ((nrows * 4 + 255) / 256) * 256, o_tk = o_rl + ((nrows * 4 + 255) / 256) * 256;
-  d_misc.ensure(o_tk + tasks.size() * sizeof(int2) + 256);
+  std::vector<JacTask> jtasks;
+  for (const int2& t : tasks) {
+    const EigBlock& e = blks[t.x];
+    jtasks.push_back(JacTask{e.off, e.n, e.nbc, e.bw, e.bc_off, t.x, t.y});
+  }
+  d_misc.ensure(o_tk + jtasks.size() * sizeof(JacTask) + 256);
   char* mb = d_misc.as<char>();
   h2d((int*)(mb + o_rb), row_block.data(), nrows, st);
   h2d((int*)(mb + o_rl), row_local.data(), nrows, st);
-  h2d((int2*)(mb + o_tk), tasks.data(), tasks.size(), st);
+  h2d((JacTask*)(mb + o_tk), jtasks.data(), jtasks.size(), st);
   const EigBlock* dblk = d_eig.as<EigBlock>();
 
   const size_t bytes64 = std::max<long long>(1, tot_elems) * sizeof(double);
@@ -433,7 +438,7 @@ void Chain::eigensolve_mixed() {
       if (sw == 0 || hflags[3 * b + (sw - 1) % 3]) nbc_act = std::max(nbc_act, blks[b].nbc);
     prof_begin();
     for (int r = 0; r < nbc_act - 1; r++)
-      launch_jacobi_round(dblk, (const int2*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
+      launch_jacobi_round((const JacTask*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
                           d_dgram.as<double>(), d_dgv.as<int>(), jac_tol, jac_ang, jac_inner, st);
     prof_end("jacobi_round", nbc_act - 1, fl, by);
     d2h(hflags.data(), d_flags.as<int>(), hflags.size(), st);

@@ -134,6 +134,14 @@ struct EigBlock {
   int bc_off;      // offset of the block's block columns (prefix of nbc)
 };
 
+// One Jacobi task (a pair slot j of block b's tournament) with the block's fields, so that a
+// workgroup reads everything it needs to start in one record.
+struct JacTask {
+  long long off;
+  int n, nbc, bw, bc_off;
+  int b, j;
+};
+
 // Batched GEMM problem: C = alpha * op(A) * op(B) + beta * C0, all row-major.
 struct GemmProb {
   const double* A;

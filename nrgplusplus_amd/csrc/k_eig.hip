@@ -66,35 +66,48 @@ constexpr int JAC_BW = 16;
 constexpr int JAC_W = 2 * JAC_BW;
 constexpr int JAC_CHUNK = 768;         // rows per LDS chunk of the Gram pass
 constexpr int JAC_CS = JAC_CHUNK + 4;  // column stride of the chunk: 16 columns x 4 rows fall in distinct banks
-constexpr int GP = JAC_W + 1;          // padded row of G and V
+// G and V carry a zero row and column JAC_DUMMY, read in place of a missing member of a rotation
+// pair, and a row and column JAC_SINK that take its writes, so that a rotation step has no branches
+constexpr int JAC_DUMMY = JAC_W, JAC_SINK = JAC_W + 1;
+constexpr int GP = JAC_W + 3;          // row stride of G and V (rows and columns 0..JAC_SINK)
+constexpr int GSZ = (JAC_W + 2) * GP;
 
 // sum of the four partial 16 x 16 blocks at P, P + 256, P + 512, P + 768
 __device__ __forceinline__ double sum4(const double* P, int t) { return (P[t] + P[256 + t]) + (P[512 + t] + P[768 + t]); }
 
-__global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __restrict__ blocks,
-                                                              const int2* __restrict__ tasks, float* __restrict__ W,
+__global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __restrict__ tasks, float* __restrict__ W,
                                                               int sweep, int round, int* __restrict__ flags,
                                                               double* __restrict__ dgram, int* __restrict__ dgv,
                                                               float tol, float ang_tol, int max_inner) {
   extern __shared__ float4 chunk4[];  // [JAC_W * JAC_CS] floats, column-major, 16-byte aligned
   float* chunk = reinterpret_cast<float*>(chunk4);
-  __shared__ double G[JAC_W * GP], G2[JAC_W * GP];
-  __shared__ double Vd[JAC_W * GP], Gpart[JAC_W * GP];
+  __shared__ double G[GSZ], G2[GSZ];
+  __shared__ double Vd[GSZ], Gpart[1024];
   __shared__ int s_slots[32 * 16];
   __shared__ float Vf[JAC_W * GP];
-  __shared__ int s_rot_any;
-  const int2 task = tasks[blockIdx.x];
-  const int b = task.x, j = task.y;
-  const EigBlock blk = blocks[b];
-  const int N = blk.nbc;
+  const JacTask task = tasks[blockIdx.x];
+  const int b = task.b, j = task.j;
+  const int N = task.nbc;
   if (round >= N - 1) return;
-  // slot (sweep+1)%3 is cleared every sweep, also for converged blocks, so that the host never reads
-  // a flag left over from three sweeps earlier
-  if (round == 0 && j == 0 && threadIdx.x == 0) flags[3 * b + (sweep + 1) % 3] = 0;
-  if (flags[3 * b + (sweep + 2) % 3] == 0) return;
   int p, q;
   tournament_pair(N, round, j, p, q);
-  const int n = blk.n, bw = blk.bw;
+  const int n = task.n, bw = task.bw;
+  const int tid = threadIdx.x;
+  // the sweep flag, the validity of the carried Gram blocks and the blocks themselves are read at
+  // once (the blocks unconditionally: both slots exist whatever their state)
+  double* dgp = dgram + ((long long)task.bc_off + p) * 256;
+  double* dgq = dgram + ((long long)task.bc_off + q) * 256;
+  const int prev = flags[3 * b + (sweep + 2) % 3];
+  const int vp = dgv[task.bc_off + p], vq = dgv[task.bc_off + q];
+  double pre_p = 0.0, pre_q = 0.0;
+  if (tid < 256) {
+    pre_p = dgp[tid];
+    pre_q = dgq[tid];
+  }
+  // slot (sweep+1)%3 is cleared every sweep, also for converged blocks, so that the host never reads
+  // a flag left over from three sweeps earlier
+  if (round == 0 && j == 0 && tid == 0) flags[3 * b + (sweep + 1) % 3] = 0;
+  if (prev == 0) return;
   const int c0p = p * bw, c0q = q * bw;
   const int wp = c0p < n ? min(bw, n - c0p) : 0;
   const int wq = c0q < n ? min(bw, n - c0q) : 0;
@@ -103,8 +116,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
   // pair after round 0 (which rotates the pairs inside each block column)
   if (round > 0 && (wp == 0 || wq == 0)) return;
   const int wl = bw + wq;  // local columns in use: [0, wp) and [bw, wl)
-  float* Wb = W + blk.off;
-  const int tid = threadIdx.x;
+  float* Wb = W + task.off;
   auto valid = [&](int i) { return i < wp || (i >= bw && i < wl); };
   auto gcol = [&](int i) { return i < bw ? c0p + i : c0q + (i - bw); };
   const bool resident = n <= JAC_CHUNK;
@@ -112,17 +124,12 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
   // are valid, only the cross block is formed from P.  They are the Gram blocks of the exact rotated
   // columns, which the FP32 panel holds to rounding.  A block column keeps its block current in every
   // round that touches it, so a stored block is never stale.
-  double* dgp = dgram + ((long long)blk.bc_off + p) * 256;
-  double* dgq = dgram + ((long long)blk.bc_off + q) * 256;
   const bool keepP = wp > 0, keepQ = wq > 0;
-  const bool fast = resident && keepP && keepQ && dgv[blk.bc_off + p] && dgv[blk.bc_off + q];
-  // the carried blocks are fetched now, so that their latency overlaps the panel load (entries
-  // outside a partial block column's width are zero)
-  double pre_p = 0.0, pre_q = 0.0;
-  if (fast && tid < 256) {
+  const bool fast = resident && keepP && keepQ && vp && vq;
+  if (tid < 256) {  // entries outside a partial block column's width are zero
     const int a = tid >> 4, c = tid & 15;
-    if (a < wp && c < wp) pre_p = dgp[tid];
-    if (a < wq && c < wq) pre_q = dgq[tid];
+    if (!(a < wp && c < wp)) pre_p = 0.0;
+    if (!(a < wq && c < wq)) pre_q = 0.0;
   }
   // ---- Gram matrix on the FP64 matrix cores (16 x 16 x 4 steps).  Job of wave v: the p x q block
   // (fast: waves 0-3), or the p x p, p x q and q x q blocks (waves 0-3, 4-7, 8-11); the four waves of
@@ -196,7 +203,6 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
       }
     }
   }
-  if (tid == 0) s_rot_any = 0;
   __syncthreads();
   if (!fast) {  // keep the freshly formed diagonal blocks
     if (tid < 256) {
@@ -207,8 +213,8 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
       }
     }
     if (tid == 0) {
-      if (keepP) dgv[blk.bc_off + p] = 1;
-      if (keepQ) dgv[blk.bc_off + q] = 1;
+      if (keepP) dgv[task.bc_off + p] = 1;
+      if (keepQ) dgv[task.bc_off + q] = 1;
     }
   }
   {
@@ -268,14 +274,11 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     v = c < wb ? base + c : -1;
   };
   const double tol2 = (double)tol * (double)tol;
-  // rotation of slot (u, v) from the current G (identity for a lone column or a pair below tolerance).
-  // Branch-free, so that a thread's two rotations and its block reads overlap: the entries are read
-  // at clamped indices and the result is selected at the end (a zero or tiny ga gives inf/NaN on
-  // the discarded side only).
+  // rotation of slot (u, v) (read indices) from the current G: identity for a lone column (its
+  // partner is the zero dummy, ga = 0) or a pair below tolerance.  Branch-free: a zero or tiny ga
+  // gives inf/NaN on the discarded side of the final selection only.
   auto rot = [&](const double* Gc, int u, int v, double& cs, double& sn) {
-    const bool pair = u >= 0 && v >= 0;
-    const int uu = pair ? u : 0, vv = pair ? v : 0;
-    const double al = Gc[uu * GP + uu], be = Gc[vv * GP + vv], ga = Gc[uu * GP + vv];
+    const double al = Gc[u * GP + u], be = Gc[v * GP + v], ga = Gc[u * GP + v];
     // the angle to FP32 accuracy; cos and sin then normalised in FP64 (cs^2 + sn^2 = 1 to rounding,
     // one Newton step on the FP32 reciprocal square root), so the accumulated V stays orthogonal
     const float zeta = (float)(be - al) * __builtin_amdgcn_rcpf((float)(2.0 * ga));
@@ -285,83 +288,83 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const EigBlock* __
     double z = (double)__builtin_amdgcn_rsqf((float)xx);
     z = z * (1.5 - 0.5 * xx * z * z);
     // a rotation below the angle tolerance changes the vectors by less than FP32 can hold
-    const bool go = pair & (ga != 0.0) & (ga * ga > tol2 * (al * be)) & (fabsf(tf) > ang_tol);
+    const bool go = (ga != 0.0) & (ga * ga > tol2 * (al * be)) & (fabsf(tf) > ang_tol);
     cs = go ? z : 1.0;
     sn = go ? z * t : 0.0;
   };
-  // slot table of the schedule (one period: steps_per x 16), packed (u + 1) | (v + 1) << 8
+  // slot table of the schedule (one period: steps_per x 16): read and write indices of the pair,
+  // packed ru | rv << 8 | wu << 16 | wv << 24 (a missing member reads JAC_DUMMY, writes JAC_SINK)
   for (int e = tid; e < steps_per * 16; e += JAC_THREADS) {
     int u, v;
     slot(e >> 4, e & 15, u, v);
-    s_slots[e] = (u + 1) | ((v + 1) << 8);
+    const int ru = u >= 0 ? u : JAC_DUMMY, rv = v >= 0 ? v : JAC_DUMMY;
+    const int wu = u >= 0 ? u : JAC_SINK, wv2 = v >= 0 ? v : JAC_SINK;
+    s_slots[e] = ru | (rv << 8) | (wu << 16) | (wv2 << 24);
   }
-  for (int e = tid; e < JAC_W * JAC_W; e += JAC_THREADS) {
-    const int x = e >> 5, y = e & 31;
-    Vd[x * GP + y] = x == y ? 1.0 : 0.0;
+  // V = I; the dummy row and column zero in V and in both G buffers (G2 held partial blocks)
+  for (int e = tid; e < (JAC_W + 2) * (JAC_W + 2); e += JAC_THREADS) {
+    const int x = e / (JAC_W + 2), y = e - x * (JAC_W + 2);
+    Vd[x * GP + y] = x == y && x < JAC_W ? 1.0 : 0.0;
+    if (x == JAC_DUMMY || y == JAC_DUMMY) {
+      G[x * GP + y] = 0.0;
+      G2[x * GP + y] = 0.0;
+    }
   }
   __syncthreads();
   // thread (a, c) < 256 rotates the entries (rows of slot a) x (columns of slot c) of G into the
   // other buffer, and the same entries of V in place (V' = V J needs only slot c's rotation); the
-  // slots partition the columns, so every entry is written once.  Threads 256 and up only keep the
+  // slots partition the columns, so every entry is written once.  Lane l computes the rotation of
+  // slot l%16, so slot a's comes from lane a of the same wave.  Threads 256 and up only keep the
   // barrier.
   double* Gc = G;
   double* Gn = G2;
   const int sa_i = tid >> 4, sc_i = tid & 15;
+  bool rotated = false;
+  int ea = 0, ec = 0;
+  if (tid < 256) {
+    ea = s_slots[sa_i];
+    ec = s_slots[sc_i];
+  }
   for (int st = 0, row = 0; st < nsteps; st++) {
+    row = row + 1 == steps_per ? 0 : row + 1;  // the next step's row of the table
     if (tid < 256) {
-      const int pa = s_slots[row * 16 + sa_i], pcs = s_slots[row * 16 + sc_i];
-      const int ua = (pa & 0xff) - 1, va = (pa >> 8) - 1, uc = (pcs & 0xff) - 1, vc = (pcs >> 8) - 1;
-      if ((ua >= 0 || va >= 0) && (uc >= 0 || vc >= 0)) {
-        double ca, sa, cc, sc;
-        rot(Gc, ua, va, ca, sa);
-        rot(Gc, uc, vc, cc, sc);
-        const int ia = max(ua, 0), ja = max(va, 0), ic = max(uc, 0), jc = max(vc, 0);
-        const double g00 = Gc[ia * GP + ic], g01 = Gc[ia * GP + jc], g10 = Gc[ja * GP + ic], g11 = Gc[ja * GP + jc];
-        const double v00 = Vd[ia * GP + ic], v01 = Vd[ia * GP + jc], v10 = Vd[ja * GP + ic], v11 = Vd[ja * GP + jc];
-        if (sa != 0.0 || sc != 0.0) s_rot_any = 1;
-        const bool a0 = ua >= 0, a1 = va >= 0, c0 = uc >= 0, c1 = vc >= 0;
-        const double b00 = a0 && c0 ? g00 : 0.0, b01 = a0 && c1 ? g01 : 0.0;
-        const double b10 = a1 && c0 ? g10 : 0.0, b11 = a1 && c1 ? g11 : 0.0;
-        // columns, then rows
-        const double r00 = cc * b00 - sc * b01, r01 = sc * b00 + cc * b01;
-        const double r10 = cc * b10 - sc * b11, r11 = sc * b10 + cc * b11;
-        if (a0) {
-          if (c0) {
-            Gn[ia * GP + ic] = ca * r00 - sa * r10;
-            Vd[ia * GP + ic] = cc * v00 - sc * v01;
-          }
-          if (c1) {
-            Gn[ia * GP + jc] = ca * r01 - sa * r11;
-            Vd[ia * GP + jc] = sc * v00 + cc * v01;
-          }
-        }
-        if (a1) {
-          if (c0) {
-            Gn[ja * GP + ic] = sa * r00 + ca * r10;
-            Vd[ja * GP + ic] = cc * v10 - sc * v11;
-          }
-          if (c1) {
-            Gn[ja * GP + jc] = sa * r01 + ca * r11;
-            Vd[ja * GP + jc] = sc * v10 + cc * v11;
-          }
-        }
-      }
+      const int rua = ea & 0xff, rva = (ea >> 8) & 0xff, wua = (ea >> 16) & 0xff, wva = ea >> 24;
+      const int ruc = ec & 0xff, rvc = (ec >> 8) & 0xff, wuc = (ec >> 16) & 0xff, wvc = ec >> 24;
+      ea = s_slots[row * 16 + sa_i];
+      ec = s_slots[row * 16 + sc_i];
+      double cc, sc;
+      rot(Gc, ruc, rvc, cc, sc);
+      const double g00 = Gc[rua * GP + ruc], g01 = Gc[rua * GP + rvc], g10 = Gc[rva * GP + ruc], g11 = Gc[rva * GP + rvc];
+      const double v00 = Vd[rua * GP + ruc], v01 = Vd[rua * GP + rvc], v10 = Vd[rva * GP + ruc], v11 = Vd[rva * GP + rvc];
+      const double ca = __shfl(cc, sa_i, 64), sa = __shfl(sc, sa_i, 64);
+      rotated |= sc != 0.0;
+      // columns, then rows
+      const double r00 = cc * g00 - sc * g01, r01 = sc * g00 + cc * g01;
+      const double r10 = cc * g10 - sc * g11, r11 = sc * g10 + cc * g11;
+      Gn[wua * GP + wuc] = ca * r00 - sa * r10;
+      Gn[wua * GP + wvc] = ca * r01 - sa * r11;
+      Gn[wva * GP + wuc] = sa * r00 + ca * r10;
+      Gn[wva * GP + wvc] = sa * r01 + ca * r11;
+      Vd[wua * GP + wuc] = cc * v00 - sc * v01;
+      Vd[wua * GP + wvc] = sc * v00 + cc * v01;
+      Vd[wva * GP + wuc] = cc * v10 - sc * v11;
+      Vd[wva * GP + wvc] = sc * v10 + cc * v11;
     }
     __syncthreads();
     double* t = Gc;
     Gc = Gn;
     Gn = t;
-    row = row + 1 == steps_per ? 0 : row + 1;
   }
+  const bool any_rot = __syncthreads_or(rotated);
+  if (!any_rot) return;  // P unchanged: nothing to write
   const int ex = tid >> 5, ey = tid & 31;
   Vf[ex * GP + ey] = (float)Vd[ex * GP + ey];
-  if (s_rot_any) {  // the diagonal blocks of G' = V^T G V for the next round
+  {  // the diagonal blocks of G' = V^T G V for the next round
     const double gv = Gc[ex * GP + ey];
     if (keepP && ex < wp && ey < wp) dgp[ex * JAC_BW + ey] = gv;
     if (keepQ && ex >= bw && ey >= bw && ex < wl && ey < wl) dgq[(ex - bw) * JAC_BW + (ey - bw)] = gv;
   }
   __syncthreads();
-  if (!s_rot_any) return;  // P unchanged: nothing to write
   // ---- P <- P V in FP32
   if (resident) {
     // (P V)^T = V^T P^T on the FP32 matrix cores in 16-row tiles (wave v takes tiles v, v + 16, ...):
@@ -1108,12 +1111,11 @@ size_t final_sort_lds(int nmax) {
   return (size_t)np2 * 12 + 64;
 }
 
-void launch_jacobi_round(const EigBlock* blocks, const int2* tasks, int n_tasks, float* W, int sweep, int round,
-                         int* flags, double* dgram, int* dgv, float tol, float ang_tol, int max_inner, hipStream_t s) {
+void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int sweep, int round, int* flags, double* dgram,
+                         int* dgv, float tol, float ang_tol, int max_inner, hipStream_t s) {
   if (n_tasks <= 0) return;
-  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CS * sizeof(float), s, blocks,
-                     tasks, W, sweep, round, flags, dgram, dgv,
-                     tol, ang_tol, max_inner);
+  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CS * sizeof(float), s, tasks, W,
+                     sweep, round, flags, dgram, dgv, tol, ang_tol, max_inner);
   NRGK_HIP(hipGetLastError());
 }
 void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
