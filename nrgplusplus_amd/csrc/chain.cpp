@@ -384,16 +384,10 @@ void Chain::eigensolve_mixed() {
   d_eig.ensure(nb * sizeof(EigBlock) + 256);
   h2d(d_eig.as<EigBlock>(), blks.data(), nb, st);
   const size_t o_rb = 0, o_rl = ((nrows * 4 + 255) / 256) * 256, o_tk = o_rl + ((nrows * 4 + 255) / 256) * 256;
-  std::vector<JacTask> jtasks;
-  for (const int2& t : tasks) {
-    const EigBlock& e = blks[t.x];
-    jtasks.push_back(JacTask{e.off, e.n, e.nbc, e.bw, e.bc_off, t.x, t.y});
-  }
-  d_misc.ensure(o_tk + jtasks.size() * sizeof(JacTask) + 256);
+  d_misc.ensure(o_tk + tasks.size() * sizeof(JacTask) + 256);
   char* mb = d_misc.as<char>();
   h2d((int*)(mb + o_rb), row_block.data(), nrows, st);
   h2d((int*)(mb + o_rl), row_local.data(), nrows, st);
-  h2d((JacTask*)(mb + o_tk), jtasks.data(), jtasks.size(), st);
   const EigBlock* dblk = d_eig.as<EigBlock>();
 
   const size_t bytes64 = std::max<long long>(1, tot_elems) * sizeof(double);
@@ -403,13 +397,12 @@ void Chain::eigensolve_mixed() {
   NRGK_HIP(hipMemsetAsync(d_sigma_bits.p, 0, std::max(1, nb) * 2 * sizeof(unsigned), st));
   launch_shift_convert(dblk, (const int*)(mb + o_rb), (const int*)(mb + o_rl), nrows, dH.as<double>(),
                        d_sigma_bits.as<unsigned>(), d_sigma.as<float>(), dW.as<float>(), st);
-  // Jacobi sweeps
-  std::vector<int> flags0(3 * std::max(1, nb), 0);
-  for (int b = 0; b < nb; b++) flags0[3 * b + 2] = 1;
-  d_flags.ensure(flags0.size() * sizeof(int));
-  h2d(d_flags.as<int>(), flags0.data(), flags0.size(), st);
+  // Jacobi sweeps.  The tasks of the blocks still rotating are listed per sweep (a block that a sweep
+  // leaves unrotated has converged), blocks with the longest tournaments first, so that round r
+  // launches the prefix of tasks whose block has a round r.
+  d_flags.ensure(std::max(1, nb) * sizeof(int));
   const int max_sweeps = 20;
-  // Gram blocks of the full-width block columns, carried between rounds (invalid until first formed)
+  // Gram blocks of the block columns, carried between rounds (invalid until first formed)
   d_dgram.ensure(std::max(1, n_bc) * 256 * sizeof(double));
   d_dgv.ensure(std::max(1, n_bc) * sizeof(int));
   NRGK_HIP(hipMemsetAsync(d_dgv.p, 0, std::max(1, n_bc) * sizeof(int), st));
@@ -418,34 +411,45 @@ void Chain::eigensolve_mixed() {
   const float jac_tol = knob("NRGK_JAC_TOL", 5e-7);
   const float jac_ang = knob("NRGK_JAC_ANGLE", 1e-7);
   const int jac_inner = (int)knob("NRGK_JAC_INNER", 1);  // Jacobi sweeps over a task's Gram matrix
-  std::vector<int> hflags(flags0.size());
+  std::vector<int> order(nb);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return blks[x].nbc > blks[y].nbc; });
+  std::vector<int> hflags(std::max(1, nb), 1);
+  std::vector<JacTask> sweep_tasks;
   sweeps = 0;
   bool converged = false;
   for (int sw = 0; sw < max_sweeps && nb > 0; sw++) {
-    double fl = 0, by = 0;
-    if (prof_on)  // blocks still rotating: per round, Gram (2 n w^2) and update (2 n w^2) of every pair
-      for (int b = 0; b < nb; b++)
-        if (sw == 0 || hflags[3 * b + (sw - 1) % 3]) {
-          const double n = blks[b].n, w = std::min(blks[b].n, 2 * blks[b].bw);
-          const double rounds = blks[b].nbc - 1, pairs = std::ceil(n / w);
-          fl += rounds * pairs * 4.0 * n * w * w;
-          by += rounds * pairs * 3.0 * n * w * 4.0;  // read twice, write once (FP32)
-        }
-    // rounds this sweep: as many as the widest block that rotated in the previous sweep needs (the
-    // tasks of converged blocks return at once; round 0 always runs, it clears their next flag)
+    sweep_tasks.clear();
     int nbc_act = 2;
-    for (int b = 0; b < nb; b++)
-      if (sw == 0 || hflags[3 * b + (sw - 1) % 3]) nbc_act = std::max(nbc_act, blks[b].nbc);
+    double fl = 0, by = 0;
+    for (int bi : order) {
+      if (!hflags[bi]) continue;
+      const EigBlock& e = blks[bi];
+      nbc_act = std::max(nbc_act, e.nbc);
+      for (int jj = 0; jj < e.nbc / 2; jj++) sweep_tasks.push_back(JacTask{e.off, e.n, e.nbc, e.bw, e.bc_off, bi, jj});
+      if (prof_on) {  // per round, Gram (2 n w^2) and update (2 n w^2) of every pair
+        const double n = e.n, w = std::min(e.n, 2 * e.bw);
+        const double rounds = e.nbc - 1, pairs = std::ceil(n / w);
+        fl += rounds * pairs * 4.0 * n * w * w;
+        by += rounds * pairs * 3.0 * n * w * 4.0;  // read twice, write once (FP32)
+      }
+    }
+    JacTask* dt = (JacTask*)(mb + o_tk);
+    h2d(dt, sweep_tasks.data(), sweep_tasks.size(), st);
+    NRGK_HIP(hipMemsetAsync(d_flags.p, 0, std::max(1, nb) * sizeof(int), st));
     prof_begin();
-    for (int r = 0; r < nbc_act - 1; r++)
-      launch_jacobi_round((const JacTask*)(mb + o_tk), (int)tasks.size(), dW.as<float>(), sw, r, d_flags.as<int>(),
-                          d_dgram.as<double>(), d_dgv.as<int>(), jac_tol, jac_ang, jac_inner, st);
+    size_t cnt = sweep_tasks.size();
+    for (int r = 0; r < nbc_act - 1; r++) {
+      while (cnt > 0 && sweep_tasks[cnt - 1].nbc - 1 <= r) cnt--;  // blocks whose tournament has ended
+      launch_jacobi_round(dt, (int)cnt, dW.as<float>(), r, d_flags.as<int>(), d_dgram.as<double>(), d_dgv.as<int>(),
+                          jac_tol, jac_ang, jac_inner, st);
+    }
     prof_end("jacobi_round", nbc_act - 1, fl, by);
-    d2h(hflags.data(), d_flags.as<int>(), hflags.size(), st);
+    d2h(hflags.data(), d_flags.as<int>(), std::max(1, nb), st);
     NRGK_HIP(hipStreamSynchronize(st));
     sweeps = sw + 1;
     bool any = false;
-    for (int b = 0; b < nb; b++) any |= hflags[3 * b + sw % 3] != 0;
+    for (int bi = 0; bi < nb; bi++) any |= hflags[bi] != 0;
     if (!any) {
       converged = true;
       break;

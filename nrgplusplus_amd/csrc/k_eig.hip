@@ -52,8 +52,8 @@ __device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p,
   }
 }
 
-// One tournament round for all blocks.  flags[3*b + s%3] is set when block b rotated during sweep s;
-// a block whose previous sweep did not rotate has converged and its tasks return at once.
+// One tournament round for the tasks of the blocks still rotating (the host drops a block once a
+// sweep leaves it unrotated).  flags[b] is set when block b rotates during the sweep.
 //
 // A task owns the pair (p, q) of block columns of W (column-major, length n).  Its local columns
 // are p's at 0..wp-1 and q's at bw..bw+wq-1 (bw <= 16; the columns between exist only beside the
@@ -76,13 +76,13 @@ constexpr int GSZ = (JAC_W + 2) * GP;
 __device__ __forceinline__ double sum4(const double* P, int t) { return (P[t] + P[256 + t]) + (P[512 + t] + P[768 + t]); }
 
 __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __restrict__ tasks, float* __restrict__ W,
-                                                              int sweep, int round, int* __restrict__ flags,
+                                                              int round, int* __restrict__ flags,
                                                               double* __restrict__ dgram, int* __restrict__ dgv,
                                                               float tol, float ang_tol, int max_inner) {
   extern __shared__ float4 chunk4[];  // [JAC_W * JAC_CS] floats, column-major, 16-byte aligned
   float* chunk = reinterpret_cast<float*>(chunk4);
-  __shared__ double G[GSZ], G2[GSZ];
-  __shared__ double Vd[GSZ], Gpart[1024];
+  __shared__ double G[GSZ], G2[GSZ], Vd[GSZ];
+  __shared__ double Gpart[3 * 1024];  // partial Gram blocks of the matrix-core waves
   __shared__ int s_slots[32 * 16];
   __shared__ float Vf[JAC_W * GP];
   const JacTask task = tasks[blockIdx.x];
@@ -93,21 +93,6 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
   tournament_pair(N, round, j, p, q);
   const int n = task.n, bw = task.bw;
   const int tid = threadIdx.x;
-  // the sweep flag, the validity of the carried Gram blocks and the blocks themselves are read at
-  // once (the blocks unconditionally: both slots exist whatever their state)
-  double* dgp = dgram + ((long long)task.bc_off + p) * 256;
-  double* dgq = dgram + ((long long)task.bc_off + q) * 256;
-  const int prev = flags[3 * b + (sweep + 2) % 3];
-  const int vp = dgv[task.bc_off + p], vq = dgv[task.bc_off + q];
-  double pre_p = 0.0, pre_q = 0.0;
-  if (tid < 256) {
-    pre_p = dgp[tid];
-    pre_q = dgq[tid];
-  }
-  // slot (sweep+1)%3 is cleared every sweep, also for converged blocks, so that the host never reads
-  // a flag left over from three sweeps earlier
-  if (round == 0 && j == 0 && tid == 0) flags[3 * b + (sweep + 1) % 3] = 0;
-  if (prev == 0) return;
   const int c0p = p * bw, c0q = q * bw;
   const int wp = c0p < n ? min(bw, n - c0p) : 0;
   const int wq = c0q < n ? min(bw, n - c0q) : 0;
@@ -123,126 +108,29 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
   // Gram blocks of the two block columns (16 x 16 slots, FP64), kept from their last round: when both
   // are valid, only the cross block is formed from P.  They are the Gram blocks of the exact rotated
   // columns, which the FP32 panel holds to rounding.  A block column keeps its block current in every
-  // round that touches it, so a stored block is never stale.
+  // round that touches it, so a stored block is never stale.  Their validity and the blocks are read
+  // together (both slots exist whatever their state).
+  double* dgp = dgram + ((long long)task.bc_off + p) * 256;
+  double* dgq = dgram + ((long long)task.bc_off + q) * 256;
+  const int vp = dgv[task.bc_off + p], vq = dgv[task.bc_off + q];
+  double pre_p = 0.0, pre_q = 0.0;
+  if (tid < 256) {
+    pre_p = dgp[tid];
+    pre_q = dgq[tid];
+  }
   const bool keepP = wp > 0, keepQ = wq > 0;
   const bool fast = resident && keepP && keepQ && vp && vq;
-  if (tid < 256) {  // entries outside a partial block column's width are zero
-    const int a = tid >> 4, c = tid & 15;
-    if (!(a < wp && c < wp)) pre_p = 0.0;
-    if (!(a < wq && c < wq)) pre_q = 0.0;
-  }
-  // ---- Gram matrix on the FP64 matrix cores (16 x 16 x 4 steps).  Job of wave v: the p x q block
-  // (fast: waves 0-3), or the p x p, p x q and q x q blocks (waves 0-3, 4-7, 8-11); the four waves of
-  // a job take every fourth group of 4 rows, two accumulators in flight.  Lane l supplies column
-  // l%16 of its left block as A and of its right block as B, at row 4 kb + l/16; entry (i, j) of the
-  // product lands in lane 16 (i%4) + j, register i/4.  The partial blocks meet in G2, Vd and Gpart,
-  // all free until the Jacobi steps.
-  const int wv = tid >> 6, lane = tid & 63, lx = lane & 15;
-  const int job = fast ? (wv < 4 ? 1 : -1) : (wv < 12 ? wv >> 2 : -1), kq = wv & 3;
-  const float* ca = chunk + (job == 2 ? bw + lx : lx) * JAC_CS + (lane >> 4);
-  const float* cb = chunk + (job == 0 ? lx : bw + lx) * JAC_CS + (lane >> 4);
-  // a 16-wide tile of p reaches into q when bw < 16: those lanes supply zeros
-  const bool am = job == 2 || lx < bw, bm = job != 0 || lx < bw;
-  dbl4 m0 = {0.0, 0.0, 0.0, 0.0}, m1 = {0.0, 0.0, 0.0, 0.0};
-  for (int r0 = 0; r0 < n; r0 += JAC_CHUNK) {
-    const int rows = min(JAC_CHUNK, n - r0);
-    const int rows4 = (rows + 3) & ~3;
-    // thread = (column tid/32, rows tid%32 + 32k): one column pointer per thread, 128-byte runs, all
-    // of a chunk's loads in flight before their LDS stores; rows are filled up to a multiple of 4
-    {
-      const int i = tid >> 5, l = tid & 31;
-      const bool ok = valid(i);
-      const float* src = Wb + (long long)gcol(ok ? i : 0) * n + r0 + l;
-      float* dst = chunk + i * JAC_CS + l;
-      const int lim = ok ? rows - l : 0, lim4 = rows4 - l;
-      float t[JAC_CHUNK / 32];
-#pragma unroll
-      for (int k = 0; k < JAC_CHUNK / 32; k++) t[k] = 32 * k < lim ? src[32 * k] : 0.0f;
-#pragma unroll
-      for (int k = 0; k < JAC_CHUNK / 32; k++)
-        if (32 * k < lim4) dst[32 * k] = t[k];
-    }
-    if (fast && tid < 256) {  // (resident: this is the only chunk)
-      const int a = tid >> 4, c = tid & 15;
-      if (a < bw && c < bw) {
-        G[a * GP + c] = pre_p;
-        G[(bw + a) * GP + bw + c] = pre_q;
-      }
-    }
-    __syncthreads();
-    if (job >= 0) {
-      const int nkb = rows4 >> 2;
-      int kb = kq;
-      for (; kb + 4 < nkb; kb += 8) {
-        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
-                                                  0, 0, 0);
-        m1 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb + 16] : 0.0,
-                                                  bm ? (double)cb[4 * kb + 16] : 0.0, m1, 0, 0, 0);
-      }
-      if (kb < nkb)
-        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
-                                                  0, 0, 0);
-    }
-    if (!resident) __syncthreads();  // the next chunk overwrites this one
-  }
-  if (job >= 0) {
-    double* part = (job == 0 ? G2 : job == 1 ? Vd : Gpart) + kq * 256 + (lane >> 4) * 16 + lx;
-#pragma unroll
-    for (int r = 0; r < 4; r++) part[64 * r] = m0[r] + m1[r];
-  }
-  __syncthreads();
-  if (tid < 256) {
-    const int a = tid >> 4, c = tid & 15;
-    if (a < bw && c < bw) {
-      const double gx = sum4(Vd, tid);
-      G[a * GP + bw + c] = gx;
-      G[(bw + c) * GP + a] = gx;
-      if (!fast) {
-        G[a * GP + c] = sum4(G2, tid);
-        G[(bw + a) * GP + bw + c] = sum4(Gpart, tid);
-      }
-    }
-  }
-  __syncthreads();
-  if (!fast) {  // keep the freshly formed diagonal blocks
-    if (tid < 256) {
-      const int a = tid >> 4, c = tid & 15;
-      if (a < bw && c < bw) {
-        if (keepP) dgp[tid] = G[a * GP + c];
-        if (keepQ) dgq[tid] = G[(bw + a) * GP + bw + c];
-      }
-    }
-    if (tid == 0) {
-      if (keepP) dgv[task.bc_off + p] = 1;
-      if (keepQ) dgv[task.bc_off + q] = 1;
-    }
-  }
-  {
-    // a task whose columns are already orthogonal to the tolerance has nothing to rotate (the zero
-    // columns between p and q have zero entries)
-    const int x = tid >> 5, y = tid & 31;
-    bool need = false;
-    if (x < y && y < wl) {
-      const double ga = G[x * GP + y];
-      need = ga * ga > ((double)tol * (double)tol) * (G[x * GP + x] * G[y * GP + y]);
-    }
-    if (!__syncthreads_or(need)) return;
-  }
-  // ---- cyclic Jacobi on G.  A step rotates up to 16 disjoint column pairs ("slots"); column x of slot
-  // (u, v), u < v, moves as P'(:,u) = c P(:,u) - s P(:,v), P'(:,v) = s P(:,u) + c P(:,v), so
-  // G' = J^T G J and V' = V J.  Every thread that needs a slot's rotation computes it from the same
-  // three entries of the current G (so all copies agree exactly).  One barrier per step.
-  // Step schedule.  Each pair of columns is rotated once per outer sweep: the pairs inside p and
-  // inside q in round 0 (a circle tournament of bwp = bw rounded up to even players on each block
-  // at once, bwp - 1 steps, slots 0..bwp/2-1 on p and bwp/2..bwp-1 on q), the bw x bw cross pairs
-  // in every round (step k, slot i: p-column i with q-column (i + k) mod bw).  A column without a
-  // partner in a step sits in a slot with -1.
+  // ---- step schedule of the Jacobi rotations.  A step rotates up to 16 disjoint column pairs
+  // ("slots").  Each pair of columns is rotated once per outer sweep: the pairs inside p and inside q
+  // in round 0 (a circle tournament of bwp = bw rounded up to even players on each block at once,
+  // bwp - 1 steps, slots 0..bwp/2-1 on p and bwp/2..bwp-1 on q), the bw x bw cross pairs in every
+  // round (step k, slot i: p-column i with q-column (i + k) mod bw).  A column without a partner in
+  // a step sits in a slot with -1.
   const int bwp = bw + (bw & 1);
   const int n_intra = round == 0 ? bwp - 1 : 0;
   const int steps_per = n_intra + bw;
   const int nsteps = max_inner * steps_per;
   auto slot = [&](int st, int sl, int& u, int& v) {
-    while (st >= steps_per) st -= steps_per;
     if (st >= n_intra) {  // cross step (uniform branch)
       int jj = sl + st - n_intra;
       jj -= jj >= bw ? bw : 0;
@@ -273,6 +161,130 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
     u = a < wb ? base + a : -1;
     v = c < wb ? base + c : -1;
   };
+  // set-up that needs no data, done while the panel loads are in flight: the slot table (one period,
+  // steps_per x 16: read and write indices of the pair packed ru | rv << 8 | wu << 16 | wv << 24, a
+  // missing member reading JAC_DUMMY and writing JAC_SINK), V = I, and the dummy row and column of
+  // both G buffers set to zero
+  auto setup = [&]() {
+    for (int e = tid; e < steps_per * 16; e += JAC_THREADS) {
+      int u, v;
+      slot(e >> 4, e & 15, u, v);
+      const int ru = u >= 0 ? u : JAC_DUMMY, rv = v >= 0 ? v : JAC_DUMMY;
+      const int wu = u >= 0 ? u : JAC_SINK, wv2 = v >= 0 ? v : JAC_SINK;
+      s_slots[e] = ru | (rv << 8) | (wu << 16) | (wv2 << 24);
+    }
+    for (int e = tid; e < (JAC_W + 2) * (JAC_W + 2); e += JAC_THREADS) {
+      const int x = e / (JAC_W + 2), y = e - x * (JAC_W + 2);
+      Vd[x * GP + y] = x == y && x < JAC_W ? 1.0 : 0.0;
+      if (x == JAC_DUMMY || y == JAC_DUMMY) {
+        G[x * GP + y] = 0.0;
+        G2[x * GP + y] = 0.0;
+      }
+    }
+  };
+  // ---- Gram matrix on the FP64 matrix cores (16 x 16 x 4 steps).  Job of wave v: the p x q block
+  // (fast: waves 0-7, each taking every eighth group of 4 rows), or the p x p, p x q and q x q blocks
+  // (waves 0-3, 4-7, 8-11, every fourth group); two accumulators in flight.  Lane l supplies column
+  // l%16 of its left block as A and of its right block as B, at row 4 kb + l/16; entry (i, j) of the
+  // product lands in lane 16 (i%4) + j, register i/4.  The partial blocks meet in Gpart.
+  const int wv = tid >> 6, lane = tid & 63, lx = lane & 15;
+  const int job = fast ? (wv < 8 ? 1 : -1) : (wv < 12 ? wv >> 2 : -1);
+  const int kq = fast ? wv : wv & 3, kstride = fast ? 8 : 4;
+  const float* ca = chunk + (job == 2 ? bw + lx : lx) * JAC_CS + (lane >> 4);
+  const float* cb = chunk + (job == 0 ? lx : bw + lx) * JAC_CS + (lane >> 4);
+  // a 16-wide tile of p reaches into q when bw < 16: those lanes supply zeros
+  const bool am = job == 2 || lx < bw, bm = job != 0 || lx < bw;
+  dbl4 m0 = {0.0, 0.0, 0.0, 0.0}, m1 = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = 0; r0 < n; r0 += JAC_CHUNK) {
+    const int rows = min(JAC_CHUNK, n - r0);
+    const int rows4 = (rows + 3) & ~3;
+    // thread = (column tid/32, rows tid%32 + 32k): one column pointer per thread, 128-byte runs, all
+    // of a chunk's loads in flight before their LDS stores; rows are filled up to a multiple of 4
+    {
+      const int i = tid >> 5, l = tid & 31;
+      const bool ok = valid(i);
+      const float* src = Wb + (long long)gcol(ok ? i : 0) * n + r0 + l;
+      float* dst = chunk + i * JAC_CS + l;
+      const int lim = ok ? rows - l : 0, lim4 = rows4 - l;
+      float t[JAC_CHUNK / 32];
+#pragma unroll
+      for (int k = 0; k < JAC_CHUNK / 32; k++) t[k] = 32 * k < lim ? src[32 * k] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < JAC_CHUNK / 32; k++)
+        if (32 * k < lim4) dst[32 * k] = t[k];
+    }
+    if (r0 == 0) setup();
+    if (fast && tid < 256) {  // (resident: this is the only chunk); entries outside a partial block
+      const int a = tid >> 4, c = tid & 15;  // column's width are zero
+      if (a < bw && c < bw) {
+        G[a * GP + c] = a < wp && c < wp ? pre_p : 0.0;
+        G[(bw + a) * GP + bw + c] = a < wq && c < wq ? pre_q : 0.0;
+      }
+    }
+    __syncthreads();
+    if (job >= 0) {
+      const int nkb = rows4 >> 2;
+      int kb = kq;
+      for (; kb + kstride < nkb; kb += 2 * kstride) {
+        const int k2 = kb + kstride;
+        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
+                                                  0, 0, 0);
+        m1 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * k2] : 0.0, bm ? (double)cb[4 * k2] : 0.0, m1,
+                                                  0, 0, 0);
+      }
+      if (kb < nkb)
+        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
+                                                  0, 0, 0);
+    }
+    if (!resident) __syncthreads();  // the next chunk overwrites this one
+  }
+  if (job >= 0) {
+    double* part = Gpart + (fast ? 0 : job * 1024) + kq * 256 + (lane >> 4) * 16 + lx;
+#pragma unroll
+    for (int r = 0; r < 4; r++) part[64 * r] = m0[r] + m1[r];
+  }
+  __syncthreads();
+  if (tid < 256) {
+    const int a = tid >> 4, c = tid & 15;
+    if (a < bw && c < bw) {
+      const double gx = fast ? sum4(Gpart, tid) + sum4(Gpart + 1024, tid) : sum4(Gpart + 1024, tid);
+      G[a * GP + bw + c] = gx;
+      G[(bw + c) * GP + a] = gx;
+      if (!fast) {
+        G[a * GP + c] = sum4(Gpart, tid);
+        G[(bw + a) * GP + bw + c] = sum4(Gpart + 2048, tid);
+      }
+    }
+  }
+  __syncthreads();
+  if (!fast) {  // keep the freshly formed diagonal blocks
+    if (tid < 256) {
+      const int a = tid >> 4, c = tid & 15;
+      if (a < bw && c < bw) {
+        if (keepP) dgp[tid] = G[a * GP + c];
+        if (keepQ) dgq[tid] = G[(bw + a) * GP + bw + c];
+      }
+    }
+    if (tid == 0) {
+      if (keepP) dgv[task.bc_off + p] = 1;
+      if (keepQ) dgv[task.bc_off + q] = 1;
+    }
+  }
+  {
+    // a task whose columns are already orthogonal to the tolerance has nothing to rotate (the zero
+    // columns between p and q have zero entries)
+    const int x = tid >> 5, y = tid & 31;
+    bool need = false;
+    if (x < y && y < wl) {
+      const double ga = G[x * GP + y];
+      need = ga * ga > ((double)tol * (double)tol) * (G[x * GP + x] * G[y * GP + y]);
+    }
+    if (!__syncthreads_or(need)) return;
+  }
+  // ---- cyclic Jacobi on G.  Column x of slot (u, v), u < v, moves as P'(:,u) = c P(:,u) - s P(:,v),
+  // P'(:,v) = s P(:,u) + c P(:,v), so G' = J^T G J and V' = V J.  Every thread that needs a slot's
+  // rotation computes it from the same three entries of the current G (so all copies agree exactly).
+  // One barrier per step.
   const double tol2 = (double)tol * (double)tol;
   // rotation of slot (u, v) (read indices) from the current G: identity for a lone column (its
   // partner is the zero dummy, ga = 0) or a pair below tolerance.  Branch-free: a zero or tiny ga
@@ -292,25 +304,6 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
     cs = go ? z : 1.0;
     sn = go ? z * t : 0.0;
   };
-  // slot table of the schedule (one period: steps_per x 16): read and write indices of the pair,
-  // packed ru | rv << 8 | wu << 16 | wv << 24 (a missing member reads JAC_DUMMY, writes JAC_SINK)
-  for (int e = tid; e < steps_per * 16; e += JAC_THREADS) {
-    int u, v;
-    slot(e >> 4, e & 15, u, v);
-    const int ru = u >= 0 ? u : JAC_DUMMY, rv = v >= 0 ? v : JAC_DUMMY;
-    const int wu = u >= 0 ? u : JAC_SINK, wv2 = v >= 0 ? v : JAC_SINK;
-    s_slots[e] = ru | (rv << 8) | (wu << 16) | (wv2 << 24);
-  }
-  // V = I; the dummy row and column zero in V and in both G buffers (G2 held partial blocks)
-  for (int e = tid; e < (JAC_W + 2) * (JAC_W + 2); e += JAC_THREADS) {
-    const int x = e / (JAC_W + 2), y = e - x * (JAC_W + 2);
-    Vd[x * GP + y] = x == y && x < JAC_W ? 1.0 : 0.0;
-    if (x == JAC_DUMMY || y == JAC_DUMMY) {
-      G[x * GP + y] = 0.0;
-      G2[x * GP + y] = 0.0;
-    }
-  }
-  __syncthreads();
   // thread (a, c) < 256 rotates the entries (rows of slot a) x (columns of slot c) of G into the
   // other buffer, and the same entries of V in place (V' = V J needs only slot c's rotation); the
   // slots partition the columns, so every entry is written once.  Lane l computes the rotation of
@@ -398,7 +391,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
           }
       }
     }
-    if (tid == 0) flags[3 * b + sweep % 3] = 1;
+    if (tid == 0) flags[b] = 1;
     return;
   }
   // larger panels: one row per thread from global memory (every value of the row is read before it
@@ -415,7 +408,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
       Wb[(long long)gcol(c) * n + r] = s;
     }
   }
-  if (tid == 0) flags[3 * b + sweep % 3] = 1;
+  if (tid == 0) flags[b] = 1;
 }
 
 // Column norms of the converged W give lambda + sigma; columns are normalised, sorted by lambda
@@ -1111,11 +1104,11 @@ size_t final_sort_lds(int nmax) {
   return (size_t)np2 * 12 + 64;
 }
 
-void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int sweep, int round, int* flags, double* dgram,
-                         int* dgv, float tol, float ang_tol, int max_inner, hipStream_t s) {
+void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int round, int* flags, double* dgram, int* dgv,
+                         float tol, float ang_tol, int max_inner, hipStream_t s) {
   if (n_tasks <= 0) return;
   hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CS * sizeof(float), s, tasks, W,
-                     sweep, round, flags, dgram, dgv, tol, ang_tol, max_inner);
+                     round, flags, dgram, dgv, tol, ang_tol, max_inner);
   NRGK_HIP(hipGetLastError());
 }
 void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,

@@ -11,8 +11,8 @@ void launch_assemble(const AsmSector* sectors, int n_sectors, int n_rows, const 
 void launch_shift_convert(const EigBlock* blocks, const int* row_block, const int* row_local, int n_rows,
                           const double* H, unsigned int* sigma_bits, float* sigma_out, float* W, hipStream_t s);
 // k_eig.hip
-void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int sweep, int round,
-                         int* flags, double* dgram, int* dgv, float tol, float ang_tol, int max_inner, hipStream_t s);
+void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int round, int* flags, double* dgram, int* dgv,
+                         float tol, float ang_tol, int max_inner, hipStream_t s);
 void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
                            size_t lds, hipStream_t s);
 void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym, const int4* all_tiles, int n_all,
