@@ -422,6 +422,7 @@ int nrgk_chain_refine_stats(nrgk_chain* ch, int* oa_steps, int* unconverged) {
 int nrgk_chain_profile(nrgk_chain* ch, int enable) {
   return guard([&] {
     NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    ch->c->sync();
     ch->c->prof_on = enable != 0;
     if (enable) ch->c->kstats.clear();
   });
@@ -429,6 +430,7 @@ int nrgk_chain_profile(nrgk_chain* ch, int enable) {
 int nrgk_chain_kernel_stats(nrgk_chain* ch, nrgk_kernel_stat* out, int cap, int* n_out) {
   return guard([&] {
     NRGK_REQUIRE(ch && n_out && (out || cap == 0), NRGK_ERR_INVALID, "null argument");
+    ch->c->sync();  // runs still in flight are resolved
     int i = 0;
     for (auto& [name, k] : ch->c->kstats) {
       if (i < cap) {

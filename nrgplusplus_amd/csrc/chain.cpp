@@ -90,34 +90,67 @@ Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, 
                     &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act,
                     &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv})
     b->ctx = ctx;
+  stg.ctx = ctx;
   NRGK_HIP(hipSetDevice(ctx->device));
   NRGK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   for (auto& e : ev) NRGK_HIP(hipEventCreate(&e));
-  for (auto& e : pev) NRGK_HIP(hipEventCreate(&e));
   upload_bath();
 }
 
+void Chain::sync() {
+  NRGK_HIP(hipStreamSynchronize(st));
+  stg.reset();
+  prof_resolve();
+}
+
+hipEvent_t Chain::take_event() {
+  if (ev_pool.empty()) {
+    hipEvent_t e;
+    NRGK_HIP(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = ev_pool.back();
+  ev_pool.pop_back();
+  return e;
+}
 void Chain::prof_begin() {
-  if (prof_on) NRGK_HIP(hipEventRecord(pev[0], st));
+  if (!prof_on) return;
+  prof_open = take_event();
+  NRGK_HIP(hipEventRecord(prof_open, st));
 }
 void Chain::prof_end(const char* family, long long launches, double flops, double bytes) {
-  if (!prof_on) return;
-  NRGK_HIP(hipEventRecord(pev[1], st));
-  NRGK_HIP(hipEventSynchronize(pev[1]));
-  float t = 0;
-  NRGK_HIP(hipEventElapsedTime(&t, pev[0], pev[1]));
-  KStat& k = kstats[family];
-  k.ms += t;
-  k.launches += launches;
-  k.flops += flops;
-  k.bytes += bytes;
+  if (!prof_on || !prof_open) return;
+  hipEvent_t e1 = take_event();
+  NRGK_HIP(hipEventRecord(e1, st));
+  prof_pending.push_back(ProfRun{family, prof_open, e1, launches, flops, bytes});
+  prof_open = nullptr;
+}
+void Chain::prof_resolve() {
+  for (auto& r : prof_pending) {
+    float t = 0;
+    NRGK_HIP(hipEventSynchronize(r.e1));
+    NRGK_HIP(hipEventElapsedTime(&t, r.e0, r.e1));
+    KStat& k = kstats[r.family];
+    k.ms += t;
+    k.launches += r.launches;
+    k.flops += r.flops;
+    k.bytes += r.bytes;
+    ev_pool.push_back(r.e0);
+    ev_pool.push_back(r.e1);
+  }
+  prof_pending.clear();
 }
 
 Chain::~Chain() {
   if (st) (void)hipStreamSynchronize(st);
   plain_destroy(*this);
   for (auto& e : ev) (void)hipEventDestroy(e);
-  for (auto& e : pev) (void)hipEventDestroy(e);
+  for (auto& r : prof_pending) {
+    ev_pool.push_back(r.e0);
+    ev_pool.push_back(r.e1);
+  }
+  if (prof_open) ev_pool.push_back(prof_open);
+  for (auto& e : ev_pool) (void)hipEventDestroy(e);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -144,7 +177,7 @@ void Chain::upload_bath() {
       f.insert(f.end(), blk.begin(), blk.end());
     }
   upload(d_bathF, f, st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  sync();
 }
 
 // First add_bath_site: impurity data and discard_higher_energies on it (nrgcore.hpp:72-78).
@@ -288,20 +321,12 @@ void Chain::assemble(const std::vector<double>& t, double rescale) {
     sec.con_end = (int)cons.size();
     secs.push_back(sec);
   }
-  // one upload: sectors | subs | cons
-  const size_t b0 = secs.size() * sizeof(AsmSector), b1 = subs.size() * sizeof(AsmSub), b2 = cons.size() * sizeof(AsmCon);
-  const size_t o1 = (b0 + 255) & ~size_t(255), o2 = (o1 + b1 + 255) & ~size_t(255);
-  d_asm.ensure(o2 + b2 + 256);
-  char* base = d_asm.as<char>();
-  if (b0) NRGK_HIP(hipMemcpyAsync(base, secs.data(), b0, hipMemcpyHostToDevice, st));
-  if (b1) NRGK_HIP(hipMemcpyAsync(base + o1, subs.data(), b1, hipMemcpyHostToDevice, st));
-  if (b2) NRGK_HIP(hipMemcpyAsync(base + o2, cons.data(), b2, hipMemcpyHostToDevice, st));
+  const AsmSector* d_secs = stg.put(secs, st);
+  const AsmSub* d_subs = stg.put(subs, st);
+  const AsmCon* d_cons = stg.put(cons, st);
   dH.ensure(std::max<long long>(1, tot_elems) * sizeof(double));
-  launch_assemble((const AsmSector*)base, (int)secs.size(), row0, (const AsmSub*)(base + o1),
-                  (const AsmCon*)(base + o2), d_eprev, d_bathE.as<double>(), fbuf[fcur].as<double>(),
-                  d_bathF.as<double>(), rescale, dH.as<double>(), st);
-  // the host vectors must stay alive until the copies have been consumed
-  NRGK_HIP(hipStreamSynchronize(st));
+  launch_assemble(d_secs, (int)secs.size(), row0, d_subs, d_cons, d_eprev, d_bathE.as<double>(),
+                  fbuf[fcur].as<double>(), d_bathF.as<double>(), rescale, dH.as<double>(), st);
 }
 
 void Chain::run_gemms(std::vector<GemmProb>& probs) {
@@ -323,8 +348,7 @@ void Chain::run_gemms(std::vector<GemmProb>& probs) {
   }
   if (live.empty()) return;
   const int tiles = prepare_gemm(live);
-  d_gemm.ensure(live.size() * sizeof(GemmProb));
-  h2d(d_gemm.as<GemmProb>(), live.data(), live.size(), st);
+  const GemmProb* dprobs = stg.put(live, st);
   double fl = 0, by = 0;
   if (prof_on)
     for (auto& p : live) {
@@ -334,9 +358,8 @@ void Chain::run_gemms(std::vector<GemmProb>& probs) {
       by += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (p.C0 ? 2.0 : 1.0) * outs);
     }
   prof_begin();
-  launch_gemm(d_gemm.as<GemmProb>(), (int)live.size(), tiles, st);
+  launch_gemm(dprobs, (int)live.size(), tiles, st);
   prof_end("gemm_f64", 1, fl, by);
-  NRGK_HIP(hipStreamSynchronize(st));  // keeps the descriptor buffer reusable by the next batch
 }
 
 // tuning knobs read from the environment (defaults are the tuned values)
@@ -381,13 +404,12 @@ void Chain::eigensolve_mixed() {
   }
   const int nb = (int)blks.size();
   const int nrows = (int)row_block.size();
-  d_eig.ensure(nb * sizeof(EigBlock) + 256);
-  h2d(d_eig.as<EigBlock>(), blks.data(), nb, st);
-  const size_t o_rb = 0, o_rl = ((nrows * 4 + 255) / 256) * 256, o_tk = o_rl + ((nrows * 4 + 255) / 256) * 256;
-  d_misc.ensure(o_tk + tasks.size() * sizeof(JacTask) + 256);
+  upload_to(d_eig, blks.data(), nb);
+  const size_t o_rb = 0, o_rl = ((nrows * 4 + 255) / 256) * 256;
+  d_misc.ensure(o_rl + std::max(1, nrows) * sizeof(int));
+  upload_to(d_misc, row_block.data(), nrows, o_rb);
+  upload_to(d_misc, row_local.data(), nrows, o_rl);
   char* mb = d_misc.as<char>();
-  h2d((int*)(mb + o_rb), row_block.data(), nrows, st);
-  h2d((int*)(mb + o_rl), row_local.data(), nrows, st);
   const EigBlock* dblk = d_eig.as<EigBlock>();
 
   const size_t bytes64 = std::max<long long>(1, tot_elems) * sizeof(double);
@@ -400,7 +422,6 @@ void Chain::eigensolve_mixed() {
   // Jacobi sweeps.  The tasks of the blocks still rotating are listed per sweep (a block that a sweep
   // leaves unrotated has converged), blocks with the longest tournaments first, so that round r
   // launches the prefix of tasks whose block has a round r.
-  d_flags.ensure(std::max(1, nb) * sizeof(int));
   const int max_sweeps = 20;
   // Gram blocks of the block columns, carried between rounds (invalid until first formed)
   d_dgram.ensure(std::max(1, n_bc) * 256 * sizeof(double));
@@ -434,19 +455,19 @@ void Chain::eigensolve_mixed() {
         by += rounds * pairs * 3.0 * n * w * 4.0;  // read twice, write once (FP32)
       }
     }
-    JacTask* dt = (JacTask*)(mb + o_tk);
-    h2d(dt, sweep_tasks.data(), sweep_tasks.size(), st);
-    NRGK_HIP(hipMemsetAsync(d_flags.p, 0, std::max(1, nb) * sizeof(int), st));
+    const JacTask* dt = stg.put(sweep_tasks, st);
+    int* dflags = stg.put_zero<int>(nb, st);
     prof_begin();
     size_t cnt = sweep_tasks.size();
     for (int r = 0; r < nbc_act - 1; r++) {
       while (cnt > 0 && sweep_tasks[cnt - 1].nbc - 1 <= r) cnt--;  // blocks whose tournament has ended
-      launch_jacobi_round(dt, (int)cnt, dW.as<float>(), r, d_flags.as<int>(), d_dgram.as<double>(), d_dgv.as<int>(),
-                          jac_tol, jac_ang, jac_inner, st);
+      launch_jacobi_round(dt, (int)cnt, dW.as<float>(), r, dflags, d_dgram.as<double>(), d_dgv.as<int>(), jac_tol,
+                          jac_ang, jac_inner, st);
     }
     prof_end("jacobi_round", nbc_act - 1, fl, by);
-    d2h(hflags.data(), d_flags.as<int>(), std::max(1, nb), st);
-    NRGK_HIP(hipStreamSynchronize(st));
+    const int* hf = stg.get(dflags, nb, st);
+    sync();
+    std::copy(hf, hf + nb, hflags.begin());
     sweeps = sw + 1;
     bool any = false;
     for (int bi = 0; bi < nb; bi++) any |= hflags[bi] != 0;
@@ -528,11 +549,9 @@ void Chain::eigensolve_mixed() {
   const double e_tol = knob("NRGK_OA_ETOL", 1.0), d_tol = knob("NRGK_OA_DTOL", 1e-12);
   const int dbg_oa = (int)knob("NRGK_DEBUG_OA", 0);
   std::vector<EigBlock> act = blks;
-  d_conv.ensure(std::max(1, nb) * 3 * sizeof(double));
   oa_steps = 0;
   d_cid.ensure(std::max<long long>(1, N_tot) * sizeof(int));
   d_sidx.ensure(std::max<long long>(1, N_tot) * sizeof(int));
-  d_offmax.ensure(std::max(1, nb) * sizeof(double));
   for (int it = 0; it < oa_max && !act.empty(); it++) {
     const int na = (int)act.size();
     // 32 x 32 tiles of the active blocks: all of them, and the (i <= j) half for the symmetrisation
@@ -545,17 +564,11 @@ void Chain::eigensolve_mixed() {
           if (ti <= tj) tiles_sym.push_back(int4{a, ti, tj, 0});
         }
     }
-    const size_t blk_bytes = (na * sizeof(EigBlock) + 15) & ~size_t(15);  // int4 tiles start 16-byte aligned
-    const size_t meta = blk_bytes + (tiles_all.size() + tiles_sym.size()) * sizeof(int4);
-    d_eig_act.ensure(meta);
-    EigBlock* m_blk = d_eig_act.as<EigBlock>();
-    int4* m_all = (int4*)((char*)d_eig_act.p + blk_bytes);
-    int4* m_sym = m_all + tiles_all.size();
-    h2d(m_blk, act.data(), na, st);
-    h2d(m_all, tiles_all.data(), tiles_all.size(), st);
-    h2d(m_sym, tiles_sym.data(), tiles_sym.size(), st);
-    NRGK_HIP(hipMemsetAsync(d_conv.p, 0, na * 3 * sizeof(double), st));
-    NRGK_HIP(hipMemsetAsync(d_offmax.p, 0, na * sizeof(double), st));
+    const EigBlock* m_blk = stg.put(act, st);
+    const int4* m_all = stg.put(tiles_all, st);
+    const int4* m_sym = stg.put(tiles_sym, st);
+    double* m_conv = stg.put_zero<double>(3 * na, st);
+    double* m_offmax = stg.put_zero<double>(na, st);
     std::vector<GemmProb> g1, g2, g3;
     for (auto& b : act) {
       const int n = b.n;
@@ -583,15 +596,16 @@ void Chain::eigensolve_mixed() {
       for (auto& b : act) oa_bytes += 8.0 * 9.0 * (double)b.n * b.n;
     prof_begin();
     launch_oa(m_blk, na, m_sym, (int)tiles_sym.size(), m_all, (int)tiles_all.size(), dS.as<double>(), dR.as<double>(),
-              dE.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_sidx.as<int>(), d_offmax.as<double>(),
-              d_stats.as<int>(), d_conv.as<double>(), dP.as<double>(), dX2.as<double>(), dQ.as<double>(),
+              dE.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_sidx.as<int>(), m_offmax,
+              d_stats.as<int>(), m_conv, dP.as<double>(), dX2.as<double>(), dQ.as<double>(),
               d_rrv.as<double>(), nmax, e_tol, d_tol, st);
     prof_end("oa_update", 5, 0.0, oa_bytes);
     check("oa_update (E)", dE, false, false);
     check("oa_update (lam)", dlam_tmp, false, true);
-    std::vector<double> conv(3 * na);
-    d2h(conv.data(), d_conv.as<double>(), conv.size(), st);
-    run_gemms(g3);  // synchronises the stream
+    const double* h_conv = stg.get(m_conv, 3 * na, st);
+    run_gemms(g3);
+    sync();
+    const std::vector<double> conv(h_conv, h_conv + 3 * na);
     std::swap(dX, dX2);
     check("X+XE", dX, false, false);
     oa_steps = it + 1;
@@ -606,15 +620,14 @@ void Chain::eigensolve_mixed() {
     }
     // the converged blocks' vectors into both iterate buffers (the device applies the same test)
     if (next.size() < act.size())
-      launch_copy_converged(m_blk, na, d_conv.as<double>(), e_tol, d_tol, dX.as<double>(), dX2.as<double>(), st);
+      launch_copy_converged(m_blk, na, m_conv, e_tol, d_tol, dX.as<double>(), dX2.as<double>(), st);
     act.swap(next);
   }
   oa_unconverged = (int)act.size();
   launch_final_sort(dblk, nb, dX.as<double>(), dlam_tmp.as<double>(), dU.as<double>(), dlam_raw.as<double>(),
                     final_sort_lds(nmax), st);
-  int hs[4];
-  d2h(hs, d_stats.as<int>(), 4, st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  const int* hs = stg.get(d_stats.as<int>(), 4, st);
+  sync();
   maxcluster = hs[0];
   unresolved = hs[1];
 }
@@ -636,12 +649,12 @@ void Chain::add_bath_site(const double* t, int nt, double rescale) {
       h_keep[s].resize((size_t)cur_n[s] * cur_n[s]);
       d2h(h_keep[s].data(), dH.as<double>() + cur_off[s], h_keep[s].size(), st);
     }
-    NRGK_HIP(hipStreamSynchronize(st));
+    sync();
   }
   if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
   else eigensolve_mixed();
   NRGK_HIP(hipEventRecord(ev[2], st));
-  NRGK_HIP(hipStreamSynchronize(st));
+  sync();
   NRGK_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
   NRGK_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
   phase = ASSEMBLED;
@@ -652,24 +665,21 @@ void Chain::update_internal_state() {
   NRGK_REQUIRE(phase == ASSEMBLED, NRGK_ERR_STATE, "nrgk: update_internal_state without add_bath_site");
   cnt++;
   const int S = (int)cur_qn.size();
-  std::vector<int> bn(cur_n), beo(cur_eoff);
-  const size_t o1 = ((S * 4 + 255) / 256) * 256;
-  d_trunc.ensure(2 * o1 + sizeof(TruncOut) + 256);
-  char* tb = d_trunc.as<char>();
-  h2d((int*)tb, bn.data(), S, st);
-  h2d((int*)(tb + o1), beo.data(), S, st);
-  TruncOut* dout = (TruncOut*)(tb + 2 * o1);
+  const int* d_bn = stg.put(cur_n, st);
+  const int* d_beo = stg.put(cur_eoff, st);
+  d_trunc.ensure(sizeof(TruncOut) + 256);
+  TruncOut* dout = (TruncOut*)d_trunc.p;
   d_kept.ensure(std::max(1, S) * sizeof(int));
   const int lnext = lcur ^ 1;
   dlam_shift[lnext].ensure(std::max<long long>(1, N_tot) * sizeof(double));
   NRGK_HIP(hipEventRecord(ev[3], st));
-  launch_truncate((const int*)tb, (const int*)(tb + o1), S, dlam_raw.as<double>(), N_tot, (long long)K, errbar,
-                  d_kept.as<int>(), dout, dlam_shift[lnext].as<double>(), st);
-  TruncOut out{};
-  kept.assign(S, 0);
-  d2h(kept.data(), d_kept.as<int>(), S, st);
-  d2h(&out, dout, 1, st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  launch_truncate(d_bn, d_beo, S, dlam_raw.as<double>(), N_tot, (long long)K, errbar, d_kept.as<int>(), dout,
+                  dlam_shift[lnext].as<double>(), st);
+  const int* h_kept = stg.get(d_kept.as<int>(), S, st);
+  const TruncOut* h_out = stg.get(dout, 1, st);
+  sync();
+  kept.assign(h_kept, h_kept + S);
+  const TruncOut out = *h_out;
   NRGK_HIP(hipEventRecord(ev[4], st));
   all_eig.clear();  // sorted lazily from dlam_raw
   no_kept = (size_t)out.n_kept;
@@ -677,7 +687,7 @@ void Chain::update_internal_state() {
   else relgs.push_back(out.e_min);
   rotate_fdag();
   NRGK_HIP(hipEventRecord(ev[5], st));
-  NRGK_HIP(hipStreamSynchronize(st));
+  sync();
   NRGK_HIP(hipEventElapsedTime(&ms[2], ev[3], ev[4]));
   NRGK_HIP(hipEventElapsedTime(&ms[3], ev[4], ev[5]));
   pre_qn = cur_qn;
@@ -758,16 +768,11 @@ void Chain::rotate_fdag() {
     g.B = dY.as<double>() + (intptr_t)g.B;
     g.C = fbuf[nf].as<double>() + (intptr_t)g.C;
   }
-  const size_t b0 = groups.size() * sizeof(RotGroup), b1 = segs.size() * sizeof(RotSegment),
-               b2 = tris.size() * sizeof(RotTri);
-  const size_t o1 = (b0 + 255) & ~size_t(255), o2 = (o1 + b1 + 255) & ~size_t(255);
-  d_rot.ensure(o2 + b2 + 256);
-  char* rb = d_rot.as<char>();
-  if (b0) NRGK_HIP(hipMemcpyAsync(rb, groups.data(), b0, hipMemcpyHostToDevice, st));
-  if (b1) NRGK_HIP(hipMemcpyAsync(rb + o1, segs.data(), b1, hipMemcpyHostToDevice, st));
-  if (b2) NRGK_HIP(hipMemcpyAsync(rb + o2, tris.data(), b2, hipMemcpyHostToDevice, st));
-  launch_fdag_gather((const RotGroup*)rb, (int)groups.size(), (const RotSegment*)(rb + o1), (const RotTri*)(rb + o2),
-                     dU.as<double>(), d_bathF.as<double>(), dY.as<double>(), st);
+  const RotGroup* d_groups = stg.put(groups, st);
+  const RotSegment* d_segs = stg.put(segs, st);
+  const RotTri* d_tris = stg.put(tris, st);
+  launch_fdag_gather(d_groups, (int)groups.size(), d_segs, d_tris, dU.as<double>(), d_bathF.as<double>(),
+                     dY.as<double>(), st);
   run_gemms(gp);
   fprev = std::move(fnew);
   fcur = nf;
@@ -777,7 +782,7 @@ void Chain::ensure_all_eig() {
   if (!all_eig.empty() || N_tot == 0 || phase == FRESH) return;
   std::vector<double> v(N_tot);
   d2h(v.data(), dlam_raw.as<double>(), N_tot, st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  sync();
   std::sort(v.begin(), v.end());
   all_eig = std::move(v);
 }
@@ -805,7 +810,7 @@ void Chain::set_state(const nrgk_site_desc* d, int cnt_) {
       f.insert(f.end(), blk.begin(), blk.end());
     }
   upload(fbuf[fcur], f, st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  sync();
   cnt = cnt_;
   phase = TRUNCATED;
   h_last_shift = flat;

@@ -96,11 +96,37 @@ struct Chain {
     double ms = 0, flops = 0, bytes = 0;
     long long launches = 0;
   };
+  // small uploads and readbacks of the chain's stream go through page-locked staging; sync()
+  // drains the stream and recycles it
+  Staging stg;
+  void sync();
+  // host data to a persistent device buffer, without blocking the host (staged, then copied)
+  template <class T>
+  T* upload_to(DevBuf& buf, const T* src, size_t n, size_t byte_off = 0) {
+    buf.ensure(byte_off + std::max<size_t>(1, n) * sizeof(T));
+    T* dst = reinterpret_cast<T*>(buf.as<char>() + byte_off);
+    if (n) {
+      const T* d = stg.put(src, n, st);
+      NRGK_HIP(hipMemcpyAsync(dst, d, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
+    return dst;
+  }
   bool prof_on = false;
-  hipEvent_t pev[2];
   std::map<std::string, KStat> kstats;
+  // event pairs recorded around each run, resolved at the next sync() (recording never blocks)
+  struct ProfRun {
+    const char* family;
+    hipEvent_t e0, e1;
+    long long launches;
+    double flops, bytes;
+  };
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<ProfRun> prof_pending;
+  hipEvent_t prof_open = nullptr;
+  hipEvent_t take_event();
   void prof_begin();
   void prof_end(const char* family, long long launches, double flops, double bytes);
+  void prof_resolve();
 
   Chain(Ctx* c, const nrgk_site_desc* imp, const nrgk_site_desc* bath, size_t K, int mode);
   ~Chain();

@@ -5,6 +5,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
+#include <algorithm>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -84,6 +86,96 @@ struct Ctx {
     used += bytes;
   }
   void refund(size_t bytes) { used -= bytes < used ? bytes : used; }
+};
+
+// Page-locked staging for one stream's small transfers.  put() copies host data into pinned memory
+// and enqueues one asynchronous copy to its device twin, returning the device address; get()
+// enqueues a device-to-host copy into pinned memory and returns the host address, readable once the
+// stream is synchronised.  Regions are handed out front to back and stay valid until reset(), which
+// the owner calls after synchronising the stream.  Running out of room opens a further chunk (no
+// region moves while in use); reset() merges the chunks into one.
+struct Staging {
+  struct Chunk {
+    char* host = nullptr;
+    char* dev = nullptr;
+    size_t cap = 0, used = 0;
+  };
+  std::vector<Chunk> chunks;
+  Ctx* ctx = nullptr;
+  Staging() = default;
+  Staging(const Staging&) = delete;
+  Staging& operator=(const Staging&) = delete;
+  ~Staging() { release(); }
+  void release() {
+    for (auto& c : chunks) {
+      if (c.host) (void)hipHostFree(c.host);
+      if (c.dev) {
+        (void)hipFree(c.dev);
+        if (ctx) ctx->refund(c.cap);
+      }
+    }
+    chunks.clear();
+  }
+  void add_chunk(size_t cap) {
+    Chunk c;
+    c.cap = cap;
+    if (ctx) ctx->charge(cap);
+    hipError_t e = hipHostMalloc((void**)&c.host, cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&c.dev, cap);
+    if (e != hipSuccess) {
+      if (c.host) (void)hipHostFree(c.host);
+      if (ctx) ctx->refund(cap);
+      throw Error(NRGK_ERR_NOMEM, std::string("nrgk: staging allocation (") + std::to_string(cap) + "): " +
+                                      hipGetErrorString(e));
+    }
+    chunks.push_back(c);
+  }
+  // an aligned region of `bytes`: (host, device) addresses
+  std::pair<char*, char*> take(size_t bytes) {
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (chunks.empty() || chunks.back().used + need > chunks.back().cap)
+      add_chunk(std::max<size_t>({need, chunks.empty() ? 0 : 2 * chunks.back().cap, size_t(1) << 20}));
+    Chunk& c = chunks.back();
+    const size_t o = c.used;
+    c.used += need;
+    return {c.host + o, c.dev + o};
+  }
+  template <class T>
+  T* put(const T* src, size_t n, hipStream_t s) {
+    auto [h, d] = take(std::max<size_t>(1, n) * sizeof(T));
+    if (n) {
+      std::memcpy(h, src, n * sizeof(T));
+      NRGK_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    return reinterpret_cast<T*>(d);
+  }
+  template <class T>
+  T* put(const std::vector<T>& v, hipStream_t s) {
+    return put(v.data(), v.size(), s);
+  }
+  template <class T>
+  T* put_zero(size_t n, hipStream_t s) {
+    auto [h, d] = take(std::max<size_t>(1, n) * sizeof(T));
+    std::memset(h, 0, std::max<size_t>(1, n) * sizeof(T));
+    NRGK_HIP(hipMemcpyAsync(d, h, std::max<size_t>(1, n) * sizeof(T), hipMemcpyHostToDevice, s));
+    return reinterpret_cast<T*>(d);
+  }
+  template <class T>
+  const T* get(const T* dsrc, size_t n, hipStream_t s) {
+    auto [h, d] = take(std::max<size_t>(1, n) * sizeof(T));
+    (void)d;
+    if (n) NRGK_HIP(hipMemcpyAsync(h, dsrc, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    return reinterpret_cast<const T*>(h);
+  }
+  void reset() {
+    if (chunks.size() > 1) {
+      size_t total = 0;
+      for (auto& c : chunks) total += c.cap;
+      release();
+      add_chunk(total);
+    }
+    if (!chunks.empty()) chunks.back().used = 0;
+  }
 };
 
 template <class T>

@@ -90,9 +90,8 @@ void Obs::update() {
   dC.ensure(std::max<long long>(1, coff) * sizeof(double));
   dY.ensure(std::max<long long>(1, coff) * sizeof(double));
   NRGK_HIP(hipMemsetAsync(dC.p, 0, std::max<long long>(1, coff) * sizeof(double), c.st));
-  dseg.ensure(std::max<size_t>(1, segs.size()) * sizeof(ObsSeg));
-  h2d(dseg.as<ObsSeg>(), segs.data(), segs.size(), c.st);
-  launch_obs_scatter(dseg.as<ObsSeg>(), (int)segs.size(), buf[cur].as<double>(), dC.as<double>(), c.st);
+  const ObsSeg* d_segs = c.stg.put(segs, c.st);
+  launch_obs_scatter(d_segs, (int)segs.size(), buf[cur].as<double>(), dC.as<double>(), c.st);
   for (auto& pr : pairs) {
     const int ni = c.cur_n[pr.i], nj = c.cur_n[pr.j];
     double* Cm = dC.as<double>() + pr.c_off;
@@ -106,6 +105,7 @@ void Obs::update() {
   }
   c.run_gemms(g1);
   c.run_gemms(g2);
+  c.sync();
   blocks = std::move(nbk);
   cur = nbuf;
 }
@@ -142,8 +142,7 @@ void Fdm::record() {
   sh->coupled = c.coupled;
   sh->sub_kidx = c.sub_kidx;
   sh->pre_kept = c.pre_kept;
-  sh->E.resize(c.N_tot);
-  d2h(sh->E.data(), c.dlam_raw.as<double>(), c.N_tot, c.st);
+  const double* h_e = c.stg.get(c.dlam_raw.as<double>(), c.N_tot, c.st);
   if (obs) {
     sh->dU.p = arena_alloc(std::max<long long>(1, c.tot_elems) * sizeof(double));
     NRGK_HIP(hipMemcpyAsync(sh->dU.p, c.dU.p, c.tot_elems * sizeof(double), hipMemcpyDeviceToDevice, c.st));
@@ -156,7 +155,8 @@ void Fdm::record() {
     NRGK_HIP(hipMemcpyAsync(sh->dO.p, obs->buf[obs->cur].p, tot * sizeof(double), hipMemcpyDeviceToDevice, c.st));
     sh->ops = obs->blocks;
   }
-  NRGK_HIP(hipStreamSynchronize(c.st));
+  c.sync();
+  sh->E.assign(h_e, h_e + c.N_tot);
   shells.push_back(std::move(sh));
 }
 
