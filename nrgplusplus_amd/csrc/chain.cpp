@@ -432,6 +432,9 @@ void Chain::eigensolve_mixed() {
   const float jac_tol = knob("NRGK_JAC_TOL", 5e-7);
   const float jac_ang = knob("NRGK_JAC_ANGLE", 1e-7);
   const int jac_inner = (int)knob("NRGK_JAC_INNER", 1);  // Jacobi sweeps over a task's Gram matrix
+  // stop a block after a sweep whose pairs all started near orthogonal (0: only after a sweep
+  // without rotations)
+  const bool jac_quad = knob("NRGK_JAC_QUAD", 1) != 0;
   std::vector<int> order(nb);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return blks[x].nbc > blks[y].nbc; });
@@ -456,7 +459,7 @@ void Chain::eigensolve_mixed() {
       }
     }
     const JacTask* dt = stg.put(sweep_tasks, st);
-    int* dflags = stg.put_zero<int>(nb, st);
+    int* dflags = stg.put_zero<int>(2 * nb, st);
     prof_begin();
     size_t cnt = sweep_tasks.size();
     for (int r = 0; r < nbc_act - 1; r++) {
@@ -465,12 +468,15 @@ void Chain::eigensolve_mixed() {
                           jac_ang, jac_inner, st);
     }
     prof_end("jacobi_round", nbc_act - 1, fl, by);
-    const int* hf = stg.get(dflags, nb, st);
+    const int* hf = stg.get(dflags, 2 * nb, st);
     sync();
-    std::copy(hf, hf + nb, hflags.begin());
     sweeps = sw + 1;
+    // a block goes on while it rotated and some pair was still far from orthogonal
     bool any = false;
-    for (int bi = 0; bi < nb; bi++) any |= hflags[bi] != 0;
+    for (int bi = 0; bi < nb; bi++) {
+      hflags[bi] = hflags[bi] && hf[2 * bi] && (jac_quad ? hf[2 * bi + 1] : 1);
+      any |= hflags[bi] != 0;
+    }
     if (!any) {
       converged = true;
       break;

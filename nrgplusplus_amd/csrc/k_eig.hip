@@ -52,8 +52,12 @@ __device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p,
   }
 }
 
-// One tournament round for the tasks of the blocks still rotating (the host drops a block once a
-// sweep leaves it unrotated).  flags[b] is set when block b rotates during the sweep.
+// One tournament round for the tasks of the blocks still rotating.  flags[2b] is set when block b
+// rotates during the sweep, flags[2b + 1] when some pair of it starts a round more than JAC_FAR from
+// orthogonal (cosine of the angle between the columns).  The host drops a block after a sweep that
+// leaves it unrotated, or whose pairs all started within JAC_FAR: Jacobi converges quadratically, so
+// that sweep has taken them to about JAC_FAR^2, below the rotation tolerance, and a further sweep
+// would only find that out.
 //
 // A task owns the pair (p, q) of block columns of W (column-major, length n).  Its local columns
 // are p's at 0..wp-1 and q's at bw..bw+wq-1 (bw <= 16; the columns between exist only beside the
@@ -70,6 +74,7 @@ constexpr int JAC_CS = JAC_CHUNK + 4;  // column stride of the chunk: 16 columns
 // pair, and a row and column JAC_SINK that take its writes, so that a rotation step has no branches
 constexpr int JAC_DUMMY = JAC_W, JAC_SINK = JAC_W + 1;
 constexpr int GP = JAC_W + 3;          // row stride of G and V (rows and columns 0..JAC_SINK)
+constexpr double JAC_FAR = 1e-4;
 constexpr int GSZ = (JAC_W + 2) * GP;
 
 // sum of the four partial 16 x 16 blocks at P, P + 256, P + 512, P + 768
@@ -274,11 +279,14 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
     // a task whose columns are already orthogonal to the tolerance has nothing to rotate (the zero
     // columns between p and q have zero entries)
     const int x = tid >> 5, y = tid & 31;
-    bool need = false;
+    bool need = false, far = false;
     if (x < y && y < wl) {
-      const double ga = G[x * GP + y];
-      need = ga * ga > ((double)tol * (double)tol) * (G[x * GP + x] * G[y * GP + y]);
+      const double ga = G[x * GP + y], ab = G[x * GP + x] * G[y * GP + y];
+      need = ga * ga > ((double)tol * (double)tol) * ab;
+      far = ga * ga > JAC_FAR * JAC_FAR * ab;
     }
+    // flags[2b + 1]: some pair of the block was at least JAC_FAR from orthogonal during the sweep
+    if (__syncthreads_or(far) && tid == 0) flags[2 * b + 1] = 1;
     if (!__syncthreads_or(need)) return;
   }
   // ---- cyclic Jacobi on G.  Column x of slot (u, v), u < v, moves as P'(:,u) = c P(:,u) - s P(:,v),
@@ -391,7 +399,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
           }
       }
     }
-    if (tid == 0) flags[b] = 1;
+    if (tid == 0) flags[2 * b] = 1;
     return;
   }
   // larger panels: one row per thread from global memory (every value of the row is read before it
@@ -408,7 +416,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
       Wb[(long long)gcol(c) * n + r] = s;
     }
   }
-  if (tid == 0) flags[b] = 1;
+  if (tid == 0) flags[2 * b] = 1;
 }
 
 // Column norms of the converged W give lambda + sigma; columns are normalised, sorted by lambda
