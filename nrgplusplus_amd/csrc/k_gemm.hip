@@ -7,14 +7,15 @@
 // grid (prefix sums in GemmProb::tile0), so one launch covers all uneven blocks of an iteration.
 // A problem marked sym (a product known to be symmetric) gets only its upper tiles; the caller
 // reads the upper triangle.
-// The 64x16 / 16x64 operand slabs are staged in LDS (double-buffered) and each of the 256 threads
-// keeps a 4x4 FP64 accumulator tile in registers.
+// The 64x32 / 32x64 operand slabs are staged in LDS (double-buffered); the products run on the FP64
+// matrix cores: each of the 4 waves owns a 32x32 quarter of the tile as 2x2 blocks of 16x16x4 steps.
 #include "common.hpp"
 
 namespace nrgk {
 
 constexpr int GT = 64;   // tile edge
-constexpr int GK = 16;   // k-slab
+constexpr int GK = 32;   // k-slab
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int find_prob(const GemmProb* p, int n, int tile) {
   int lo = 0, hi = n - 1;
@@ -26,32 +27,50 @@ __device__ __forceinline__ int find_prob(const GemmProb* p, int n, int tile) {
   return lo;
 }
 
-__device__ __forceinline__ void load_a(const GemmProb& P, int m0, int k0, double (*As)[GT + 1]) {
+// Operand slabs go global -> registers -> LDS: the registers of the next slab are fetched before
+// the current slab's products, so the global latency overlaps them.  Thread t holds elements
+// t + 256 l (l < GK/4) of the 64 x GK slab, walked along the contiguous direction of the operand.
+constexpr int GL = GT * GK / 256;
+__device__ __forceinline__ void fetch_a(const GemmProb& P, int m0, int k0, double* r) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int l = 0; l < 4; l++) {
+  for (int l = 0; l < GL; l++) {
     const int idx = tid + 256 * l;
     int mm, kk;
     if (!P.transA) { kk = idx % GK; mm = idx / GK; }
     else { mm = idx % GT; kk = idx / GT; }
     const int m = m0 + mm, k = k0 + kk;
-    double v = 0.0;
-    if (m < P.m && k < P.k) v = P.transA ? P.A[(long long)k * P.lda + m] : P.A[(long long)m * P.lda + k];
-    As[kk][mm] = v;
+    r[l] = (m < P.m && k < P.k) ? (P.transA ? P.A[(long long)k * P.lda + m] : P.A[(long long)m * P.lda + k]) : 0.0;
   }
 }
-__device__ __forceinline__ void load_b(const GemmProb& P, int n0, int k0, double (*Bs)[GT + 1]) {
+__device__ __forceinline__ void store_a(const GemmProb& P, const double* r, double (*As)[GT + 1]) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int l = 0; l < 4; l++) {
+  for (int l = 0; l < GL; l++) {
+    const int idx = tid + 256 * l;
+    if (!P.transA) As[idx % GK][idx / GK] = r[l];
+    else As[idx / GT][idx % GT] = r[l];
+  }
+}
+__device__ __forceinline__ void fetch_b(const GemmProb& P, int n0, int k0, double* r) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int l = 0; l < GL; l++) {
     const int idx = tid + 256 * l;
     int nn, kk;
     if (!P.transB) { nn = idx % GT; kk = idx / GT; }
     else { kk = idx % GK; nn = idx / GK; }
     const int nc = n0 + nn, k = k0 + kk;
-    double v = 0.0;
-    if (nc < P.n && k < P.k) v = P.transB ? P.B[(long long)nc * P.ldb + k] : P.B[(long long)k * P.ldb + nc];
-    Bs[kk][nn] = v;
+    r[l] = (nc < P.n && k < P.k) ? (P.transB ? P.B[(long long)nc * P.ldb + k] : P.B[(long long)k * P.ldb + nc]) : 0.0;
+  }
+}
+__device__ __forceinline__ void store_b(const GemmProb& P, const double* r, double (*Bs)[GT + 1]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int l = 0; l < GL; l++) {
+    const int idx = tid + 256 * l;
+    if (!P.transB) Bs[idx / GT][idx % GT] = r[l];
+    else Bs[idx % GK][idx / GK] = r[l];
   }
 }
 
@@ -77,51 +96,57 @@ __global__ void __launch_bounds__(256) k_gemm64(const GemmProb* __restrict__ pro
   const int n0 = tn * GT;
   __shared__ double As[2][GK][GT + 1];
   __shared__ double Bs[2][GK][GT + 1];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4];
+  // wave w: rows wm..wm+31, columns wn..wn+31 of the tile.  Lane l supplies A(m, k) for m = l%16 of
+  // a 16-row block and k = l/16 of a 4-step, B(k, n) likewise; entry (i, j) of a 16x16 block lands
+  // in lane 16 (i%4) + j, register i/4.
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  dbl4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int j = 0; j < 2; j++) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
   const int nk = (P.k + GK - 1) / GK;
+  double ra[GL], rb[GL];
   if (nk > 0) {
-    load_a(P, m0, 0, As[0]);
-    load_b(P, n0, 0, Bs[0]);
+    fetch_a(P, m0, 0, ra);
+    fetch_b(P, n0, 0, rb);
   }
-  __syncthreads();
   for (int ks = 0; ks < nk; ks++) {
     const int cur = ks & 1;
-    if (ks + 1 < nk) {
-      load_a(P, m0, (ks + 1) * GK, As[cur ^ 1]);
-      load_b(P, n0, (ks + 1) * GK, Bs[cur ^ 1]);
-    }
-#pragma unroll
-    for (int kk = 0; kk < GK; kk++) {
-      double a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) a[i] = As[cur][kk][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) b[j] = Bs[cur][kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-    }
+    // (the buffer written here was last read two slabs ago, before the previous barrier)
+    store_a(P, ra, As[cur]);
+    store_b(P, rb, Bs[cur]);
     __syncthreads();
-  }
+    if (ks + 1 < nk) {
+      fetch_a(P, m0, (ks + 1) * GK, ra);
+      fetch_b(P, n0, (ks + 1) * GK, rb);
+    }
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int m = m0 + ty + 16 * i;
-    if (m >= P.m) continue;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int nc = n0 + tx + 16 * j;
-      if (nc >= P.n) continue;
-      double v = P.alpha * acc[i][j];
-      if (P.beta != 0.0) v += P.beta * P.C0[(long long)m * P.ldc0 + nc];
-      P.C[(long long)m * P.ldc + nc] = v;
+    for (int kk = 0; kk < GK; kk += 4) {
+      const double a0 = As[cur][kk + lk][wm + li], a1 = As[cur][kk + lk][wm + 16 + li];
+      const double b0 = Bs[cur][kk + lk][wn + li], b1 = Bs[cur][kk + lk][wn + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
   }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int m = m0 + wm + 16 * bi + 4 * r + lk;
+      if (m >= P.m) continue;
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        const int nc = n0 + wn + 16 * bj + li;
+        if (nc >= P.n) continue;
+        double v = P.alpha * acc[bi][bj][r];
+        if (P.beta != 0.0) v += P.beta * P.C0[(long long)m * P.ldc0 + nc];
+        P.C[(long long)m * P.ldc + nc] = v;
+      }
+    }
 }
 
 // Host helper: fills tile0/tiles_n, uploads the problem list and launches one grid.
