@@ -435,6 +435,7 @@ void Chain::eigensolve_mixed() {
   // stop a block after a sweep whose pairs all started near orthogonal (0: only after a sweep
   // without rotations)
   const bool jac_quad = knob("NRGK_JAC_QUAD", 1) != 0;
+  const double jac_far = knob("NRGK_JAC_FAR", 1e-4);
   std::vector<int> order(nb);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return blks[x].nbc > blks[y].nbc; });
@@ -465,7 +466,7 @@ void Chain::eigensolve_mixed() {
     for (int r = 0; r < nbc_act - 1; r++) {
       while (cnt > 0 && sweep_tasks[cnt - 1].nbc - 1 <= r) cnt--;  // blocks whose tournament has ended
       launch_jacobi_round(dt, (int)cnt, dW.as<float>(), r, dflags, d_dgram.as<double>(), d_dgv.as<int>(), jac_tol,
-                          jac_ang, jac_inner, st);
+                          jac_ang, jac_inner, jac_far, st);
     }
     prof_end("jacobi_round", nbc_act - 1, fl, by);
     const int* hf = stg.get(dflags, 2 * nb, st);
@@ -494,7 +495,18 @@ void Chain::eigensolve_mixed() {
   d_rrv.ensure(std::max<long long>(1, N_tot) * 3 * sizeof(double));
   dlam_tmp.ensure(std::max<long long>(1, N_tot) * sizeof(double));
   dlam_raw.ensure(std::max<long long>(1, N_tot) * sizeof(double));
-  launch_normalize_sort(dblk, nb, dW.as<float>(), d_sigma.as<float>(), dX.as<double>(), dlam_tmp.as<double>(),
+  // 32 x 32 tiles of every block, for the gathers into X and U
+  std::vector<int4> tiles_blk;
+  for (int bi = 0; bi < nb; bi++) {
+    const int T = (blks[bi].n + 31) / 32;
+    for (int ti = 0; ti < T; ti++)
+      for (int tj = 0; tj < T; tj++) tiles_blk.push_back(int4{bi, ti, tj, 0});
+  }
+  const int4* d_tiles_blk = upload_to(d_eig_act, tiles_blk.data(), tiles_blk.size());
+  d_cid.ensure(std::max<long long>(1, N_tot) * sizeof(int));
+  d_sidx.ensure(std::max<long long>(1, N_tot) * sizeof(int));
+  launch_normalize_sort(dblk, nb, d_tiles_blk, (int)tiles_blk.size(), dW.as<float>(), d_sigma.as<float>(),
+                        dX.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_rrv.as<double>(),
                         normalize_sort_lds(nmax), st);
   // NRGK_DEBUG_FINITE=1: report the first stage and block that produces a non-finite value
   const bool dbg = knob("NRGK_DEBUG_FINITE", 0) != 0;
@@ -556,8 +568,6 @@ void Chain::eigensolve_mixed() {
   const int dbg_oa = (int)knob("NRGK_DEBUG_OA", 0);
   std::vector<EigBlock> act = blks;
   oa_steps = 0;
-  d_cid.ensure(std::max<long long>(1, N_tot) * sizeof(int));
-  d_sidx.ensure(std::max<long long>(1, N_tot) * sizeof(int));
   for (int it = 0; it < oa_max && !act.empty(); it++) {
     const int na = (int)act.size();
     // 32 x 32 tiles of the active blocks: all of them, and the (i <= j) half for the symmetrisation
@@ -630,8 +640,8 @@ void Chain::eigensolve_mixed() {
     act.swap(next);
   }
   oa_unconverged = (int)act.size();
-  launch_final_sort(dblk, nb, dX.as<double>(), dlam_tmp.as<double>(), dU.as<double>(), dlam_raw.as<double>(),
-                    final_sort_lds(nmax), st);
+  launch_final_sort(dblk, nb, d_tiles_blk, (int)tiles_blk.size(), dX.as<double>(), dlam_tmp.as<double>(),
+                    dU.as<double>(), dlam_raw.as<double>(), d_sidx.as<int>(), final_sort_lds(nmax), st);
   const int* hs = stg.get(d_stats.as<int>(), 4, st);
   sync();
   maxcluster = hs[0];

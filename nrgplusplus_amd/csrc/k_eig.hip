@@ -12,6 +12,7 @@
 //   cluster's k x k matrix), then eigenvalues sorted ascending with their columns, as dsyevd
 //   returns them (eigenvectors in the columns of U, qmatrix.hpp:390-391).
 #include "common.hpp"
+#include "kernels.hpp"
 
 namespace nrgk {
 
@@ -54,10 +55,10 @@ __device__ __forceinline__ void tournament_pair(int N, int round, int j, int& p,
 
 // One tournament round for the tasks of the blocks still rotating.  flags[2b] is set when block b
 // rotates during the sweep, flags[2b + 1] when some pair of it starts a round more than JAC_FAR from
-// orthogonal (cosine of the angle between the columns).  The host drops a block after a sweep that
-// leaves it unrotated, or whose pairs all started within JAC_FAR: Jacobi converges quadratically, so
-// that sweep has taken them to about JAC_FAR^2, below the rotation tolerance, and a further sweep
-// would only find that out.
+// orthogonal (cosine of the angle between the columns; far2 = JAC_FAR^2).  The host drops a block after
+// a sweep that leaves it unrotated, or whose pairs all started within JAC_FAR: Jacobi converges
+// quadratically, so that sweep has taken them to about JAC_FAR^2, below the rotation tolerance, and
+// a further sweep would only find that out.
 //
 // A task owns the pair (p, q) of block columns of W (column-major, length n).  Its local columns
 // are p's at 0..wp-1 and q's at bw..bw+wq-1 (bw <= 16; the columns between exist only beside the
@@ -74,7 +75,6 @@ constexpr int JAC_CS = JAC_CHUNK + 4;  // column stride of the chunk: 16 columns
 // pair, and a row and column JAC_SINK that take its writes, so that a rotation step has no branches
 constexpr int JAC_DUMMY = JAC_W, JAC_SINK = JAC_W + 1;
 constexpr int GP = JAC_W + 3;          // row stride of G and V (rows and columns 0..JAC_SINK)
-constexpr double JAC_FAR = 1e-4;
 constexpr int GSZ = (JAC_W + 2) * GP;
 
 // sum of the four partial 16 x 16 blocks at P, P + 256, P + 512, P + 768
@@ -83,7 +83,7 @@ __device__ __forceinline__ double sum4(const double* P, int t) { return (P[t] + 
 __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __restrict__ tasks, float* __restrict__ W,
                                                               int round, int* __restrict__ flags,
                                                               double* __restrict__ dgram, int* __restrict__ dgv,
-                                                              float tol, float ang_tol, int max_inner) {
+                                                              float tol, float ang_tol, int max_inner, double far2) {
   extern __shared__ float4 chunk4[];  // [JAC_W * JAC_CS] floats, column-major, 16-byte aligned
   float* chunk = reinterpret_cast<float*>(chunk4);
   __shared__ double G[GSZ], G2[GSZ], Vd[GSZ];
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
     if (x < y && y < wl) {
       const double ga = G[x * GP + y], ab = G[x * GP + x] * G[y * GP + y];
       need = ga * ga > ((double)tol * (double)tol) * ab;
-      far = ga * ga > JAC_FAR * JAC_FAR * ab;
+      far = ga * ga > far2 * ab;
     }
     // flags[2b + 1]: some pair of the block was at least JAC_FAR from orthogonal during the sweep
     if (__syncthreads_or(far) && tid == 0) flags[2 * b + 1] = 1;
@@ -422,8 +422,8 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
 // Column norms of the converged W give lambda + sigma; columns are normalised, sorted by lambda
 // and written as FP64 X (row-major, eigenvectors in columns).
 __global__ void __launch_bounds__(1024) k_normalize_sort(const EigBlock* __restrict__ blocks, const float* __restrict__ W,
-                                                         const float* __restrict__ sigma, double* __restrict__ X,
-                                                         double* __restrict__ lam) {
+                                                         const float* __restrict__ sigma, double* __restrict__ lam,
+                                                         int* __restrict__ perm, double* __restrict__ nrm_s) {
   extern __shared__ double sh[];
   const EigBlock blk = blocks[blockIdx.x];
   const int n = blk.n;
@@ -471,13 +471,36 @@ __global__ void __launch_bounds__(1024) k_normalize_sort(const EigBlock* __restr
       __syncthreads();
     }
   }
-  double* Xb = X + blk.off;
-  for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) {
-    const int r = (int)(e / n), c = (int)(e % n);
-    const int src = idx[c];
-    Xb[e] = (double)Wb[(long long)src * n + r] / nrm[src];
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    lam[blk.e_off + c] = key[c];
+    perm[blk.e_off + c] = idx[c];
+    nrm_s[blk.e_off + c] = nrm[idx[c]];
   }
-  for (int c = threadIdx.x; c < n; c += blockDim.x) lam[blk.e_off + c] = key[c];
+}
+
+// X(r, c) = W(:, perm c)(r) / |W(:, perm c)| in FP64 (row-major X, eigenvectors in columns), one
+// 32 x 32 tile of one block per workgroup: columns are read along W's contiguous direction into LDS
+// and the tile is written along X's rows.
+__global__ void __launch_bounds__(256) k_gather_x(const EigBlock* __restrict__ blocks, const int4* __restrict__ tiles,
+                                                  const float* __restrict__ W, const int* __restrict__ perm,
+                                                  const double* __restrict__ nrm_s, double* __restrict__ X) {
+  __shared__ float T[32][33];
+  const int4 t = tiles[blockIdx.x];
+  const EigBlock blk = blocks[t.x];
+  const int n = blk.n, r0 = 32 * t.y, c0 = 32 * t.z;
+  const float* Wb = W + blk.off;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k, c = e >> 5, rr = e & 31;
+    if (c0 + c < n && r0 + rr < n) T[c][rr] = Wb[(long long)perm[blk.e_off + c0 + c] * n + r0 + rr];
+  }
+  __syncthreads();
+  double* Xb = X + blk.off;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k, rr = e >> 5, c = e & 31;
+    if (c0 + c < n && r0 + rr < n) Xb[(long long)(r0 + rr) * n + c0 + c] = (double)T[c][rr] / nrm_s[blk.e_off + c0 + c];
+  }
 }
 
 // ---------------------------------------------------------------- Ogita-Aishima update
@@ -1050,9 +1073,9 @@ void launch_copy_converged(const EigBlock* blocks, int nb, const double* conv, d
 }
 
 // Final ordering: U(:, c) = X(:, perm[c]) with eigenvalues ascending (dsyevd order).
-__global__ void __launch_bounds__(1024) k_final_sort(const EigBlock* __restrict__ blocks, const double* __restrict__ X,
-                                                     const double* __restrict__ lam_in, double* __restrict__ U,
-                                                     double* __restrict__ lam_out) {
+__global__ void __launch_bounds__(1024) k_final_sort(const EigBlock* __restrict__ blocks,
+                                                     const double* __restrict__ lam_in, double* __restrict__ lam_out,
+                                                     int* __restrict__ perm) {
   extern __shared__ double sh[];
   const EigBlock blk = blocks[blockIdx.x];
   const int n = blk.n;
@@ -1084,13 +1107,29 @@ __global__ void __launch_bounds__(1024) k_final_sort(const EigBlock* __restrict_
       }
       __syncthreads();
     }
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    lam_out[blk.e_off + c] = key[c];
+    perm[blk.e_off + c] = idx[c];
+  }
+}
+
+// U(r, c) = X(r, perm c), one 32 x 32 tile of one block per workgroup
+__global__ void __launch_bounds__(256) k_gather_u(const EigBlock* __restrict__ blocks, const int4* __restrict__ tiles,
+                                                  const double* __restrict__ X, const int* __restrict__ perm,
+                                                  double* __restrict__ U) {
+  const int4 t = tiles[blockIdx.x];
+  const EigBlock blk = blocks[t.x];
+  const int n = blk.n, r0 = 32 * t.y, c0 = 32 * t.z;
   const double* Xb = X + blk.off;
   double* Ub = U + blk.off;
-  for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) {
-    const int r = (int)(e / n), c = (int)(e % n);
-    Ub[e] = Xb[(long long)r * n + idx[c]];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = threadIdx.x + 256 * k, rr = e >> 5, c = e & 31;
+    if (c0 + c < n && r0 + rr < n) {
+      const long long row = (long long)(r0 + rr) * n;
+      Ub[row + c0 + c] = Xb[row + perm[blk.e_off + c0 + c]];
+    }
   }
-  for (int c = threadIdx.x; c < n; c += blockDim.x) lam_out[blk.e_off + c] = key[c];
 }
 
 size_t normalize_sort_lds(int nmax) {
@@ -1113,16 +1152,20 @@ size_t final_sort_lds(int nmax) {
 }
 
 void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int round, int* flags, double* dgram, int* dgv,
-                         float tol, float ang_tol, int max_inner, hipStream_t s) {
+                         float tol, float ang_tol, int max_inner, double jac_far, hipStream_t s) {
   if (n_tasks <= 0) return;
   hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CS * sizeof(float), s, tasks, W,
-                     round, flags, dgram, dgv, tol, ang_tol, max_inner);
+                     round, flags, dgram, dgv, tol, ang_tol, max_inner, jac_far * jac_far);
   NRGK_HIP(hipGetLastError());
 }
-void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
-                           size_t lds, hipStream_t s) {
+void launch_normalize_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const float* W,
+                           const float* sigma, double* X, double* lam, int* perm, double* nrm_s, size_t lds,
+                           hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, W, sigma, X, lam);
+  hipLaunchKernelGGL(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, W, sigma, lam, perm, nrm_s);
+  NRGK_HIP(hipGetLastError());
+  if (n_tiles <= 0) return;
+  hipLaunchKernelGGL(k_gather_x, dim3(n_tiles), dim3(256), 0, s, blocks, tiles, W, perm, nrm_s, X);
   NRGK_HIP(hipGetLastError());
 }
 void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym, const int4* all_tiles, int n_all,
@@ -1143,13 +1186,17 @@ void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym,
   hipLaunchKernelGGL(k_oa_mask, dim3(n_all), dim3(256), 0, s, blocks, all_tiles, cid, conv, e_tol, d_tol, E);
   NRGK_HIP(hipGetLastError());
 }
-void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
-                       double* lam_out, size_t lds, hipStream_t s) {
+void launch_final_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const double* X,
+                       const double* lam_in, double* U, double* lam_out, int* perm, size_t lds, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_final_sort, dim3(nb), dim3(1024), lds, s, blocks, X, lam_in, U, lam_out);
+  hipLaunchKernelGGL(k_final_sort, dim3(nb), dim3(1024), lds, s, blocks, lam_in, lam_out, perm);
+  NRGK_HIP(hipGetLastError());
+  if (n_tiles <= 0) return;
+  hipLaunchKernelGGL(k_gather_u, dim3(n_tiles), dim3(256), 0, s, blocks, tiles, X, perm, U);
   NRGK_HIP(hipGetLastError());
 }
 void set_kernel_lds_limits(int bytes) {
+  set_trunc_lds_limit();
   NRGK_HIP(hipFuncSetAttribute((const void*)k_jacobi_round, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 JAC_W * JAC_CS * sizeof(float)));
   NRGK_HIP(hipFuncSetAttribute((const void*)k_normalize_sort, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));

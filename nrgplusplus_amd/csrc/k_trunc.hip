@@ -52,13 +52,22 @@ __device__ T block_reduce(T v, T* sh, Op op) {
   return r;
 }
 
+constexpr long long TRUNC_LDS = 16384;  // eigenvalues staged in LDS up to this count (128 KB)
+
 __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, const int* __restrict__ beoff, int S,
                                                    const double* __restrict__ lam, long long N_tot, long long K,
                                                    double errbar, int* __restrict__ kept, TruncOut* __restrict__ out,
                                                    double* __restrict__ lam_shift) {
   __shared__ double shd[16];
   __shared__ long long shl[16];
+  extern __shared__ double lds_lam[];  // all eigenvalues, when they fit (the searches then run in LDS)
   const int tid = threadIdx.x;
+  const bool in_lds = N_tot <= TRUNC_LDS;
+  if (in_lds) {
+    for (long long i = tid; i < N_tot; i += blockDim.x) lds_lam[i] = lam[i];
+    __syncthreads();
+  }
+  const double* L = in_lds ? lds_lam : lam;
   double mn = 1e308, mx = -1e308;
   for (int b = tid; b < S; b += blockDim.x)
     if (bn[b] > 0) {
@@ -75,7 +84,7 @@ __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, c
       const unsigned long long mid = lo + ((hi - lo) >> 1);
       const double v = okey_inv(mid);
       long long c = 0;
-      for (int b = tid; b < S; b += blockDim.x) c += count_le(lam + beoff[b], bn[b], v);
+      for (int b = tid; b < S; b += blockDim.x) c += count_le(L + beoff[b], bn[b], v);
       c = block_reduce(c, shl, [](long long a, long long d) { return a + d; });
       if (c >= K + 1) hi = mid;
       else lo = mid + 1;
@@ -85,7 +94,7 @@ __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, c
   const double x = e_cut + errbar;
   long long nk = 0;
   for (int b = tid; b < S; b += blockDim.x) {
-    const int k = trunc ? count_lt(lam + beoff[b], bn[b], x) : bn[b];
+    const int k = trunc ? count_lt(L + beoff[b], bn[b], x) : bn[b];
     kept[b] = k;
     nk += k;
   }
@@ -102,9 +111,15 @@ __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, c
   }
 }
 
+void set_trunc_lds_limit() {
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_truncate, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(TRUNC_LDS * sizeof(double))));
+}
+
 void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
                      double errbar, int* kept, TruncOut* out, double* lam_shift, hipStream_t s) {
-  hipLaunchKernelGGL(k_truncate, dim3(1), dim3(1024), 0, s, bn, beoff, S, lam, N_tot, K, errbar, kept, out, lam_shift);
+  const size_t lds = N_tot <= TRUNC_LDS ? (size_t)std::max<long long>(1, N_tot) * sizeof(double) : 0;
+  hipLaunchKernelGGL(k_truncate, dim3(1), dim3(1024), lds, s, bn, beoff, S, lam, N_tot, K, errbar, kept, out, lam_shift);
   NRGK_HIP(hipGetLastError());
 }
 

@@ -12,17 +12,18 @@ void launch_shift_convert(const EigBlock* blocks, const int* row_block, const in
                           const double* H, unsigned int* sigma_bits, float* sigma_out, float* W, hipStream_t s);
 // k_eig.hip
 void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int round, int* flags, double* dgram, int* dgv,
-                         float tol, float ang_tol, int max_inner, hipStream_t s);
-void launch_normalize_sort(const EigBlock* blocks, int nb, const float* W, const float* sigma, double* X, double* lam,
-                           size_t lds, hipStream_t s);
+                         float tol, float ang_tol, int max_inner, double jac_far, hipStream_t s);
+void launch_normalize_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const float* W,
+                           const float* sigma, double* X, double* lam, int* perm, double* nrm_s, size_t lds,
+                           hipStream_t s);
 void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym, const int4* all_tiles, int n_all,
                double* S, double* Rp, double* E, double* lam, int* cid, int* sidx, double* offmax, int* stats,
                double* conv, double* scrW, double* scrG, double* scrT, double* rrv, int nmax, double e_tol,
                double d_tol, hipStream_t s);
 void launch_copy_converged(const EigBlock* blocks, int nb, const double* conv, double e_tol, double d_tol,
                            const double* src, double* dst, hipStream_t s);
-void launch_final_sort(const EigBlock* blocks, int nb, const double* X, const double* lam_in, double* U,
-                       double* lam_out, size_t lds, hipStream_t s);
+void launch_final_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const double* X,
+                       const double* lam_in, double* U, double* lam_out, int* perm, size_t lds, hipStream_t s);
 size_t normalize_sort_lds(int nmax);
 size_t final_sort_lds(int nmax);
 void set_kernel_lds_limits(int bytes);
@@ -30,6 +31,7 @@ void set_kernel_lds_limits(int bytes);
 int prepare_gemm(std::vector<GemmProb>& probs);
 void launch_gemm(const GemmProb* d_probs, int n_probs, int tiles, hipStream_t s);
 // k_trunc.hip
+void set_trunc_lds_limit();
 void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
                      double errbar, int* kept, TruncOut* out, double* lam_shift, hipStream_t s);
 // k_rotate.hip
