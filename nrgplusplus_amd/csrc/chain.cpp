@@ -436,22 +436,37 @@ void Chain::eigensolve_mixed() {
   // without rotations)
   const bool jac_quad = knob("NRGK_JAC_QUAD", 1) != 0;
   const double jac_far = knob("NRGK_JAC_FAR", 1e-4);
+  // Workgroup g of a launch runs on XCD g mod 8 (round-robin dispatch).  Every block is kept on one
+  // XCD for the whole solve, so that its columns stay in that XCD's L2 from round to round: blocks,
+  // longest tournaments first, go to the XCD with the fewest tasks so far, and slot k of XCD x's
+  // task list sits at grid index 8 k + x (unused slots hold an empty task, which returns at once).
+  constexpr int NXCD = 8;
   std::vector<int> order(nb);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return blks[x].nbc > blks[y].nbc; });
+  std::vector<int> xcd_of(std::max(1, nb), 0);
+  {
+    int load[NXCD] = {0};
+    for (int bi : order) {
+      const int x = (int)(std::min_element(load, load + NXCD) - load);
+      xcd_of[bi] = x;
+      load[x] += blks[bi].nbc / 2;
+    }
+  }
   std::vector<int> hflags(std::max(1, nb), 1);
-  std::vector<JacTask> sweep_tasks;
+  std::vector<JacTask> xtasks[NXCD], grid_tasks;
   sweeps = 0;
   bool converged = false;
   for (int sw = 0; sw < max_sweeps && nb > 0; sw++) {
-    sweep_tasks.clear();
+    for (auto& v : xtasks) v.clear();
     int nbc_act = 2;
     double fl = 0, by = 0;
     for (int bi : order) {
       if (!hflags[bi]) continue;
       const EigBlock& e = blks[bi];
       nbc_act = std::max(nbc_act, e.nbc);
-      for (int jj = 0; jj < e.nbc / 2; jj++) sweep_tasks.push_back(JacTask{e.off, e.n, e.nbc, e.bw, e.bc_off, bi, jj});
+      for (int jj = 0; jj < e.nbc / 2; jj++)
+        xtasks[xcd_of[bi]].push_back(JacTask{e.off, e.n, e.nbc, e.bw, e.bc_off, bi, jj});
       if (prof_on) {  // per round, Gram (2 n w^2) and update (2 n w^2) of every pair
         const double n = e.n, w = std::min(e.n, 2 * e.bw);
         const double rounds = e.nbc - 1, pairs = std::ceil(n / w);
@@ -459,14 +474,24 @@ void Chain::eigensolve_mixed() {
         by += rounds * pairs * 3.0 * n * w * 4.0;  // read twice, write once (FP32)
       }
     }
-    const JacTask* dt = stg.put(sweep_tasks, st);
+    size_t maxlen = 0;
+    for (auto& v : xtasks) maxlen = std::max(maxlen, v.size());
+    grid_tasks.assign(NXCD * maxlen, JacTask{0, 0, 0, 1, 0, 0, 0});
+    for (int x = 0; x < NXCD; x++)
+      for (size_t k = 0; k < xtasks[x].size(); k++) grid_tasks[NXCD * k + x] = xtasks[x][k];
+    const JacTask* dt = stg.put(grid_tasks, st);
     int* dflags = stg.put_zero<int>(2 * nb, st);
     prof_begin();
-    size_t cnt = sweep_tasks.size();
+    size_t cnt[NXCD];
+    for (int x = 0; x < NXCD; x++) cnt[x] = xtasks[x].size();
     for (int r = 0; r < nbc_act - 1; r++) {
-      while (cnt > 0 && sweep_tasks[cnt - 1].nbc - 1 <= r) cnt--;  // blocks whose tournament has ended
-      launch_jacobi_round(dt, (int)cnt, dW.as<float>(), r, dflags, d_dgram.as<double>(), d_dgv.as<int>(), jac_tol,
-                          jac_ang, jac_inner, jac_far, st);
+      size_t len = 0;
+      for (int x = 0; x < NXCD; x++) {  // tasks of blocks whose tournament has ended drop off the end
+        while (cnt[x] > 0 && xtasks[x][cnt[x] - 1].nbc - 1 <= r) cnt[x]--;
+        len = std::max(len, cnt[x]);
+      }
+      launch_jacobi_round(dt, (int)(NXCD * len), dW.as<float>(), r, dflags, d_dgram.as<double>(), d_dgv.as<int>(),
+                          jac_tol, jac_ang, jac_inner, jac_far, st);
     }
     prof_end("jacobi_round", nbc_act - 1, fl, by);
     const int* hf = stg.get(dflags, 2 * nb, st);
