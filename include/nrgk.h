@@ -154,6 +154,9 @@ typedef struct {
   double bytes;
 } nrgk_kernel_stat;
 int nrgk_chain_profile(nrgk_chain* chain, int enable);
+/* restricts the timing to one family (NULL or "": all), so that a caller timing one kernel records
+ * one event pair per run of that family only */
+int nrgk_chain_profile_only(nrgk_chain* chain, const char* family);
 int nrgk_chain_kernel_stats(nrgk_chain* chain, nrgk_kernel_stat* out, int cap, int* n_out);
 
 /* ---------------------------------------------------------------- observables (sysOperator.hpp) */

@@ -102,6 +102,7 @@ def lib():
             "nrgk_chain_step_stats": (i, [vp, dp, ip, ip]),
             "nrgk_chain_refine_stats": (i, [vp, ip, ip]),
             "nrgk_chain_profile": (i, [vp, i]),
+            "nrgk_chain_profile_only": (i, [vp, C.c_char_p]),
             "nrgk_chain_kernel_stats": (i, [vp, vp, i, ip]),
         }
         optional = {
@@ -439,8 +440,10 @@ class NrgCore:
         return dict(ms_assemble=ms[0], ms_eig=ms[1], ms_truncate=ms[2], ms_rotate=ms[3], jacobi_sweeps=sw.value,
                     max_cluster=mc.value, oa_steps=oa.value, oa_unconverged=unc.value)
 
-    def profile(self, enable=True):
-        """Start (clearing the totals) or stop per-kernel-family device timing."""
+    def profile(self, enable=True, only=None):
+        """Start (clearing the totals) or stop per-kernel-family device timing; `only` restricts it to
+        one family."""
+        _check(lib().nrgk_chain_profile_only(self._h, only.encode() if only else None))
         _check(lib().nrgk_chain_profile(self._h, 1 if enable else 0))
 
     def kernel_stats(self):

@@ -427,6 +427,12 @@ int nrgk_chain_profile(nrgk_chain* ch, int enable) {
     if (enable) ch->c->kstats.clear();
   });
 }
+int nrgk_chain_profile_only(nrgk_chain* ch, const char* family) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    ch->c->prof_only = family ? family : "";
+  });
+}
 int nrgk_chain_kernel_stats(nrgk_chain* ch, nrgk_kernel_stat* out, int cap, int* n_out) {
   return guard([&] {
     NRGK_REQUIRE(ch && n_out && (out || cap == 0), NRGK_ERR_INVALID, "null argument");

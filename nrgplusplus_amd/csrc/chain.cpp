@@ -113,8 +113,8 @@ hipEvent_t Chain::take_event() {
   ev_pool.pop_back();
   return e;
 }
-void Chain::prof_begin() {
-  if (!prof_on) return;
+void Chain::prof_begin(const char* family) {
+  if (!prof_on || (!prof_only.empty() && prof_only != family)) return;
   prof_open = take_event();
   NRGK_HIP(hipEventRecord(prof_open, st));
 }
@@ -342,7 +342,7 @@ void Chain::run_gemms(std::vector<GemmProb>& probs) {
   if (!large.empty()) {
     double fl = 0;
     for (auto& p : large) fl += 2.0 * p.m * p.n * p.k;
-    prof_begin();
+    prof_begin("gemm_rocblas");
     plain_gemms(*this, large);  // synchronises the stream
     prof_end("gemm_rocblas", (long long)large.size(), fl, 0.0);
   }
@@ -357,7 +357,7 @@ void Chain::run_gemms(std::vector<GemmProb>& probs) {
       fl += 2.0 * outs * p.k;
       by += 8.0 * ((double)p.m * p.k + (double)p.k * p.n + (p.C0 ? 2.0 : 1.0) * outs);
     }
-  prof_begin();
+  prof_begin("gemm_f64");
   launch_gemm(dprobs, (int)live.size(), tiles, st);
   prof_end("gemm_f64", 1, fl, by);
 }
@@ -467,11 +467,11 @@ void Chain::eigensolve_mixed() {
       nbc_act = std::max(nbc_act, e.nbc);
       for (int jj = 0; jj < e.nbc / 2; jj++)
         xtasks[xcd_of[bi]].push_back(JacTask{e.off, e.n, e.nbc, e.bw, e.bc_off, bi, jj});
-      if (prof_on) {  // per round, Gram (2 n w^2) and update (2 n w^2) of every pair
+      if (prof_on) {  // per task and round: the cross Gram block (2 n (w/2)^2) and P V (2 n w^2)
         const double n = e.n, w = std::min(e.n, 2 * e.bw);
         const double rounds = e.nbc - 1, pairs = std::ceil(n / w);
-        fl += rounds * pairs * 4.0 * n * w * w;
-        by += rounds * pairs * 3.0 * n * w * 4.0;  // read twice, write once (FP32)
+        fl += rounds * pairs * 2.5 * n * w * w;
+        by += rounds * pairs * (n <= 768 ? 2.0 : 3.0) * n * w * 4.0;  // FP32 panel read once, written once
       }
     }
     size_t maxlen = 0;
@@ -481,7 +481,7 @@ void Chain::eigensolve_mixed() {
       for (size_t k = 0; k < xtasks[x].size(); k++) grid_tasks[NXCD * k + x] = xtasks[x][k];
     const JacTask* dt = stg.put(grid_tasks, st);
     int* dflags = stg.put_zero<int>(2 * nb, st);
-    prof_begin();
+    prof_begin("jacobi_round");
     size_t cnt[NXCD];
     for (int x = 0; x < NXCD; x++) cnt[x] = xtasks[x].size();
     for (int r = 0; r < nbc_act - 1; r++) {
@@ -635,7 +635,7 @@ void Chain::eigensolve_mixed() {
     double oa_bytes = 0;  // symmetrise S, Rp (read and write both), E from S and Rp, masking pass
     if (prof_on)
       for (auto& b : act) oa_bytes += 8.0 * 9.0 * (double)b.n * b.n;
-    prof_begin();
+    prof_begin("oa_update");
     launch_oa(m_blk, na, m_sym, (int)tiles_sym.size(), m_all, (int)tiles_all.size(), dS.as<double>(), dR.as<double>(),
               dE.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_sidx.as<int>(), m_offmax,
               d_stats.as<int>(), m_conv, dP.as<double>(), dX2.as<double>(), dQ.as<double>(),

@@ -124,7 +124,8 @@ struct Chain {
   std::vector<ProfRun> prof_pending;
   hipEvent_t prof_open = nullptr;
   hipEvent_t take_event();
-  void prof_begin();
+  std::string prof_only;  // a single family to time (empty: all)
+  void prof_begin(const char* family);
   void prof_end(const char* family, long long launches, double flops, double bytes);
   void prof_resolve();
 
