@@ -90,6 +90,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
   __shared__ double Gpart[3 * 1024];  // partial Gram blocks of the matrix-core waves
   __shared__ int s_slots[32 * 16];
   __shared__ float Vf[JAC_W * GP];
+  __shared__ int s_any;
   const JacTask task = tasks[blockIdx.x];
   const int b = task.b, j = task.j;
   const int N = task.nbc;
@@ -171,6 +172,7 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
   // missing member reading JAC_DUMMY and writing JAC_SINK), V = I, and the dummy row and column of
   // both G buffers set to zero
   auto setup = [&]() {
+    if (tid == 0) s_any = 0;
     for (int e = tid; e < steps_per * 16; e += JAC_THREADS) {
       int u, v;
       slot(e >> 4, e & 15, u, v);
@@ -188,13 +190,13 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
     }
   };
   // ---- Gram matrix on the FP64 matrix cores (16 x 16 x 4 steps).  Job of wave v: the p x q block
-  // (fast: waves 0-7, each taking every eighth group of 4 rows), or the p x p, p x q and q x q blocks
-  // (waves 0-3, 4-7, 8-11, every fourth group); two accumulators in flight.  Lane l supplies column
+  // (fast: waves 0-11, each taking every twelfth group of 4 rows), or the p x p, p x q and q x q
+  // blocks (waves 0-3, 4-7, 8-11, every fourth group); two accumulators in flight.  Lane l supplies column
   // l%16 of its left block as A and of its right block as B, at row 4 kb + l/16; entry (i, j) of the
   // product lands in lane 16 (i%4) + j, register i/4.  The partial blocks meet in Gpart.
   const int wv = tid >> 6, lane = tid & 63, lx = lane & 15;
-  const int job = fast ? (wv < 8 ? 1 : -1) : (wv < 12 ? wv >> 2 : -1);
-  const int kq = fast ? wv : wv & 3, kstride = fast ? 8 : 4;
+  const int job = wv < 12 ? (fast ? 1 : wv >> 2) : -1;
+  const int kq = fast ? wv : wv & 3, kstride = fast ? 12 : 4;
   const float* ca = chunk + (job == 2 ? bw + lx : lx) * JAC_CS + (lane >> 4);
   const float* cb = chunk + (job == 0 ? lx : bw + lx) * JAC_CS + (lane >> 4);
   // a 16-wide tile of p reaches into q when bw < 16: those lanes supply zeros
@@ -230,14 +232,20 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
     if (job >= 0) {
       const int nkb = rows4 >> 2;
       int kb = kq;
-      for (; kb + kstride < nkb; kb += 2 * kstride) {
-        const int k2 = kb + kstride;
-        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
-                                                  0, 0, 0);
-        m1 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * k2] : 0.0, bm ? (double)cb[4 * k2] : 0.0, m1,
-                                                  0, 0, 0);
+      // four groups per iteration: all eight operands are read before the products
+      for (; kb + 3 * kstride < nkb; kb += 4 * kstride) {
+        double xa[4], xb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          xa[u] = am ? (double)ca[4 * (kb + u * kstride)] : 0.0;
+          xb[u] = bm ? (double)cb[4 * (kb + u * kstride)] : 0.0;
+        }
+        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[0], xb[0], m0, 0, 0, 0);
+        m1 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[1], xb[1], m1, 0, 0, 0);
+        m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[2], xb[2], m0, 0, 0, 0);
+        m1 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[3], xb[3], m1, 0, 0, 0);
       }
-      if (kb < nkb)
+      for (; kb < nkb; kb += kstride)
         m0 = __builtin_amdgcn_mfma_f64_16x16x4f64(am ? (double)ca[4 * kb] : 0.0, bm ? (double)cb[4 * kb] : 0.0, m0,
                                                   0, 0, 0);
     }
@@ -252,7 +260,8 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
   if (tid < 256) {
     const int a = tid >> 4, c = tid & 15;
     if (a < bw && c < bw) {
-      const double gx = fast ? sum4(Gpart, tid) + sum4(Gpart + 1024, tid) : sum4(Gpart + 1024, tid);
+      const double gx =
+          fast ? (sum4(Gpart, tid) + sum4(Gpart + 1024, tid)) + sum4(Gpart + 2048, tid) : sum4(Gpart + 1024, tid);
       G[a * GP + bw + c] = gx;
       G[(bw + c) * GP + a] = gx;
       if (!fast) {
@@ -285,9 +294,14 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
       need = ga * ga > ((double)tol * (double)tol) * ab;
       far = ga * ga > far2 * ab;
     }
-    // flags[2b + 1]: some pair of the block was at least JAC_FAR from orthogonal during the sweep
-    if (__syncthreads_or(far) && tid == 0) flags[2 * b + 1] = 1;
-    if (!__syncthreads_or(need)) return;
+    // one barrier for both tests; flags[2b + 1]: some pair of the block was at least JAC_FAR from
+    // orthogonal during the sweep
+    const unsigned long long bneed = __ballot(need), bfar = __ballot(far);
+    if ((tid & 63) == 0 && (bneed | bfar)) atomicOr(&s_any, (bneed ? 1 : 0) | (bfar ? 2 : 0));
+    __syncthreads();
+    const int any = s_any;
+    if ((any & 2) && tid == 0) flags[2 * b + 1] = 1;
+    if (!(any & 1)) return;
   }
   // ---- cyclic Jacobi on G.  Column x of slot (u, v), u < v, moves as P'(:,u) = c P(:,u) - s P(:,v),
   // P'(:,v) = s P(:,u) + c P(:,v), so G' = J^T G J and V' = V J.  Every thread that needs a slot's
@@ -368,35 +382,60 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
   __syncthreads();
   // ---- P <- P V in FP32
   if (resident) {
-    // (P V)^T = V^T P^T on the FP32 matrix cores in 16-row tiles (wave v takes tiles v, v + 16, ...):
-    // per tile and half h of the new columns, 8 steps of 16 x 16 x 4 over the 32 old columns.  Lane l
-    // keeps V(4 ks + l/16, 16 h + l%16) (A) in registers and reads P(row0 + l%16, 4 ks + l/16) (B);
-    // entry (i, j) = (P V)(row0 + j, 16 h + i) lands in lane 16 (i/4) + j, register i%4 and goes
-    // straight to W (16 consecutive rows of 4 columns per store).  Tile rows past n read the
-    // column's padding and are not stored.
+    // (P V)^T = V^T P^T on the FP32 matrix cores in 16-row tiles: per tile and half h of the new
+    // columns, 8 steps of 16 x 16 x 4 over the 32 old columns.  Lane l keeps V(4 ks + l/16, 16 h + l%16)
+    // (A) in registers and reads P(row0 + l%16, 4 ks + l/16) (B); entry (i, j) = (P V)(row0 + j,
+    // 16 h + i) lands in lane 16 (i/4) + j, register i%4 and goes straight to W (16 consecutive rows
+    // of 4 columns per store).  Tile rows past n read the column's padding and are not stored.
     const int li = lane & 15, lk = lane >> 4;
     float va[2][8];
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
       for (int ks = 0; ks < 8; ks++) va[h][ks] = Vf[(4 * ks + lk) * GP + 16 * h + li];
-    const int ntile = (n + 15) >> 4;
-    for (int t = wv; t < ntile; t += JAC_THREADS / 64) {
-      const int row = 16 * t + li;
-      float pb[8];
+    // the eight columns this lane stores: 16 h + 4 (l/16) + r (absent ones flagged)
+    float* colp[2][4];
+    bool colv[2][4];
 #pragma unroll
-      for (int ks = 0; ks < 8; ks++) pb[ks] = chunk[(4 * ks + lk) * JAC_CS + row];
+    for (int h = 0; h < 2; h++)
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        flt4 m = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int r = 0; r < 4; r++) {
+        const int col = 16 * h + 4 * lk + r;
+        colv[h][r] = valid(col);
+        colp[h][r] = Wb + (long long)gcol(colv[h][r] ? col : 0) * n;
+      }
+    auto store = [&](const flt4& m, int row, int h) {
+      if (row < n)
 #pragma unroll
-        for (int ks = 0; ks < 8; ks++) m = __builtin_amdgcn_mfma_f32_16x16x4f32(va[h][ks], pb[ks], m, 0, 0, 0);
-        if (row < n)
+        for (int r = 0; r < 4; r++)
+          if (colv[h][r]) colp[h][r][row] = m[r];
+    };
+    // two tiles per pass (waves take tiles v, v + 16, v + 32, ...): four independent accumulator
+    // chains keep the matrix pipeline busy
+    const int ntile = (n + 15) >> 4, nw = JAC_THREADS / 64;
+    for (int t0 = wv; t0 < ntile; t0 += 2 * nw) {
+      const int t1 = t0 + nw;
+      const int row0 = 16 * t0 + li, row1 = 16 * t1 + li;
+      const bool two = t1 < ntile;
+      float pb0[8], pb1[8];
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int col = 16 * h + 4 * lk + r;
-            if (valid(col)) Wb[(long long)gcol(col) * n + row] = m[r];
-          }
+      for (int ks = 0; ks < 8; ks++) {
+        pb0[ks] = chunk[(4 * ks + lk) * JAC_CS + row0];
+        pb1[ks] = two ? chunk[(4 * ks + lk) * JAC_CS + row1] : 0.0f;
+      }
+      flt4 m00 = {0.0f, 0.0f, 0.0f, 0.0f}, m01 = m00, m10 = m00, m11 = m00;
+#pragma unroll
+      for (int ks = 0; ks < 8; ks++) {
+        m00 = __builtin_amdgcn_mfma_f32_16x16x4f32(va[0][ks], pb0[ks], m00, 0, 0, 0);
+        m01 = __builtin_amdgcn_mfma_f32_16x16x4f32(va[1][ks], pb0[ks], m01, 0, 0, 0);
+        m10 = __builtin_amdgcn_mfma_f32_16x16x4f32(va[0][ks], pb1[ks], m10, 0, 0, 0);
+        m11 = __builtin_amdgcn_mfma_f32_16x16x4f32(va[1][ks], pb1[ks], m11, 0, 0, 0);
+      }
+      store(m00, row0, 0);
+      store(m01, row0, 1);
+      if (two) {
+        store(m10, row1, 0);
+        store(m11, row1, 1);
       }
     }
     if (tid == 0) flags[2 * b] = 1;
