@@ -715,6 +715,119 @@ __device__ void rr_wide_cluster(const double* Sb, const double* Rb, double* Eb, 
   __syncthreads();
 }
 
+// Rayleigh-Ritz of one cluster of 2..RR_SMALL members by one wave, with the algebra of the workgroup
+// path below: M = 1.5 I - 0.5 Rp_JJ, T = M S_JJ M, one-sided Jacobi on T - cen I + sig I (lane 8a + b
+// holds entry (a, b); a column's rows are the lanes b, b + 8, ...), Q and mu ascending, G = M Q,
+// E(i, J) <- E(i, J) G for the rows outside the cluster (unless the block takes its last step) and
+// E_JJ = G - I.  ws: 3 x 64 doubles of this wave's LDS.
+constexpr int RR_SMALL = 8;
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ void rr_small_wave(const double* Sb, const double* Rb, double* Eb, int n, int k, const int* J,
+                              const int* cid, int cl, double* lam, double* ws, bool map_rows) {
+  const int lane = threadIdx.x & 63, a = lane >> 3, b = lane & 7;
+  const bool in = a < k && b < k;
+  double* M = ws;
+  double* T = ws + 64;
+  double* Q = ws + 128;
+  const int ja = J[min(a, k - 1)], jb = J[min(b, k - 1)];
+  M[lane] = in ? (a == b ? 1.5 : 0.0) - 0.5 * Rb[(long long)ja * n + jb] : 0.0;
+  wave_lds_sync();
+  double sm = 0.0;  // (S_JJ M)(a, b)
+  if (in)
+    for (int l = 0; l < k; l++) sm += Sb[(long long)ja * n + J[l]] * M[l * 8 + b];
+  T[lane] = sm;
+  wave_lds_sync();
+  double t = 0.0;  // (M S_JJ M)(a, b)
+  if (in)
+    for (int l = 0; l < k; l++) t += M[a * 8 + l] * T[l * 8 + b];
+  // shift by the cluster centre and make it positive definite (as the workgroup path)
+  const double cen = wave_sum(in && a == b ? t : 0.0) / k;
+  double ra = in ? fabs(t - (a == b ? cen : 0.0)) : 0.0;
+#pragma unroll
+  for (int o = 1; o < 8; o <<= 1) ra += __shfl_xor(ra, o, 64);
+#pragma unroll
+  for (int o = 8; o < 64; o <<= 1) ra = fmax(ra, __shfl_xor(ra, o, 64));
+  const double sig = ra > 1e-30 * fmax(1.0, fabs(cen)) ? 1.05 * ra : 1.0;
+  double w = in ? (a == b ? (t - cen) + sig : t) : 0.0;
+  // one-sided Jacobi on the columns, round-robin over kp players
+  const int kp = (k + 1) & ~1;
+  auto col_sum = [&](double v) {
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+  };
+  for (int sweep = 0; sweep < 30; sweep++) {
+    bool rot = false;
+    for (int step = 0; step < kp - 1; step++) {
+      int partner = -1;
+      for (int pj = 0; pj < kp / 2; pj++) {
+        int x, y;
+        tournament_pair(kp, step, pj, x, y);
+        if (x == b) partner = y;
+        if (y == b) partner = x;
+      }
+      const bool pair = b < k && partner >= 0 && partner < k;
+      const double y = __shfl(w, 8 * a + (pair ? partner : b), 64);
+      const double nx = col_sum(w * w), ny = col_sum(y * y), xy = col_sum(w * y);
+      // the pair's rotation from (first, second) = (min, max) column.  The sums may differ by an ulp
+      // from row to row (a product can be fused into the first add), which at a degenerate pair
+      // would change the angle: every lane takes the rotation of row 0 of the first column.
+      const bool first = b < partner;
+      const double al = first ? nx : ny, be = first ? ny : nx;
+      const bool go_l = pair && xy != 0.0 && fabs(xy) > 1e-15 * sqrt(al * be);
+      const double zeta = (be - al) / (2.0 * xy);
+      const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+      const double cs_l = go_l ? 1.0 / sqrt(1.0 + tt * tt) : 1.0, sn_l = go_l ? cs_l * tt : 0.0;
+      const int src = pair ? min(b, partner) : b;
+      const double cs = __shfl(cs_l, src, 64), sn = __shfl(sn_l, src, 64);
+      const bool go = __shfl(go_l ? 1 : 0, src, 64) != 0;
+      if (go) {
+        w = first ? cs * w - sn * y : sn * y + cs * w;
+        rot = true;
+      }
+    }
+    if (__ballot(rot) == 0) break;
+  }
+  // columns ascending by norm (ties by index): Q(:, rank) = W(:, b) / |W(:, b)|, mu = |W(:, b)|
+  const double nb = __shfl(sqrt(col_sum(w * w)), b, 64);  // row 0's value for the whole column
+  int rank = 0;
+  for (int c = 0; c < k; c++) {
+    const double nc = __shfl(nb, c, 64);
+    rank += (nc < nb) || (nc == nb && c < b);
+  }
+  if (in) Q[a * 8 + rank] = w / nb;
+  if (a == 0 && b < k) lam[J[rank]] = nb - sig + cen;
+  wave_lds_sync();
+  double g = 0.0;  // G = M Q
+  if (in)
+    for (int l = 0; l < k; l++) g += M[a * 8 + l] * Q[l * 8 + b];
+  T[lane] = g;
+  wave_lds_sync();
+  int jl[RR_SMALL];
+#pragma unroll
+  for (int l = 0; l < RR_SMALL; l++) jl[l] = J[l < k ? l : 0];
+  for (int i = lane; i < n && map_rows; i += 64) {
+    if (cid[i] == cl) continue;
+    double* Ei = Eb + (long long)i * n;
+    double old[RR_SMALL];
+#pragma unroll
+    for (int l = 0; l < RR_SMALL; l++) old[l] = l < k ? Ei[jl[l]] : 0.0;
+#pragma unroll
+    for (int c = 0; c < RR_SMALL; c++) {
+      if (c >= k) break;
+      double v = 0.0;
+#pragma unroll
+      for (int l = 0; l < RR_SMALL; l++) v += old[l] * T[l * 8 + c];
+      Ei[jl[c]] = v;
+    }
+  }
+  if (in) Eb[(long long)ja * n + jb] = g - (a == b ? 1.0 : 0.0);
+}
+
 // The OA step as four launches over the active blocks:
 //   k_oa_sym      32x32 tile pairs (i,j)/(j,i): S and Rp, computed on and above the diagonal only,
 //                 completed by mirroring (exactly symmetric S and Rp keep E + E^T = -(Rp - I) exact,
@@ -954,8 +1067,9 @@ __global__ void __launch_bounds__(256) k_oa_entries(const EigBlock* __restrict__
   }
 }
 
-// Rayleigh-Ritz on every cluster of two or more, in sorted order: M = 1.5 I - 0.5 Rp_JJ,
-// (M S_JJ M) Q = Q mu, E_JJ = M Q - I and the out-of-cluster rows of E(:,J) mapped through M Q.
+// Rayleigh-Ritz on every cluster of two or more: M = 1.5 I - 0.5 Rp_JJ, (M S_JJ M) Q = Q mu,
+// E_JJ = M Q - I and the out-of-cluster rows of E(:,J) mapped through M Q.  Clusters of up to
+// RR_SMALL members one wave each, the wider ones in turn by the whole workgroup.
 __global__ void __launch_bounds__(1024) k_oa_rr(const EigBlock* __restrict__ blocks, const double* __restrict__ S,
                                                 const double* __restrict__ Rp, double* __restrict__ E,
                                                 double* __restrict__ lam, const int* __restrict__ cid_g,
@@ -997,10 +1111,22 @@ __global__ void __launch_bounds__(1024) k_oa_rr(const EigBlock* __restrict__ blo
   }
   __syncthreads();
   const int ncl = s_nc;
+  // clusters of 2..RR_SMALL members: one wave each, concurrently (they touch disjoint columns of E)
+  {
+    double* wscr = (double*)(((uintptr_t)(cstart + n + 2) + 7) & ~(uintptr_t)7);
+    const int wave = tid >> 6;
+    for (int c = wave; c < ncl; c += blockDim.x / 64) {
+      const int s0 = cstart[2 * c], k = cstart[2 * c + 1] - s0;
+      if (k <= RR_SMALL)
+        rr_small_wave(Sb, Rb, Eb, n, k, idx + s0, cidb, cidb[idx[s0]], lam + blk.e_off, wscr + 192 * wave, !last);
+    }
+    __syncthreads();
+  }
   auto Ss = [&](int i, int j) { return Sb[(long long)i * n + j]; };
   auto Rs = [&](int i, int j) { return Rb[(long long)i * n + j]; };
   for (int c = 0; c < ncl; c++) {
     const int s0 = cstart[2 * c], k = cstart[2 * c + 1] - s0;
+    if (k <= RR_SMALL) continue;
     const int cl = cidb[idx[s0]];
     if (k > RR_MAX) {
       rr_wide_cluster(Sb, Rb, Eb, n, k, idx + s0, cidb, cl, scrW + blk.off, scrG + blk.off, scrT + blk.off,
@@ -1182,7 +1308,8 @@ size_t oa_prep_lds(int nmax) {
   return (size_t)np2 * 12 + (size_t)nmax * 4 + 64;
 }
 size_t oa_rr_lds(int nmax) {
-  return (size_t)(2 * RR_MAX * RR_MAX + 2 * RR_MAX + 32) * 8 + (size_t)RR_MAX * 4 + (size_t)(nmax + 2) * 4 + 64;
+  return (size_t)(2 * RR_MAX * RR_MAX + 2 * RR_MAX + 32) * 8 + (size_t)RR_MAX * 4 + (size_t)(nmax + 2) * 4 + 64 +
+         16 * 192 * sizeof(double);
 }
 size_t final_sort_lds(int nmax) {
   int np2 = 1;
