@@ -947,7 +947,8 @@ __global__ void __launch_bounds__(256) k_oa_sym(const EigBlock* __restrict__ blo
 // rounding noise of S_ij (about sqrt(n) eps max|lt|) the OA quotient is noise over gap, so pairs
 // closer than 1e5 times that noise are handled as clusters, which caps the out-of-cluster corrections
 // made of pure rounding at about 1e-5 and keeps the iteration at its fixed point once converged.
-// Clusters are the runs of sorted lt with gaps <= delta; cid = cluster number, sidx = sorted order.
+// Clusters are the runs of sorted lt with gaps <= delta; cid = sorted position of the cluster's first
+// member, sidx = sorted order.
 __global__ void __launch_bounds__(1024) k_oa_prep(const EigBlock* __restrict__ blocks, const double* __restrict__ S,
                                                   const double* __restrict__ Rp, const double* __restrict__ offmax,
                                                   double* __restrict__ lam, int* __restrict__ cid_g,
@@ -963,7 +964,9 @@ __global__ void __launch_bounds__(1024) k_oa_prep(const EigBlock* __restrict__ b
   int* idx = (int*)(key + np2);  // [np2]
   int* head = idx + np2;         // [n] 1 where a cluster starts (sorted position)
   __shared__ double red[16];
+  __shared__ int s_maxk;
   const int tid = threadIdx.x;
+  if (tid == 0) s_maxk = 1;
   const double* Sb = S + blk.off;
   const double* Rb = Rp + blk.off;
   double ml = 0.0;
@@ -1002,20 +1005,23 @@ __global__ void __launch_bounds__(1024) k_oa_prep(const EigBlock* __restrict__ b
   if (tid == 0) conv[3 * blockIdx.x + 2] = noise;
   for (int s = tid; s < n; s += blockDim.x) head[s] = (s == 0) || (key[s] - key[s - 1] > delta);
   __syncthreads();
-  if (tid == 0) {  // cluster numbers (prefix count) and the widest cluster
-    int c = -1, maxk = 1, run = 0;
-    for (int s = 0; s < n; s++) {
-      if (head[s]) {
-        c++;
-        run = 0;
-      }
-      run++;
-      maxk = max(maxk, run);
-      cid_g[blk.e_off + idx[s]] = c;
-      sidx_g[blk.e_off + s] = idx[s];
+  // cluster id = sorted position of the cluster's first member (ids are only compared), and the
+  // widest cluster, all positions in parallel
+  for (int s = tid; s < n; s += blockDim.x) {
+    int s0 = s;
+    while (!head[s0]) s0--;  // head[0] is set
+    cid_g[blk.e_off + idx[s]] = s0;
+    sidx_g[blk.e_off + s] = idx[s];
+    if (head[s]) {
+      int e = s + 1;
+      while (e < n && !head[e]) e++;
+      atomicMax(&s_maxk, e - s);
     }
-    atomicMax(&stats[0], maxk);
-    if (maxk > RR_MAX) atomicOr(&stats[1], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    atomicMax(&stats[0], s_maxk);
+    if (s_maxk > RR_MAX) atomicOr(&stats[1], 1);
   }
 }
 
@@ -1093,27 +1099,30 @@ __global__ void __launch_bounds__(1024) k_oa_rr(const EigBlock* __restrict__ blo
   double* rrN = rrMu + RR_MAX;           // [RR_MAX]
   double* red = rrN + RR_MAX;            // [32]
   int* rrP = (int*)(red + 32);           // [RR_MAX]
-  int* cstart = rrP + RR_MAX;            // [n + 1] cluster starts of width >= 2 (sorted positions)
+  int* cstart = rrP + RR_MAX;            // [n + 2] cluster (start, end) of width >= 2 (sorted positions)
   __shared__ int s_nc;
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    int nc = 0, s0 = 0;
-    for (int s = 1; s <= n; s++)
-      if (s == n || cidb[idx[s]] != cidb[idx[s - 1]]) {
-        if (s - s0 >= 2) {
-          cstart[2 * nc] = s0;
-          cstart[2 * nc + 1] = s;
-          nc++;
-        }
-        s0 = s;
-      }
-    s_nc = nc;
+  // clusters are runs of equal id in sorted order: ids staged in LDS, every run start found in
+  // parallel (the list order does not matter, clusters are disjoint)
+  int* csort = cstart + n + 2;  // [n] cluster id by sorted position
+  if (tid == 0) s_nc = 0;
+  for (int s = tid; s < n; s += blockDim.x) csort[s] = cidb[idx[s]];
+  __syncthreads();
+  for (int s = tid; s < n; s += blockDim.x) {
+    if (s > 0 && csort[s] == csort[s - 1]) continue;
+    int e = s + 1;
+    while (e < n && csort[e] == csort[s]) e++;
+    if (e - s >= 2) {
+      const int slot = atomicAdd(&s_nc, 1);
+      cstart[2 * slot] = s;
+      cstart[2 * slot + 1] = e;
+    }
   }
   __syncthreads();
   const int ncl = s_nc;
   // clusters of 2..RR_SMALL members: one wave each, concurrently (they touch disjoint columns of E)
   {
-    double* wscr = (double*)(((uintptr_t)(cstart + n + 2) + 7) & ~(uintptr_t)7);
+    double* wscr = (double*)(((uintptr_t)(csort + n + 2) + 7) & ~(uintptr_t)7);
     const int wave = tid >> 6;
     for (int c = wave; c < ncl; c += blockDim.x / 64) {
       const int s0 = cstart[2 * c], k = cstart[2 * c + 1] - s0;
@@ -1308,7 +1317,7 @@ size_t oa_prep_lds(int nmax) {
   return (size_t)np2 * 12 + (size_t)nmax * 4 + 64;
 }
 size_t oa_rr_lds(int nmax) {
-  return (size_t)(2 * RR_MAX * RR_MAX + 2 * RR_MAX + 32) * 8 + (size_t)RR_MAX * 4 + (size_t)(nmax + 2) * 4 + 64 +
+  return (size_t)(2 * RR_MAX * RR_MAX + 2 * RR_MAX + 32) * 8 + (size_t)RR_MAX * 4 + (size_t)(nmax + 2) * 8 + 64 +
          16 * 192 * sizeof(double);
 }
 size_t final_sort_lds(int nmax) {
