@@ -520,19 +520,25 @@ void Chain::eigensolve_mixed() {
   d_rrv.ensure(std::max<long long>(1, N_tot) * 3 * sizeof(double));
   dlam_tmp.ensure(std::max<long long>(1, N_tot) * sizeof(double));
   dlam_raw.ensure(std::max<long long>(1, N_tot) * sizeof(double));
-  // 32 x 32 tiles of every block, for the gathers into X and U
+  // 32 x 32 tiles of every block, for the gathers into X and U, then the 32-column groups for the
+  // column norms
   std::vector<int4> tiles_blk;
   for (int bi = 0; bi < nb; bi++) {
     const int T = (blks[bi].n + 31) / 32;
     for (int ti = 0; ti < T; ti++)
       for (int tj = 0; tj < T; tj++) tiles_blk.push_back(int4{bi, ti, tj, 0});
   }
+  const size_t n_tiles_blk = tiles_blk.size();
+  for (int bi = 0; bi < nb; bi++)
+    for (int c0 = 0; c0 < blks[bi].n; c0 += 32) tiles_blk.push_back(int4{bi, 0, c0, 0});
   const int4* d_tiles_blk = upload_to(d_eig_act, tiles_blk.data(), tiles_blk.size());
+  const int4* d_colg = d_tiles_blk + n_tiles_blk;
+  const int n_colg = (int)(tiles_blk.size() - n_tiles_blk);
   d_cid.ensure(std::max<long long>(1, N_tot) * sizeof(int));
   d_sidx.ensure(std::max<long long>(1, N_tot) * sizeof(int));
-  launch_normalize_sort(dblk, nb, d_tiles_blk, (int)tiles_blk.size(), dW.as<float>(), d_sigma.as<float>(),
+  launch_normalize_sort(dblk, nb, d_tiles_blk, (int)n_tiles_blk, d_colg, n_colg, dW.as<float>(), d_sigma.as<float>(),
                         dX.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_rrv.as<double>(),
-                        normalize_sort_lds(nmax), st);
+                        d_rrv.as<double>() + std::max<long long>(1, N_tot), normalize_sort_lds(nmax), st);
   // NRGK_DEBUG_FINITE=1: report the first stage and block that produces a non-finite value
   const bool dbg = knob("NRGK_DEBUG_FINITE", 0) != 0;
   auto check = [&](const char* stage, const DevBuf& buf, bool is_float, bool per_block_vec) {
@@ -665,7 +671,7 @@ void Chain::eigensolve_mixed() {
     act.swap(next);
   }
   oa_unconverged = (int)act.size();
-  launch_final_sort(dblk, nb, d_tiles_blk, (int)tiles_blk.size(), dX.as<double>(), dlam_tmp.as<double>(),
+  launch_final_sort(dblk, nb, d_tiles_blk, (int)n_tiles_blk, dX.as<double>(), dlam_tmp.as<double>(),
                     dU.as<double>(), dlam_raw.as<double>(), d_sidx.as<int>(), final_sort_lds(nmax), st);
   const int* hs = stg.get(d_stats.as<int>(), 4, st);
   sync();

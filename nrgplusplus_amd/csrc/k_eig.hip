@@ -460,7 +460,26 @@ __global__ void __launch_bounds__(JAC_THREADS) k_jacobi_round(const JacTask* __r
 
 // Column norms of the converged W give lambda + sigma; columns are normalised, sorted by lambda
 // and written as FP64 X (row-major, eigenvectors in columns).
-__global__ void __launch_bounds__(1024) k_normalize_sort(const EigBlock* __restrict__ blocks, const float* __restrict__ W,
+// Column norms of W, 32 columns of one block per workgroup (one wave per column at a time)
+__global__ void __launch_bounds__(256) k_col_norms(const EigBlock* __restrict__ blocks, const int4* __restrict__ groups,
+                                                   const float* __restrict__ W, double* __restrict__ nrm_g) {
+  const int4 g = groups[blockIdx.x];
+  const EigBlock blk = blocks[g.x];
+  const int n = blk.n, c1 = min(g.z + 32, n);
+  const float* Wb = W + blk.off;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = g.z + wave; c < c1; c += 4) {
+    double s = 0.0;
+    for (int r = lane; r < n; r += 64) {
+      const double v = Wb[(long long)c * n + r];
+      s += v * v;
+    }
+    s = wave_sum(s);
+    if (lane == 0) nrm_g[blk.e_off + c] = sqrt(s);
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_normalize_sort(const EigBlock* __restrict__ blocks, const double* __restrict__ nrm_g,
                                                          const float* __restrict__ sigma, double* __restrict__ lam,
                                                          int* __restrict__ perm, double* __restrict__ nrm_s) {
   extern __shared__ double sh[];
@@ -472,17 +491,7 @@ __global__ void __launch_bounds__(1024) k_normalize_sort(const EigBlock* __restr
   double* key = sh;                 // [np2]
   int* idx = (int*)(sh + np2);      // [np2]
   double* nrm = sh + np2 + (np2 + 1) / 2 + 1;  // [n]
-  const float* Wb = W + blk.off;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int c = wave; c < n; c += 16) {
-    double s = 0.0;
-    for (int r = lane; r < n; r += 64) {
-      const double v = Wb[(long long)c * n + r];
-      s += v * v;
-    }
-    s = wave_sum(s);
-    if (lane == 0) nrm[c] = sqrt(s);
-  }
+  for (int c = threadIdx.x; c < n; c += blockDim.x) nrm[c] = nrm_g[blk.e_off + c];
   __syncthreads();
   const double sg = sigma[blockIdx.x];
   for (int i = threadIdx.x; i < np2; i += blockDim.x) {
@@ -1333,11 +1342,13 @@ void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int round,
                      round, flags, dgram, dgv, tol, ang_tol, max_inner, jac_far * jac_far);
   NRGK_HIP(hipGetLastError());
 }
-void launch_normalize_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const float* W,
-                           const float* sigma, double* X, double* lam, int* perm, double* nrm_s, size_t lds,
-                           hipStream_t s) {
+void launch_normalize_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const int4* groups,
+                           int n_groups, const float* W, const float* sigma, double* X, double* lam, int* perm,
+                           double* nrm_s, double* nrm_g, size_t lds, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, W, sigma, lam, perm, nrm_s);
+  hipLaunchKernelGGL(k_col_norms, dim3(n_groups), dim3(256), 0, s, blocks, groups, W, nrm_g);
+  NRGK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, nrm_g, sigma, lam, perm, nrm_s);
   NRGK_HIP(hipGetLastError());
   if (n_tiles <= 0) return;
   hipLaunchKernelGGL(k_gather_x, dim3(n_tiles), dim3(256), 0, s, blocks, tiles, W, perm, nrm_s, X);

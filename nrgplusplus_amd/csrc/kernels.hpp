@@ -13,9 +13,9 @@ void launch_shift_convert(const EigBlock* blocks, const int* row_block, const in
 // k_eig.hip
 void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int round, int* flags, double* dgram, int* dgv,
                          float tol, float ang_tol, int max_inner, double jac_far, hipStream_t s);
-void launch_normalize_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const float* W,
-                           const float* sigma, double* X, double* lam, int* perm, double* nrm_s, size_t lds,
-                           hipStream_t s);
+void launch_normalize_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const int4* groups,
+                           int n_groups, const float* W, const float* sigma, double* X, double* lam, int* perm,
+                           double* nrm_s, double* nrm_g, size_t lds, hipStream_t s);
 void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym, const int4* all_tiles, int n_all,
                double* S, double* Rp, double* E, double* lam, int* cid, int* sidx, double* offmax, int* stats,
                double* conv, double* scrW, double* scrG, double* scrT, double* rrv, int nmax, double e_tol,
