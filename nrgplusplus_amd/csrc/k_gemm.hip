@@ -7,14 +7,14 @@
 // grid (prefix sums in GemmProb::tile0), so one launch covers all uneven blocks of an iteration.
 // A problem marked sym (a product known to be symmetric) gets only its upper tiles; the caller
 // reads the upper triangle.
-// The 64x32 / 32x64 operand slabs are staged in LDS (double-buffered); the products run on the FP64
+// The 64x16 / 16x64 operand slabs are staged in LDS (double-buffered); the products run on the FP64
 // matrix cores: each of the 4 waves owns a 32x32 quarter of the tile as 2x2 blocks of 16x16x4 steps.
 #include "common.hpp"
 
 namespace nrgk {
 
 constexpr int GT = 64;   // tile edge
-constexpr int GK = 32;   // k-slab
+constexpr int GK = 16;   // k-slab
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int find_prob(const GemmProb* p, int n, int tile) {
@@ -75,9 +75,13 @@ __device__ __forceinline__ void store_b(const GemmProb& P, const double* r, doub
 }
 
 __global__ void __launch_bounds__(256) k_gemm64(const GemmProb* __restrict__ probs, int n_probs) {
-  const int pi = find_prob(probs, n_probs, blockIdx.x);
+  // workgroup g runs on XCD g mod 8: XCD x takes the contiguous range of tiles [start(x), start(x+1)),
+  // so the tiles of a row panel (and of one problem) share that XCD's L2
+  const int T = gridDim.x, q = T >> 3, r = T & 7, x = blockIdx.x & 7;
+  const int tile = x * q + min(x, r) + (blockIdx.x >> 3);
+  const int pi = find_prob(probs, n_probs, tile);
   const GemmProb P = probs[pi];
-  const int t = blockIdx.x - P.tile0;
+  const int t = tile - P.tile0;
   int tm, tn;
   if (P.sym) {  // upper tiles only (tm <= tn), row by row
     const int T = P.tiles_n;

@@ -56,33 +56,47 @@ void Obs::update() {
   std::vector<std::vector<std::pair<int, int>>> by_ii(c.pre_qn.size());
   for (int s = 0; s < S; s++)
     for (size_t k = 0; k < c.coupled[s].size(); k++) by_ii[c.coupled[s][k].first].push_back({s, (int)k});
+  struct Item {
+    int i, j, k, kp;
+    long long o_off;
+    int o_cols;
+  };
+  std::vector<Item> items;
   for (int p = 0; p < nops; p++) {
-    // (i, j) pairs in sector order; a pair exists when some (ii,bb) in i and (ii',bb) in j have O(ii,ii')
-    std::map<std::pair<int, int>, std::vector<std::pair<int, int>>> pj;  // (i,j) -> (k, kp)
+    // (i, j) pairs in sector order; a pair exists when some (ii,bb) in i and (ii',bb) in j have
+    // O(ii,ii').  The (k, kp) of a pair keep their discovery order (a stable sort on (i, j)).
+    items.clear();
     for (auto& [key, ref] : blocks[p]) {
       const int ii = key.first, iip = key.second;
       if (ii >= (int)by_ii.size() || iip >= (int)by_ii.size()) continue;
       for (auto [i, k] : by_ii[ii])
         for (auto [j, kp] : by_ii[iip])
-          if (c.coupled[i][k].second == c.coupled[j][kp].second) pj[{i, j}].push_back({k, kp});
+          if (c.coupled[i][k].second == c.coupled[j][kp].second)
+            items.push_back(Item{i, j, k, kp, ref.off, ref.cols});
     }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item& x, const Item& y) { return x.i != y.i ? x.i < y.i : x.j < y.j; });
     (void)nb;
-    for (auto& [ij, lst] : pj) {
-      const int i = ij.first, j = ij.second;
+    for (size_t q0 = 0; q0 < items.size();) {
+      size_t q1 = q0 + 1;
+      while (q1 < items.size() && items[q1].i == items[q0].i && items[q1].j == items[q0].j) q1++;
+      const int i = items[q0].i, j = items[q0].j;
       const int ni = c.cur_n[i], nj = c.cur_n[j];
       Pair pr{p, i, j, coff, ooff};
-      for (auto [k, kp] : lst) {
+      for (size_t q = q0; q < q1; q++) {
+        const int k = items[q].k, kp = items[q].kp;
         const int ii = c.coupled[i][k].first, bb = c.coupled[i][k].second;
         const int iip = c.coupled[j][kp].first;
         const int kii = c.pre_kept[ii], kiip = c.pre_kept[iip], dbb = c.bath.dim(bb);
         if ((long long)kii * kiip * dbb == 0) continue;
-        const BlockRef& o = blocks[p].at({ii, iip});
-        segs.push_back(ObsSeg{coff, nj, c.sub_kidx[i][k], c.sub_kidx[j][kp], kii, kiip, dbb, o.off, o.cols});
+        segs.push_back(ObsSeg{coff, nj, c.sub_kidx[i][k], c.sub_kidx[j][kp], kii, kiip, dbb, items[q].o_off,
+                              items[q].o_cols});
       }
-      nbk[p][ij] = BlockRef{ooff, ni, nj};
+      nbk[p][{i, j}] = BlockRef{ooff, ni, nj};
       coff += (long long)ni * nj;
       ooff += (long long)ni * nj;
       pairs.push_back(pr);
+      q0 = q1;
     }
   }
   const int nbuf = cur ^ 1;
