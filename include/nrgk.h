@@ -69,6 +69,10 @@ int nrgk_site_kondo(double J, double S, nrgk_site** out);
 /* f^dagger_sigma chi on a spinhalf site, chi = (-1)^N (examples/fdmSpectrumSiam/main.cpp:86-106);
  * the operators of the returned site are the observable blocks */
 int nrgk_site_fdag_chi_spinhalf(nrgk_site** out);
+/* the double occupancy n_up n_dn of a spinhalf site (spinhalf::doubleOccupancy / getDoubleOcc,
+ * nrgcore/include/models/spinhalf.hpp:106-114, 215): one system operator for the thermodynamic
+ * expectation values */
+int nrgk_site_double_occupancy_spinhalf(nrgk_site** out);
 /* impurity.f_dag_operator.push_back(f[0]); push_back(f[1]) (examples/twoChannelSiam/main.cpp:24-25) */
 int nrgk_site_duplicate_ops(nrgk_site* site);
 /* a view of the site; pointers stay valid until the site is destroyed or modified */
@@ -198,6 +202,17 @@ int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out);
  * scale_offset = 0).  Writes entropy S[nT] and specific heat C[nT]. */
 int nrgk_thermo(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, const double* T,
                 int nT, double* S, double* C);
+/* fdmThermodynamics::setSystemOperator (static/fdmThermodynamics.hpp:39-41): from the next
+ * nrgk_fdm_record on, each recorded shell also keeps the diagonal of the (i, i) blocks of the
+ * operators of sysop in that shell's eigenbasis (calcThermodynamics, fdmThermodynamics.hpp:63-87).
+ * sysop must belong to the recorder's chain and be updated (nrgk_obs_update) before each record. */
+int nrgk_fdm_set_system_operator(nrgk_fdm* rec, nrgk_obs* sysop);
+/* qsysOPeratorExpValue of saveFinalData (fdmThermodynamics.hpp:191-211): the thermal expectation
+ * value of each system operator, sum_m w_m sum_{l discarded in m} exp((gs_m - E_l)/T)/Z'_m O_ll,
+ * with the shell weights and energies of nrgk_thermo.  out: [n_ops][nT].  Every shell used must
+ * have been recorded after nrgk_fdm_set_system_operator. */
+int nrgk_thermo_expectation(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, const double* T,
+                            int nT, double* out);
 
 /* ---------------------------------------------------------------- broadening */
 /* log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44): out[o] = (1/w_o) sum_k

@@ -85,6 +85,12 @@ class site {
     check(nrgk_site_fdag_chi_spinhalf(&s));
     return site(s);
   }
+  // spinhalf::getDoubleOcc (nrgcore/include/models/spinhalf.hpp:106-114, 215): n_up n_dn
+  static site doubleOccupancySpinhalf() {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_double_occupancy_spinhalf(&s));
+    return site(s);
+  }
   // the impurity f^dagger list doubled for two channels (examples/twoChannelSiam/main.cpp:24-25)
   site& duplicate_ops() {
     check(nrgk_site_duplicate_ops(h_.get()));
@@ -358,17 +364,32 @@ class fdmThermodynamics {
     h_.reset(f, [](nrgk_fdm* p) { nrgk_fdm_destroy(p); });
   }
   void record() { check(nrgk_fdm_record(h_.get())); }
+  // setSystemOperator (fdmThermodynamics.hpp:39-41): shells recorded from now on keep the diagonals of
+  // the operators, which the caller updates (update_system_operator) before each record
+  void setSystemOperator(system_operator& ops) {
+    check(nrgk_fdm_set_system_operator(h_.get(), ops.handle()));
+    n_sysop_ = ops.size();
+  }
   void calcThermodynamics(const std::vector<double>& T, double lambda, int scale_offset = -1, int first_cnt = 0) {
     temperatureArray = T;
     entropy.assign(T.size(), 0.0);
     specificHeat.assign(T.size(), 0.0);
     check(nrgk_thermo(h_.get(), lambda, scale_offset, first_cnt, T.data(), (int)T.size(), entropy.data(),
                       specificHeat.data()));
+    qsysOPeratorExpValue.clear();
+    if (n_sysop_) {
+      std::vector<double> v(n_sysop_ * T.size());
+      check(nrgk_thermo_expectation(h_.get(), lambda, scale_offset, first_cnt, T.data(), (int)T.size(), v.data()));
+      for (size_t q = 0; q < n_sysop_; q++)
+        qsysOPeratorExpValue.emplace_back(v.begin() + q * T.size(), v.begin() + (q + 1) * T.size());
+    }
   }
   std::vector<double> temperatureArray, entropy, specificHeat;
+  std::vector<std::vector<double>> qsysOPeratorExpValue;  // [op][T] (fdmThermodynamics.hpp:148-150, 206)
 
  private:
   std::shared_ptr<nrgk_fdm> h_;
+  size_t n_sysop_ = 0;
 };
 
 }  // namespace nrgk

@@ -72,6 +72,7 @@ def lib():
             "nrgk_site_two_channel": (i, [pp]),
             "nrgk_site_kondo": (i, [d, d, pp]),
             "nrgk_site_fdag_chi_spinhalf": (i, [pp]),
+            "nrgk_site_double_occupancy_spinhalf": (i, [pp]),
             "nrgk_site_duplicate_ops": (i, [vp]),
             "nrgk_site_view": (i, [vp, sd]),
             "nrgk_site_destroy": (i, [vp]),
@@ -116,6 +117,8 @@ def lib():
             "nrgk_fdm_backward": (i, [vp, d, i, i, dp, dp, dp, dp, i, ip]),
             "nrgk_fdm_energy_points": (i, [vp, dp]),
             "nrgk_thermo": (i, [vp, d, i, i, dp, i, dp, dp]),
+            "nrgk_fdm_set_system_operator": (i, [vp, vp]),
+            "nrgk_thermo_expectation": (i, [vp, d, i, i, dp, i, dp]),
             "nrgk_broaden": (i, [vp, dp, dp, i, dp, i, d, dp]),
         }
         for name, (res, args) in sig.items():
@@ -176,6 +179,13 @@ class Site:
     def fdag_chi_spinhalf(cls):
         p = C.c_void_p()
         _check(lib().nrgk_site_fdag_chi_spinhalf(C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def double_occupancy_spinhalf(cls):
+        """n_up n_dn of a spinhalf site (spinhalf::getDoubleOcc, spinhalf.hpp:106-114, 215)"""
+        p = C.c_void_p()
+        _check(lib().nrgk_site_double_occupancy_spinhalf(C.byref(p)))
         return cls(p)
 
     @classmethod
@@ -515,6 +525,7 @@ class FdmRecorder:
 
     def __init__(self, chain, op=None, n_bins=100000, e_min=1e-10, e_max=10.0):
         self.chain, self.op = chain, op
+        self.sysop = None
         self.n_bins = n_bins
         p = C.c_void_p()
         _check(lib().nrgk_fdm_create(chain._h, op._h if op is not None else None, n_bins, e_min, e_max, C.byref(p)))
@@ -545,6 +556,21 @@ class FdmRecorder:
         Cv = np.zeros(len(T))
         _check(lib().nrgk_thermo(self._h, lam, scale_offset, first_cnt, _dp(T), len(T), _dp(S), _dp(Cv)))
         return S, Cv
+
+    def setSystemOperator(self, op):
+        """fdmThermodynamics::setSystemOperator (static/fdmThermodynamics.hpp:39-41): shells recorded
+        from now on keep the diagonals of ``op`` (a SystemOperator updated before each record)."""
+        _check(lib().nrgk_fdm_set_system_operator(self._h, op._h))
+        self.sysop = op
+
+    def expectation_values(self, lam, T, scale_offset=-1, first_cnt=0):
+        """qsysOPeratorExpValue (fdmThermodynamics.hpp:191-211): [n_ops][len(T)]"""
+        if self.sysop is None:
+            raise RuntimeError("expectation_values: no system operator set (setSystemOperator)")
+        T = np.ascontiguousarray(T, np.float64)
+        out = np.zeros(self.sysop.n_ops * len(T))
+        _check(lib().nrgk_thermo_expectation(self._h, lam, scale_offset, first_cnt, _dp(T), len(T), _dp(out)))
+        return out.reshape(self.sysop.n_ops, len(T))
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _LIB is not None:

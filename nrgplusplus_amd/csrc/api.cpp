@@ -391,6 +391,23 @@ int nrgk_thermo(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, c
     rec->f->thermo(lambda, scale_offset, first_cnt, T, nT, S, C);
   });
 }
+int nrgk_fdm_set_system_operator(nrgk_fdm* rec, nrgk_obs* sysop) {
+  return guard([&] {
+    NRGK_REQUIRE(rec && sysop, NRGK_ERR_INVALID, "nrgk_fdm_set_system_operator: null argument");
+    NRGK_REQUIRE(sysop->o->ch == rec->f->ch, NRGK_ERR_INVALID,
+                 "nrgk_fdm_set_system_operator: the operator belongs to another chain");
+    rec->f->sysop = sysop->o.get();
+  });
+}
+int nrgk_thermo_expectation(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, const double* T, int nT,
+                            double* out) {
+  return guard([&] {
+    NRGK_REQUIRE(rec && T && out && nT > 0, NRGK_ERR_INVALID, "nrgk_thermo_expectation: null argument");
+    NRGK_REQUIRE(rec->f->sysop, NRGK_ERR_STATE, "nrgk_thermo_expectation: no system operator set");
+    std::vector<double> S(nT), C(nT);
+    rec->f->thermo(lambda, scale_offset, first_cnt, T, nT, S.data(), C.data(), out);
+  });
+}
 int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* w, int n, const double* omega, int n_omega, double b,
                  double* out) {
   return guard([&] {

@@ -295,6 +295,11 @@ struct PtContrib {    // sub-block (ii, bb) of sector i: rows kidx + it + kii*il
   long long rho_off;  // rho_i (n_i x n_i)
   int ni, kidx, dbb;
 };
+struct DiagItem {     // diagonal of block (i, i) of one system operator
+  long long src;      // element offset of the n x n block, -1 if the operator has no such block
+  long long dst;      // op * N_tot + first state of sector i
+  int n;
+};
 
 struct TruncOut {
   double e_cut, e_min, e_max;

@@ -131,6 +131,38 @@ __global__ void __launch_bounds__(256) k_thermo(const double* __restrict__ E, co
   }
 }
 
+// Thermal expectation values of system operators: for each (temperature, operator) and shell m,
+// sum over the shell's discarded states of exp_reg((gs_m - E)/T) * O_ll, O_ll aligned with E
+// (fdmThermodynamics.hpp:191-206 before the w_m / Z'_m weighting).  One workgroup per (T, op).
+__global__ void __launch_bounds__(256) k_thermo_ops(const double* __restrict__ E, const double* __restrict__ Q,
+                                                    long long q_stride, const int* __restrict__ shell_off,
+                                                    const double* __restrict__ gs, int n_shells,
+                                                    const double* __restrict__ T, double* __restrict__ out) {
+  const int t = blockIdx.x, q = blockIdx.y, nq = gridDim.y;
+  const double kT = T[t];
+  const double* Qq = Q + q * q_stride;
+  __shared__ double red[4];
+  for (int m = 0; m < n_shells; m++) {
+    double s = 0;
+    for (int e = shell_off[m] + threadIdx.x; e < shell_off[m + 1]; e += blockDim.x)
+      s += regulate_exp((gs[m] - E[e]) / kT) * Qq[e];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[((size_t)t * nq + q) * n_shells + m] = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+  }
+}
+
+// Diagonal of the (i, i) blocks of the system operators, one workgroup per (operator, sector):
+// dst[j] = src[j * (n + 1)] for j < n, zeros where the operator has no (i, i) block.
+__global__ void __launch_bounds__(256) k_diag_gather(const DiagItem* __restrict__ items, const double* __restrict__ O,
+                                                     double* __restrict__ out) {
+  const DiagItem it = items[blockIdx.x];
+  for (int j = threadIdx.x; j < it.n; j += blockDim.x)
+    out[it.dst + j] = it.src >= 0 ? O[it.src + (long long)j * (it.n + 1)] : 0.0;
+}
+
 __global__ void __launch_bounds__(256) k_broaden(const double* __restrict__ x, const double* __restrict__ w, int n,
                                                  const double* __restrict__ omega, int no, double b,
                                                  double* __restrict__ out) {
@@ -178,6 +210,17 @@ void launch_thermo(const double* E, const int* shell_off, const double* gs, int 
                    double* z, double* m1, double* m2, hipStream_t s) {
   if (nT <= 0 || n_shells <= 0) return;
   hipLaunchKernelGGL(k_thermo, dim3(nT), dim3(256), 0, s, E, shell_off, gs, n_shells, T, z, m1, m2);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_thermo_ops(const double* E, const double* Q, long long q_stride, int nq, const int* shell_off,
+                       const double* gs, int n_shells, const double* T, int nT, double* out, hipStream_t s) {
+  if (nT <= 0 || n_shells <= 0 || nq <= 0) return;
+  hipLaunchKernelGGL(k_thermo_ops, dim3(nT, nq), dim3(256), 0, s, E, Q, q_stride, shell_off, gs, n_shells, T, out);
+  NRGK_HIP(hipGetLastError());
+}
+void launch_diag_gather(const DiagItem* items, int n_items, const double* O, double* out, hipStream_t s) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(k_diag_gather, dim3(n_items), dim3(256), 0, s, items, O, out);
   NRGK_HIP(hipGetLastError());
 }
 void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,

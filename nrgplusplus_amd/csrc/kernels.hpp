@@ -46,6 +46,9 @@ void launch_partial_trace(const void* items, int n_items, const void* cons, cons
                           hipStream_t s);
 void launch_thermo(const double* E, const int* shell_off, const double* gs, int n_shells, const double* T, int nT,
                    double* z, double* m1, double* m2, hipStream_t s);
+void launch_thermo_ops(const double* E, const double* Q, long long q_stride, int nq, const int* shell_off,
+                       const double* gs, int n_shells, const double* T, int nT, double* out, hipStream_t s);
+void launch_diag_gather(const DiagItem* items, int n_items, const double* O, double* out, hipStream_t s);
 void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,
                     hipStream_t s);
 void launch_transpose(const double* in, double* out, int n, hipStream_t s);

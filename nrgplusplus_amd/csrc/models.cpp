@@ -308,6 +308,19 @@ Site* site_fdag_chi_spinhalf() {
   return new Site(make_site(B, E, B.block_ops(ops)));
 }
 
+// The double occupancy n_up n_dn of the spin-1/2 impurity in its sector blocks (the reference's
+// spinhalf::doubleOccupancy, nrgcore/include/models/spinhalf.hpp:106-114): a system operator for
+// the thermodynamic expectation values.
+Site* site_double_occupancy_spinhalf() {
+  Basis B;
+  B.jordan_wigner(2);
+  B.charge_and_spin();
+  const M n_up = mul(B.fdag[0], tr(B.fdag[0])), n_dn = mul(B.fdag[1], tr(B.fdag[1]));
+  std::vector<std::vector<double>> E;
+  for (auto& b : B.blocks) E.push_back(std::vector<double>(b.size(), 0.0));
+  return new Site(make_site(B, E, B.block_ops({mul(n_up, n_dn)})));
+}
+
 }  // namespace nrgk
 
 struct nrgk_site {
@@ -341,6 +354,13 @@ int nrgk_site_kondo(double J, double S, nrgk_site** out) {
 int nrgk_site_fdag_chi_spinhalf(nrgk_site** out) {
   if (!out) return NRGK_ERR_INVALID;
   nrgk::Site* s = nrgk::site_fdag_chi_spinhalf();
+  *out = new nrgk_site{std::move(*s)};
+  delete s;
+  return NRGK_OK;
+}
+int nrgk_site_double_occupancy_spinhalf(nrgk_site** out) {
+  if (!out) return NRGK_ERR_INVALID;
+  nrgk::Site* s = nrgk::site_double_occupancy_spinhalf();
   *out = new nrgk_site{std::move(*s)};
   delete s;
   return NRGK_OK;

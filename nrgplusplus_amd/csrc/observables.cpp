@@ -128,7 +128,7 @@ void Obs::update() {
 Fdm::Fdm(Chain* c, Obs* o, int nb, double emin, double emax) : ch(c), obs(o), nbins(nb), e_min(emin), e_max(emax) {
   NRGK_REQUIRE(nb > 0 && emin > 0 && emax > emin, NRGK_ERR_INVALID, "nrgk: bad FDM binning");
   delE = (nbins - 1.0) / std::log(e_max / e_min);  // fdmSpectrum.hpp:351-354
-  for (DevBuf* b : {&dpos, &dneg, &dsum, &dbf, &dred[0], &dred[1], &dRA, &dAR, &dT1, &drho, &ditems, &dpt, &dE, &dth})
+  for (DevBuf* b : {&dpos, &dneg, &dsum, &dbf, &dred[0], &dred[1], &dRA, &dAR, &dT1, &drho, &ditems, &dpt, &dE, &dth, &dQ})
     b->ctx = ch->ctx;
 }
 
@@ -169,8 +169,29 @@ void Fdm::record() {
     NRGK_HIP(hipMemcpyAsync(sh->dO.p, obs->buf[obs->cur].p, tot * sizeof(double), hipMemcpyDeviceToDevice, c.st));
     sh->ops = obs->blocks;
   }
+  // the system operators' diagonals in this shell's eigenbasis (fdmThermodynamics.hpp:63-87)
+  const double* h_q = nullptr;
+  if (sysop) {
+    const int S = (int)c.cur_n.size();
+    std::vector<DiagItem> items;
+    for (int q = 0; q < sysop->nops; q++)
+      for (int i = 0; i < S; i++) {
+        auto f = sysop->blocks[q].find({i, i});
+        NRGK_REQUIRE(f == sysop->blocks[q].end() || (f->second.rows == c.cur_n[i] && f->second.cols == c.cur_n[i]),
+                     NRGK_ERR_STATE, "nrgk: system operator not updated for this iteration");
+        items.push_back({f == sysop->blocks[q].end() ? -1LL : f->second.off, (long long)q * c.N_tot + c.cur_eoff[i],
+                         c.cur_n[i]});
+      }
+    const size_t nq = (size_t)sysop->nops * c.N_tot;
+    dQ.ensure(std::max<size_t>(1, nq) * sizeof(double));
+    launch_diag_gather(c.stg.put(items, c.st), (int)items.size(), sysop->buf[sysop->cur].as<double>(),
+                       dQ.as<double>(), c.st);
+    h_q = c.stg.get(dQ.as<double>(), nq, c.st);
+    sh->q_ops = sysop->nops;
+  }
   c.sync();
   sh->E.assign(h_e, h_e + c.N_tot);
+  if (h_q) sh->Q.assign(h_q, h_q + (size_t)sh->q_ops * c.N_tot);
   shells.push_back(std::move(sh));
 }
 
@@ -390,7 +411,8 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
 }
 
 // fdmThermodynamics (fdmThermodynamics.hpp:42-236) over the recorded shells.
-void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S_out, double* C_out) {
+void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S_out, double* C_out,
+                 double* ops_out) {
   Chain& c = *ch;
   hipStream_t st = c.st;
   std::vector<Shell*> use;
@@ -401,11 +423,15 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
   size_t bathDim = 0;
   for (int b = 0; b < c.bath.S; b++) bathDim += c.bath.dim(b);
   // calcThermodynamics: discarded (index >= kept of this shell) and all energies times the scale
+  const int nq = ops_out ? (sysop ? sysop->nops : 0) : 0;
   std::vector<double> disc;
+  std::vector<std::vector<double>> qd(nq);  // O_ll of the discarded states, aligned with disc
   std::vector<int> off{0};
   std::vector<double> gs(ns);
   for (int m = 0; m < ns; m++) {
     Shell& sh = *use[m];
+    NRGK_REQUIRE(sh.q_ops >= nq && (nq == 0 || !sh.Q.empty()), NRGK_ERR_STATE,
+                 "nrgk: a shell was recorded without the system operator");
     const double scale = std::pow(lambda, -((double)sh.cnt + scale_offset) / 2.0);
     const std::vector<int> kc = m == 0 ? std::vector<int>(sh.n.size(), 0) : use[m - 1]->pre_kept;
     NRGK_REQUIRE(kc.size() == sh.n.size(), NRGK_ERR_STATE, "nrgk: recorded shells are not consecutive");
@@ -413,7 +439,10 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
     for (size_t i = 0; i < sh.n.size(); i++)
       for (int j = 0; j < sh.n[i]; j++) mn = std::min(mn, sh.E[sh.eoff[i] + j] * scale);
     for (size_t i = 0; i < sh.n.size(); i++)
-      for (int j = kc[i]; j < sh.n[i]; j++) disc.push_back(sh.E[sh.eoff[i] + j] * scale - mn);
+      for (int j = kc[i]; j < sh.n[i]; j++) {
+        disc.push_back(sh.E[sh.eoff[i] + j] * scale - mn);
+        for (int q = 0; q < nq; q++) qd[q].push_back(sh.Q[(size_t)q * sh.E.size() + sh.eoff[i] + j]);
+      }
     off.push_back((int)disc.size());
     gs[m] = mn;
   }
@@ -435,7 +464,16 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
   h2d(dT, T, nT, st);
   h2d(doff, off.data(), off.size(), st);
   launch_thermo(dE.as<double>(), doff, dgs, ns, dT, nT, dz, dm1, dm2, st);
-  std::vector<double> z(nz), m1(nz), m2(nz);
+  std::vector<double> z(nz), m1(nz), m2(nz), mq((size_t)nq * nz);
+  if (nq) {
+    const size_t nd = std::max<size_t>(1, disc.size());
+    dQ.ensure(((size_t)nq * nd + (size_t)nq * nz) * sizeof(double));
+    double* dq = dQ.as<double>();
+    double* dmq = dq + (size_t)nq * nd;
+    for (int q = 0; q < nq; q++) h2d(dq + (size_t)q * nd, qd[q].data(), qd[q].size(), st);
+    launch_thermo_ops(dE.as<double>(), dq, (long long)nd, nq, doff, dgs, ns, dT, nT, dmq, st);
+    d2h(mq.data(), dmq, mq.size(), st);
+  }
   d2h(z.data(), dz, nz, st);
   d2h(m1.data(), dm1, nz, st);
   d2h(m2.data(), dm2, nz, st);
@@ -462,6 +500,16 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
     }
     S_out[t] = std::log(lpart) + eAv / kT;
     C_out[t] = (eSq - eAv * eAv) / (kT * kT);
+    // qsysOPeratorExpValue[iq][itm] (fdmThermodynamics.hpp:191-206)
+    for (int q = 0; q < nq; q++) {
+      double v = 0;
+      for (int m = 0; m < ns; m++) {
+        const double zp = z[(size_t)t * ns + m];
+        if (zp == 0.0) continue;
+        v += wm[m] * mq[((size_t)t * nq + q) * ns + m] / zp;
+      }
+      ops_out[(size_t)q * nT + t] = v;
+    }
   }
 }
 

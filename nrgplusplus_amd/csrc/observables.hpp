@@ -27,6 +27,10 @@ struct Shell {
   std::vector<std::vector<int>> sub_kidx;
   std::vector<int> pre_kept;  // eigenvaluesQ_kept_indices sizes at save time (previous iteration)
   std::vector<double> E;      // eigenvalues (pre-shift), concatenated
+  // diagonal of the (i, i) blocks of the system operators, [op][state] as E (0 where no block);
+  // empty when the shell was recorded without a system operator
+  std::vector<double> Q;
+  int q_ops = 0;
   // U blocks, eigenvalues and observable blocks of the shell: sub-allocations of the recorder's arena
   struct Ref {
     void* p = nullptr;
@@ -41,6 +45,7 @@ struct Shell {
 struct Fdm {
   Chain* ch = nullptr;
   Obs* obs = nullptr;
+  Obs* sysop = nullptr;  // fdmThermodynamics::setSystemOperator
   int nbins = 0;
   double e_min = 1e-10, e_max = 10, delE = 0;
   std::vector<std::unique_ptr<Shell>> shells;
@@ -48,12 +53,14 @@ struct Fdm {
   std::vector<std::unique_ptr<DevBuf>> chunks;
   size_t chunk_used = 0;
   void* arena_alloc(size_t bytes);
-  DevBuf dpos, dneg, dsum, dbf, dred[2], dRA, dAR, dT1, drho, ditems, dpt, dE, dth;
+  DevBuf dpos, dneg, dsum, dbf, dred[2], dRA, dAR, dT1, drho, ditems, dpt, dE, dth, dQ;
   Fdm(Chain* c, Obs* o, int nb, double emin, double emax);
   void record();
   void backward(double lambda, int scale_offset, int first_cnt, double* pos, double* neg, double* traces,
                 double* sums, int cap, int* n_shells);
-  void thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S, double* C);
+  // ops_out ([sysop ops][nT], may be null): the system operators' thermal expectation values
+  void thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S, double* C,
+              double* ops_out = nullptr);
 };
 
 }  // namespace nrgk

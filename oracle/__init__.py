@@ -87,6 +87,7 @@ def lib():
             "orc_fdag_block": (None, [vp, i, i, dp]),
             "orc_run_rgflow": (i, [C.POINTER(OrcDriver), dp, l, lp]),
             "orc_run_thermo": (i, [C.POINTER(OrcDriver), i, dp, i, dp, dp, i]),
+            "orc_run_thermo_ops": (i, [C.POINTER(OrcDriver), i, dp, i, i, dp, dp, dp, ip]),
             "orc_run_fdm": (i, [C.POINTER(OrcDriver), i, dp, dp, dp, dp, ip]),
             "orc_broaden": (None, [dp, dp, i, dp, i, d, dp]),
             "orc_time_forward": (i, [C.POINTER(OrcDriver), i, i, dp]),
@@ -295,6 +296,19 @@ def run_thermo(drv, T, min_iter0, low_clamp_zero=False):
     Cv = np.zeros(len(T))
     _check(lib().orc_run_thermo(C.byref(drv), min_iter0, _dp(T), len(T), _dp(S), _dp(Cv), int(low_clamp_zero)))
     return S, Cv
+
+
+def run_thermo_ops(drv, T, min_iter0, op_kind):
+    """entropy, specific heat and the thermal expectation values [n_ops][len(T)] of the system
+    operators op_kind 0: {n_up n_dn}, 1: {n_up, n_dn, identity} of a spinhalf impurity"""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    S = np.zeros(len(T))
+    Cv = np.zeros(len(T))
+    V = np.zeros(3 * len(T))
+    nq = C.c_int()
+    _check(lib().orc_run_thermo_ops(C.byref(drv), min_iter0, _dp(T), len(T), op_kind, _dp(S), _dp(Cv), _dp(V),
+                                    C.byref(nq)))
+    return S, Cv, V[: nq.value * len(T)].reshape(nq.value, len(T))
 
 
 def run_fdm(drv, nbins):

@@ -831,8 +831,13 @@ struct FdmThermo {
     if (x < -100) return low_clamp_zero ? 0.0 : std::exp(-100);
     return std::exp(x);
   }
-  // calcThermodynamics (fdmThermodynamics.hpp:42-89), without the optional system operator
-  void calc(const Shell& sh, double scale) {
+  // setSystemOperator (fdmThermodynamics.hpp:39-41) records, per shell and operator, the (i, i)
+  // diagonal entries of the discarded states keyed by their index among the discarded energies.
+  std::vector<std::vector<std::map<size_t, double>>> qsysOPeratorValue;
+  std::vector<std::vector<double>> qsysOPeratorExpValue;  // [op][T], filled by finish()
+  // calcThermodynamics (fdmThermodynamics.hpp:42-89); sysop: the system operators in the shell's
+  // eigenbasis (the reference reads them through its stored pointer at this point)
+  void calc(const Shell& sh, double scale, const std::vector<BlockOp>* sysop = nullptr) {
     if (currentKeptIndex.empty()) currentKeptIndex.assign(sh.current_sysmQ.size(), {});
     else currentKeptIndex = previoudKeptIndex;
     previoudKeptIndex = sh.kept_indices;
@@ -844,6 +849,19 @@ struct FdmThermo {
     for (size_t i = 0; i < sh.eigenvaluesQ.size(); i++)
       for (size_t j = 0; j < sh.eigenvaluesQ[i].size(); j++) ev.push_back(sh.eigenvaluesQ[i][j] * scale);
     allEigValues.push_back(ev);
+    if (sysop) {
+      std::vector<std::map<size_t, double>> qvalue(sysop->size());
+      for (size_t iq = 0; iq < sysop->size(); iq++) {
+        size_t icounter = 0;
+        for (size_t i = 0; i < sh.eigenvaluesQ.size(); i++) {
+          const size_t kept = currentKeptIndex[i].size();
+          if (const Mat* q = (*sysop)[iq].get(i, i))
+            for (size_t j = kept; j < sh.eigenvaluesQ[i].size(); j++) qvalue[iq][icounter + j - kept] = (*q)(j, j);
+          icounter += sh.eigenvaluesQ[i].size() - kept;
+        }
+      }
+      qsysOPeratorValue.push_back(qvalue);
+    }
   }
   // saveFinalData (fdmThermodynamics.hpp:101-236): returns (entropy, specificHeat)
   std::pair<std::vector<double>, std::vector<double>> finish() {
@@ -857,6 +875,8 @@ struct FdmThermo {
     double g0 = gs[0];
     for (auto& x : gs) x = x - g0;
     std::vector<double> S(temperatureArray.size()), C(temperatureArray.size());
+    const size_t nq = qsysOPeratorValue.empty() ? 0 : qsysOPeratorValue[0].size();
+    qsysOPeratorExpValue.assign(nq, std::vector<double>(temperatureArray.size(), 0.0));
     for (size_t itm = 0; itm < temperatureArray.size(); itm++) {
       double kBT = temperatureArray[itm];
       int cnt = nrgMaxIterations;
@@ -878,6 +898,16 @@ struct FdmThermo {
           eAv += (a - gs[id]) * wm[id] * blm;
           eSq += (a - gs[id]) * (a - gs[id]) * wm[id] * blm;
         }
+      // qsysOPeratorExpValue (fdmThermodynamics.hpp:191-206)
+      for (size_t iq = 0; iq < nq; iq++) {
+        double tvalue = 0;
+        for (size_t id = 0; id < discardedEigValues.size(); id++)
+          for (auto const& [ie, qval] : qsysOPeratorValue[id][iq]) {
+            double blm = regulate_exp((gs[id] - discardedEigValues[id][ie]) / kBT) / Zp[id];
+            tvalue += qval * wm[id] * blm;
+          }
+        qsysOPeratorExpValue[iq][itm] = tvalue;
+      }
       S[itm] = std::log(lpart) + (eAv / kBT);
       C[itm] = (eSq - eAv * eAv) / (kBT * kBT);
     }

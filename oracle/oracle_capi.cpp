@@ -280,6 +280,58 @@ int orc_run_thermo(const OrcDriver* d, int min_iter0, const double* T, int nT, d
   });
 }
 
+// Thermodynamics driver with system operators on a spinhalf impurity (fdmThermodynamics::
+// setSystemOperator, static/fdmThermodynamics.hpp:39-41, 63-87, 191-211): the operators are rotated
+// with update_system_operator after every add_bath_site and saved beside each shell, then read back
+// by calcThermodynamics in the backward loop of examples/entropySIAM/main.cpp:79-92.
+// op_kind 0: {n_up n_dn} (spinhalf.hpp:114); 1: {n_up, n_dn, identity}.  V_out: [n_ops][nT].
+int orc_run_thermo_ops(const OrcDriver* d, int min_iter0, const double* T, int nT, int op_kind, double* S_out,
+                       double* C_out, double* V_out, int* n_ops) {
+  return guard([&] {
+    if (d->imp_kind != 0) throw std::runtime_error("orc_run_thermo_ops: spinhalf impurity only");
+    SiteModel imp = driver_imp(*d), bath = driver_bath(*d);
+    FermionBasis fb(2);
+    const Mat nu = dot(fb.fdag_full[0], transpose(fb.fdag_full[0]));
+    const Mat nd = dot(fb.fdag_full[1], transpose(fb.fdag_full[1]));
+    std::vector<BlockOp> ops = op_kind == 0 ? imp.double_occupancy : fb.block_operators({nu, nd, eye(4)});
+    *n_ops = (int)ops.size();
+    Chain c(imp, bath);
+    c.set_parameters((size_t)d->K);
+    size_t nop = imp.f_dag_operator.size();
+    std::map<int, Shell> shells;
+    size_t minIterations = (size_t)min_iter0;
+    if (d->first_V > 0) {
+      c.add_bath_site(std::vector<double>(nop, d->first_V), 1.0);
+      update_system_operator(c, ops);
+      c.update_internal_state();
+    }
+    for (int in = 0; in < d->nsites; in++) {
+      double rescale = d->first_V > 0 ? (in > 0 ? std::sqrt(d->Lambda) : 1.0) : std::sqrt(d->Lambda);
+      c.add_bath_site(std::vector<double>(nop, hopping(in, d->Lambda)), rescale);
+      update_system_operator(c, ops);
+      if (c.checkHigherEnergyDiscarded()) {
+        shells[c.nrg_iterations_cnt] = save_shell(c);
+        shells[c.nrg_iterations_cnt].ops = ops;
+      } else {
+        minIterations++;
+      }
+      c.update_internal_state();
+    }
+    FdmThermo th(c.nrg_iterations_cnt, c.bath_eigenvaluesQ, std::vector<double>(T, T + nT));
+    for (size_t in = (size_t)d->nsites; in > minIterations; in--) {
+      double sc = std::pow(d->Lambda, -(1.0 * (double)in - 1.0) / 2.0);
+      c.nrg_iterations_cnt--;
+      const Shell& sh = shells.at(c.nrg_iterations_cnt);
+      th.calc(sh, sc, &sh.ops);
+    }
+    auto [S, C] = th.finish();
+    std::memcpy(S_out, S.data(), S.size() * sizeof(double));
+    std::memcpy(C_out, C.data(), C.size() * sizeof(double));
+    for (size_t q = 0; q < th.qsysOPeratorExpValue.size(); q++)
+      std::memcpy(V_out + q * nT, th.qsysOPeratorExpValue[q].data(), nT * sizeof(double));
+  });
+}
+
 // T = 0 FDM spectral function of f^dagger chi (examples/fdmSpectrumSiam/main.cpp:61-187, with the
 // hopping list sized to the operator count).  pos/neg: [n_ops][nbins]; traces/sums per shell.
 int orc_run_fdm(const OrcDriver* d, int nbins, double* pos, double* neg, double* traces, double* sums,

@@ -197,6 +197,62 @@ def test_fdm_spectrum_matches_oracle(nk, oracle_mod):
             assert np.abs(ag - ao).max() <= 1e-6 * max(1e-12, np.abs(ao).max())
 
 
+def _occupation_ops(nk):
+    """n_up, n_dn and the identity on the spinhalf impurity's sectors (1x1 blocks, zero blocks left
+    out as the reference's get_block_operators does, fermionBasis.hpp:291-318)."""
+    qn, dims, en, chi, _ = nk.Site.spinhalf(0.0, 0.0).export()
+    assert np.all(dims == 1)
+    ops = [[(i, i, np.array([[float(qn[i][0])]])) for i in range(len(dims)) if qn[i][0]],
+           [(i, i, np.array([[float(qn[i][1])]])) for i in range(len(dims)) if qn[i][1]],
+           [(i, i, np.eye(1)) for i in range(len(dims))]]
+    return nk.Site.from_arrays(qn, dims, en, chi, ops)
+
+
+def test_thermal_expectation_values_match_oracle(nk, oracle_mod):
+    """fdmThermodynamics with system operators (setSystemOperator / qsysOPeratorExpValue,
+    fdmThermodynamics.hpp:39-41, 63-87, 191-211) on a truncated SIAM chain: double occupancy,
+    n_up, n_dn and the identity against the oracle to 1e-8 absolute (eigenvalues to 1e-10; sums over
+    degenerate multiplets are basis-independent), the identity to 1 within 1e-12."""
+    eps, U, G, L, K, ns = -0.15, 0.3, 0.01, 2.0, 128, 24
+    V = np.sqrt(2 * G / np.pi)
+    T = np.logspace(-10, 0, 41)
+    g = nk.NrgCore(nk.Site.spinhalf(eps, U), nk.Site.spinhalf(0, 0), K)
+    dbl = nk.SystemOperator(g, nk.Site.double_occupancy_spinhalf())
+    occ = nk.SystemOperator(g, _occupation_ops(nk))
+    rec_d, rec_o = nk.FdmRecorder(g, None), nk.FdmRecorder(g, None)
+    rec_d.setSystemOperator(dbl)
+    rec_o.setSystemOperator(occ)
+    g.add_bath_site([V, V], 1.0)
+    dbl.update()
+    occ.update()
+    g.update_internal_state()
+    min_it = 0
+    for n in range(ns):
+        t = nk.hopping(n, L)
+        g.add_bath_site([t, t], 1.0 if n == 0 else np.sqrt(L))
+        dbl.update()
+        occ.update()
+        if g.checkHigherEnergyDiscarded():
+            rec_d.record()
+            rec_o.record()
+        else:
+            min_it += 1
+        g.update_internal_state()
+    vd = rec_d.expectation_values(L, T, scale_offset=-1, first_cnt=min_it + 1)
+    vo = rec_o.expectation_values(L, T, scale_offset=-1, first_cnt=min_it + 1)
+    S, Cv = rec_o.thermodynamics(L, T, scale_offset=-1, first_cnt=min_it + 1)
+    drv = oracle_mod.driver(imp_kind=0, eps=eps, U=U, bath_kind=0, Lambda=L, first_V=V, nsites=ns, K=K)
+    oS, oC, od = oracle_mod.run_thermo_ops(drv, T, 0, 0)
+    _, _, oo = oracle_mod.run_thermo_ops(drv, T, 0, 1)
+    np.testing.assert_allclose(vo[2], 1.0, atol=1e-12)
+    assert np.abs(vd - od).max() < 1e-8, np.abs(vd - od).max()
+    assert np.abs(vo - oo).max() < 1e-8, np.abs(vo - oo).max()
+    assert np.abs(S - oS).max() < 1e-8
+    # the recorder without a system operator refuses the expectation values
+    with pytest.raises(RuntimeError):
+        nk.FdmRecorder(g, None).expectation_values(L, T)
+
+
 def test_broadening_matches_oracle(nk, oracle_mod):
     rng = np.random.default_rng(1)
     x = np.sort(10 ** rng.uniform(-9, 1, 300))

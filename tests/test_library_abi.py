@@ -97,6 +97,17 @@ def test_fdag_chi_observable(nk, oracle_mod):
         np.testing.assert_array_equal(a, b @ chi)
 
 
+def test_double_occupancy_observable(nk):
+    """n_up n_dn (spinhalf.hpp:106-114) from the f-dagger matrices of the same basis: f f^T products."""
+    f = _full(nk.Site.spinhalf(0.0, 0.0))
+    d = _full(nk.Site.double_occupancy_spinhalf())
+    assert len(d) == 1
+    np.testing.assert_array_equal(d[0], (f[0] @ f[0].T) @ (f[1] @ f[1].T))
+    qn, dims, _, _, ops = nk.Site.double_occupancy_spinhalf().export()
+    # one 1x1 block, in the doubly occupied sector
+    assert [(i, j) for i, j, _ in ops[0]] == [(int(np.nonzero((qn == [1, 1]).all(1))[0][0]),) * 2]
+
+
 def test_duplicate_ops(nk):
     s = nk.Site.spinhalf(-0.2, 0.4).duplicate_ops()
     _, _, _, _, ops = s.export()

@@ -220,3 +220,45 @@ def test_fdm_reduced_density_matrices_have_unit_trace(oracle_mod):
     assert len(tr) >= 8
     np.testing.assert_allclose(tr, 1.0, atol=1e-12)
     assert np.all(pos >= 0) and np.all(neg >= 0)
+
+
+# ---------------------------------------------------------------- static observables
+def test_thermal_expectation_equals_exact_diagonalisation(oracle_mod):
+    """With K equal to the 256 states of iteration 1, the one recorded shell is the full exact
+    eigenbasis of the 10-orbital chain, with every state discarded and w_m = 1.  Then
+    qsysOPeratorExpValue (fdmThermodynamics.hpp:191-206) is the thermal average
+    sum_l x_l <l|O|l> / sum_l x_l over the exact Jordan-Wigner chain, x_l = exp((E_0 - E_l) s / T) with
+    s = 1/Lambda and the exponent clamped at -100 (regulateExp, fdmThermodynamics.hpp:240-249), and
+    the entropy is ln sum_l x_l + <(E - E_0) s>/T.  Operators: n_up n_dn, n_up, n_dn and the identity."""
+    eps, U, G, L = -0.2, 0.5, 0.02, 2.0
+    V = np.sqrt(2 * G / np.pi)
+    drv = oracle_mod.driver(imp_kind=0, eps=eps, U=U, bath_kind=0, Lambda=L, first_V=V, nsites=3, K=256)
+    T = np.logspace(-4, 1, 31)
+    S, _, occ = oracle_mod.run_thermo_ops(drv, T, 0, 1)
+    _, _, dbl = oracle_mod.run_thermo_ops(drv, T, 0, 0)
+    ts = [oracle_mod.hopping(n, L) for n in range(3)]
+    H, cu, cd = _ed_siam_final(eps, U, V, ts, [1.0, np.sqrt(L), np.sqrt(L)])
+    w, Q = np.linalg.eigh(H)
+    nu, nd = cu @ cu.T, cd @ cd.T
+    x = np.exp(np.clip((w[0] - w)[None, :] / L / T[:, None], -100, 100))
+    Z = x.sum(1)
+    for got, O in ((dbl[0], nu @ nd), (occ[0], nu), (occ[1], nd), (occ[2], np.eye(len(w)))):
+        diag = np.einsum("ij,ik,kj->j", Q, O, Q)
+        np.testing.assert_allclose(got, x @ diag / Z, rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(S, np.log(Z) + (x @ ((w - w[0]) / L)) / Z / T, rtol=1e-9, atol=1e-12)
+
+
+def test_thermal_occupation_at_particle_hole_symmetry(oracle_mod):
+    """At eps = -U/2 the truncated chain is particle-hole symmetric and every shell's discarded set is
+    closed under the symmetry, so <n_up> = <n_dn> = 1/2 at every temperature; the identity averages
+    to 1 (normalised shell weights w_m over Z'_m).  Double occupancy lies in [0, 1/4] for U > 0."""
+    U, L = 0.5, 2.5
+    drv = oracle_mod.driver(imp_kind=0, eps=-U / 2, U=U, bath_kind=0, Lambda=L,
+                            first_V=np.sqrt(2 * 0.02 / np.pi), nsites=20, K=128)
+    T = np.logspace(-9, 0, 40)
+    _, _, occ = oracle_mod.run_thermo_ops(drv, T, 0, 1)
+    _, _, dbl = oracle_mod.run_thermo_ops(drv, T, 0, 0)
+    np.testing.assert_allclose(occ[0], 0.5, atol=1e-9)
+    np.testing.assert_allclose(occ[1], 0.5, atol=1e-9)
+    np.testing.assert_allclose(occ[2], 1.0, atol=1e-12)
+    assert np.all(dbl[0] > 0) and np.all(dbl[0] < 0.25)
