@@ -234,18 +234,52 @@ void Chain::import_impurity() {
 void Chain::create_next_basis() {
   cur_qn.clear();
   coupled.clear();
-  // sectors in first-seen order over (i, j); the index map replaces the reference's linear search
+  // sectors in first-seen order over (i, j); a hash of the quantum numbers replaces the reference's
+  // linear search.  Up to four quantum numbers in [-32768, 32767] pack into one 64-bit key (open
+  // addressing, linear probing); other labels go through an ordered map.
+  const size_t combos = pre_qn.size() * (size_t)bath.S;
+  bool packable = bath.nqn <= 4;
+  for (int x = 0; x < bath.nqn && packable; x++) {
+    long long lo = 0, hi = 0, blo = 0, bhi = 0;
+    for (auto& q : pre_qn) lo = std::min<long long>(lo, q[x]), hi = std::max<long long>(hi, q[x]);
+    for (auto& q : bath.qn) blo = std::min<long long>(blo, q[x]), bhi = std::max<long long>(bhi, q[x]);
+    packable = lo + blo >= -32768 && hi + bhi <= 32767;
+  }
+  auto pack = [&](const std::vector<int>& q) {
+    unsigned long long k = 0;
+    for (int x = 0; x < bath.nqn; x++) k = (k << 16) | (unsigned long long)(unsigned short)(q[x] + 32768);
+    return k;
+  };
+  size_t cap = 16;
+  while (cap < 2 * combos) cap <<= 1;
+  std::vector<unsigned long long> hkey(cap);
+  std::vector<int> hval(cap, -1);
   std::map<std::vector<int>, int> index;
   std::vector<int> tm;
   for (size_t i = 0; i < pre_qn.size(); i++)
     for (int j = 0; j < bath.S; j++) {
       tm = pre_qn[i];
       for (int x = 0; x < bath.nqn; x++) tm[x] += bath.qn[j][x];
-      auto it = index.find(tm);
-      if (it != index.end()) {
-        coupled[it->second].push_back({(int)i, j});
+      const unsigned long long key = packable ? pack(tm) : 0;
+      int found = -1;
+      size_t slot = 0;
+      if (packable) {
+        slot = (size_t)((key * 0x9E3779B97F4A7C15ull) >> 20) & (cap - 1);
+        while (hval[slot] >= 0 && hkey[slot] != key) slot = (slot + 1) & (cap - 1);
+        found = hval[slot];
       } else {
-        index.emplace(tm, (int)cur_qn.size());
+        auto it = index.find(tm);
+        if (it != index.end()) found = it->second;
+      }
+      if (found >= 0) {
+        coupled[found].push_back({(int)i, j});
+      } else {
+        if (packable) {
+          hkey[slot] = key;
+          hval[slot] = (int)cur_qn.size();
+        } else {
+          index.emplace(tm, (int)cur_qn.size());
+        }
         cur_qn.push_back(tm);
         coupled.push_back({{(int)i, j}});
       }
@@ -749,41 +783,53 @@ void Chain::update_internal_state() {
 // f-dagger of the new site in the kept basis (nrgcore::set_current_fdag_operator, nrgcore.hpp:401-477)
 void Chain::rotate_fdag() {
   const int S = (int)cur_qn.size();
+  const int SB = bath.S, Sp = (int)pre_qn.size();
   // (ii, bb) -> (sector, sub index)
-  std::map<std::pair<int, int>, std::pair<int, int>> where;
+  std::vector<std::pair<int, int>> where((size_t)Sp * SB, {-1, -1});
   for (int s = 0; s < S; s++)
-    for (size_t k = 0; k < coupled[s].size(); k++) where[coupled[s][k]] = {s, (int)k};
+    for (size_t k = 0; k < coupled[s].size(); k++)
+      where[(size_t)coupled[s][k].first * SB + coupled[s][k].second] = {s, (int)k};
   std::vector<RotTri> tris;
   std::vector<RotGroup> groups;
   std::vector<RotSegment> segs;
   std::vector<GemmProb> gp;
   std::vector<OpMap> fnew(imp.nops);
   long long yoff = 0, foff = 0;
-  struct Pending {
-    int p, i, j;
+  // coupled sub-block pairs of one operator: sub-block k of sector i, sub-block sj of sector j, bath
+  // block F_bath[p](bb, bbp)
+  struct Trip {
+    int i, j, k, sj, bbp;
+    BlockRef fb;
   };
-  std::vector<Pending> pend;
+  std::vector<Trip> tl;
   for (int p = 0; p < imp.nops; p++) {
-    // triples (p, i, j) in the reference's map order: (p, i, j) lexicographic
-    std::map<std::pair<int, int>, std::vector<std::tuple<int, int, int>>> trip;  // (i,j) -> (sub_i, sub_j, fb key)
+    // triples (p, i, j) in the reference's map order, (i, j) lexicographic; the sub-block pairs of a
+    // triple keep their discovery order (a stable sort on (i, j))
+    tl.clear();
     for (int i = 0; i < S; i++)
       for (size_t k = 0; k < coupled[i].size(); k++) {
         auto [ii, bb] = coupled[i][k];
         for (auto& [key, ref] : bathF[p]) {
           if (key.first != bb) continue;
-          const int bbp = key.second;
-          auto w = where.find({ii, bbp});
-          if (w == where.end()) continue;
-          trip[{i, w->second.first}].push_back({(int)k, w->second.second, bbp});
+          const auto w = where[(size_t)ii * SB + key.second];
+          if (w.first < 0) continue;
+          tl.push_back(Trip{i, w.first, (int)k, w.second, key.second, ref});
         }
       }
-    for (auto& [ij, lst] : trip) {
-      const int i = ij.first, j = ij.second;
+    std::stable_sort(tl.begin(), tl.end(), [](const Trip& x, const Trip& y) { return x.i != y.i ? x.i < y.i : x.j < y.j; });
+    fnew[p].reserve(tl.size());
+    for (size_t q0 = 0; q0 < tl.size();) {
+      size_t q1 = q0 + 1;
+      while (q1 < tl.size() && tl[q1].i == tl[q0].i && tl[q1].j == tl[q0].j) q1++;
+      const int i = tl[q0].i, j = tl[q0].j;
       const int ki = kept[i], kj = kept[j], ni = cur_n[i], nj = cur_n[j];
       BlockRef ref{foff, ki, kj};
-      fnew[p][ij] = ref;
+      fnew[p][{i, j}] = ref;
       foff += (long long)ki * kj;
-      if (ki == 0 || kj == 0 || ni == 0) continue;
+      if (ki == 0 || kj == 0 || ni == 0) {
+        q0 = q1;
+        continue;
+      }
       const int tri = (int)tris.size();
       tris.push_back(RotTri{cur_off[j], nj, kj, yoff});
       // one group per sub-block of sector i (rows without a coupling stay zero)
@@ -792,20 +838,19 @@ void Chain::rotate_fdag() {
         const int kii = pre_kept[ii], dbb = bath.dim(bb);
         if (kii * dbb == 0) continue;
         RotGroup g{tri, sub_kidx[i][k], kii, dbb, (int)segs.size(), 0};
-        for (auto& [sk, sj, bbp] : lst) {
-          if (sk != (int)k) continue;
-          const BlockRef fb = bathF[p].at({bb, bbp});
-          segs.push_back(RotSegment{sub_kidx[j][sj], bath.dim(bbp), (int)fb.off, fb.cols});
+        for (size_t q = q0; q < q1; q++) {
+          if (tl[q].k != (int)k) continue;
+          segs.push_back(RotSegment{sub_kidx[j][tl[q].sj], bath.dim(tl[q].bbp), (int)tl[q].fb.off, tl[q].fb.cols});
         }
         g.seg_end = (int)segs.size();
         groups.push_back(g);
       }
-      pend.push_back({p, i, j});
       gp.push_back(GemmProb{dU.as<double>() + cur_off[i], nullptr, nullptr, nullptr, ki, kj, ni, ni, kj, kj, 0, 1, 0,
                             1.0, 0.0, 0, 0});
       gp.back().B = (const double*)(intptr_t)yoff;  // patched below once Y is allocated
       gp.back().C = (double*)(intptr_t)ref.off;
       yoff += (long long)ni * kj;
+      q0 = q1;
     }
   }
   const int nf = fcur ^ 1;

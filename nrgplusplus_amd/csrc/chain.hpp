@@ -1,7 +1,9 @@
 // Host-side chain state of libnrgk (what nrgcore<imp,bath> holds, nrgcore.hpp:512-550), with the
 // numerical state resident on the device.
 #pragma once
+#include <algorithm>
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <memory>
 #include <utility>
@@ -15,7 +17,54 @@ struct BlockRef {
   long long off;  // element offset in its arena
   int rows, cols; // leading dimension == cols
 };
-using OpMap = std::map<std::pair<int, int>, BlockRef>;
+// Block table of one operator: (row sector, column sector) -> block, iterated in key order like the
+// reference's std::map-based qOperator (qOperator.hpp:18-158).  Entries live in one sorted vector:
+// the producers add keys in ascending order (an append), lookups are binary searches, and copying a
+// table (one per recorded shell) is a single allocation.
+class OpMap {
+ public:
+  using key_type = std::pair<int, int>;
+  using value_type = std::pair<key_type, BlockRef>;
+  using iterator = std::vector<value_type>::iterator;
+  using const_iterator = std::vector<value_type>::const_iterator;
+  iterator begin() { return v_.begin(); }
+  iterator end() { return v_.end(); }
+  const_iterator begin() const { return v_.begin(); }
+  const_iterator end() const { return v_.end(); }
+  size_t size() const { return v_.size(); }
+  bool empty() const { return v_.empty(); }
+  void clear() { v_.clear(); }
+  void reserve(size_t n) { v_.reserve(n); }
+  iterator find(const key_type& k) {
+    auto it = lower(k);
+    return it != v_.end() && it->first == k ? it : v_.end();
+  }
+  const_iterator find(const key_type& k) const {
+    auto it = std::lower_bound(v_.begin(), v_.end(), k, [](const value_type& a, const key_type& b) { return a.first < b; });
+    return it != v_.end() && it->first == k ? it : v_.end();
+  }
+  size_t count(const key_type& k) const { return find(k) != end() ? 1 : 0; }
+  const BlockRef& at(const key_type& k) const {
+    auto it = find(k);
+    if (it == end()) throw std::out_of_range("OpMap::at");
+    return it->second;
+  }
+  BlockRef& operator[](const key_type& k) {
+    if (v_.empty() || v_.back().first < k) {  // in-order insertion
+      v_.push_back({k, BlockRef{0, 0, 0}});
+      return v_.back().second;
+    }
+    auto it = lower(k);
+    if (it == v_.end() || it->first != k) it = v_.insert(it, {k, BlockRef{0, 0, 0}});
+    return it->second;
+  }
+
+ private:
+  iterator lower(const key_type& k) {
+    return std::lower_bound(v_.begin(), v_.end(), k, [](const value_type& a, const key_type& b) { return a.first < b; });
+  }
+  std::vector<value_type> v_;
+};
 
 // A site model (impurity or bath) in the reference's model concept.
 struct SiteData {
