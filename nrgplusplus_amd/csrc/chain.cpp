@@ -355,9 +355,12 @@ void Chain::assemble(const std::vector<double>& t, double rescale) {
     sec.con_end = (int)cons.size();
     secs.push_back(sec);
   }
-  const AsmSector* d_secs = stg.put(secs, st);
-  const AsmSub* d_subs = stg.put(subs, st);
-  const AsmCon* d_cons = stg.put(cons, st);
+  Staging::Region rg = stg.open(Staging::span<AsmSector>(secs.size()) + Staging::span<AsmSub>(subs.size()) +
+                                Staging::span<AsmCon>(cons.size()));
+  const AsmSector* d_secs = stg.place(rg, secs);
+  const AsmSub* d_subs = stg.place(rg, subs);
+  const AsmCon* d_cons = stg.place(rg, cons);
+  stg.commit(rg, st);
   dH.ensure(std::max<long long>(1, tot_elems) * sizeof(double));
   launch_assemble(d_secs, (int)secs.size(), row0, d_subs, d_cons, d_eprev, d_bathE.as<double>(),
                   fbuf[fcur].as<double>(), d_bathF.as<double>(), rescale, dH.as<double>(), st);
@@ -513,8 +516,10 @@ void Chain::eigensolve_mixed() {
     grid_tasks.assign(NXCD * maxlen, JacTask{0, 0, 0, 1, 0, 0, 0});
     for (int x = 0; x < NXCD; x++)
       for (size_t k = 0; k < xtasks[x].size(); k++) grid_tasks[NXCD * k + x] = xtasks[x][k];
-    const JacTask* dt = stg.put(grid_tasks, st);
-    int* dflags = stg.put_zero<int>(2 * nb, st);
+    Staging::Region rg = stg.open(Staging::span<JacTask>(grid_tasks.size()) + Staging::span<int>(2 * nb));
+    const JacTask* dt = stg.place(rg, grid_tasks);
+    int* dflags = stg.place_zero<int>(rg, 2 * nb);
+    stg.commit(rg, st);
     prof_begin("jacobi_round");
     size_t cnt[NXCD];
     for (int x = 0; x < NXCD; x++) cnt[x] = xtasks[x].size();
@@ -645,11 +650,15 @@ void Chain::eigensolve_mixed() {
           if (ti <= tj) tiles_sym.push_back(int4{a, ti, tj, 0});
         }
     }
-    const EigBlock* m_blk = stg.put(act, st);
-    const int4* m_all = stg.put(tiles_all, st);
-    const int4* m_sym = stg.put(tiles_sym, st);
-    double* m_conv = stg.put_zero<double>(3 * na, st);
-    double* m_offmax = stg.put_zero<double>(na, st);
+    Staging::Region rg = stg.open(Staging::span<EigBlock>(act.size()) + Staging::span<int4>(tiles_all.size()) +
+                                  Staging::span<int4>(tiles_sym.size()) + Staging::span<double>(3 * na) +
+                                  Staging::span<double>(na));
+    const EigBlock* m_blk = stg.place(rg, act);
+    const int4* m_all = stg.place(rg, tiles_all);
+    const int4* m_sym = stg.place(rg, tiles_sym);
+    double* m_conv = stg.place_zero<double>(rg, 3 * na);
+    double* m_offmax = stg.place_zero<double>(rg, na);
+    stg.commit(rg, st);
     std::vector<GemmProb> g1, g2, g3;
     for (auto& b : act) {
       const int n = b.n;
@@ -746,8 +755,10 @@ void Chain::update_internal_state() {
   NRGK_REQUIRE(phase == ASSEMBLED, NRGK_ERR_STATE, "nrgk: update_internal_state without add_bath_site");
   cnt++;
   const int S = (int)cur_qn.size();
-  const int* d_bn = stg.put(cur_n, st);
-  const int* d_beo = stg.put(cur_eoff, st);
+  Staging::Region rg = stg.open(Staging::span<int>(cur_n.size()) + Staging::span<int>(cur_eoff.size()));
+  const int* d_bn = stg.place(rg, cur_n);
+  const int* d_beo = stg.place(rg, cur_eoff);
+  stg.commit(rg, st);
   d_trunc.ensure(sizeof(TruncOut) + 256);
   TruncOut* dout = (TruncOut*)d_trunc.p;
   d_kept.ensure(std::max(1, S) * sizeof(int));
@@ -860,9 +871,12 @@ void Chain::rotate_fdag() {
     g.B = dY.as<double>() + (intptr_t)g.B;
     g.C = fbuf[nf].as<double>() + (intptr_t)g.C;
   }
-  const RotGroup* d_groups = stg.put(groups, st);
-  const RotSegment* d_segs = stg.put(segs, st);
-  const RotTri* d_tris = stg.put(tris, st);
+  Staging::Region rg = stg.open(Staging::span<RotGroup>(groups.size()) + Staging::span<RotSegment>(segs.size()) +
+                                Staging::span<RotTri>(tris.size()));
+  const RotGroup* d_groups = stg.place(rg, groups);
+  const RotSegment* d_segs = stg.place(rg, segs);
+  const RotTri* d_tris = stg.place(rg, tris);
+  stg.commit(rg, st);
   launch_fdag_gather(d_groups, (int)groups.size(), d_segs, d_tris, dU.as<double>(), d_bathF.as<double>(),
                      dY.as<double>(), st);
   run_gemms(gp);

@@ -160,6 +160,47 @@ struct Staging {
     NRGK_HIP(hipMemcpyAsync(d, h, std::max<size_t>(1, n) * sizeof(T), hipMemcpyHostToDevice, s));
     return reinterpret_cast<T*>(d);
   }
+  // Several arrays sent with one copy: open() a region for their total span() bytes, place() each
+  // array (host side only; returns its device address), then commit() enqueues a single
+  // host-to-device copy of the region.
+  struct Region {
+    char* h = nullptr;
+    char* d = nullptr;
+    size_t used = 0, cap = 0;
+  };
+  template <class T>
+  static size_t span(size_t n) {
+    return (std::max<size_t>(1, n) * sizeof(T) + 255) & ~size_t(255);
+  }
+  Region open(size_t bytes) {
+    auto [h, d] = take(bytes);
+    return Region{h, d, 0, (bytes + 255) & ~size_t(255)};
+  }
+  template <class T>
+  T* place(Region& r, const T* src, size_t n) {
+    const size_t sz = span<T>(n);
+    if (r.used + sz > r.cap) throw Error(NRGK_ERR_STATE, "nrgk: staging region overflow");
+    if (n) std::memcpy(r.h + r.used, src, n * sizeof(T));
+    T* d = reinterpret_cast<T*>(r.d + r.used);
+    r.used += sz;
+    return d;
+  }
+  template <class T>
+  T* place(Region& r, const std::vector<T>& v) {
+    return place(r, v.data(), v.size());
+  }
+  template <class T>
+  T* place_zero(Region& r, size_t n) {
+    const size_t sz = span<T>(n);
+    if (r.used + sz > r.cap) throw Error(NRGK_ERR_STATE, "nrgk: staging region overflow");
+    std::memset(r.h + r.used, 0, sz);
+    T* d = reinterpret_cast<T*>(r.d + r.used);
+    r.used += sz;
+    return d;
+  }
+  void commit(const Region& r, hipStream_t s) {
+    if (r.used) NRGK_HIP(hipMemcpyAsync(r.d, r.h, r.used, hipMemcpyHostToDevice, s));
+  }
   template <class T>
   const T* get(const T* dsrc, size_t n, hipStream_t s) {
     auto [h, d] = take(std::max<size_t>(1, n) * sizeof(T));

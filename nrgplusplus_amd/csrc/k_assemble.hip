@@ -100,31 +100,62 @@ __device__ __forceinline__ float from_ordered_bits(unsigned o) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
+// Gershgorin bounds per block: the largest absolute row sum (rad) and the largest R_i - H_ii
+// (-(lower bound)), as order-preserving bits combined by atomicMax.  A workgroup takes GG_ROWS
+// consecutive rows (a wave per row, GG_ROWS / 4 rows per wave) and issues one pair of atomics per
+// block among them, so that the rows of a large block do not all contend for the same two words.
+constexpr int GG_ROWS = 16;
 __global__ void __launch_bounds__(256) k_gershgorin(const EigBlock* __restrict__ blocks, const int* __restrict__ row_block,
-                                                    const int* __restrict__ row_local, const double* __restrict__ H,
-                                                    unsigned int* __restrict__ sigma_bits) {
-  const int grow = blockIdx.x;
-  const int b = row_block[grow];
-  const int r = row_local[grow];
-  const EigBlock blk = blocks[b];
-  const double* row = H + blk.off + (long long)r * blk.n;
-  double s = 0.0;
-  for (int c = threadIdx.x; c < blk.n; c += blockDim.x) s += fabs(row[c]);
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+                                                    const int* __restrict__ row_local, int n_rows,
+                                                    const double* __restrict__ H, unsigned int* __restrict__ sigma_bits) {
+  __shared__ int sb[GG_ROWS];
+  __shared__ unsigned srad[GG_ROWS], slow[GG_ROWS];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int q = 0; q < GG_ROWS / 4; q++) {
+    const int lr = wv * (GG_ROWS / 4) + q;
+    const int grow = blockIdx.x * GG_ROWS + lr;
+    if (grow < n_rows) {  // uniform over the wave
+      const int b = row_block[grow], r = row_local[grow];
+      const EigBlock blk = blocks[b];
+      const double* row = H + blk.off + (long long)r * blk.n;
+      double s = 0.0;
+      for (int c = lane; c < blk.n; c += 64) s += fabs(row[c]);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) {
+        const double d = row[r];
+        // R_i - H_ii, rounded up so that the float bound is never below the exact one
+        const double low = (s - fabs(d)) - d;
+        float lf = (float)low;
+        if ((double)lf < low) lf = nextafterf(lf, INFINITY);
+        sb[lr] = b;
+        srad[lr] = __float_as_uint((float)s);
+        slow[lr] = ordered_bits(lf);
+      }
+    } else if (lane == 0) {
+      sb[lr] = -1;
+    }
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const double d = row[r];
-    atomicMax(&sigma_bits[2 * b], __float_as_uint((float)red[0]));
-    // R_i - H_ii, rounded up so that the float bound is never below the exact one
-    const double low = (red[0] - fabs(d)) - d;
-    float lf = (float)low;
-    if ((double)lf < low) lf = nextafterf(lf, INFINITY);
-    atomicMax(&sigma_bits[2 * b + 1], ordered_bits(lf));
+    int cur = -1;
+    unsigned mr = 0, ml = 0;
+    for (int lr = 0; lr <= GG_ROWS; lr++) {
+      const int b = lr < GG_ROWS ? sb[lr] : -1;
+      if (b != cur && cur >= 0) {
+        atomicMax(&sigma_bits[2 * cur], mr);
+        atomicMax(&sigma_bits[2 * cur + 1], ml);
+      }
+      if (b < 0) break;
+      if (b != cur) {
+        cur = b;
+        mr = srad[lr];
+        ml = slow[lr];
+      } else {
+        mr = max(mr, srad[lr]);
+        ml = max(ml, slow[lr]);
+      }
+    }
   }
 }
 
@@ -164,7 +195,8 @@ void launch_assemble(const AsmSector* sectors, int n_sectors, int n_rows, const 
 void launch_shift_convert(const EigBlock* blocks, const int* row_block, const int* row_local, int n_rows,
                           const double* H, unsigned int* sigma_bits, float* sigma_out, float* W, hipStream_t s) {
   if (n_rows <= 0) return;
-  hipLaunchKernelGGL(k_gershgorin, dim3(n_rows), dim3(256), 0, s, blocks, row_block, row_local, H, sigma_bits);
+  hipLaunchKernelGGL(k_gershgorin, dim3((n_rows + GG_ROWS - 1) / GG_ROWS), dim3(256), 0, s, blocks, row_block, row_local,
+                     n_rows, H, sigma_bits);
   NRGK_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_to_fp32_shift, dim3(n_rows), dim3(256), 0, s, blocks, row_block, row_local, H, sigma_bits,
                      sigma_out, W);

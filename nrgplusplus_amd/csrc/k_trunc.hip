@@ -2,11 +2,11 @@
 //
 // The reference sorts all N_tot eigenvalues and keeps, per sector, every state with
 // E < sorted[K] + 1e-8 (strict, K 0-based), then shifts all energies by the minimum if and only if
-// it truncated.  Each block's eigenvalues are already ascending, so sorted[K] is found without a
-// global sort: a bisection over the order-preserving 64-bit key of an FP64 value, where each step
-// counts the elements <= the probe with one binary search per block.  The result is the exact
-// element value, and the kept counts are lower_bound(sorted[K] + 1e-8) per block; both are
-// bit-identical to the reference for identical eigenvalues.
+// it truncated.  sorted[K] is found without a global sort: a radix select over the order-preserving
+// 64-bit key of an FP64 value, eight passes of one 8-bit digit each (per-wave histograms in LDS, then
+// the digit whose prefix range holds rank K).  The result is the exact element value, and the kept
+// counts are lower_bound(sorted[K] + 1e-8) per block (each block's eigenvalues are ascending); both
+// are bit-identical to the reference for identical eigenvalues.
 #include "common.hpp"
 
 namespace nrgk {
@@ -20,16 +20,7 @@ __device__ __forceinline__ double okey_inv(unsigned long long k) {
   unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
   return __longlong_as_double((long long)b);
 }
-// number of elements <= v (upper_bound) and < v (lower_bound) in an ascending list
-__device__ __forceinline__ int count_le(const double* a, int n, double v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int m = (lo + hi) >> 1;
-    if (a[m] <= v) lo = m + 1;
-    else hi = m;
-  }
-  return lo;
-}
+// number of elements < v (lower_bound) in an ascending list
 __device__ __forceinline__ int count_lt(const double* a, int n, double v) {
   int lo = 0, hi = n;
   while (lo < hi) {
@@ -79,17 +70,50 @@ __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, c
   const bool trunc = N_tot > K;
   double e_cut = mx;
   if (trunc) {
-    unsigned long long lo = okey(mn), hi = okey(mx);
-    while (lo < hi) {
-      const unsigned long long mid = lo + ((hi - lo) >> 1);
-      const double v = okey_inv(mid);
-      long long c = 0;
-      for (int b = tid; b < S; b += blockDim.x) c += count_le(L + beoff[b], bn[b], v);
-      c = block_reduce(c, shl, [](long long a, long long d) { return a + d; });
-      if (c >= K + 1) hi = mid;
-      else lo = mid + 1;
+    // radix select of the element of rank K (0-based) among all N_tot keys
+    __shared__ int hist[16][256];  // one histogram per wave
+    __shared__ int s_digit;
+    __shared__ long long s_rank;
+    const int wv = tid >> 6, lane = tid & 63, nwv = blockDim.x >> 6;
+    unsigned long long prefix = 0, mask = 0;
+    long long rank = K;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+      for (int e = tid; e < 16 * 256; e += blockDim.x) (&hist[0][0])[e] = 0;
+      __syncthreads();
+      for (long long i = tid; i < N_tot; i += blockDim.x) {
+        const unsigned long long k = okey(L[i]);
+        if ((k & mask) == prefix) atomicAdd(&hist[wv][(k >> shift) & 255], 1);
+      }
+      __syncthreads();
+      if (tid < 256) {
+        int c = 0;
+        for (int w = 0; w < nwv; w++) c += hist[w][tid];
+        // inclusive scan of the 256 bin counts: within each wave, then across the four waves
+        int incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(incl, o, 64);
+          if (lane >= o) incl += y;
+        }
+        hist[0][tid] = incl;  // bins of this wave, read back below after the barrier
+        hist[1][tid] = c;
+      }
+      __syncthreads();
+      if (tid < 256) {
+        long long before = hist[0][tid] - hist[1][tid];
+        for (int w = 0; w < (tid >> 6); w++) before += hist[0][64 * w + 63];
+        if (before <= rank && rank < before + hist[1][tid]) {
+          s_digit = tid;
+          s_rank = rank - before;
+        }
+      }
+      __syncthreads();
+      prefix |= (unsigned long long)s_digit << shift;
+      mask |= 255ull << shift;
+      rank = s_rank;
+      __syncthreads();
     }
-    e_cut = okey_inv(lo);
+    e_cut = okey_inv(prefix);
   }
   const double x = e_cut + errbar;
   long long nk = 0;
