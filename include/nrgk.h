@@ -214,6 +214,23 @@ int nrgk_fdm_set_system_operator(nrgk_fdm* rec, nrgk_obs* sysop);
 int nrgk_thermo_expectation(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, const double* T,
                             int nT, double* out);
 
+/* ---------------------------------------------------------------- z-shifted discretisation */
+/* The Wilson chain of a bath discretised on the z-shifted logarithmic mesh (Campo-Oliveira levels),
+ * the reference's z-averaging generator: calcTn (independFiles/zAverage/hoppingElement.cpp:16-105,
+ * flat band) and zAverage (independFiles/zAverage/zAverage.hpp:12-195, tabulated density of states).
+ * Host computation in quadruple precision with full reorthogonalisation (the reference: 1000-digit
+ * MPFR, plain three-term Lanczos).
+ *   z: twist in (0, 1] (z = 0 is the same mesh as z = 1); lambda > 1; D: half bandwidth
+ *   n_x = 0: flat band rho = 1/(2D); else rho tabulated at ascending x[0..n_x), linear between points
+ *   quad: 0 exact integrals of that interpolant; 1 the reference's 100-point left sums
+ *         (zAverage/quadrature.hpp:13-26)
+ *   n_intervals: intervals per half band (0: 2 n_sites + 20; calcTn uses 11)
+ *   normalize: 1 normalise the coupling vector (zAverage.hpp:131-143); 0 keep it (calcTn)
+ * Outputs t_n Lambda^{n/2} and eps_n Lambda^{n/2}, n < n_sites (getHoppingElement's scaling,
+ * zAverage.hpp:33-43); entries past the star's m - 1 hoppings are 0.  No GPU work. */
+int nrgk_zdisc_chain(double z, double lambda, double D, const double* x, const double* rho, int n_x, int quad,
+                     int n_intervals, int normalize, int n_sites, double* t_out, double* eps_out);
+
 /* ---------------------------------------------------------------- broadening */
 /* log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44): out[o] = (1/w_o) sum_k
  * weight_k exp(-(ln(w_o/x_k)/b - b/4)^2)/(b sqrt(pi)), on the device */

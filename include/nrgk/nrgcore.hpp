@@ -353,6 +353,21 @@ class fdmSpectrum {
   size_t n_bins_, n_ops_;
 };
 
+// Wilson chain of the z-shifted logarithmic discretisation of a bath (independFiles/zAverage:
+// calcTn, hoppingElement.cpp:16-105, for the flat band; zAverage.hpp:12-195 for a density tabulated
+// at x): {t_n Lambda^{n/2}, eps_n Lambda^{n/2}}, n < n_sites.  quad 0: exact integrals of the
+// tabulated density, 1: the reference's 100-point left sums.
+inline std::pair<std::vector<double>, std::vector<double>> z_chain(double z, double lambda, int n_sites,
+                                                                   double D = 1.0, const std::vector<double>& x = {},
+                                                                   const std::vector<double>& rho = {}, int quad = 0,
+                                                                   int n_intervals = 0, bool normalize = true) {
+  if (x.size() != rho.size()) throw std::runtime_error("z_chain: x and rho differ in length");
+  std::vector<double> t(n_sites), e(n_sites);
+  check(nrgk_zdisc_chain(z, lambda, D, x.data(), rho.data(), (int)x.size(), quad, n_intervals, normalize ? 1 : 0,
+                         n_sites, t.data(), e.data()));
+  return {t, e};
+}
+
 // fdmThermodynamics (static/fdmThermodynamics.hpp:17-260): shells recorded during the forward pass,
 // entropy and specific heat on a temperature grid.
 class fdmThermodynamics {

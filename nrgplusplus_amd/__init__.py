@@ -120,6 +120,7 @@ def lib():
             "nrgk_fdm_set_system_operator": (i, [vp, vp]),
             "nrgk_thermo_expectation": (i, [vp, d, i, i, dp, i, dp]),
             "nrgk_broaden": (i, [vp, dp, dp, i, dp, i, d, dp]),
+            "nrgk_zdisc_chain": (i, [d, d, d, dp, dp, i, i, i, i, i, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -587,6 +588,26 @@ def broaden(x, w, omega, b, ctx=None):
     out = np.zeros(len(omega))
     _check(lib().nrgk_broaden(ctx._h, _dp(x), _dp(w), len(x), _dp(omega), len(omega), float(b), _dp(out)))
     return out
+
+
+def z_chain(z, lam, n_sites, D=1.0, dos=None, quad="exact", n_intervals=0, normalize=True):
+    """Wilson chain (t_n Lambda^{n/2}, eps_n Lambda^{n/2}) of the z-shifted logarithmic discretisation
+    of a bath (independFiles/zAverage/hoppingElement.cpp:16-105, zAverage.hpp:12-195).  dos: None for
+    the flat band, or (x, rho) tabulated on ascending x over [-D, D]; quad: "exact" or "reference"
+    (the reference's 100-point left sums)."""
+    t = np.zeros(n_sites)
+    e = np.zeros(n_sites)
+    if dos is None:
+        x = r = np.zeros(1)
+        nx = 0
+    else:
+        x = np.ascontiguousarray(dos[0], np.float64)
+        r = np.ascontiguousarray(dos[1], np.float64)
+        nx = len(x)
+    q = {"exact": 0, "reference": 1}[quad]
+    _check(lib().nrgk_zdisc_chain(z, lam, D, _dp(x), _dp(r), nx, q, n_intervals, int(normalize), n_sites, _dp(t),
+                                  _dp(e)))
+    return t, e
 
 
 def hopping(n, lam):

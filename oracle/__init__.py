@@ -87,6 +87,7 @@ def lib():
             "orc_fdag_block": (None, [vp, i, i, dp]),
             "orc_run_rgflow": (i, [C.POINTER(OrcDriver), dp, l, lp]),
             "orc_run_thermo": (i, [C.POINTER(OrcDriver), i, dp, i, dp, dp, i]),
+            "orc_run_thermo_t": (i, [C.POINTER(OrcDriver), dp, i, dp, i, dp, dp]),
             "orc_run_thermo_ops": (i, [C.POINTER(OrcDriver), i, dp, i, i, dp, dp, dp, ip]),
             "orc_run_fdm": (i, [C.POINTER(OrcDriver), i, dp, dp, dp, dp, ip]),
             "orc_broaden": (None, [dp, dp, i, dp, i, d, dp]),
@@ -295,6 +296,17 @@ def run_thermo(drv, T, min_iter0, low_clamp_zero=False):
     S = np.zeros(len(T))
     Cv = np.zeros(len(T))
     _check(lib().orc_run_thermo(C.byref(drv), min_iter0, _dp(T), len(T), _dp(S), _dp(Cv), int(low_clamp_zero)))
+    return S, Cv
+
+
+def run_thermo_hoppings(drv, hop, T, min_iter0):
+    """run_thermo with the chain's hopping per site given (e.g. a z-shifted discretisation)"""
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    hop = np.ascontiguousarray(hop, dtype=np.float64)
+    assert len(hop) >= drv.nsites
+    S = np.zeros(len(T))
+    Cv = np.zeros(len(T))
+    _check(lib().orc_run_thermo_t(C.byref(drv), _dp(hop), min_iter0, _dp(T), len(T), _dp(S), _dp(Cv)))
     return S, Cv
 
 

@@ -247,8 +247,8 @@ int orc_run_rgflow(const OrcDriver* d, double* eig_out, long cap, long* counts) 
 
 // Thermodynamics driver (examples/entropySIAM/main.cpp:13-96, examples/entropyKondo/main.cpp:35-119).
 // min_iter0 = the driver's initial minIterations (0 for SIAM, 1 for the Kondo/free drivers).
-int orc_run_thermo(const OrcDriver* d, int min_iter0, const double* T, int nT, double* S_out, double* C_out,
-                   int low_clamp_zero) {
+static int run_thermo(const OrcDriver* d, const double* hop, int min_iter0, const double* T, int nT, double* S_out,
+                      double* C_out, int low_clamp_zero) {
   return guard([&] {
     SiteModel imp = driver_imp(*d), bath = driver_bath(*d);
     Chain c(imp, bath);
@@ -262,7 +262,7 @@ int orc_run_thermo(const OrcDriver* d, int min_iter0, const double* T, int nT, d
     size_t minIterations = (size_t)min_iter0;
     for (int in = 0; in < d->nsites; in++) {
       double rescale = d->first_V > 0 ? (in > 0 ? std::sqrt(d->Lambda) : 1.0) : std::sqrt(d->Lambda);
-      c.add_bath_site(std::vector<double>(nop, hopping(in, d->Lambda)), rescale);
+      c.add_bath_site(std::vector<double>(nop, hop ? hop[in] : hopping(in, d->Lambda)), rescale);
       if (c.checkHigherEnergyDiscarded()) shells[c.nrg_iterations_cnt] = save_shell(c);
       else minIterations++;
       c.update_internal_state();
@@ -278,6 +278,17 @@ int orc_run_thermo(const OrcDriver* d, int min_iter0, const double* T, int nT, d
     std::memcpy(S_out, S.data(), S.size() * sizeof(double));
     std::memcpy(C_out, C.data(), C.size() * sizeof(double));
   });
+}
+
+int orc_run_thermo(const OrcDriver* d, int min_iter0, const double* T, int nT, double* S_out, double* C_out,
+                   int low_clamp_zero) {
+  return run_thermo(d, nullptr, min_iter0, T, nT, S_out, C_out, low_clamp_zero);
+}
+// the same driver with the chain's hoppings given per site (a z-shifted discretisation's t_n(z)
+// in place of hopping(n, Lambda))
+int orc_run_thermo_t(const OrcDriver* d, const double* hop, int min_iter0, const double* T, int nT, double* S_out,
+                     double* C_out) {
+  return run_thermo(d, hop, min_iter0, T, nT, S_out, C_out, 0);
 }
 
 // Thermodynamics driver with system operators on a spinhalf impurity (fdmThermodynamics::

@@ -9,6 +9,8 @@ lists per iteration, entropy, specific heat and the temperature grid.
   examples/entropyKondo/resultsJ0.250000.h5  Kondo J=0.25, S=1/2, Lambda=2, K=1024
   examples/entropySIAM/resultSIAMU0.200000.h5  SIAM U=0.2, eps=-0.08, Gamma=0.01, Lambda=2.5, K=512
   examples/entropySIAM/resultSIAMZero.h5       same driver with impurity (0,0)
+  independFiles/zAverage/hoppingElementsL3.h5  z = 1, Lambda = 3 hopping elements of the flat band
+                                               (calcTn, hoppingElement.cpp:16-105)
 
 Run once in a container that has /root/reference mounted:
     python tests/golden/make_fixtures.py
@@ -30,6 +32,7 @@ FILES = {
     "kondo_J025_L2_K1024": "examples/entropyKondo/resultsJ0.250000.h5",
     "siam_U02_L25_K512": "examples/entropySIAM/resultSIAMU0.200000.h5",
     "siam_zero_L25_K512": "examples/entropySIAM/resultSIAMZero.h5",
+    "zdisc_flat_z1_L3": "independFiles/zAverage/hoppingElementsL3.h5",
 }
 
 

@@ -253,6 +253,41 @@ def test_thermal_expectation_values_match_oracle(nk, oracle_mod):
         nk.FdmRecorder(g, None).expectation_values(L, T)
 
 
+def test_z_averaged_thermodynamics_matches_oracle(nk, oracle_mod):
+    """z-averaging (the z-averaged SIAM workload at a smaller K and length: eps = -0.25, U = 0.5,
+    Gamma = 0.015, Lambda = 4): one chain per twist z = k/4 with the hoppings of the z-shifted
+    discretisation (times A_Lambda, so that z = 1 is Wilson's chain), thermodynamics per chain
+    against the oracle driven with the same hoppings (entropy to 1e-8 absolute where the exp clamp
+    does not dominate), then averaged over z."""
+    L, K, ns = 4.0, 256, 18
+    eps, U, G = -0.25, 0.5, 0.015
+    V = np.sqrt(2 * G / np.pi)
+    A = 0.5 * np.log(L) * (1 + 1 / L) / (1 - 1 / L)
+    T = np.logspace(-6, 0, 25)
+    S_avg = np.zeros(len(T))
+    for z in (0.25, 0.5, 0.75, 1.0):
+        hop = nk.z_chain(z, L, ns)[0] * A
+        g = nk.NrgCore(nk.Site.spinhalf(eps, U), nk.Site.spinhalf(0, 0), K)
+        rec = nk.FdmRecorder(g, None)
+        g.add_bath_site([V, V], 1.0)
+        g.update_internal_state()
+        min_it = 0
+        for n in range(ns):
+            g.add_bath_site([hop[n], hop[n]], 1.0 if n == 0 else np.sqrt(L))
+            if g.checkHigherEnergyDiscarded():
+                rec.record()
+            else:
+                min_it += 1
+            g.update_internal_state()
+        S, Cv = rec.thermodynamics(L, T, scale_offset=-1, first_cnt=min_it + 1)
+        drv = oracle_mod.driver(imp_kind=0, eps=eps, U=U, bath_kind=0, Lambda=L, first_V=V, nsites=ns, K=K)
+        oS, oC = oracle_mod.run_thermo_hoppings(drv, hop, T, 0)
+        m = T >= 1e-5
+        assert np.abs(S - oS)[m].max() < 1e-8, (z, np.abs(S - oS)[m].max())
+        S_avg += S / 4
+    assert np.all(np.isfinite(S_avg))
+
+
 def test_broadening_matches_oracle(nk, oracle_mod):
     rng = np.random.default_rng(1)
     x = np.sort(10 ** rng.uniform(-9, 1, 300))
