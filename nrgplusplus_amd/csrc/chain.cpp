@@ -635,6 +635,9 @@ void Chain::eigensolve_mixed() {
   // show up there.  Converged blocks leave the batch; both X buffers then hold their final vectors.
   const int oa_max = (int)knob("NRGK_OA_ITERS", 12);
   const double e_tol = knob("NRGK_OA_ETOL", 1.0), d_tol = knob("NRGK_OA_DTOL", 1e-12);
+  // a full step from residuals within e_pred noise and a defect within d_pred is the block's final
+  // one (its update lands at the noise; the verifying step is skipped)
+  const double e_pred = knob("NRGK_OA_EPRED", 300.0), d_pred = knob("NRGK_OA_DPRED", 1e-6);
   const int dbg_oa = (int)knob("NRGK_DEBUG_OA", 0);
   std::vector<EigBlock> act = blks;
   oa_steps = 0;
@@ -706,11 +709,13 @@ void Chain::eigensolve_mixed() {
       if (dbg_oa > 0 && act[a].n >= dbg_oa)
         std::fprintf(stderr, "nrgk oa step %d block off=%lld n=%d residual/noise=%.3g orth=%.3g\n", it, act[a].off,
                      act[a].n, conv[3 * a] / conv[3 * a + 2], conv[3 * a + 1]);
-      if (conv[3 * a] > e_tol * conv[3 * a + 2] || conv[3 * a + 1] > d_tol) next.push_back(act[a]);
+      const bool last = conv[3 * a] <= e_tol * conv[3 * a + 2] && conv[3 * a + 1] <= d_tol;
+      const bool pred = conv[3 * a] <= e_pred * conv[3 * a + 2] && conv[3 * a + 1] <= d_pred;
+      if (!last && !pred) next.push_back(act[a]);
     }
-    // the converged blocks' vectors into both iterate buffers (the device applies the same test)
+    // the finished blocks' vectors into both iterate buffers (the device applies the same test)
     if (next.size() < act.size())
-      launch_copy_converged(m_blk, na, m_conv, e_tol, d_tol, dX.as<double>(), dX2.as<double>(), st);
+      launch_copy_converged(m_blk, na, m_conv, e_tol, d_tol, e_pred, d_pred, dX.as<double>(), dX2.as<double>(), st);
     act.swap(next);
   }
   oa_unconverged = (int)act.size();

@@ -858,6 +858,15 @@ constexpr int OT = 32;
 __device__ __forceinline__ bool oa_converged(const double* conv, int b, double e_tol, double d_tol) {
   return conv[3 * b] <= e_tol * conv[3 * b + 2] && conv[3 * b + 1] <= d_tol;
 }
+// A block leaves the batch after its last step, or after a full step taken from residuals within
+// e_pred times the noise and an orthogonality defect of at most d_pred: that step's update lands at
+// the noise (the residuals fall quadratically, to rho^2 / gap with every out-of-cluster gap at least
+// 1e5 noise, so e_pred = 300 leaves below one noise unit; the defect squares, 1e-6 -> 1e-12), so the
+// verifying step that would follow is skipped.
+__device__ __forceinline__ bool oa_done(const double* conv, int b, double e_tol, double d_tol, double e_pred,
+                                        double d_pred) {
+  return oa_converged(conv, b, e_tol, d_tol) || (conv[3 * b] <= e_pred * conv[3 * b + 2] && conv[3 * b + 1] <= d_pred);
+}
 
 __global__ void __launch_bounds__(256) k_oa_mask(const EigBlock* __restrict__ blocks, const int4* __restrict__ tiles,
                                                  const int* __restrict__ cid_g, const double* __restrict__ conv,
@@ -1240,18 +1249,20 @@ __global__ void __launch_bounds__(1024) k_oa_rr(const EigBlock* __restrict__ blo
 // them to the other iterate buffer as well (one launch for all of them: grid = blocks x 8).
 __global__ void __launch_bounds__(256) k_copy_converged(const EigBlock* __restrict__ blocks,
                                                         const double* __restrict__ conv, double e_tol, double d_tol,
-                                                        const double* __restrict__ src, double* __restrict__ dst) {
+                                                        double e_pred, double d_pred, const double* __restrict__ src,
+                                                        double* __restrict__ dst) {
   const int a = blockIdx.x >> 3, part = blockIdx.x & 7;
-  if (!oa_converged(conv, a, e_tol, d_tol)) return;
+  if (!oa_done(conv, a, e_tol, d_tol, e_pred, d_pred)) return;
   const EigBlock blk = blocks[a];
   const long long nn = (long long)blk.n * blk.n;
   for (long long e = part * 256LL + threadIdx.x; e < nn; e += 8 * 256LL) dst[blk.off + e] = src[blk.off + e];
 }
 
 void launch_copy_converged(const EigBlock* blocks, int nb, const double* conv, double e_tol, double d_tol,
-                           const double* src, double* dst, hipStream_t s) {
+                           double e_pred, double d_pred, const double* src, double* dst, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_copy_converged, dim3(nb * 8), dim3(256), 0, s, blocks, conv, e_tol, d_tol, src, dst);
+  hipLaunchKernelGGL(k_copy_converged, dim3(nb * 8), dim3(256), 0, s, blocks, conv, e_tol, d_tol, e_pred, d_pred, src,
+                     dst);
   NRGK_HIP(hipGetLastError());
 }
 

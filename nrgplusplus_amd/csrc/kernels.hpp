@@ -21,7 +21,7 @@ void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym,
                double* conv, double* scrW, double* scrG, double* scrT, double* rrv, int nmax, double e_tol,
                double d_tol, hipStream_t s);
 void launch_copy_converged(const EigBlock* blocks, int nb, const double* conv, double e_tol, double d_tol,
-                           const double* src, double* dst, hipStream_t s);
+                           double e_pred, double d_pred, const double* src, double* dst, hipStream_t s);
 void launch_final_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const double* X,
                        const double* lam_in, double* U, double* lam_out, int* perm, size_t lds, hipStream_t s);
 size_t normalize_sort_lds(int nmax);
