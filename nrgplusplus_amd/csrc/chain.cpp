@@ -481,13 +481,25 @@ void Chain::eigensolve_mixed() {
   std::vector<int> order(nb);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return blks[x].nbc > blks[y].nbc; });
+  // The balance counts the tasks that do work: after round 0 the pair holding the empty block column
+  // that pads an odd count returns at once.  Blocks with rounds after the first are balanced on their
+  // tasks per later round (32 CUs per XCD: more than 32 tasks on one XCD need a second wave), the
+  // one-round blocks on their round-0 tasks.
   std::vector<int> xcd_of(std::max(1, nb), 0);
   {
-    int load[NXCD] = {0};
+    int load[NXCD] = {0}, load0[NXCD] = {0};
     for (int bi : order) {
-      const int x = (int)(std::min_element(load, load + NXCD) - load);
+      const EigBlock& e = blks[bi];
+      const int w = e.nbc > 2 ? e.nbc / 2 - ((e.nbc - 1) * e.bw >= e.n ? 1 : 0) : 0;
+      int x = 0;
+      for (int y = 1; y < NXCD; y++) {
+        const bool better = w > 0 ? (load[y] < load[x] || (load[y] == load[x] && load0[y] < load0[x]))
+                                  : (load0[y] < load0[x] || (load0[y] == load0[x] && load[y] < load[x]));
+        if (better) x = y;
+      }
       xcd_of[bi] = x;
-      load[x] += blks[bi].nbc / 2;
+      load[x] += w;
+      load0[x] += e.nbc / 2;
     }
   }
   std::vector<int> hflags(std::max(1, nb), 1);
