@@ -73,6 +73,37 @@ int nrgk_site_fdag_chi_spinhalf(nrgk_site** out);
  * nrgcore/include/models/spinhalf.hpp:106-114, 215): one system operator for the thermodynamic
  * expectation values */
 int nrgk_site_double_occupancy_spinhalf(nrgk_site** out);
+/* spinless(eps) (nrgcore/include/models/spinnless.hpp): one orbital, charge only, H = eps n */
+int nrgk_site_spinless(double eps, nrgk_site** out);
+/* spinnlessTwoLead(eps) (nrgcore/include/models/spinlessTwoLead.hpp): two orbitals, charge only,
+ * H = eps n_0; both f^dagger couple to the chain (the bath of the resonant-level driver) */
+int nrgk_site_spinless_two_lead(double eps, nrgk_site** out);
+/* resonantTwoLead(eps, V_L, V_R, U) (nrgcore/include/models/resonantTwoLead.hpp:20-74): a level and
+ * the first site of two leads, charge only.  observables = 0: the chain operators (the two lead
+ * f^dagger in the eigenbasis); 1: the same sectors with the operators impurityFdag and
+ * impurityNparticle, for nrgk_obs_create */
+int nrgk_site_resonant_two_lead(double eps, double v_left, double v_right, double U, int observables,
+                                nrgk_site** out);
+/* spinhalfSC (nrgcore/include/models/spinhalfSC.hpp:27-123): two orbitals, spin only;
+ * H = eps (n_up + n_dn) + h (n_up - n_dn) + U n_up n_dn + delta (f_up f_dn + h.c.).  The reference's
+ * addSCDelta(delta) replaces H by the pairing term alone: eps = h = U = 0 here. */
+int nrgk_site_spinhalf_sc(double eps, double U, double h, double delta, nrgk_site** out);
+/* a free two-orbital Wilson site, H = 0, in a symmetry: 0 charge and spin (freeModel.hpp), 1 charge
+ * only, 2 spin only (the bath of the superconducting impurities) */
+int nrgk_site_free(int symmetry, nrgk_site** out);
+/* Any model as the reference's constructors build one (e.g. resonantTwoLead.hpp:31-62): n_states
+ * basis states with n_qn quantum numbers each (qn_state[n_states][n_qn]), a symmetric Hamiltonian
+ * H[n_states][n_states] that does not couple different quantum numbers, and n_ops operators
+ * ops[n_ops][n_states][n_states].  States are grouped into sectors in first-appearance order, each
+ * sector diagonalised (eigenvalues ascending), and every nonzero operator block rotated into the
+ * eigenbasis, U_i^T F U_j.  chi = (-1)^(sum of the quantum numbers). */
+int nrgk_site_from_hamiltonian(int n_states, int n_qn, const int* qn_state, const double* H, int n_ops,
+                               const double* ops, nrgk_site** out);
+/* The Jordan-Wigner Fock basis of fermionBasis (nrgcore/include/models/fermionBasis.hpp:20-442):
+ * f^dagger_i of n_orb orbitals as full 2^n_orb x 2^n_orb matrices (fdag[n_orb][D][D], may be NULL)
+ * and the quantum numbers of each basis state for a symmetry (0 charge and spin: 2 numbers;
+ * 1 charge only, 2 spin only: 1 number; qn_state[D][n], may be NULL).  n_orb <= 12. */
+int nrgk_fock_basis(int n_orb, int symmetry, double* fdag, int* qn_state);
 /* impurity.f_dag_operator.push_back(f[0]); push_back(f[1]) (examples/twoChannelSiam/main.cpp:24-25) */
 int nrgk_site_duplicate_ops(nrgk_site* site);
 /* a view of the site; pointers stay valid until the site is destroyed or modified */

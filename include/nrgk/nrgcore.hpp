@@ -91,6 +91,42 @@ class site {
     check(nrgk_site_double_occupancy_spinhalf(&s));
     return site(s);
   }
+  // spinless(eps), spinnlessTwoLead(eps), resonantTwoLead(eps, V_L, V_R, U), spinhalfSC with a pairing
+  // term, and a free Wilson site in any symmetry (nrgcore/include/models/*.hpp)
+  static site spinless(double eps = 0.0) {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_spinless(eps, &s));
+    return site(s);
+  }
+  static site spinlessTwoLead(double eps = 0.0) {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_spinless_two_lead(eps, &s));
+    return site(s);
+  }
+  static site resonantTwoLead(double eps, double v_left, double v_right, double U, bool observables = false) {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_resonant_two_lead(eps, v_left, v_right, U, observables ? 1 : 0, &s));
+    return site(s);
+  }
+  static site spinhalfSC(double eps, double U, double h = 0.0, double delta = 0.0) {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_spinhalf_sc(eps, U, h, delta, &s));
+    return site(s);
+  }
+  static site freeSite(int symmetry = 0) {
+    nrgk_site* s = nullptr;
+    check(nrgk_site_free(symmetry, &s));
+    return site(s);
+  }
+  // any model: per-state quantum numbers, a Hamiltonian and full operators (row-major)
+  static site fromHamiltonian(int n_states, int n_qn, const std::vector<int>& qn_state, const std::vector<double>& H,
+                              const std::vector<std::vector<double>>& ops) {
+    std::vector<double> flat;
+    for (auto& o : ops) flat.insert(flat.end(), o.begin(), o.end());
+    nrgk_site* s = nullptr;
+    check(nrgk_site_from_hamiltonian(n_states, n_qn, qn_state.data(), H.data(), (int)ops.size(), flat.data(), &s));
+    return site(s);
+  }
   // the impurity f^dagger list doubled for two channels (examples/twoChannelSiam/main.cpp:24-25)
   site& duplicate_ops() {
     check(nrgk_site_duplicate_ops(h_.get()));

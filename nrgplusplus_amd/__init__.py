@@ -73,6 +73,13 @@ def lib():
             "nrgk_site_kondo": (i, [d, d, pp]),
             "nrgk_site_fdag_chi_spinhalf": (i, [pp]),
             "nrgk_site_double_occupancy_spinhalf": (i, [pp]),
+            "nrgk_site_spinless": (i, [d, pp]),
+            "nrgk_site_spinless_two_lead": (i, [d, pp]),
+            "nrgk_site_resonant_two_lead": (i, [d, d, d, d, i, pp]),
+            "nrgk_site_spinhalf_sc": (i, [d, d, d, d, pp]),
+            "nrgk_site_free": (i, [i, pp]),
+            "nrgk_site_from_hamiltonian": (i, [i, i, ip, dp, i, dp, pp]),
+            "nrgk_fock_basis": (i, [i, i, dp, ip]),
             "nrgk_site_duplicate_ops": (i, [vp]),
             "nrgk_site_view": (i, [vp, sd]),
             "nrgk_site_destroy": (i, [vp]),
@@ -190,6 +197,58 @@ class Site:
         return cls(p)
 
     @classmethod
+    def spinless(cls, eps=0.0):
+        """spinless(eps) (models/spinnless.hpp): one orbital, charge only"""
+        p = C.c_void_p()
+        _check(lib().nrgk_site_spinless(eps, C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def spinless_two_lead(cls, eps=0.0):
+        """spinnlessTwoLead(eps) (models/spinlessTwoLead.hpp): two orbitals, charge only"""
+        p = C.c_void_p()
+        _check(lib().nrgk_site_spinless_two_lead(eps, C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def resonant_two_lead(cls, eps, v_left, v_right, U, observables=False):
+        """resonantTwoLead (models/resonantTwoLead.hpp:20-74); observables=True gives the same
+        sectors with the operators impurityFdag and impurityNparticle"""
+        p = C.c_void_p()
+        _check(lib().nrgk_site_resonant_two_lead(eps, v_left, v_right, U, int(observables), C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def spinhalf_sc(cls, eps=0.0, U=0.0, h=0.0, delta=0.0):
+        """spinhalfSC (models/spinhalfSC.hpp:27-123) with the pairing term delta (f_up f_dn + h.c.)"""
+        p = C.c_void_p()
+        _check(lib().nrgk_site_spinhalf_sc(eps, U, h, delta, C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def free(cls, symmetry="charge_and_spin"):
+        """a free two-orbital Wilson site (freeModel.hpp), in one of the three symmetries"""
+        p = C.c_void_p()
+        _check(lib().nrgk_site_free(SYMMETRY[symmetry], C.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def from_hamiltonian(cls, qn_state, H, ops):
+        """any model: per-state quantum numbers [D][n_qn], a symmetric D x D Hamiltonian that keeps
+        them, and full D x D operators; sectors diagonalised and operators rotated into the eigenbasis"""
+        qn = np.ascontiguousarray(qn_state, np.int32)
+        if qn.ndim == 1:
+            qn = qn[:, None]
+        D, nq = qn.shape
+        H = np.ascontiguousarray(H, np.float64)
+        assert H.shape == (D, D)
+        o = np.ascontiguousarray(np.array(ops, np.float64).reshape(len(ops), D, D)) if len(ops) else np.zeros(1)
+        p = C.c_void_p()
+        _check(lib().nrgk_site_from_hamiltonian(D, nq, _ip(qn.reshape(-1)), _dp(H.reshape(-1)), len(ops),
+                                                _dp(o.reshape(-1)), C.byref(p)))
+        return cls(p)
+
+    @classmethod
     def from_arrays(cls, qn, dims, energies, chi, ops):
         """ops: list over operators of [(bi, bj, 2-D block)]"""
         qn = np.ascontiguousarray(qn, np.int32).reshape(-1)
@@ -262,6 +321,21 @@ class Site:
 
 
 # ---------------------------------------------------------------- context
+SYMMETRY = {"charge_and_spin": 0, "charge_only": 1, "spin_only": 2}
+
+
+def fock_basis(n_orb, symmetry="charge_and_spin"):
+    """fermionBasis (models/fermionBasis.hpp:20-442): the Jordan-Wigner f^dagger_i as full
+    2^n_orb x 2^n_orb matrices and the quantum numbers of each basis state"""
+    D = 1 << n_orb
+    sym = SYMMETRY[symmetry]
+    nq = 2 if sym == 0 else 1
+    f = np.zeros(n_orb * D * D)
+    q = np.zeros(D * nq, np.int32)
+    _check(lib().nrgk_fock_basis(n_orb, sym, _dp(f), _ip(q)))
+    return list(f.reshape(n_orb, D, D)), q.reshape(D, nq)
+
+
 class Context:
     """One HIP device.  Chains allocate from its capacity (HBM admission)."""
 

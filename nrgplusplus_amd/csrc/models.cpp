@@ -6,9 +6,12 @@
 //     and <s|f_i^dagger|s'> = (-1)^(number of occupied orbitals below i);
 //     chargeAndSpin quantum numbers [N(even orbitals), N(odd orbitals)] (:122-141);
 //     sectors in first-appearance order (:230-268); a block is kept iff sum |elem| > 0 (:291-318).
+//   chargeOnly [N] (:103-117) and spinOnly [N(even) - N(odd)] (:59-72, 150-193) quantum numbers;
 //   spinhalf (nrgcore/include/models/spinhalf.hpp:81-124), twoChannel
-//   (nrgcore/src/models/twoChannel.cpp:12-48), spinhalfKondo (nrgcore/include/models/spinhalfKondo.hpp:28-166)
-//   and the f^dagger chi observable of examples/fdmSpectrumSiam/main.cpp:86-106.
+//   (nrgcore/src/models/twoChannel.cpp:12-48), spinhalfKondo (nrgcore/include/models/spinhalfKondo.hpp:28-166),
+//   spinless, spinnlessTwoLead, resonantTwoLead, spinhalfSC and freeModel (nrgcore/include/models/),
+//   the f^dagger chi observable of examples/fdmSpectrumSiam/main.cpp:86-106, and a generic builder
+//   from a Hamiltonian on any basis with per-state quantum numbers.
 // Small blocks are diagonalised with a cyclic FP64 Jacobi (eigenvalues ascending, vectors in columns).
 #include <algorithm>
 #include <cmath>
@@ -140,6 +143,35 @@ struct Basis {
       for (size_t j = 0; j < occ.size(); j++) (j % 2 == 0 ? up : dn) += (int)occ[j][s];
       qn_state.push_back({up, dn});
     }
+    group();
+  }
+  // chargeOnly: [N] (fermionBasis.hpp:103-117)
+  void charge_only() {
+    qn_state.clear();
+    for (size_t s = 0; s < occ[0].size(); s++) {
+      int n = 0;
+      for (size_t j = 0; j < occ.size(); j++) n += (int)occ[j][s];
+      qn_state.push_back({n});
+    }
+    group();
+  }
+  // spinOnly: [N(even orbitals) - N(odd orbitals)] (fermionBasis.hpp:59-72, 150-193)
+  void spin_only() {
+    qn_state.clear();
+    for (size_t s = 0; s < occ[0].size(); s++) {
+      int sz = 0;
+      for (size_t j = 0; j < occ.size(); j++) sz += (j % 2 == 0 ? 1 : -1) * (int)occ[j][s];
+      qn_state.push_back({sz});
+    }
+    group();
+  }
+  void symmetry(int sym) {
+    if (sym == 0) charge_and_spin();
+    else if (sym == 1) charge_only();
+    else spin_only();
+  }
+  // sectors in first-appearance order, states ascending within a sector (fermionBasis.hpp:230-268)
+  void group() {
     std::set<std::vector<int>> seen;
     unique_qn.clear();
     blocks.clear();
@@ -208,6 +240,7 @@ static Site make_site(const Basis& B, const std::vector<std::vector<double>>& E,
                       const std::vector<std::map<std::pair<int, int>, M>>& ops) {
   Site s;
   s.n_sectors = (int)B.unique_qn.size();
+  s.n_qn = B.unique_qn.empty() ? 0 : (int)B.unique_qn[0].size();
   for (auto& q : B.unique_qn) {
     s.qn.insert(s.qn.end(), q.begin(), q.end());
     int tot = 0;
@@ -294,6 +327,158 @@ Site* site_kondo(double J, double S) {
   return new Site(make_site(B, E, ops));
 }
 
+// the sectors, energies and parities of s with other operator blocks
+static Site make_site_qn(const Site& s0, const std::vector<std::map<std::pair<int, int>, M>>& ops) {
+  Site s = s0;
+  s.nblocks.clear();
+  s.bi.clear();
+  s.bj.clear();
+  s.data.clear();
+  s.n_ops = (int)ops.size();
+  for (auto& op : ops) {
+    s.nblocks.push_back((int)op.size());
+    for (auto& [k, m] : op) {
+      s.bi.push_back(k.first);
+      s.bj.push_back(k.second);
+      s.data.insert(s.data.end(), m.a.begin(), m.a.end());
+    }
+  }
+  return s;
+}
+
+// A site from a Hamiltonian on any basis whose states carry quantum numbers: states grouped into
+// sectors (first appearance), each sector block diagonalised ascending, the operators rotated into
+// the eigenbasis, U_i^T F U_j, where their unrotated block is nonzero -- the construction every model
+// constructor of the reference repeats (e.g. resonantTwoLead.hpp:31-62, spinhalfSC.hpp:53-86).
+static Site site_from_states(Basis& B, const M& H, const std::vector<M>& ops) {
+  const int D = (int)B.qn_state.size();
+  std::vector<int> sec(D);
+  for (size_t i = 0; i < B.blocks.size(); i++)
+    for (int st : B.blocks[i]) sec[st] = (int)i;
+  for (int i = 0; i < D; i++)
+    for (int j = 0; j < D; j++)
+      if (sec[i] != sec[j] && H(i, j) != 0.0)
+        throw Error(NRGK_ERR_INVALID, "nrgk: the Hamiltonian couples different quantum-number sectors");
+  std::vector<std::vector<double>> E;
+  std::vector<M> Ub;
+  for (auto& hb : B.block_diag(H)) {
+    M U;
+    E.push_back(small_eigh(hb, U));
+    Ub.push_back(U);
+  }
+  auto bops = B.block_ops(ops);
+  for (auto& op : bops)
+    for (auto& [k, m] : op) m = mul(tr(Ub[k.first]), mul(m, Ub[k.second]));
+  return make_site(B, E, bops);
+}
+
+static M number(const M& fd) { return mul(fd, tr(fd)); }
+
+// spinless(eps) (nrgcore/include/models/spinnless.hpp): one orbital, chargeOnly, H = eps n
+Site* site_spinless(double eps) {
+  Basis B;
+  B.jordan_wigner(1);
+  B.charge_only();
+  return new Site(site_from_states(B, scal(number(B.fdag[0]), eps), B.fdag));
+}
+
+// spinnlessTwoLead(eps) (nrgcore/include/models/spinlessTwoLead.hpp): two orbitals, chargeOnly,
+// H = eps n_0, both f^dagger coupling to the chain
+Site* site_spinless_two_lead(double eps) {
+  Basis B;
+  B.jordan_wigner(2);
+  B.charge_only();
+  return new Site(site_from_states(B, scal(number(B.fdag[0]), eps), B.fdag));
+}
+
+// resonantTwoLead(eps, V_L, V_R, U) (nrgcore/include/models/resonantTwoLead.hpp:20-62): a level
+// (orbital 0) and the first site of each lead (orbitals 1, 2), chargeOnly;
+// H = eps n_0 + V_L (f0+ f1 + f1+ f0) + V_R (f0+ f2 + f2+ f0) + U (n_0 - I)(n_1 + n_2 - I), the
+// identity as there (its `half_id` is n_label.id(), qmatrix.hpp:251-262).  The chain couples to the
+// two lead orbitals;
+// with observables = 1 the operators are instead the level's f^dagger and n_0 = f0+ f0 blocks
+// (impurityFdag, impurityNparticle) for update_system_operator.
+Site* site_resonant_two_lead(double eps, double vl, double vr, double U, bool observables) {
+  Basis B;
+  B.jordan_wigner(3);
+  B.charge_only();
+  const M& f0 = B.fdag[0];
+  const M& f1 = B.fdag[1];
+  const M& f2 = B.fdag[2];
+  const M n0 = number(f0), n1 = number(f1), n2 = number(f2);
+  const M I = eye(n0.r);
+  M H = scal(n0, eps);
+  H = lin(H, 1.0, lin(mul(f0, tr(f1)), 1.0, mul(f1, tr(f0)), 1.0), vl);
+  H = lin(H, 1.0, lin(mul(f0, tr(f2)), 1.0, mul(f2, tr(f0)), 1.0), vr);
+  H = lin(H, 1.0, mul(lin(n0, 1.0, I, -1.0), lin(lin(n1, 1.0, n2, 1.0), 1.0, I, -1.0)), U);
+  if (!observables) return new Site(site_from_states(B, H, {f1, f2}));
+  // impurityNparticle: F F^T of the rotated level operator, on the diagonal blocks (resonantTwoLead.hpp:64-74)
+  Site s = site_from_states(B, H, {f0});
+  std::vector<std::map<std::pair<int, int>, M>> ops(2);
+  size_t off = 0;
+  std::map<std::pair<int, int>, M> fd;
+  for (int b = 0; b < s.nblocks[0]; b++) {
+    const int i = s.bi[b], j = s.bj[b];
+    M m(s.dims[i], s.dims[j]);
+    std::copy(s.data.begin() + off, s.data.begin() + off + m.a.size(), m.a.begin());
+    off += m.a.size();
+    fd[{i, j}] = m;
+  }
+  ops[0] = fd;
+  for (auto& [k, m] : fd) {
+    M nn = mul(m, tr(m));
+    auto& slot = ops[1][{k.first, k.first}];
+    slot = slot.r ? lin(slot, 1.0, nn, 1.0) : nn;
+  }
+  return new Site(make_site_qn(s, ops));
+}
+
+// spinhalfSC(eps, U, h) with a pairing term (nrgcore/include/models/spinhalfSC.hpp:27-123): two
+// orbitals, spinOnly; H = eps (n_up + n_dn) + h (n_up - n_dn) + U n_up n_dn + delta (f_up f_dn + h.c.).
+// The reference's addSCDelta (:89-123) replaces H by the pairing term alone; the same site follows
+// here with eps = h = U = 0.
+Site* site_spinhalf_sc(double eps, double U, double h, double delta) {
+  Basis B;
+  B.jordan_wigner(2);
+  B.spin_only();
+  const M nu = number(B.fdag[0]), nd = number(B.fdag[1]);
+  M H = lin(lin(lin(nu, 1.0, nd, 1.0), eps, lin(nu, 1.0, nd, -1.0), h), 1.0, mul(nu, nd), U);
+  const M pair = lin(mul(tr(B.fdag[0]), tr(B.fdag[1])), 1.0, mul(B.fdag[1], B.fdag[0]), 1.0);
+  H = lin(H, 1.0, pair, delta);
+  return new Site(site_from_states(B, H, B.fdag));
+}
+
+// a free Wilson site in any of the three symmetries (freeModel.hpp for chargeAndSpin; the spinOnly
+// site is the bath of the superconducting impurities, the chargeOnly one spinnlessTwoLead(0))
+Site* site_free(int symmetry) {
+  Basis B;
+  B.jordan_wigner(2);
+  B.symmetry(symmetry);
+  return new Site(site_from_states(B, M(4, 4), B.fdag));
+}
+
+// a site from user data: D states with n_qn quantum numbers each, H (D x D) and n_ops operators
+Site* site_from_hamiltonian(int D, int n_qn, const int* qn_state, const double* H, int n_ops, const double* ops) {
+  Basis B;
+  B.qn_state.clear();
+  for (int i = 0; i < D; i++) B.qn_state.push_back(std::vector<int>(qn_state + (size_t)i * n_qn, qn_state + (size_t)(i + 1) * n_qn));
+  B.group();
+  M Hm(D, D);
+  std::copy(H, H + (size_t)D * D, Hm.a.begin());
+  std::vector<M> om;
+  for (int p = 0; p < n_ops; p++) {
+    M o(D, D);
+    std::copy(ops + (size_t)p * D * D, ops + (size_t)(p + 1) * D * D, o.a.begin());
+    om.push_back(o);
+  }
+  for (int i = 0; i < D; i++)
+    for (int j = 0; j < i; j++)
+      if (Hm(i, j) != Hm(j, i)) throw Error(NRGK_ERR_INVALID, "nrgk: the Hamiltonian is not symmetric");
+  Site s = site_from_states(B, Hm, om);
+  s.n_qn = n_qn;
+  return new Site(s);
+}
+
 Site* site_fdag_chi_spinhalf() {
   Basis B;
   B.jordan_wigner(2);
@@ -329,6 +514,22 @@ struct nrgk_site {
 
 using nrgk::Error;
 
+template <class F>
+static int wrap(F&& make, nrgk_site** out) {
+  try {
+    nrgk::Site* s = make();
+    *out = new nrgk_site{std::move(*s)};
+    delete s;
+    return NRGK_OK;
+  } catch (const nrgk::Error& e) {
+    nrgk::set_last_error(e.what());
+    return e.status;
+  } catch (const std::exception& e) {
+    nrgk::set_last_error(e.what());
+    return NRGK_ERR_INVALID;
+  }
+}
+
 extern "C" {
 int nrgk_site_spinhalf(double eps, double U, double h, nrgk_site** out) {
   if (!out) return NRGK_ERR_INVALID;
@@ -356,6 +557,46 @@ int nrgk_site_fdag_chi_spinhalf(nrgk_site** out) {
   nrgk::Site* s = nrgk::site_fdag_chi_spinhalf();
   *out = new nrgk_site{std::move(*s)};
   delete s;
+  return NRGK_OK;
+}
+int nrgk_site_spinless(double eps, nrgk_site** out) {
+  if (!out) return NRGK_ERR_INVALID;
+  return wrap([&] { return nrgk::site_spinless(eps); }, out);
+}
+int nrgk_site_spinless_two_lead(double eps, nrgk_site** out) {
+  if (!out) return NRGK_ERR_INVALID;
+  return wrap([&] { return nrgk::site_spinless_two_lead(eps); }, out);
+}
+int nrgk_site_resonant_two_lead(double eps, double v_left, double v_right, double U, int observables,
+                                nrgk_site** out) {
+  if (!out) return NRGK_ERR_INVALID;
+  return wrap([&] { return nrgk::site_resonant_two_lead(eps, v_left, v_right, U, observables != 0); }, out);
+}
+int nrgk_site_spinhalf_sc(double eps, double U, double h, double delta, nrgk_site** out) {
+  if (!out) return NRGK_ERR_INVALID;
+  return wrap([&] { return nrgk::site_spinhalf_sc(eps, U, h, delta); }, out);
+}
+int nrgk_site_free(int symmetry, nrgk_site** out) {
+  if (!out || symmetry < 0 || symmetry > 2) return NRGK_ERR_INVALID;
+  return wrap([&] { return nrgk::site_free(symmetry); }, out);
+}
+int nrgk_site_from_hamiltonian(int n_states, int n_qn, const int* qn_state, const double* H, int n_ops,
+                               const double* ops, nrgk_site** out) {
+  if (!out || n_states <= 0 || n_qn <= 0 || !qn_state || !H || n_ops < 0 || (n_ops > 0 && !ops))
+    return NRGK_ERR_INVALID;
+  return wrap([&] { return nrgk::site_from_hamiltonian(n_states, n_qn, qn_state, H, n_ops, ops); }, out);
+}
+int nrgk_fock_basis(int n_orb, int symmetry, double* fdag, int* qn_state) {
+  if (n_orb <= 0 || n_orb > 12 || symmetry < 0 || symmetry > 2) return NRGK_ERR_INVALID;
+  nrgk::Basis B;
+  B.jordan_wigner(n_orb);
+  B.symmetry(symmetry);
+  const size_t D = (size_t)1 << n_orb;
+  if (fdag)
+    for (int i = 0; i < n_orb; i++) std::copy(B.fdag[i].a.begin(), B.fdag[i].a.end(), fdag + (size_t)i * D * D);
+  if (qn_state)
+    for (size_t st = 0; st < D; st++)
+      std::copy(B.qn_state[st].begin(), B.qn_state[st].end(), qn_state + st * B.qn_state[st].size());
   return NRGK_OK;
 }
 int nrgk_site_double_occupancy_spinhalf(nrgk_site** out) {

@@ -288,6 +288,71 @@ def test_z_averaged_thermodynamics_matches_oracle(nk, oracle_mod):
     assert np.all(np.isfinite(S_avg))
 
 
+def _jw(n_orb):
+    D = 1 << n_orb
+    ops = []
+    for i in range(n_orb):
+        m = np.zeros((D, D))
+        for st in range(D):
+            if not (st >> i) & 1:
+                m[st | (1 << i), st] = (-1) ** bin(st & ((1 << i) - 1)).count("1")
+        ops.append(m)
+    return ops
+
+
+def _chain_vs_ed(nk, imp, bath, H_imp, links, n_imp, n_sites, L):
+    """Untruncated chain of a given impurity and two-orbital bath against exact diagonalisation of
+    the whole fermion chain: H_0 = H_imp, H_{k+1} = r_k H_k + t_k sum_c (a+_c b_c + h.c.), where a_c
+    are the impurity (k = 0) or previous site's orbitals of channel c and b_c the new site's."""
+    g = nk.NrgCore(imp, bath, 100000)
+    n_orb = n_imp + 2 * n_sites
+    f = _jw(n_orb)
+    H = np.zeros((1 << n_orb, 1 << n_orb))
+    H[...] = H_imp(f)
+    prev = list(links)
+    for k in range(n_sites):
+        t, r = nk.hopping(k, L), (1.0 if k == 0 else np.sqrt(L))
+        g.add_bath_site([t, t], r)
+        new = [n_imp + 2 * k, n_imp + 2 * k + 1]
+        H = r * H
+        for a, b in zip(prev, new):
+            hop = f[a] @ f[b].T
+            H = H + t * (hop + hop.T)
+        prev = new
+        g.update_internal_state()  # all_eigenvalue is refreshed there (nrgcore.hpp:105-109, 269-327)
+        got = np.sort(g.all_eigenvalue)
+        # the ED space holds the sites not yet added as idle orbitals: 4 copies per idle site
+        ed = np.linalg.eigvalsh(H)
+        idle = n_sites - 1 - k
+        exp = np.sort(np.repeat(got, 4 ** idle))
+        assert np.abs(exp - ed).max() < 1e-9, (k, np.abs(exp - ed).max())
+
+
+def test_resonant_level_chain_equals_exact_diagonalisation(nk):
+    """examples/resonantLevel/main.cpp with the hopping list sized to the two lead operators (the
+    driver passes four and would throw): resonantTwoLead impurity, spinnlessTwoLead bath, Lambda = 3."""
+    eps, vl, vr, U = -0.025, 0.11, 0.07, 0.05
+
+    def H_imp(f):
+        n = [x @ x.T for x in f[:3]]
+        I = np.eye(f[0].shape[0])
+        return eps * n[0] + vl * (f[0] @ f[1].T + f[1] @ f[0].T) + vr * (f[0] @ f[2].T + f[2] @ f[0].T) \
+            + U * (n[0] - I) @ (n[1] + n[2] - I)
+
+    _chain_vs_ed(nk, nk.Site.resonant_two_lead(eps, vl, vr, U), nk.Site.spinless_two_lead(), H_imp, [1, 2], 3, 3, 3.0)
+
+
+def test_superconducting_impurity_chain_equals_exact_diagonalisation(nk):
+    """spinhalfSC with a pairing term on a spin-only free chain (only S_z conserved)."""
+    e, u, h, d = -0.1, 0.3, 0.02, 0.05
+
+    def H_imp(f):
+        nu, nd = f[0] @ f[0].T, f[1] @ f[1].T
+        return e * (nu + nd) + h * (nu - nd) + u * nu @ nd + d * (f[0].T @ f[1].T + f[1] @ f[0])
+
+    _chain_vs_ed(nk, nk.Site.spinhalf_sc(e, u, h, d), nk.Site.free("spin_only"), H_imp, [0, 1], 2, 3, 2.0)
+
+
 def test_broadening_matches_oracle(nk, oracle_mod):
     rng = np.random.default_rng(1)
     x = np.sort(10 ** rng.uniform(-9, 1, 300))

@@ -125,3 +125,92 @@ def test_cpp_nrgcore_mirror_compiles():
     src = os.path.join(root, "nrgplusplus_amd", "cpp", "rgflow_siam.cpp")
     r = subprocess.run(["g++", "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+# ---------------------------------------------------------------- other model families
+def _jw(n_orb):
+    """f^dagger_i with orbital i as bit i of the Fock index and the sign of the occupied orbitals
+    below i (the reference's Jordan-Wigner order, fermionBasis.hpp:397-420)"""
+    D = 1 << n_orb
+    ops = []
+    for i in range(n_orb):
+        m = np.zeros((D, D))
+        for s in range(D):
+            if not (s >> i) & 1:
+                m[s | (1 << i), s] = (-1) ** bin(s & ((1 << i) - 1)).count("1")
+        ops.append(m)
+    return ops
+
+
+@pytest.mark.parametrize("sym", ["charge_and_spin", "charge_only", "spin_only"])
+def test_fock_basis(nk, sym):
+    f, q = nk.fock_basis(3, sym)
+    for a, b in zip(f, _jw(3)):
+        np.testing.assert_array_equal(a, b)
+    occ = np.array([[(s >> i) & 1 for i in range(3)] for s in range(8)])
+    expect = {"charge_and_spin": np.stack([occ[:, 0] + occ[:, 2], occ[:, 1]], 1),
+              "charge_only": occ.sum(1)[:, None],
+              "spin_only": (occ[:, 0] + occ[:, 2] - occ[:, 1])[:, None]}[sym]
+    np.testing.assert_array_equal(q, expect)
+
+
+def _check_site(site, qn_state, H, ops):
+    """sectors in first-appearance order, per-sector eigenvalues of H (numpy), chi = (-1)^sum(qn), and
+    every operator block's Frobenius norm equal to its unrotated block's (orthogonal rotation)"""
+    qn, dims, en, chi, got_ops = site.export()
+    keys = []
+    for q in map(tuple, qn_state):
+        if q not in keys:
+            keys.append(q)
+    np.testing.assert_array_equal(qn, np.array(keys).reshape(len(keys), -1))
+    idx = [[s for s in range(len(qn_state)) if tuple(qn_state[s]) == k] for k in keys]
+    np.testing.assert_array_equal(dims, [len(x) for x in idx])
+    np.testing.assert_array_equal(chi, [(-1.0) ** sum(k) for k in keys])
+    off = 0
+    for x in idx:
+        np.testing.assert_allclose(en[off:off + len(x)], np.linalg.eigvalsh(H[np.ix_(x, x)]), atol=1e-14)
+        off += len(x)
+    assert len(got_ops) == len(ops)
+    for blocks, F in zip(got_ops, ops):
+        want = {(i, j): np.linalg.norm(F[np.ix_(idx[i], idx[j])]) for i in range(len(idx)) for j in range(len(idx))}
+        assert sorted((i, j) for i, j, _ in blocks) == sorted(k for k, v in want.items() if v > 0)
+        for i, j, b in blocks:
+            assert abs(np.linalg.norm(b) - want[(i, j)]) < 1e-13
+
+
+def test_reference_models(nk):
+    f3, _ = nk.fock_basis(3, "charge_only")
+    f1, f2 = _jw(1), _jw(2)
+    q1 = np.array([[(s & 1)] for s in range(2)])
+    _check_site(nk.Site.spinless(0.3), q1, 0.3 * f1[0] @ f1[0].T, f1)
+    q2 = np.array([[bin(s).count("1")] for s in range(4)])
+    _check_site(nk.Site.spinless_two_lead(-0.2), q2, -0.2 * f2[0] @ f2[0].T, f2)
+    # resonantTwoLead.hpp:31-40: eps n0 + V_L(f0+ f1 + h.c.) + V_R(f0+ f2 + h.c.) + U (n0 - 1)(n1 + n2 - 1)
+    eps, vl, vr, U = -0.025, 0.11, 0.07, 0.05
+    f = _jw(3)
+    n = [x @ x.T for x in f]
+    I = np.eye(8)
+    H = eps * n[0] + vl * (f[0] @ f[1].T + f[1] @ f[0].T) + vr * (f[0] @ f[2].T + f[2] @ f[0].T) \
+        + U * (n[0] - I) @ (n[1] + n[2] - I)
+    q3 = np.array([[bin(s).count("1")] for s in range(8)])
+    _check_site(nk.Site.resonant_two_lead(eps, vl, vr, U), q3, H, [f[1], f[2]])
+    # impurityFdag and impurityNparticle = sum_j F_ij F_ij^T on (i, i) (resonantTwoLead.hpp:63-74)
+    _check_site(nk.Site.resonant_two_lead(eps, vl, vr, U, observables=True), q3, H, [f[0], n[0]])
+    # spinhalfSC with pairing: sector Sz = 0 holds |0> and |up dn>
+    e, u, h, d = -0.1, 0.3, 0.02, 0.05
+    nu, nd = f2[0] @ f2[0].T, f2[1] @ f2[1].T
+    Hs = e * (nu + nd) + h * (nu - nd) + u * nu @ nd + d * (f2[0].T @ f2[1].T + f2[1] @ f2[0])
+    qs = np.array([[(s & 1) - ((s >> 1) & 1)] for s in range(4)])
+    _check_site(nk.Site.spinhalf_sc(e, u, h, d), qs, Hs, f2)
+    _check_site(nk.Site.free("spin_only"), qs, np.zeros((4, 4)), f2)
+    # a user model through the generic builder equals the library's own
+    _check_site(nk.Site.from_hamiltonian(q3, H, [f[1], f[2]]), q3, H, [f[1], f[2]])
+
+
+def test_model_builder_rejects_symmetry_breaking(nk):
+    f = _jw(2)
+    q = np.array([[bin(s).count("1")] for s in range(4)])
+    with pytest.raises(nk.NrgkError):  # f0+ + f0 changes N
+        nk.Site.from_hamiltonian(q, f[0] + f[0].T, [f[0]])
+    with pytest.raises(nk.NrgkError):  # not symmetric
+        nk.Site.from_hamiltonian(q, f[0] @ f[1].T, [f[0]])
