@@ -222,6 +222,15 @@ int nrgk_fdm_record(nrgk_fdm* fdm);
  * weights summed over shells; traces/sums: per processed shell (rhoTrace, specSum), may be NULL. */
 int nrgk_fdm_backward(nrgk_fdm* fdm, double lambda, int scale_offset, int first_cnt, double* pos,
                       double* neg, double* traces, double* sums, int cap_shells, int* n_shells);
+/* NrgData::saveCurrentData + saveqOperator (nrgcore/include/nrgcore/nrgData.hpp:134-205) of every
+ * recorded shell, in an HDF5 file of the reference's layout (h5stream.hpp:244-478): /NrgItr{cnt}/
+ * current_hamiltonQ{s}, current_sysmQ{s}, pre_sysmQ{s}, eigenvaluesQ{s}, coupled_nQ_index{s},
+ * eigenvaluesQ_kept_indices{s}; /Eigenvalues{cnt}; and, when the recorder has an observable and
+ * op_group is not NULL, {op_group}{cnt-1} with attribute operatorSize and at{p}/qmat{k},
+ * idVector{k}, colVector, rowVector.  The HDF5 library is loaded when first needed. */
+int nrgk_fdm_save(nrgk_fdm* rec, const char* path, const char* op_group);
+/* a one-dimensional dataset of an HDF5 file as doubles (n: its length; out may be NULL) */
+int nrgk_h5_read(const char* path, const char* dataset, double* out, size_t cap, size_t* n);
 /* bin centres e_min*exp(i/delE) (fdmSpectrum.hpp:323-327) */
 int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out);
 

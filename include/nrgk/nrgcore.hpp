@@ -380,6 +380,10 @@ class fdmSpectrum {
     return e;
   }
   nrgk_fdm* handle() const { return h_.get(); }
+  // NrgData::saveCurrentData + saveqOperator of every recorded shell, in the reference's HDF5 layout
+  void save(const std::string& path, const std::string& op_group = "/nparticle") {
+    check(nrgk_fdm_save(h_.get(), path.c_str(), op_group.empty() ? nullptr : op_group.c_str()));
+  }
 
   std::vector<std::vector<double>> positiveWeight, negativeWeight;
   std::vector<double> rhoTraces;

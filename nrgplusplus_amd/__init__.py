@@ -124,6 +124,8 @@ def lib():
             "nrgk_fdm_backward": (i, [vp, d, i, i, dp, dp, dp, dp, i, ip]),
             "nrgk_fdm_energy_points": (i, [vp, dp]),
             "nrgk_thermo": (i, [vp, d, i, i, dp, i, dp, dp]),
+            "nrgk_fdm_save": (i, [vp, C.c_char_p, C.c_char_p]),
+            "nrgk_h5_read": (i, [C.c_char_p, C.c_char_p, dp, sz, szp]),
             "nrgk_fdm_set_system_operator": (i, [vp, vp]),
             "nrgk_thermo_expectation": (i, [vp, d, i, i, dp, i, dp]),
             "nrgk_broaden": (i, [vp, dp, dp, i, dp, i, d, dp]),
@@ -380,6 +382,7 @@ class NrgCore:
         _check(lib().nrgk_chain_create(self.ctx._h, C.byref(di), C.byref(db), max_kept, mode, C.byref(p)))
         self._h = p
         self.n_ops = di.n_ops
+        self.n_qn = di.n_qn
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _LIB is not None:
@@ -442,7 +445,7 @@ class NrgCore:
         """(current_sysmQ as [S, n_qn], block dimensions [S])"""
         s = C.c_int()
         _check(lib().nrgk_chain_sectors(self._h, C.byref(s), None, None))
-        qn = np.zeros((s.value, 2), np.int32)
+        qn = np.zeros((s.value, self.n_qn), np.int32)
         dims = np.zeros(s.value, np.int32)
         _check(lib().nrgk_chain_sectors(self._h, C.byref(s), _ip(qn), _ip(dims)))
         return qn, dims
@@ -632,6 +635,11 @@ class FdmRecorder:
         _check(lib().nrgk_thermo(self._h, lam, scale_offset, first_cnt, _dp(T), len(T), _dp(S), _dp(Cv)))
         return S, Cv
 
+    def save(self, path, op_group="/nparticle"):
+        """NrgData::saveCurrentData + saveqOperator of every recorded shell (nrgData.hpp:134-205), in
+        the reference's HDF5 layout; op_group=None leaves the observable out"""
+        _check(lib().nrgk_fdm_save(self._h, os.fsencode(path), op_group.encode() if op_group else None))
+
     def setSystemOperator(self, op):
         """fdmThermodynamics::setSystemOperator (static/fdmThermodynamics.hpp:39-41): shells recorded
         from now on keep the diagonals of ``op`` (a SystemOperator updated before each record)."""
@@ -662,6 +670,15 @@ def broaden(x, w, omega, b, ctx=None):
     out = np.zeros(len(omega))
     _check(lib().nrgk_broaden(ctx._h, _dp(x), _dp(w), len(x), _dp(omega), len(omega), float(b), _dp(out)))
     return out
+
+
+def h5_read(path, dataset):
+    """a one-dimensional dataset of an HDF5 file, as doubles"""
+    n = C.c_size_t()
+    _check(lib().nrgk_h5_read(os.fsencode(path), dataset.encode(), None, 0, C.byref(n)))
+    out = np.zeros(max(1, n.value))
+    _check(lib().nrgk_h5_read(os.fsencode(path), dataset.encode(), _dp(out), len(out), C.byref(n)))
+    return out[: n.value]
 
 
 def z_chain(z, lam, n_sites, D=1.0, dos=None, quad="exact", n_intervals=0, normalize=True):

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "chain.hpp"
+#include "h5io.hpp"
 #include "kernels.hpp"
 #include "observables.hpp"
 
@@ -168,7 +169,7 @@ int nrgk_chain_all_eigenvalues(nrgk_chain* ch, double* out, size_t cap, size_t* 
     NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
     Chain& c = *ch->c;
     NRGK_REQUIRE(c.phase != Chain::FRESH || !c.all_eig.empty(), NRGK_ERR_STATE, "nrgk: no eigenvalues yet");
-    if (c.phase == Chain::TRUNCATED) c.ensure_all_eig();
+    c.ensure_all_eig();
     if (n) *n = c.all_eig.size();
     if (out) {
       NRGK_REQUIRE(cap >= c.all_eig.size(), NRGK_ERR_INVALID, "nrgk: output capacity too small");
@@ -389,6 +390,23 @@ int nrgk_thermo(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, c
   return guard([&] {
     NRGK_REQUIRE(rec && T && S && C && nT > 0, NRGK_ERR_INVALID, "nrgk_thermo: null argument");
     rec->f->thermo(lambda, scale_offset, first_cnt, T, nT, S, C);
+  });
+}
+int nrgk_fdm_save(nrgk_fdm* rec, const char* path, const char* op_group) {
+  return guard([&] {
+    NRGK_REQUIRE(rec && path, NRGK_ERR_INVALID, "nrgk_fdm_save: null argument");
+    rec->f->save(path, op_group ? op_group : "");
+  });
+}
+int nrgk_h5_read(const char* path, const char* dataset, double* out, size_t cap, size_t* n) {
+  return guard([&] {
+    NRGK_REQUIRE(path && dataset, NRGK_ERR_INVALID, "nrgk_h5_read: null argument");
+    h5::File f(path, false);
+    const size_t m = f.size(dataset);
+    if (n) *n = m;
+    if (!out) return;
+    NRGK_REQUIRE(cap >= m, NRGK_ERR_INVALID, "nrgk_h5_read: output capacity too small");
+    f.read(dataset, h5::Type::F64, out, m);
   });
 }
 int nrgk_fdm_set_system_operator(nrgk_fdm* rec, nrgk_obs* sysop) {

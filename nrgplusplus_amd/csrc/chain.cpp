@@ -86,7 +86,7 @@ Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, 
                    std::to_string(imp.nops) + "bath_model->f_dag_operator.size() = " + std::to_string(bath.nops));
   NRGK_REQUIRE(mode == NRGK_MODE_MIXED || mode == NRGK_MODE_PLAIN64, NRGK_ERR_INVALID, "nrgk: unknown mode");
   for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
-                    &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
+                    &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_prev, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
                     &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act,
                     &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv})
     b->ctx = ctx;
@@ -745,6 +745,16 @@ void Chain::add_bath_site(const double* t, int nt, double rescale) {
   NRGK_REQUIRE(nt == (int)imp.nops, NRGK_ERR_INVALID,
                std::string("t_hopping ") + std::to_string(nt) + " =! pre_fdag_oparator " + std::to_string(imp.nops));
   if (cnt == 0 && phase == FRESH) import_impurity();
+  // all_eigenvalue keeps the previous truncation's list until the next update_internal_state
+  // (nrgcore.hpp:105-109, 269-327): when it has not been read yet, the raw eigenvalues it is sorted
+  // from are kept aside before the eigensolve overwrites them
+  if (phase == TRUNCATED && all_eig.empty() && N_tot > 0) {
+    dlam_prev.ensure((size_t)N_tot * sizeof(double));
+    NRGK_HIP(hipMemcpyAsync(dlam_prev.p, dlam_raw.p, (size_t)N_tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+    N_prev = N_tot;
+  } else {
+    N_prev = 0;
+  }
   std::vector<double> tv(t, t + nt);
   NRGK_HIP(hipEventRecord(ev[0], st));
   create_next_basis();
@@ -902,9 +912,13 @@ void Chain::rotate_fdag() {
 }
 
 void Chain::ensure_all_eig() {
-  if (!all_eig.empty() || N_tot == 0 || phase == FRESH) return;
-  std::vector<double> v(N_tot);
-  d2h(v.data(), dlam_raw.as<double>(), N_tot, st);
+  if (!all_eig.empty() || phase == FRESH) return;
+  // after add_bath_site: the previous iteration's eigenvalues, kept aside there
+  const bool prev = phase == ASSEMBLED;
+  const long long n = prev ? N_prev : N_tot;
+  if (n == 0) return;
+  std::vector<double> v(n);
+  d2h(v.data(), (prev ? dlam_prev : dlam_raw).as<double>(), n, st);
   sync();
   std::sort(v.begin(), v.end());
   all_eig = std::move(v);

@@ -127,6 +127,8 @@ struct Chain {
 
   // host mirrors
   std::vector<double> all_eig;
+  DevBuf dlam_prev;      // the previous iteration's raw eigenvalues, while all_eig is still unsorted
+  long long N_prev = 0;
   std::vector<double> relgs;
   std::vector<double> h_last_shift;  // shifted eigenvalues of the previous basis (host copy, lazily)
 

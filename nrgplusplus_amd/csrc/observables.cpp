@@ -13,6 +13,7 @@
 #include <cstring>
 #include <numeric>
 
+#include "h5io.hpp"
 #include "kernels.hpp"
 
 namespace nrgk {
@@ -156,6 +157,9 @@ void Fdm::record() {
   sh->coupled = c.coupled;
   sh->sub_kidx = c.sub_kidx;
   sh->pre_kept = c.pre_kept;
+  sh->qn = c.cur_qn;
+  sh->pre_qn = c.pre_qn;
+  sh->n_bath = c.bath.S;
   const double* h_e = c.stg.get(c.dlam_raw.as<double>(), c.N_tot, c.st);
   if (obs) {
     sh->dU.p = arena_alloc(std::max<long long>(1, c.tot_elems) * sizeof(double));
@@ -193,6 +197,79 @@ void Fdm::record() {
   sh->E.assign(h_e, h_e + c.N_tot);
   if (h_q) sh->Q.assign(h_q, h_q + (size_t)sh->q_ops * c.N_tot);
   shells.push_back(std::move(sh));
+}
+
+// The reference's temporary file of the forward pass (NrgData, nrgData.hpp:134-205) for every recorded
+// shell: /NrgItr{cnt}/ with current_hamiltonQ{s} (the n_s x n_s eigenvector block, row-major, columns
+// the eigenvectors), current_sysmQ{s}, pre_sysmQ{s}, eigenvaluesQ{s}, coupled_nQ_index{s}
+// (i * bath sectors + j) and eigenvaluesQ_kept_indices{s}; Eigenvalues{cnt} at the root; and the
+// observable as saveqOperator writes it, in the group {op_group}{cnt - 1} (the SIAM drivers' index,
+// examples/fdmSpectrumSiam/main.cpp:89-90): attribute operatorSize, at{p}/qmat{k}, idVector{k},
+// colVector, rowVector.  Eigenvalues{cnt} holds the shell's own eigenvalues, ascending (the reference
+// writes the chain's all_eigenvalue of that moment, the previous iteration's list).
+void Fdm::save(const std::string& path, const std::string& op_group) const {
+  Chain& c = *ch;
+  h5::File f(path, true);
+  std::vector<double> hU, hO;
+  for (const auto& shp : shells) {
+    const Shell& sh = *shp;
+    const std::string g = "/NrgItr" + std::to_string(sh.cnt) + "/";
+    f.group(g);
+    const int S = (int)sh.n.size();
+    if (sh.dU.p) {
+      long long tot = 0;
+      for (int i = 0; i < S; i++) tot = std::max(tot, sh.uoff[i] + (long long)sh.n[i] * sh.n[i]);
+      hU.resize(std::max<long long>(1, tot));
+      d2h(hU.data(), sh.dU.as<double>(), tot, c.st);
+    }
+    if (sh.dO.p) {
+      long long tot = 0;
+      for (auto& m : sh.ops)
+        for (auto& [k, r] : m) tot = std::max(tot, r.off + (long long)r.rows * r.cols);
+      hO.resize(std::max<long long>(1, tot));
+      d2h(hO.data(), sh.dO.as<double>(), tot, c.st);
+    }
+    NRGK_HIP(hipStreamSynchronize(c.st));
+    for (int i = 0; i < S; i++) {
+      const std::string si = std::to_string(i);
+      if (sh.dU.p)
+        f.write(g + "current_hamiltonQ" + si, h5::Type::F64, hU.data() + sh.uoff[i], (size_t)sh.n[i] * sh.n[i]);
+      f.write(g + "current_sysmQ" + si, sh.qn[i]);
+      f.write(g + "eigenvaluesQ" + si, std::vector<double>(sh.E.begin() + sh.eoff[i], sh.E.begin() + sh.eoff[i] + sh.n[i]));
+      std::vector<unsigned long> cq;
+      for (auto [ii, bb] : sh.coupled[i]) cq.push_back((unsigned long)ii * sh.n_bath + bb);
+      f.write(g + "coupled_nQ_index" + si, cq);
+    }
+    for (size_t i = 0; i < sh.pre_qn.size(); i++) {
+      const std::string si = std::to_string(i);
+      f.write(g + "pre_sysmQ" + si, sh.pre_qn[i]);
+      std::vector<unsigned long> kept(sh.pre_kept[i]);
+      for (int k = 0; k < sh.pre_kept[i]; k++) kept[k] = k;
+      f.write(g + "eigenvaluesQ_kept_indices" + si, kept);
+    }
+    std::vector<double> all = sh.E;
+    std::sort(all.begin(), all.end());
+    f.write("Eigenvalues" + std::to_string(sh.cnt), all);
+    if (sh.dO.p && !op_group.empty()) {
+      const std::string og = op_group + std::to_string(sh.cnt - 1);
+      f.attribute(og, "operatorSize", (unsigned long)sh.ops.size());
+      for (size_t p = 0; p < sh.ops.size(); p++) {
+        const std::string lg = og + "/at" + std::to_string(p) + "/";
+        f.group(lg);
+        std::vector<unsigned long> cols, rows;
+        size_t k = 0;
+        for (auto& [key, r] : sh.ops[p]) {
+          f.write(lg + "qmat" + std::to_string(k), h5::Type::F64, hO.data() + r.off, (size_t)r.rows * r.cols);
+          f.write(lg + "idVector" + std::to_string(k), std::vector<unsigned long>{(unsigned long)key.first, (unsigned long)key.second});
+          cols.push_back(r.cols);
+          rows.push_back(r.rows);
+          k++;
+        }
+        f.write(lg + "colVector", cols);
+        f.write(lg + "rowVector", rows);
+      }
+    }
+  }
 }
 
 void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, double* neg, double* traces,

@@ -1,6 +1,7 @@
 // Observables, FDM spectrum and thermodynamics on the device.
 #pragma once
 #include <memory>
+#include <string>
 
 #include "chain.hpp"
 
@@ -26,6 +27,8 @@ struct Shell {
   std::vector<std::vector<std::pair<int, int>>> coupled;
   std::vector<std::vector<int>> sub_kidx;
   std::vector<int> pre_kept;  // eigenvaluesQ_kept_indices sizes at save time (previous iteration)
+  std::vector<std::vector<int>> qn, pre_qn;  // current_sysmQ, pre_sysmQ
+  int n_bath = 0;                            // bath sectors (coupled_nQ_index = i * n_bath + j)
   std::vector<double> E;      // eigenvalues (pre-shift), concatenated
   // diagonal of the (i, i) blocks of the system operators, [op][state] as E (0 where no block);
   // empty when the shell was recorded without a system operator
@@ -56,6 +59,8 @@ struct Fdm {
   DevBuf dpos, dneg, dsum, dbf, dred[2], dRA, dAR, dT1, drho, ditems, dpt, dE, dth, dQ;
   Fdm(Chain* c, Obs* o, int nb, double emin, double emax);
   void record();
+  // NrgData::saveCurrentData + saveqOperator (nrgData.hpp:134-205) of every recorded shell
+  void save(const std::string& path, const std::string& op_group) const;
   void backward(double lambda, int scale_offset, int first_cnt, double* pos, double* neg, double* traces,
                 double* sums, int cap, int* n_shells);
   // ops_out ([sysop ops][nT], may be null): the system operators' thermal expectation values

@@ -353,6 +353,72 @@ def test_superconducting_impurity_chain_equals_exact_diagonalisation(nk):
     _chain_vs_ed(nk, nk.Site.spinhalf_sc(e, u, h, d), nk.Site.free("spin_only"), H_imp, [0, 1], 2, 3, 2.0)
 
 
+def test_fdm_recorder_saves_the_reference_layout(nk, tmp_path):
+    """NrgData::saveCurrentData + saveqOperator (nrgData.hpp:134-205) from the recorder: every dataset
+    read back equals what the chain and the observable held when the shell was recorded."""
+    L, K = 2.0, 64
+    g = nk.NrgCore(nk.Site.spinhalf(-0.15, 0.3), nk.Site.spinhalf(0, 0), K)
+    op = nk.SystemOperator(g, nk.Site.fdag_chi_spinhalf())
+    rec = nk.FdmRecorder(g, op, 200, 1e-10, 10.0)
+    V = np.sqrt(2 * 0.01 / np.pi)
+    g.add_bath_site([V, V], 1.0)
+    op.update()
+    g.update_internal_state()
+    seen = {}
+    for n in range(7):
+        t = nk.hopping(n, L)
+        prev_qn, prev_kept = g.current_sysmQ, g.kept_counts()
+        g.add_bath_site([t, t], 1.0 if n == 0 else np.sqrt(L))
+        op.update()
+        if g.checkHigherEnergyDiscarded():
+            rec.record()
+            qn, dims = g.sectors()
+            seen[g.nrg_iterations_cnt] = (qn, dims, g.eigenvaluesQ, [g.current_hamiltonQ(s) for s in range(len(dims))],
+                                          prev_qn, prev_kept, op.blocks())
+        g.update_internal_state()
+    assert len(seen) >= 3
+    path = str(tmp_path / "nrgdata.h5")
+    rec.save(path)
+    for cnt, (qn, dims, eq, U, pqn, pk, ops) in seen.items():
+        gr = "/NrgItr%d/" % cnt
+        for s in range(len(dims)):
+            np.testing.assert_array_equal(nk.h5_read(path, gr + "current_sysmQ%d" % s), qn[s])
+            np.testing.assert_array_equal(nk.h5_read(path, gr + "eigenvaluesQ%d" % s), eq[s])
+            np.testing.assert_array_equal(nk.h5_read(path, gr + "current_hamiltonQ%d" % s), U[s].reshape(-1))
+        for s in range(len(pqn)):
+            np.testing.assert_array_equal(nk.h5_read(path, gr + "pre_sysmQ%d" % s), pqn[s])
+            np.testing.assert_array_equal(nk.h5_read(path, gr + "eigenvaluesQ_kept_indices%d" % s), np.arange(pk[s]))
+        allE = np.sort(np.concatenate(eq))
+        np.testing.assert_array_equal(nk.h5_read(path, "/Eigenvalues%d" % cnt), allE)
+        og = "/nparticle%d/" % (cnt - 1)
+        for p, blocks in enumerate(ops):
+            for k, (i, j, b) in enumerate(blocks):
+                np.testing.assert_array_equal(nk.h5_read(path, og + "at%d/idVector%d" % (p, k)), [i, j])
+                np.testing.assert_array_equal(nk.h5_read(path, og + "at%d/qmat%d" % (p, k)), b.reshape(-1))
+            np.testing.assert_array_equal(nk.h5_read(path, og + "at%d/rowVector" % p), [b.shape[0] for _, _, b in blocks])
+
+
+def test_all_eigenvalue_keeps_the_previous_list_until_update(nk):
+    """all_eigenvalue is set by update_internal_state (discard_higher_energies, nrgcore.hpp:269-327):
+    after the next add_bath_site it still holds that list, whether or not it was read in between."""
+    def chain():
+        g = nk.NrgCore(nk.Site.spinhalf(-0.15, 0.3), nk.Site.spinhalf(0, 0), 64)
+        g.add_bath_site([0.08, 0.08], 1.0)
+        g.update_internal_state()
+        for n in range(4):
+            g.add_bath_site([nk.hopping(n, 2.0)] * 2, 1.0 if n == 0 else np.sqrt(2.0))
+            g.update_internal_state()
+        return g
+    a, b = chain(), chain()
+    before = a.all_eigenvalue  # read after update_internal_state
+    for g in (a, b):
+        g.add_bath_site([nk.hopping(4, 2.0)] * 2, np.sqrt(2.0))
+    np.testing.assert_array_equal(a.all_eigenvalue, before)
+    np.testing.assert_array_equal(b.all_eigenvalue, before)  # not read before add_bath_site
+    b.update_internal_state()
+    assert len(b.all_eigenvalue) != len(before) or np.any(b.all_eigenvalue != before)
+
+
 def test_broadening_matches_oracle(nk, oracle_mod):
     rng = np.random.default_rng(1)
     x = np.sort(10 ** rng.uniform(-9, 1, 300))

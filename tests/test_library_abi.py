@@ -214,3 +214,9 @@ def test_model_builder_rejects_symmetry_breaking(nk):
         nk.Site.from_hamiltonian(q, f[0] + f[0].T, [f[0]])
     with pytest.raises(nk.NrgkError):  # not symmetric
         nk.Site.from_hamiltonian(q, f[0] @ f[1].T, [f[0]])
+
+
+def test_hdf5_reader_errors(nk, tmp_path):
+    """the HDF5 library loads on demand; a missing file or dataset is an error, not a crash"""
+    with pytest.raises(nk.NrgkError):
+        nk.h5_read(str(tmp_path / "absent.h5"), "/x")
