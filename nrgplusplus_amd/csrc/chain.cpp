@@ -473,6 +473,7 @@ void Chain::eigensolve_mixed() {
   // without rotations)
   const bool jac_quad = knob("NRGK_JAC_QUAD", 1) != 0;
   const double jac_far = knob("NRGK_JAC_FAR", 1e-4);
+  const bool dbg_jac = knob("NRGK_DEBUG_JAC", 0) != 0;
   // Workgroup g of a launch runs on XCD g mod 8 (round-robin dispatch).  Every block is kept on one
   // XCD for the whole solve, so that its columns stay in that XCD's L2 from round to round: blocks,
   // longest tournaments first, go to the XCD with the fewest tasks so far, and slot k of XCD x's
@@ -553,6 +554,12 @@ void Chain::eigensolve_mixed() {
     for (int bi = 0; bi < nb; bi++) {
       hflags[bi] = hflags[bi] && hf[2 * bi] && (jac_quad ? hf[2 * bi + 1] : 1);
       any |= hflags[bi] != 0;
+    }
+    if (dbg_jac) {  // NRGK_DEBUG_JAC=1: blocks still rotating (and their largest width) after each sweep
+      int act = 0, wmax = 0;
+      for (int bi = 0; bi < nb; bi++)
+        if (hflags[bi]) act++, wmax = std::max(wmax, blks[bi].n);
+      std::fprintf(stderr, "nrgk jacobi sweep %d: %d of %d blocks go on (widest %d)\n", sw, act, nb, wmax);
     }
     if (!any) {
       converged = true;
