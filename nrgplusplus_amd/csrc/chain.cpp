@@ -412,13 +412,23 @@ static int choose_bw(int n) { return n <= 32 ? std::max(1, (n + 1) / 2) : 16; }
 // K2 + K6: FP32 block Jacobi, then two Ogita-Aishima refinement steps in FP64.
 void Chain::eigensolve_mixed() {
   const int S = (int)cur_qn.size();
+  // blocks wider than the LDS-resident Jacobi panel go to rocSOLVER on the side stream
+  const int mixed_max = (int)knob("NRGK_MIXED_MAX", 768);
+  std::vector<int> big;
+  for (int s = 0; s < S; s++)
+    if (cur_n[s] > mixed_max) big.push_back(s);
+  const size_t bytes64_all = std::max<long long>(1, tot_elems) * sizeof(double);
+  dX.ensure(bytes64_all);  // every buffer the side stream touches is sized before it starts
+  dU.ensure(bytes64_all);
+  dlam_raw.ensure(std::max<long long>(1, N_tot) * sizeof(double));
+  large_blocks_begin(*this, big);
   std::vector<EigBlock> blks;
   std::vector<int> row_block, row_local;
   std::vector<int2> tasks;
   int nbc_max = 2, n_bc = 0;
   for (int s = 0; s < S; s++) {
     const int n = cur_n[s];
-    if (n == 0) continue;
+    if (n == 0 || n > mixed_max) continue;
     EigBlock b{};
     b.off = cur_off[s];
     b.n = n;
@@ -440,7 +450,16 @@ void Chain::eigensolve_mixed() {
     blks.push_back(b);
   }
   const int nb = (int)blks.size();
+  if (nb == 0) {  // every block went to the side streams: no batched launch (empty grids)
+    large_blocks_join(*this);
+    sync();
+    large_blocks_check(*this);
+    sweeps = maxcluster = unresolved = oa_steps = oa_unconverged = 0;
+    return;
+  }
   const int nrows = (int)row_block.size();
+  int nmax_m = 1;  // widest block of the batched solve (LDS sizes)
+  for (auto& b : blks) nmax_m = std::max(nmax_m, b.n);
   upload_to(d_eig, blks.data(), nb);
   const size_t o_rb = 0, o_rl = ((nrows * 4 + 255) / 256) * 256;
   d_misc.ensure(o_rl + std::max(1, nrows) * sizeof(int));
@@ -596,7 +615,7 @@ void Chain::eigensolve_mixed() {
   d_sidx.ensure(std::max<long long>(1, N_tot) * sizeof(int));
   launch_normalize_sort(dblk, nb, d_tiles_blk, (int)n_tiles_blk, d_colg, n_colg, dW.as<float>(), d_sigma.as<float>(),
                         dX.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_rrv.as<double>(),
-                        d_rrv.as<double>() + std::max<long long>(1, N_tot), normalize_sort_lds(nmax), st);
+                        d_rrv.as<double>() + std::max<long long>(1, N_tot), normalize_sort_lds(nmax_m), st);
   // NRGK_DEBUG_FINITE=1: report the first stage and block that produces a non-finite value
   const bool dbg = knob("NRGK_DEBUG_FINITE", 0) != 0;
   auto check = [&](const char* stage, const DevBuf& buf, bool is_float, bool per_block_vec) {
@@ -710,7 +729,7 @@ void Chain::eigensolve_mixed() {
     launch_oa(m_blk, na, m_sym, (int)tiles_sym.size(), m_all, (int)tiles_all.size(), dS.as<double>(), dR.as<double>(),
               dE.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_sidx.as<int>(), m_offmax,
               d_stats.as<int>(), m_conv, dP.as<double>(), dX2.as<double>(), dQ.as<double>(),
-              d_rrv.as<double>(), nmax, e_tol, d_tol, st);
+              d_rrv.as<double>(), nmax_m, e_tol, d_tol, st);
     prof_end("oa_update", 5, 0.0, oa_bytes);
     check("oa_update (E)", dE, false, false);
     check("oa_update (lam)", dlam_tmp, false, true);
@@ -739,11 +758,13 @@ void Chain::eigensolve_mixed() {
   }
   oa_unconverged = (int)act.size();
   launch_final_sort(dblk, nb, d_tiles_blk, (int)n_tiles_blk, dX.as<double>(), dlam_tmp.as<double>(),
-                    dU.as<double>(), dlam_raw.as<double>(), d_sidx.as<int>(), final_sort_lds(nmax), st);
+                    dU.as<double>(), dlam_raw.as<double>(), d_sidx.as<int>(), final_sort_lds(nmax_m), st);
   const int* hs = stg.get(d_stats.as<int>(), 4, st);
+  large_blocks_join(*this);
   sync();
   maxcluster = hs[0];
   unresolved = hs[1];
+  large_blocks_check(*this);
 }
 
 void Chain::add_bath_site(const double* t, int nt, double rescale) {

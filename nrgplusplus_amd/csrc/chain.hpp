@@ -140,6 +140,15 @@ struct Chain {
 
   // plain-mode handles (rocSOLVER / rocBLAS), created on first use
   void* rocblas = nullptr;
+  // blocks wider than the batched solver's LDS panel: rocSOLVER on side streams (plain.cpp)
+  static constexpr int N_SIDE = 4;
+  hipStream_t st_big[N_SIDE] = {};
+  void* rocblas_big[N_SIDE] = {};
+  hipEvent_t ev_big_ready = nullptr;           // H assembled (main stream)
+  hipEvent_t ev_big_done[N_SIDE] = {};         // a side stream's blocks done
+  DevBuf d_big_work[N_SIDE], d_big_info;
+  int n_big = 0;
+  unsigned big_used = 0;  // side streams given blocks in this step (bit k: st_big[k])
 
   // per-kernel-family timing (HIP events on the chain's stream around runs of consecutive launches
   // of one family) with the algorithmic flops and bytes of those launches; off unless enabled
@@ -200,5 +209,10 @@ struct Chain {
 // plain FP64 mode (plain.cpp)
 void plain_gemms(Chain& c, const std::vector<GemmProb>& probs);
 void plain_destroy(Chain& c);
+// eigenpairs of the given sectors by rocSOLVER dsyevd on the side stream, started once the main
+// stream has reached this point; large_blocks_join makes the main stream wait for them
+void large_blocks_begin(Chain& c, const std::vector<int>& sectors);
+void large_blocks_join(Chain& c);
+void large_blocks_check(Chain& c);  // after a synchronisation: the dsyevd infos
 
 }  // namespace nrgk

@@ -6,6 +6,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "chain.hpp"
 #include "kernels.hpp"
@@ -25,6 +26,19 @@ static rocblas_handle plain_handle(Chain& c) {
 void plain_destroy(Chain& c) {
   if (c.rocblas) rocblas_destroy_handle((rocblas_handle)c.rocblas);
   c.rocblas = nullptr;
+  for (int k = 0; k < Chain::N_SIDE; k++) {
+    if (c.rocblas_big[k]) rocblas_destroy_handle((rocblas_handle)c.rocblas_big[k]);
+    c.rocblas_big[k] = nullptr;
+    if (c.st_big[k]) {
+      (void)hipStreamSynchronize(c.st_big[k]);
+      (void)hipStreamDestroy(c.st_big[k]);
+    }
+    c.st_big[k] = nullptr;
+    if (c.ev_big_done[k]) (void)hipEventDestroy(c.ev_big_done[k]);
+    c.ev_big_done[k] = nullptr;
+  }
+  if (c.ev_big_ready) (void)hipEventDestroy(c.ev_big_ready);
+  c.ev_big_ready = nullptr;
 }
 
 // Row-major C = op(A) op(B) through column-major rocBLAS: C^T = op(B)^T op(A)^T.
@@ -73,6 +87,82 @@ void Chain::eigensolve_plain() {
   sweeps = 0;
   maxcluster = 0;
   unresolved = 0;
+}
+
+
+// The crossover of the batched solver: a block wider than its LDS-resident Jacobi panel (768 rows)
+// is solved by rocSOLVER dsyevd -- Householder tridiagonalisation with blocked (GEMM) trailing
+// updates, then divide and conquer, all in FP64 -- on side streams of the chain (largest blocks
+// first, each to the stream with the least n^3 so far), while the main stream runs the batched FP32
+// Jacobi and FP64 refinement of the other blocks.  The side streams write only the rows of their
+// blocks in X (scratch), U and the raw eigenvalues, and have their own work and info arrays, so no
+// two streams share memory.  All streams keep the default priority.
+void large_blocks_begin(Chain& c, const std::vector<int>& sectors) {
+  c.n_big = (int)sectors.size();
+  c.big_used = 0;
+  if (sectors.empty()) return;
+  if (!c.ev_big_ready) {
+    NRGK_HIP(hipEventCreateWithFlags(&c.ev_big_ready, hipEventDisableTiming));
+    for (int k = 0; k < Chain::N_SIDE; k++) {
+      NRGK_HIP(hipStreamCreateWithFlags(&c.st_big[k], hipStreamNonBlocking));
+      NRGK_HIP(hipEventCreateWithFlags(&c.ev_big_done[k], hipEventDisableTiming));
+      rocblas_handle h;
+      NRGK_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, NRGK_ERR_HIP, "rocblas_create_handle failed");
+      rocblas_set_stream(h, c.st_big[k]);
+      c.rocblas_big[k] = h;
+      c.d_big_work[k].ctx = c.ctx;
+    }
+    c.d_big_info.ctx = c.ctx;
+  }
+  std::vector<int> order(sectors);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return c.cur_n[x] > c.cur_n[y]; });
+  double load[Chain::N_SIDE] = {0};
+  c.d_big_info.ensure(sectors.size() * sizeof(int));
+  NRGK_HIP(hipEventRecord(c.ev_big_ready, c.st));
+  std::vector<std::vector<int>> mine(Chain::N_SIDE);
+  for (int s : order) {
+    const int k = (int)(std::min_element(load, load + Chain::N_SIDE) - load);
+    load[k] += std::pow((double)c.cur_n[s], 3);
+    mine[k].push_back(s);
+  }
+  int slot = 0;
+  for (int k = 0; k < Chain::N_SIDE; k++) {
+    if (mine[k].empty()) continue;
+    c.big_used |= 1u << k;
+    int nmax = 1;
+    for (int s : mine[k]) nmax = std::max(nmax, c.cur_n[s]);
+    c.d_big_work[k].ensure((size_t)nmax * sizeof(double));  // dsyevd's off-diagonal E
+    NRGK_HIP(hipStreamWaitEvent(c.st_big[k], c.ev_big_ready, 0));
+    rocblas_handle h = (rocblas_handle)c.rocblas_big[k];
+    for (int s : mine[k]) {
+      const int n = c.cur_n[s];
+      double* A = c.dX.as<double>() + c.cur_off[s];
+      NRGK_HIP(hipMemcpyAsync(A, c.dH.as<double>() + c.cur_off[s], (size_t)n * n * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.st_big[k]));
+      // symmetric input: the column-major view is the same matrix; the eigenvectors come back in
+      // the rows of the row-major buffer and are transposed into U's columns
+      rocblas_status r = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n,
+                                          c.dlam_raw.as<double>() + c.cur_eoff[s], c.d_big_work[k].as<double>(),
+                                          c.d_big_info.as<int>() + slot++);
+      NRGK_REQUIRE(r == rocblas_status_success, NRGK_ERR_HIP, "rocsolver_dsyevd failed");
+      launch_transpose(A, c.dU.as<double>() + c.cur_off[s], n, c.st_big[k]);
+    }
+    NRGK_HIP(hipEventRecord(c.ev_big_done[k], c.st_big[k]));
+  }
+}
+
+void large_blocks_join(Chain& c) {
+  if (!c.n_big) return;
+  for (int k = 0; k < Chain::N_SIDE; k++)
+    if (c.big_used >> k & 1u) NRGK_HIP(hipStreamWaitEvent(c.st, c.ev_big_done[k], 0));
+}
+
+void large_blocks_check(Chain& c) {
+  if (!c.n_big) return;
+  std::vector<int> info(c.n_big);
+  d2h(info.data(), c.d_big_info.as<int>(), info.size(), c.st);
+  NRGK_HIP(hipStreamSynchronize(c.st));
+  for (int v : info) NRGK_REQUIRE(v == 0, NRGK_ERR_NOCONV, "rocsolver_dsyevd: no convergence");
 }
 
 }  // namespace nrgk

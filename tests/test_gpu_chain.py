@@ -419,6 +419,53 @@ def test_all_eigenvalue_keeps_the_previous_list_until_update(nk):
     assert len(b.all_eigenvalue) != len(before) or np.any(b.all_eigenvalue != before)
 
 
+def test_large_block_crossover_matches_oracle(nk, oracle_mod, monkeypatch):
+    """Blocks wider than the batched solver's limit go to the FP64 tridiagonalising solver on a side
+    stream while the rest take the batched path: with the limit lowered to 40 rows both kinds occur
+    in every iteration; eigenpairs to 1e-10, chain eigenvalues to 1e-8 of the oracle."""
+    monkeypatch.setenv("NRGK_MIXED_MAX", "40")
+    K, L = 256, 2.5
+    V = np.sqrt(2 * 0.02 / np.pi)
+    g = nk.NrgCore(nk.Site.spinhalf(-0.25, 0.5), nk.Site.spinhalf(0, 0), K)
+    g.keep_hamiltonians(True)
+    o = oracle_mod.Chain(oracle_mod.Model("spinhalf", eps=-0.25, U=0.5), oracle_mod.Model("spinhalf"), K)
+    ts = [V] + [oracle_mod.hopping(n, L) for n in range(8)]
+    rs = [1.0] + [1.0 if n == 0 else np.sqrt(L) for n in range(8)]
+    both = 0
+    for it, (t, r) in enumerate(zip(ts, rs)):
+        g.add_bath_site([t, t], r)
+        o.add_bath_site([t, t], r)
+        _, dims = g.sectors()
+        both += int(dims.max() > 40 and (dims[dims > 0].min() <= 40))
+        _check_eigen_step(g)
+        g.update_internal_state()
+        o.update_internal_state()
+        assert np.array_equal(g.kept_counts(), o.kept_counts()), it
+        a, b = g.all_eigenvalue, o.all_eigenvalue()
+        assert len(a) == len(b) and np.abs(a - b).max() < 1e-8, (it, np.abs(a - b).max())
+    assert both >= 4
+
+
+def test_all_blocks_on_side_streams_match_oracle(nk, oracle_mod, monkeypatch):
+    """With the limit at 0 every block is solved on the side streams and the batched solver has no
+    block at all."""
+    monkeypatch.setenv("NRGK_MIXED_MAX", "0")
+    K, L = 128, 2.5
+    V = np.sqrt(2 * 0.02 / np.pi)
+    g = nk.NrgCore(nk.Site.spinhalf(-0.25, 0.5), nk.Site.spinhalf(0, 0), K)
+    o = oracle_mod.Chain(oracle_mod.Model("spinhalf", eps=-0.25, U=0.5), oracle_mod.Model("spinhalf"), K)
+    ts = [V] + [oracle_mod.hopping(n, L) for n in range(5)]
+    rs = [1.0] + [1.0 if n == 0 else np.sqrt(L) for n in range(5)]
+    for it, (t, r) in enumerate(zip(ts, rs)):
+        g.add_bath_site([t, t], r)
+        o.add_bath_site([t, t], r)
+        g.update_internal_state()
+        o.update_internal_state()
+        assert np.array_equal(g.kept_counts(), o.kept_counts()), it
+        a, b = g.all_eigenvalue, o.all_eigenvalue()
+        assert len(a) == len(b) and np.abs(a - b).max() < 1e-8, (it, np.abs(a - b).max())
+
+
 def test_broadening_matches_oracle(nk, oracle_mod):
     rng = np.random.default_rng(1)
     x = np.sort(10 ** rng.uniform(-9, 1, 300))
