@@ -16,6 +16,8 @@ Beside the timed value the line carries:
                 one call per block), a bounded sample; speedup = value / plain
   eig_err       max |lambda_gpu - lambda_LAPACK| over every block of one checked iteration
   spectra       the FDM backward pass over all recorded shells (spectra/s)
+  z_average     16 z-shifted chains of the z-averaged thermodynamics workload, one after another and
+                stepped together (chain iterations/s)
   roofline      the dominant kernel family, from the library's per-launch HIP-event timers
   cpu_baseline  the FP64 oracle (faithful reference form, host LAPACK/BLAS) on host cores
 """
@@ -145,6 +147,61 @@ def roofline(kstats):
             "per_launch": {"flops": k["flops"] / k["launches"], "us": 1e3 * k["ms"] / k["launches"]}}
 
 
+ZCFG = dict(eps=-0.25, U=0.5, Gamma=0.015, Lambda=4.0, K=1024, nsites=45, nz=16, nT=200)
+
+
+def z_average(nk):
+    """The z-averaged thermodynamics workload: SIAM U=0.5, eps=-0.25, Gamma=0.015, Lambda=4, K=1024,
+    45 sites, one chain per twist z = k/16 (z = 0 taken as z = 1, the same mesh) with the hoppings of
+    the z-shifted discretisation, entropy and specific heat on 200 log-spaced T in [1e-9, 1], averaged
+    over z.  Timed twice: the 16 chains one after another, and stepped together (nrgk_chains_step:
+    one host worker and stream per chain).  Both must give the same entropy to the last bit."""
+    c = ZCFG
+    L, ns, nz = c["Lambda"], c["nsites"], c["nz"]
+    A = 0.5 * np.log(L) * (1 + 1 / L) / (1 - 1 / L)
+    zs = [(k if k else nz) / nz for k in range(nz)]
+    hops = [nk.z_chain(z, L, ns)[0] * A for z in zs]
+    V = np.sqrt(2 * c["Gamma"] / np.pi)
+    T = np.logspace(-9, 0, c["nT"])
+
+    def new_chain():
+        g = nk.NrgCore(nk.Site.spinhalf(c["eps"], c["U"]), nk.Site.spinhalf(0, 0), c["K"])
+        return g, nk.FdmRecorder(g, None)
+
+    def alone(k):
+        g, rec = new_chain()
+        g.add_bath_site([V, V], 1.0)
+        g.update_internal_state()
+        m = 0
+        for n in range(ns):
+            g.add_bath_site([hops[k][n]] * 2, 1.0 if n == 0 else np.sqrt(L))
+            if g.checkHigherEnergyDiscarded():
+                rec.record()
+            else:
+                m += 1
+            g.update_internal_state()
+        return rec.thermodynamics(L, T, scale_offset=-1, first_cnt=m + 1)[0]
+
+    alone(0)  # warm-up (first allocations, kernel loads)
+    t0 = time.perf_counter()
+    S_seq = np.mean([alone(k) for k in range(nz)], axis=0)
+    t1 = time.perf_counter()
+    gs, recs = zip(*[new_chain() for _ in range(nz)])
+    nk.chains_step(gs, [[V, V]] * nz, 1.0)
+    m = np.zeros(nz, int)
+    for n in range(ns):
+        m += ~nk.chains_step(gs, [[h[n]] * 2 for h in hops], 1.0 if n == 0 else np.sqrt(L), recorders=recs)
+    S_tog = np.mean([recs[k].thermodynamics(L, T, scale_offset=-1, first_cnt=int(m[k]) + 1)[0]
+                     for k in range(nz)], axis=0)
+    t2 = time.perf_counter()
+    its = nz * (ns + 1)
+    return {"workload": "z-averaged SIAM thermodynamics: U=0.5 eps=-0.25 Gamma=0.015 Lambda=4 K=1024, 45 sites, "
+                        "16 twists z=k/16, 200 temperatures",
+            "chains_together_s": t2 - t1, "chains_one_after_another_s": t1 - t0,
+            "chain_iterations_per_s": its / (t2 - t1), "speedup_together": (t1 - t0) / (t2 - t1),
+            "identical": bool(np.array_equal(S_seq, S_tog))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +210,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=0, help="0: the timed iterations")
+    ap.add_argument("--no-zavg", action="store_true", help="skip the z-averaged multi-chain sample")
     ap.add_argument("--stop-after-timed", action="store_true",
                     help="end after the timed iterations (for counter runs): no eigenvalue check, FDM pass, "
                          "plain or CPU sample")
@@ -214,6 +272,8 @@ def main():
         plain = {"value": n / (q1 - q0), "unit": "iterations/s", "steps": n,
                  "form": "rocSOLVER dsyevd + rocBLAS dgemm, one call per block, FP64"}
 
+    zavg = None if args.no_zavg else z_average(nk)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(W - 1 if W > 0 else 0)
@@ -238,6 +298,7 @@ def main():
         "plain_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "spectra": spectra,
+        "z_average": zavg,
         "step_ms_p50": float(np.percentile(step_ms, 50)),
         "step_ms_p99": float(np.percentile(step_ms, 99)),
         "kernels": {k: dict(ms_per_step=v["ms"] / K, launches_per_step=v["launches"] / K,

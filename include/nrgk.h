@@ -140,6 +140,21 @@ int nrgk_chain_add_bath_site(nrgk_chain* chain, const double* t_hopping, int n_h
 int nrgk_chain_update_internal_state(nrgk_chain* chain);
 /* nrgk_chain_add_bath_site + nrgk_chain_update_internal_state in one call */
 int nrgk_chain_step(nrgk_chain* chain, const double* t_hopping, int n_hopping, double rescale);
+/* Several independent chains (z-shifted discretisations, parameter points, requests) advanced by one
+ * Wilson site each, concurrently: one host worker and the chain's own stream per chain, so that the
+ * chains' launches overlap on the GPU.  No counterpart in the reference, whose drivers step one
+ * chain; chain i runs that driver loop (examples/fdmSpectrumSiam/main.cpp:86-95):
+ *   add_bath_site(t + i*n_hopping, n_hopping, rescale[i]);
+ *   update_system_operator on obs[i*n_obs .. i*n_obs + n_obs - 1];
+ *   if checkHigherEnergyDiscarded() and recorders (may be NULL) has recorders[i] != NULL:
+ *     record the shell into recorders[i] (rawData.saveCurrentData), recorded[i] = 1, else 0;
+ *   update_internal_state().
+ * Chains must be distinct, each observable and recorder must belong to its chain.  n_threads <= 0:
+ * min(n_chains, 16) workers.  recorded may be NULL.  Every chain completes its step; the first
+ * failing chain's status is returned, with its index in nrgk_last_error(). */
+int nrgk_chains_step(nrgk_chain* const* chains, int n_chains, const double* t_hopping, int n_hopping,
+                     const double* rescale, nrgk_obs* const* obs, int n_obs, nrgk_fdm* const* recorders,
+                     int n_threads, int* recorded);
 /* nrg_iterations_cnt / nrg_iterations_min (nrgcore.hpp:536-538) */
 int nrgk_chain_counters(nrgk_chain* chain, int* cnt, int* cnt_min);
 /* overwrite nrg_iterations_cnt (drivers decrement it, examples/fdmSpectrumSiam/main.cpp:163) */

@@ -264,6 +264,8 @@ class nrgcore {
     refresh_counters();
   }
   nrgk_chain* handle() const { return h_.get(); }
+  // re-reads the host mirrors after the chain was stepped through the C ABI (step_chains)
+  void refresh_host_mirrors() { refresh(); }
 
   // public data of the reference (nrgcore.hpp:520-542), host mirrors
   std::vector<double> all_eigenvalue;
@@ -406,6 +408,39 @@ inline std::pair<std::vector<double>, std::vector<double>> z_chain(double z, dou
   check(nrgk_zdisc_chain(z, lambda, D, x.data(), rho.data(), (int)x.size(), quad, n_intervals, normalize ? 1 : 0,
                          n_sites, t.data(), e.data()));
   return {t, e};
+}
+
+// One Wilson site on each of several independent chains, run concurrently (nrgk_chains_step; the
+// reference steps one chain at a time).  Chain i does what the reference's FDM drivers do per site
+// (examples/fdmSpectrumSiam/main.cpp:86-95): add_bath_site(t[i], rescale[i]); update_system_operator
+// on ops[i] when ops is given (one per chain); record into recorders[i] (the handle() of an
+// fdmSpectrum or fdmThermodynamics of that chain, or null) when higher energies were discarded;
+// update_internal_state.  Returns which chains recorded; the chains' host mirrors are refreshed.
+template <class chain_type, class op_type = system_operator>
+inline std::vector<bool> step_chains(const std::vector<chain_type*>& chains, const std::vector<std::vector<double>>& t,
+                                     const std::vector<double>& rescale, const std::vector<op_type*>& ops = {},
+                                     const std::vector<nrgk_fdm*>& recorders = {}, int n_threads = 0) {
+  const size_t n = chains.size();
+  if (t.size() != n || rescale.size() != n || (!ops.empty() && ops.size() != n) ||
+      (!recorders.empty() && recorders.size() != n))
+    throw std::runtime_error("step_chains: one entry per chain in t, rescale, ops and recorders");
+  const size_t nt = n ? t[0].size() : 0;
+  std::vector<double> tf;
+  for (const auto& v : t) {
+    if (v.size() != nt) throw std::runtime_error("step_chains: all chains take the same number of hoppings");
+    tf.insert(tf.end(), v.begin(), v.end());
+  }
+  std::vector<nrgk_chain*> hc(n);
+  std::vector<nrgk_obs*> ho;
+  for (size_t i = 0; i < n; i++) {
+    hc[i] = chains[i]->handle();
+    if (!ops.empty()) ho.push_back(ops[i]->handle());
+  }
+  std::vector<int> rec(n, 0);
+  check(nrgk_chains_step(hc.data(), (int)n, tf.data(), (int)nt, rescale.data(), ho.data(), ops.empty() ? 0 : 1,
+                         recorders.empty() ? nullptr : recorders.data(), n_threads, rec.data()));
+  for (auto* c : chains) c->refresh_host_mirrors();
+  return std::vector<bool>(rec.begin(), rec.end());
 }
 
 // fdmThermodynamics (static/fdmThermodynamics.hpp:17-260): shells recorded during the forward pass,

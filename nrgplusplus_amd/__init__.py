@@ -94,6 +94,7 @@ def lib():
             "nrgk_chain_add_bath_site": (i, [vp, dp, i, d]),
             "nrgk_chain_update_internal_state": (i, [vp]),
             "nrgk_chain_step": (i, [vp, dp, i, d]),
+            "nrgk_chains_step": (i, [pp, i, dp, i, dp, pp, i, pp, i, ip]),
             "nrgk_chain_counters": (i, [vp, ip, ip]),
             "nrgk_chain_set_counter": (i, [vp, i]),
             "nrgk_chain_higher_energy_discarded": (i, [vp, ip]),
@@ -659,6 +660,33 @@ class FdmRecorder:
         if getattr(self, "_h", None) is not None and _LIB is not None:
             _LIB.nrgk_fdm_destroy(self._h)
             self._h = None
+
+
+def chains_step(chains, t, rescale, observables=None, recorders=None, n_threads=0):
+    """One Wilson site on each of several independent chains, run concurrently (a host worker and
+    the chain's own stream per chain).  Chain i does what the reference's FDM drivers do per site
+    (examples/fdmSpectrumSiam/main.cpp:86-95): add_bath_site(t[i], rescale[i]), update of its
+    observables (observables[i], a list), record into recorders[i] when higher energies were
+    discarded, update_internal_state.  Returns a bool array: which chains recorded a shell."""
+    n = len(chains)
+    if n == 0:
+        return np.zeros(0, bool)
+    t = np.ascontiguousarray(np.asarray(t, np.float64).reshape(n, -1))
+    r = np.ascontiguousarray(np.broadcast_to(np.asarray(rescale, np.float64), (n,)))
+    obs = observables if observables is not None else [[] for _ in range(n)]
+    n_obs = len(obs[0])
+    if len(obs) != n or any(len(o) != n_obs for o in obs):
+        raise ValueError("chains_step: one list of equally many observables per chain")
+    ch = (C.c_void_p * n)(*[c._h.value for c in chains])
+    ob = (C.c_void_p * max(1, n * n_obs))(*[o._h.value for row in obs for o in row])
+    rec = None
+    if recorders is not None:
+        if len(recorders) != n:
+            raise ValueError("chains_step: one recorder (or None) per chain")
+        rec = (C.c_void_p * n)(*[(x._h.value if x is not None else None) for x in recorders])
+    out = np.zeros(n, np.int32)
+    _check(lib().nrgk_chains_step(ch, n, _dp(t), t.shape[1], _dp(r), ob, n_obs, rec, int(n_threads), _ip(out)))
+    return out.astype(bool)
 
 
 def broaden(x, w, omega, b, ctx=None):

@@ -1,9 +1,11 @@
 // extern "C" entry points of libnrgk (declared in include/nrgk.h).  Every call converts exceptions
 // into a status code and keeps the message for nrgk_last_error().
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "chain.hpp"
 #include "h5io.hpp"
@@ -495,6 +497,68 @@ int nrgk_chain_step_stats(nrgk_chain* ch, double* ms4, int* sweeps, int* max_clu
       for (int i = 0; i < 4; i++) ms4[i] = c.ms[i];
     if (sweeps) *sweeps = c.sweeps;
     if (max_cluster) *max_cluster = c.unresolved ? -c.maxcluster : c.maxcluster;
+  });
+}
+
+// Several independent chains advanced by one Wilson site each, concurrently.  Each chain runs on its
+// own stream; one host worker per chain at a time drives it, so that the chains' latency-bound launch
+// sequences (and the host waits between them) overlap on the GPU's hardware queues.  The per-chain
+// order is the loop of the reference's FDM drivers (examples/fdmSpectrumSiam/main.cpp:86-95).
+int nrgk_chains_step(nrgk_chain* const* chains, int n_chains, const double* t, int nt, const double* rescale,
+                     nrgk_obs* const* obs, int n_obs, nrgk_fdm* const* recorders, int n_threads, int* recorded) {
+  return guard([&] {
+    NRGK_REQUIRE(n_chains >= 0 && nt >= 0 && n_obs >= 0, NRGK_ERR_INVALID, "nrgk_chains_step: bad sizes");
+    if (n_chains == 0) return;
+    NRGK_REQUIRE(chains && rescale && (t || nt == 0) && (obs || n_obs == 0), NRGK_ERR_INVALID,
+                 "nrgk_chains_step: null argument");
+    std::vector<const Chain*> seen;
+    for (int i = 0; i < n_chains; i++) {
+      NRGK_REQUIRE(chains[i] && chains[i]->c, NRGK_ERR_INVALID, "nrgk_chains_step: null chain");
+      seen.push_back(chains[i]->c.get());
+      for (int k = 0; k < n_obs; k++) {
+        nrgk_obs* o = obs[(size_t)i * n_obs + k];
+        NRGK_REQUIRE(o && o->o->ch == chains[i]->c.get(), NRGK_ERR_INVALID,
+                     "nrgk_chains_step: an observable does not belong to its chain");
+      }
+      if (recorders && recorders[i])
+        NRGK_REQUIRE(recorders[i]->f->ch == chains[i]->c.get(), NRGK_ERR_INVALID,
+                     "nrgk_chains_step: a recorder does not belong to its chain");
+    }
+    std::sort(seen.begin(), seen.end());
+    NRGK_REQUIRE(std::adjacent_find(seen.begin(), seen.end()) == seen.end(), NRGK_ERR_INVALID,
+                 "nrgk_chains_step: a chain appears twice");
+    std::vector<int> status(n_chains, NRGK_OK), rec(n_chains, 0);
+    std::vector<std::string> msg(n_chains);
+    std::atomic<int> next{0};
+    auto work = [&] {
+      for (int i; (i = next.fetch_add(1)) < n_chains;) {
+        Chain& c = *chains[i]->c;
+        try {
+          NRGK_HIP(hipSetDevice(c.ctx->device));
+          c.add_bath_site(t + (size_t)i * nt, nt, rescale[i]);
+          for (int k = 0; k < n_obs; k++) obs[(size_t)i * n_obs + k]->o->update();
+          if (recorders && recorders[i] && c.no_kept >= c.K) {  // checkHigherEnergyDiscarded
+            recorders[i]->f->record();
+            rec[i] = 1;
+          }
+          c.update_internal_state();
+        } catch (const Error& e) {
+          status[i] = e.status;
+          msg[i] = e.what();
+        } catch (const std::exception& e) {
+          status[i] = NRGK_ERR_INVALID;
+          msg[i] = e.what();
+        }
+      }
+    };
+    const int nw = std::max(1, std::min(n_chains, n_threads > 0 ? n_threads : 16));
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nw; w++) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    if (recorded) std::copy(rec.begin(), rec.end(), recorded);
+    for (int i = 0; i < n_chains; i++)
+      if (status[i] != NRGK_OK) throw Error(status[i], "chain " + std::to_string(i) + ": " + msg[i]);
   });
 }
 

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -76,16 +77,22 @@ struct DevBuf {
 struct Ctx {
   int device = 0;
   size_t capacity = 0;
-  size_t used = 0;
+  std::atomic<size_t> used{0};  // chains of one context may be stepped from several host threads
   int lds_max = 65536;  // bytes of LDS one workgroup may allocate
   int n_cu = 0;
   void charge(size_t bytes) {
-    if (capacity && used + bytes > capacity)
-      throw Error(NRGK_ERR_NOMEM, "nrgk: device capacity exceeded (" + std::to_string(used + bytes) +
-                                      " > " + std::to_string(capacity) + " bytes)");
-    used += bytes;
+    size_t u = used.load();
+    do {
+      if (capacity && u + bytes > capacity)
+        throw Error(NRGK_ERR_NOMEM, "nrgk: device capacity exceeded (" + std::to_string(u + bytes) + " > " +
+                                        std::to_string(capacity) + " bytes)");
+    } while (!used.compare_exchange_weak(u, u + bytes));
   }
-  void refund(size_t bytes) { used -= bytes < used ? bytes : used; }
+  void refund(size_t bytes) {
+    size_t u = used.load();
+    while (!used.compare_exchange_weak(u, bytes < u ? u - bytes : 0)) {
+    }
+  }
 };
 
 // Page-locked staging for one stream's small transfers.  put() copies host data into pinned memory

@@ -508,18 +508,20 @@ def test_error_paths(nk):
 
 
 @pytest.mark.gpu
-def test_cpp_nrgcore_api_rgflow_matches_oracle(oracle_mod):
+@pytest.mark.parametrize("n_copies", [1, 3])
+def test_cpp_nrgcore_api_rgflow_matches_oracle(oracle_mod, n_copies):
     """The C++ driver written against include/nrgk/nrgcore.hpp (examples/rgflowSIAM form) runs the
     chain on the GPU; every iteration's all_eigenvalue equals the oracle's (same state counts, values
-    to 1e-9)."""
+    to 1e-9).  With 3 copies the chains are stepped together (nrgk::step_chains) and the driver
+    checks that the copies agree bit for bit."""
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "nrgplusplus_amd", "nrgk_rgflow_siam")
     if not os.path.exists(exe):
         pytest.fail("nrgk_rgflow_siam is not built (run __graft_entry__.build())")
     eps, U, G, L, K, ns = -0.25, 0.5, 0.02, 2.5, 512, 20
-    r = subprocess.run([exe, str(eps), str(U), str(G), str(L), str(K), str(ns), "100000"], capture_output=True,
-                       text=True, timeout=100)
+    r = subprocess.run([exe, str(eps), str(U), str(G), str(L), str(K), str(ns), "100000", str(n_copies)],
+                       capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stderr
     lines = [list(map(float, ln.split())) for ln in r.stdout.strip().splitlines()]
     assert len(lines) == ns + 1

@@ -220,3 +220,18 @@ def test_hdf5_reader_errors(nk, tmp_path):
     """the HDF5 library loads on demand; a missing file or dataset is an error, not a crash"""
     with pytest.raises(nk.NrgkError):
         nk.h5_read(str(tmp_path / "absent.h5"), "/x")
+
+
+def test_chains_step_argument_checks(nk):
+    """nrgk_chains_step rejects bad arguments before any chain is touched (no device needed): null
+    chain list or chain, negative sizes; an empty list is a no-op."""
+    import ctypes as C
+    lib = nk.lib()
+    one = np.ones(2)
+    r = np.ones(1)
+    assert lib.nrgk_chains_step(None, 0, None, 0, None, None, 0, None, 0, None) == 0
+    assert lib.nrgk_chains_step(None, 1, nk._dp(one), 2, nk._dp(r), None, 0, None, 0, None) != 0
+    nulls = (C.c_void_p * 1)(None)
+    assert lib.nrgk_chains_step(nulls, 1, nk._dp(one), 2, nk._dp(r), None, 0, None, 0, None) != 0
+    assert b"null chain" in lib.nrgk_last_error()
+    assert lib.nrgk_chains_step(nulls, -1, nk._dp(one), 2, nk._dp(r), None, 0, None, 0, None) != 0
