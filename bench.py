@@ -18,6 +18,8 @@ Beside the timed value the line carries:
   spectra       the FDM backward pass over all recorded shells (spectra/s)
   z_average     16 z-shifted chains of the z-averaged thermodynamics workload, one after another and
                 stepped together (chain iterations/s)
+  serving       a Poisson stream of SIAM requests (64 of the 512-request workload), one chain per
+                request on 8 host workers: p50/p99 request latency and solves/s
   roofline      the dominant kernel family, from the library's per-launch HIP-event timers
   cpu_baseline  the FP64 oracle (faithful reference form, host LAPACK/BLAS) on host cores
 """
@@ -202,6 +204,73 @@ def z_average(nk):
             "identical": bool(np.array_equal(S_seq, S_tog))}
 
 
+SCFG = dict(requests=64, rate=5.0, workers=8, seed=20261202, nbins=2000, n_omega=1000, b=0.6)
+
+
+def serving(nk):
+    """A bounded sample of the request workload: SIAM requests with U ~ Unif[0.1, 1], eps = -U/2 + d,
+    d ~ Unif[-0.1, 0.1], Gamma log-uniform in [0.005, 0.05], Lambda in {2, 2.5, 3}, K in {512, 1024,
+    2048}, 40-60 Wilson sites, each returning A(omega) (f-dagger chi, T = 0 FDM, 2000 bins,
+    log-Gaussian b = 0.6) on 1000 positive and 1000 negative frequencies.  64 requests (of the
+    workload's 512) arrive as a Poisson stream at 5 requests/s (seeded numpy generator); 8 host
+    workers each run one request at a time on its own chain and stream.  Latency = completion time -
+    arrival time."""
+    from concurrent.futures import ThreadPoolExecutor
+    c = SCFG
+    rng = np.random.default_rng(c["seed"])
+    n = c["requests"]
+    arrive = np.cumsum(rng.exponential(1.0 / c["rate"], n))
+    reqs = []
+    for _ in range(n):
+        U = rng.uniform(0.1, 1.0)
+        reqs.append(dict(U=U, eps=-U / 2 + rng.uniform(-0.1, 0.1), G=float(np.exp(rng.uniform(np.log(0.005), np.log(0.05)))),
+                         L=float(rng.choice([2.0, 2.5, 3.0])), K=int(rng.choice([512, 1024, 2048])),
+                         nit=int(rng.integers(40, 61))))
+    omega = np.logspace(-10, 1, c["n_omega"])
+
+    def solve(r):
+        g = nk.NrgCore(nk.Site.spinhalf(r["eps"], r["U"]), nk.Site.spinhalf(0, 0), r["K"])
+        op = nk.SystemOperator(g, nk.Site.fdag_chi_spinhalf())
+        rec = nk.FdmRecorder(g, op, c["nbins"], 1e-10, 10.0)
+        L, V = r["L"], np.sqrt(2 * r["G"] / np.pi)
+        g.add_bath_site([V, V], 1.0)
+        op.update()
+        g.update_internal_state()
+        m = 0
+        for k in range(r["nit"]):
+            t = nk.hopping(k, L)
+            g.add_bath_site([t, t], 1.0 if k == 0 else np.sqrt(L))
+            op.update()
+            if g.checkHigherEnergyDiscarded():
+                rec.record()
+            else:
+                m += 1
+            g.update_internal_state()
+        pos, neg, _, _ = rec.backward(L, scale_offset=-1, first_cnt=m + 1)
+        x = rec.energy_points()
+        A = np.concatenate([nk.broaden(x, neg.sum(0), omega, c["b"])[::-1], nk.broaden(x, pos.sum(0), omega, c["b"])])
+        return time.perf_counter(), float(pos.sum() + neg.sum()), bool(np.all(np.isfinite(A)))
+
+    solve(dict(U=0.3, eps=-0.15, G=0.01, L=2.0, K=512, nit=12))  # warm-up
+    t0 = time.perf_counter()
+    futs = []
+    with ThreadPoolExecutor(c["workers"]) as ex:
+        for k in range(n):
+            wait = t0 + arrive[k] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            futs.append(ex.submit(solve, reqs[k]))
+        out = [f.result() for f in futs]
+    done = np.array([o[0] - t0 for o in out])
+    lat = (done - arrive) * 1e3
+    return {"workload": "SIAM requests (U, eps, Gamma, Lambda, K, 40-60 sites drawn per request), each returning "
+                        "A(omega) on 2000 frequencies; Poisson arrivals",
+            "sample": "64 of 512 requests at 5 requests/s, seed %d, 8 workers" % c["seed"],
+            "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+            "solves_per_s": n / float(done.max() - arrive[0]),
+            "total_weight_range": [min(o[1] for o in out), max(o[1] for o in out)], "all_finite": all(o[2] for o in out)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +280,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=0, help="0: the timed iterations")
     ap.add_argument("--no-zavg", action="store_true", help="skip the z-averaged multi-chain sample")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-serving sample")
     ap.add_argument("--stop-after-timed", action="store_true",
                     help="end after the timed iterations (for counter runs): no eigenvalue check, FDM pass, "
                          "plain or CPU sample")
@@ -273,6 +343,7 @@ def main():
                  "form": "rocSOLVER dsyevd + rocBLAS dgemm, one call per block, FP64"}
 
     zavg = None if args.no_zavg else z_average(nk)
+    serve = None if args.no_serving else serving(nk)
 
     cpu = None
     if not args.no_cpu:
@@ -299,6 +370,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "spectra": spectra,
         "z_average": zavg,
+        "serving": serve,
         "step_ms_p50": float(np.percentile(step_ms, 50)),
         "step_ms_p99": float(np.percentile(step_ms, 99)),
         "kernels": {k: dict(ms_per_step=v["ms"] / K, launches_per_step=v["launches"] / K,
