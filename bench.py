@@ -271,6 +271,29 @@ def serving(nk):
             "total_weight_range": [min(o[1] for o in out), max(o[1] for o in out)], "all_finite": all(o[2] for o in out)}
 
 
+def two_channel(nk, n_steps):
+    """The two-channel SIAM chain (impurity spinhalf eps=-0.2, U=0.4 with its f-dagger duplicated to 4
+    operators, two-channel bath, Gamma1 = Gamma2 = 0.01, Lambda = 3, K = 4096): the first n_steps
+    iterations in the mixed mode (blocks above 768 rows on the rocSOLVER crossover) and in the plain
+    FP64 mode; per iteration: states, widest block, ms."""
+    L, K, U, eps, G = 3.0, 4096, 0.4, -0.2, 0.01
+    V = np.sqrt(2 * G / np.pi)
+    out = {}
+    for name, mode in (("mixed", nk.NRGK_MODE_MIXED), ("plain_fp64", nk.NRGK_MODE_PLAIN64)):
+        g = nk.NrgCore(nk.Site.spinhalf(eps, U).duplicate_ops(), nk.Site.two_channel(), K, mode=mode)
+        rows = []
+        for n in range(n_steps):
+            t = V if n == 0 else nk.hopping(n - 1, L)
+            t0 = time.perf_counter()
+            g.add_bath_site([t] * 4, 1.0 if n <= 1 else np.sqrt(L))
+            g.update_internal_state()
+            ms = (time.perf_counter() - t0) * 1e3
+            _, dims = g.sectors()
+            rows.append({"iteration": n, "states": int(dims.sum()), "widest_block": int(dims.max()), "ms": ms})
+        out[name] = rows
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +304,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=0, help="0: the timed iterations")
     ap.add_argument("--no-zavg", action="store_true", help="skip the z-averaged multi-chain sample")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-serving sample")
+    ap.add_argument("--two-channel-steps", type=int, default=0,
+                    help="also time this many iterations of the two-channel K=4096 chain (mixed and plain)")
     ap.add_argument("--stop-after-timed", action="store_true",
                     help="end after the timed iterations (for counter runs): no eigenvalue check, FDM pass, "
                          "plain or CPU sample")
@@ -344,6 +369,7 @@ def main():
 
     zavg = None if args.no_zavg else z_average(nk)
     serve = None if args.no_serving else serving(nk)
+    tc = two_channel(nk, args.two_channel_steps) if args.two_channel_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -371,6 +397,7 @@ def main():
         "spectra": spectra,
         "z_average": zavg,
         "serving": serve,
+        "two_channel": tc,
         "step_ms_p50": float(np.percentile(step_ms, 50)),
         "step_ms_p99": float(np.percentile(step_ms, 99)),
         "kernels": {k: dict(ms_per_step=v["ms"] / K, launches_per_step=v["launches"] / K,
