@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <system_error>
 #include <thread>
 
 #include "chain.hpp"
@@ -553,7 +554,11 @@ int nrgk_chains_step(nrgk_chain* const* chains, int n_chains, const double* t, i
     };
     const int nw = std::max(1, std::min(n_chains, n_threads > 0 ? n_threads : 16));
     std::vector<std::thread> pool;
-    for (int w = 1; w < nw; w++) pool.emplace_back(work);
+    try {
+      for (int w = 1; w < nw; w++) pool.emplace_back(work);
+    } catch (const std::system_error&) {
+      // no more threads: the workers that started and this thread take the remaining chains
+    }
     work();
     for (auto& th : pool) th.join();
     if (recorded) std::copy(rec.begin(), rec.end(), recorded);
