@@ -98,13 +98,18 @@ def cpu_baseline(n_skip, budget_s=20.0):
                         first_V=np.sqrt(2 * CFG["Gamma"] / np.pi), nsites=CFG["nsites"], K=CFG["K"])
     # one probe iteration sizes the sample to ~budget_s of CPU work
     t1 = oracle.time_forward(drv, n_skip, 1)
-    n = int(max(2, min(CFG["nsites"] - n_skip, budget_s / max(t1, 1e-3))))
-    secs = oracle.time_forward(drv, n_skip, n)
+    n1 = 2  # then two sites of the same chain at one thread (the reference's serial loop on one core)
+    n = int(max(2, min(CFG["nsites"] - n_skip - n1, budget_s / max(t1, 1e-3))))
+    secs, secs1 = oracle.time_forward_with_single_core(drv, n_skip, n, n1)
+    oracle.set_threads(cores)
+    k0 = n_skip + n
     return {"value": n / secs, "unit": "iterations/s", "cores": cores, "kind": "port",
             "sample": "FP64 oracle in the reference's form (per-block LAPACKE_dsyevd + cblas_dgemm via %s, "
                       "update_system_operator, shell save), Wilson sites %d..%d of the same chain"
                       % (oracle.lib().orc_lapack_library().decode(), n_skip, n_skip + n - 1),
-            "seconds": secs}
+            "seconds": secs,
+            "single_core": {"value": n1 / secs1, "unit": "iterations/s", "cores": 1,
+                            "sample": "Wilson sites %d..%d, one thread" % (k0, k0 + n1 - 1), "seconds": secs1}}
 
 
 PEAK_F64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)

@@ -92,6 +92,7 @@ def lib():
             "orc_run_fdm": (i, [C.POINTER(OrcDriver), i, dp, dp, dp, dp, ip]),
             "orc_broaden": (None, [dp, dp, i, dp, i, d, dp]),
             "orc_time_forward": (i, [C.POINTER(OrcDriver), i, i, dp]),
+            "orc_time_forward2": (i, [C.POINTER(OrcDriver), i, i, i, dp, dp]),
             "orc_hopping": (d, [i, d]),
         }
         for name, (res, args) in sig.items():
@@ -340,6 +341,14 @@ def time_forward(drv, n_skip, n_time):
     s = C.c_double(0)
     _check(lib().orc_time_forward(C.byref(drv), n_skip, n_time, C.byref(s)))
     return s.value
+
+
+def time_forward_with_single_core(drv, n_skip, n_time, n_time1):
+    """time_forward of sites [n_skip, n_skip + n_time), then the next n_time1 sites of the same chain
+    at one thread: (seconds, seconds_one_thread).  The caller restores its thread count."""
+    s, s1 = C.c_double(0), C.c_double(0)
+    _check(lib().orc_time_forward2(C.byref(drv), n_skip, n_time, n_time1, C.byref(s), C.byref(s1)))
+    return s.value, s1.value
 
 
 def set_threads(n):

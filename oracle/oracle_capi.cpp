@@ -396,7 +396,13 @@ int orc_run_fdm(const OrcDriver* d, int nbins, double* pos, double* neg, double*
 // CPU baseline: the FDM-driver forward pass in the reference's form (add_bath_site,
 // update_system_operator of f^dagger chi, shell save, update_internal_state), timed over loop sites
 // [n_skip, n_skip + n_time) with std::chrono like the reference's timer (nrgcore.hpp:66,90-93).
+// Seconds spent in loop sites [n_skip, n_skip + n_time) at the current thread count and, when
+// n_time1 > 0, in the following n_time1 sites at one thread (seconds1).
+int orc_time_forward2(const OrcDriver* d, int n_skip, int n_time, int n_time1, double* seconds, double* seconds1);
 int orc_time_forward(const OrcDriver* d, int n_skip, int n_time, double* seconds) {
+  return orc_time_forward2(d, n_skip, n_time, 0, seconds, nullptr);
+}
+int orc_time_forward2(const OrcDriver* d, int n_skip, int n_time, int n_time1, double* seconds, double* seconds1) {
   return guard([&] {
     SiteModel imp = driver_imp(*d), bath = driver_bath(*d);
     Chain c(imp, bath);
@@ -407,8 +413,9 @@ int orc_time_forward(const OrcDriver* d, int n_skip, int n_time, double* seconds
     c.add_bath_site(std::vector<double>(nop, d->first_V), 1.0);
     update_system_operator(c, ops);
     c.update_internal_state();
-    double tot = 0;
-    for (int in = 0; in < n_skip + n_time; in++) {
+    double tot = 0, tot1 = 0;
+    for (int in = 0; in < n_skip + n_time + n_time1; in++) {
+      if (in == n_skip + n_time) orc_set_threads(1);
       auto t0 = std::chrono::steady_clock::now();
       double rescale = in > 0 ? std::sqrt(d->Lambda) : 1.0;
       c.add_bath_site(std::vector<double>(nop, hopping(in, d->Lambda)), rescale);
@@ -419,9 +426,12 @@ int orc_time_forward(const OrcDriver* d, int n_skip, int n_time, double* seconds
       }
       c.update_internal_state();
       auto t1 = std::chrono::steady_clock::now();
-      if (in >= n_skip) tot += std::chrono::duration<double>(t1 - t0).count();
+      const double dt = std::chrono::duration<double>(t1 - t0).count();
+      if (in >= n_skip + n_time) tot1 += dt;
+      else if (in >= n_skip) tot += dt;
     }
     *seconds = tot;
+    if (seconds1) *seconds1 = tot1;
   });
 }
 

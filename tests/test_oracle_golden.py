@@ -262,3 +262,14 @@ def test_thermal_occupation_at_particle_hole_symmetry(oracle_mod):
     np.testing.assert_allclose(occ[1], 0.5, atol=1e-9)
     np.testing.assert_allclose(occ[2], 1.0, atol=1e-12)
     assert np.all(dbl[0] > 0) and np.all(dbl[0] < 0.25)
+
+
+def test_oracle_timing_with_single_core_tail(oracle_mod):
+    """bench.py's CPU baseline: sites timed at the current thread count, then the next sites of the
+    same chain at one thread; both spans are positive and the thread count is the caller's to restore."""
+    drv = oracle_mod.driver(imp_kind=0, eps=-0.15, U=0.3, bath_kind=0, Lambda=2.0,
+                            first_V=float(np.sqrt(2 * 0.01 / np.pi)), nsites=8, K=64)
+    s, s1 = oracle_mod.time_forward_with_single_core(drv, 2, 3, 2)
+    assert s > 0 and s1 > 0
+    s0, s10 = oracle_mod.time_forward_with_single_core(drv, 2, 3, 0)
+    assert s0 > 0 and s10 == 0
