@@ -140,6 +140,15 @@ int nrgk_chain_add_bath_site(nrgk_chain* chain, const double* t_hopping, int n_h
 int nrgk_chain_update_internal_state(nrgk_chain* chain);
 /* nrgk_chain_add_bath_site + nrgk_chain_update_internal_state in one call */
 int nrgk_chain_step(nrgk_chain* chain, const double* t_hopping, int n_hopping, double rescale);
+/* qmatrix::diag (nrgcore/include/utils/qmatrix.hpp:375-404: LAPACKE_dsyevd(ROW_MAJOR, 'V', 'U', n,
+ * A, n, w), A becomes the eigenvectors) for n_blocks symmetric blocks in one call, through the same
+ * batched eigensolver as a chain's sectors.  Block b is dims[b] x dims[b], row-major; the blocks are
+ * packed one after another in H (sum dims[b]^2 doubles, only the blocks' full storage is read).
+ * Ascending eigenvalues go to w (sum dims[b]), the eigenvectors as the columns of each block to U
+ * (packed like H).  Host arrays; mode NRGK_MODE_MIXED or NRGK_MODE_PLAIN64.  Allocates its
+ * device buffers per call. */
+int nrgk_eigh_batched(nrgk_ctx* ctx, int n_blocks, const int* dims, const double* H, double* w, double* U,
+                      int mode);
 /* Several independent chains (z-shifted discretisations, parameter points, requests) advanced by one
  * Wilson site each, concurrently: one host worker and the chain's own stream per chain, so that the
  * chains' launches overlap on the GPU.  No counterpart in the reference, whose drivers step one

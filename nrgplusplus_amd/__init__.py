@@ -95,6 +95,7 @@ def lib():
             "nrgk_chain_update_internal_state": (i, [vp]),
             "nrgk_chain_step": (i, [vp, dp, i, d]),
             "nrgk_chains_step": (i, [pp, i, dp, i, dp, pp, i, pp, i, ip]),
+            "nrgk_eigh_batched": (i, [vp, i, ip, dp, dp, dp, i]),
             "nrgk_chain_counters": (i, [vp, ip, ip]),
             "nrgk_chain_set_counter": (i, [vp, i]),
             "nrgk_chain_higher_energy_discarded": (i, [vp, ip]),
@@ -687,6 +688,31 @@ def chains_step(chains, t, rescale, observables=None, recorders=None, n_threads=
     out = np.zeros(n, np.int32)
     _check(lib().nrgk_chains_step(ch, n, _dp(t), t.shape[1], _dp(r), ob, n_obs, rec, int(n_threads), _ip(out)))
     return out.astype(bool)
+
+
+def eigh_batched(blocks, mode=None, ctx=None):
+    """qmatrix::diag (nrgcore/include/utils/qmatrix.hpp:375-404) for a list of symmetric blocks in one
+    call, through the chains' batched eigensolver: a list of (w, U), w ascending, the eigenvectors
+    in the columns of U."""
+    ctx = ctx or default_context()
+    mode = NRGK_MODE_MIXED if mode is None else mode
+    mats = [np.ascontiguousarray(b, np.float64) for b in blocks]
+    for m in mats:
+        if m.ndim != 2 or m.shape[0] != m.shape[1]:
+            raise ValueError("eigh_batched: every block must be square")
+    dims = np.array([m.shape[0] for m in mats], np.int32)
+    H = np.concatenate([m.reshape(-1) for m in mats]) if mats else np.zeros(0)
+    w = np.zeros(max(1, int(dims.sum())))
+    U = np.zeros(max(1, H.size))
+    _check(lib().nrgk_eigh_batched(ctx._h, len(dims), _ip(dims), _dp(H if H.size else np.zeros(1)), _dp(w), _dp(U),
+                                   int(mode)))
+    out, eo, uo = [], 0, 0
+    for n in dims:
+        n = int(n)
+        out.append((w[eo:eo + n].copy(), U[uo:uo + n * n].reshape(n, n).copy()))
+        eo += n
+        uo += n * n
+    return out
 
 
 def broaden(x, w, omega, b, ctx=None):

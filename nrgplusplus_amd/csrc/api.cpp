@@ -567,4 +567,24 @@ int nrgk_chains_step(nrgk_chain* const* chains, int n_chains, const double* t, i
   });
 }
 
+// qmatrix::diag (qmatrix.hpp:375-404) for a list of blocks in one call, through the chain's eigensolver.
+int nrgk_eigh_batched(nrgk_ctx* ctx, int n_blocks, const int* dims, const double* H, double* w, double* U,
+                      int mode) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx && n_blocks >= 0 && (dims || n_blocks == 0), NRGK_ERR_INVALID, "nrgk_eigh_batched: bad argument");
+    std::vector<int> d(dims, dims + n_blocks);
+    long long ne = 0;
+    for (int x : d) {
+      NRGK_REQUIRE(x >= 0, NRGK_ERR_INVALID, "nrgk_eigh_batched: negative block size");
+      ne += x;
+    }
+    NRGK_REQUIRE(mode == NRGK_MODE_MIXED || mode == NRGK_MODE_PLAIN64, NRGK_ERR_INVALID, "nrgk: unknown mode");
+    if (ne == 0) return;
+    NRGK_REQUIRE(H && w && U, NRGK_ERR_INVALID, "nrgk_eigh_batched: null array");
+    NRGK_HIP(hipSetDevice(ctx->c.device));
+    Chain solver(&ctx->c, mode);
+    solver.solve_blocks(d, H, w, U);
+  });
+}
+
 }  // extern "C"

@@ -84,6 +84,13 @@ Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, 
   NRGK_REQUIRE(imp.nops == bath.nops, NRGK_ERR_INVALID,
                "f_dag_operator.size is different on model and impurity!\nimpurityModel->f_dag_operator.size() = " +
                    std::to_string(imp.nops) + "bath_model->f_dag_operator.size() = " + std::to_string(bath.nops));
+  init_device();
+  upload_bath();
+}
+
+Chain::Chain(Ctx* c, int mode_) : ctx(c), mode(mode_) { init_device(); }
+
+void Chain::init_device() {
   NRGK_REQUIRE(mode == NRGK_MODE_MIXED || mode == NRGK_MODE_PLAIN64, NRGK_ERR_INVALID, "nrgk: unknown mode");
   for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
                     &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_prev, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
@@ -94,7 +101,36 @@ Chain::Chain(Ctx* c, const nrgk_site_desc* imp_d, const nrgk_site_desc* bath_d, 
   NRGK_HIP(hipSetDevice(ctx->device));
   NRGK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   for (auto& e : ev) NRGK_HIP(hipEventCreate(&e));
-  upload_bath();
+}
+
+// qmatrix::diag (qmatrix.hpp:375-404) for a list of blocks: the blocks are laid out as an
+// iteration's sector blocks are (packed, offsets n^2 apart) and go through the same eigensolver.
+void Chain::solve_blocks(const std::vector<int>& dims, const double* H, double* w, double* U) {
+  const int S = (int)dims.size();
+  cur_qn.assign(S, std::vector<int>(1, 0));
+  cur_n = dims;
+  cur_eoff.assign(S, 0);
+  cur_off.assign(S, 0);
+  tot_elems = 0;
+  N_tot = 0;
+  nmax = 0;
+  for (int s = 0; s < S; s++) {
+    NRGK_REQUIRE(dims[s] >= 0, NRGK_ERR_INVALID, "nrgk_eigh_batched: negative block size");
+    cur_eoff[s] = (int)N_tot;
+    cur_off[s] = tot_elems;
+    N_tot += dims[s];
+    tot_elems += (long long)dims[s] * dims[s];
+    nmax = std::max(nmax, dims[s]);
+  }
+  if (N_tot == 0) return;
+  NRGK_HIP(hipSetDevice(ctx->device));
+  dH.ensure((size_t)tot_elems * sizeof(double));
+  NRGK_HIP(hipMemcpyAsync(dH.p, H, (size_t)tot_elems * sizeof(double), hipMemcpyHostToDevice, st));
+  if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
+  else eigensolve_mixed();
+  NRGK_HIP(hipMemcpyAsync(w, dlam_raw.p, (size_t)N_tot * sizeof(double), hipMemcpyDeviceToHost, st));
+  NRGK_HIP(hipMemcpyAsync(U, dU.p, (size_t)tot_elems * sizeof(double), hipMemcpyDeviceToHost, st));
+  sync();
 }
 
 void Chain::sync() {
@@ -692,12 +728,12 @@ void Chain::eigensolve_mixed() {
         }
     }
     Staging::Region rg = stg.open(Staging::span<EigBlock>(act.size()) + Staging::span<int4>(tiles_all.size()) +
-                                  Staging::span<int4>(tiles_sym.size()) + Staging::span<double>(3 * na) +
+                                  Staging::span<int4>(tiles_sym.size()) + Staging::span<double>(4 * na) +
                                   Staging::span<double>(na));
     const EigBlock* m_blk = stg.place(rg, act);
     const int4* m_all = stg.place(rg, tiles_all);
     const int4* m_sym = stg.place(rg, tiles_sym);
-    double* m_conv = stg.place_zero<double>(rg, 3 * na);
+    double* m_conv = stg.place_zero<double>(rg, 4 * na);
     double* m_offmax = stg.place_zero<double>(rg, na);
     stg.commit(rg, st);
     std::vector<GemmProb> g1, g2, g3;
@@ -733,10 +769,10 @@ void Chain::eigensolve_mixed() {
     prof_end("oa_update", 5, 0.0, oa_bytes);
     check("oa_update (E)", dE, false, false);
     check("oa_update (lam)", dlam_tmp, false, true);
-    const double* h_conv = stg.get(m_conv, 3 * na, st);
+    const double* h_conv = stg.get(m_conv, 4 * na, st);
     run_gemms(g3);
     sync();
-    const std::vector<double> conv(h_conv, h_conv + 3 * na);
+    const std::vector<double> conv(h_conv, h_conv + 4 * na);
     std::swap(dX, dX2);
     check("X+XE", dX, false, false);
     oa_steps = it + 1;
@@ -745,10 +781,15 @@ void Chain::eigensolve_mixed() {
       // NRGK_DEBUG_OA=n: per step, the residual (in noise units) and orthogonality defect of every
       // active block at least n wide
       if (dbg_oa > 0 && act[a].n >= dbg_oa)
-        std::fprintf(stderr, "nrgk oa step %d block off=%lld n=%d residual/noise=%.3g orth=%.3g\n", it, act[a].off,
-                     act[a].n, conv[3 * a] / conv[3 * a + 2], conv[3 * a + 1]);
-      const bool last = conv[3 * a] <= e_tol * conv[3 * a + 2] && conv[3 * a + 1] <= d_tol;
-      const bool pred = conv[3 * a] <= e_pred * conv[3 * a + 2] && conv[3 * a + 1] <= d_pred;
+        std::fprintf(stderr, "nrgk oa step %d block off=%lld n=%d residual/noise=%.3g orth=%.3g rotation=%.3g\n", it,
+                     act[a].off, act[a].n, conv[4 * a] / conv[4 * a + 2], conv[4 * a + 1], conv[4 * a + 3]);
+      // the device's test (k_copy_converged) on the same numbers: the last step, or a full step whose
+      // update is predicted to land at the noise (out-of-cluster rotations r with n r^2 <= d_tol, so
+      // that X + XE loses no orthogonality beyond d_tol at second order)
+      const double rot = conv[4 * a + 3];
+      const bool last = conv[4 * a] <= e_tol * conv[4 * a + 2] && conv[4 * a + 1] <= d_tol;
+      const bool pred = conv[4 * a] <= e_pred * conv[4 * a + 2] && conv[4 * a + 1] <= d_pred &&
+                        (double)act[a].n * rot * rot <= d_tol;
       if (!last && !pred) next.push_back(act[a]);
     }
     // the finished blocks' vectors into both iterate buffers (the device applies the same test)

@@ -190,6 +190,11 @@ struct Chain {
   void prof_resolve();
 
   Chain(Ctx* c, const nrgk_site_desc* imp, const nrgk_site_desc* bath, size_t K, int mode);
+  // the eigensolver alone, without sites (nrgk_eigh_batched)
+  Chain(Ctx* c, int mode);
+  // eigenpairs of the given row-major symmetric blocks (host arrays packed one after another):
+  // ascending eigenvalues into w, eigenvectors as the columns of each block into U
+  void solve_blocks(const std::vector<int>& dims, const double* H, double* w, double* U);
   ~Chain();
   void set_bath(const nrgk_site_desc* b);
   void upload_bath();
@@ -204,6 +209,9 @@ struct Chain {
   void set_state(const nrgk_site_desc* st, int cnt);
   void ensure_all_eig();
   void run_gemms(std::vector<GemmProb>& probs);
+
+ private:
+  void init_device();
 };
 
 // plain FP64 mode (plain.cpp)

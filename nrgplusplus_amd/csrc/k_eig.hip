@@ -847,25 +847,36 @@ __device__ void rr_small_wave(const double* Sb, const double* Rb, double* Eb, in
 //   k_oa_prep     one workgroup per block: lt, delta, ascending sort, cluster ids, and the rounding
 //                 noise level sqrt(n) eps max|lt| of the entries of S
 //   k_oa_entries  32x32 tiles: E_ij (coalesced), largest out-of-cluster residual |rho_ij|
-// conv[3b..3b+2] = (largest residual, largest orthogonality defect, noise level) of active block b.
+// conv[4b..4b+3] = (largest residual, largest orthogonality defect, noise level, largest out-of-cluster
+// rotation |rho_ij / (lt_j - lt_i)|) of active block b.
 //   k_oa_rr       one workgroup per block: Rayleigh-Ritz of every cluster of two or more
 constexpr int OT = 32;
+constexpr int OA_CONV = 4;  // entries of conv per block
 
 // A block takes its last step when its out-of-cluster residuals are within e_tol times the rounding
 // noise of S and its orthogonality defect is at most d_tol.  Its out-of-cluster corrections are then
 // rounding noise over a gap, which could only spoil orthogonality: the last step applies the cluster
 // rotations and the normalisation alone.
 __device__ __forceinline__ bool oa_converged(const double* conv, int b, double e_tol, double d_tol) {
-  return conv[3 * b] <= e_tol * conv[3 * b + 2] && conv[3 * b + 1] <= d_tol;
+  return conv[OA_CONV * b] <= e_tol * conv[OA_CONV * b + 2] && conv[OA_CONV * b + 1] <= d_tol;
 }
 // A block leaves the batch after its last step, or after a full step taken from residuals within
-// e_pred times the noise and an orthogonality defect of at most d_pred: that step's update lands at
-// the noise (the residuals fall quadratically, to rho^2 / gap with every out-of-cluster gap at least
-// 1e5 noise, so e_pred = 300 leaves below one noise unit; the defect squares, 1e-6 -> 1e-12), so the
-// verifying step that would follow is skipped.
-__device__ __forceinline__ bool oa_done(const double* conv, int b, double e_tol, double d_tol, double e_pred,
+// e_pred times the noise, an orthogonality defect of at most d_pred and out-of-cluster rotations r
+// with n r^2 <= d_tol: that step's update lands at the noise (the residuals fall quadratically, to
+// rho^2 / gap with every out-of-cluster gap at least 1e5 noise, so e_pred = 300 leaves below one
+// noise unit; the normalisation defect squares, 1e-6 -> 1e-12; X + XE is orthogonal only to first
+// order in the rotations, so they must be small enough that their second-order loss stays within
+// d_tol -- a pair just outside a cluster can take a rotation of 1e-3), so the verifying step that
+// would follow is skipped.
+__device__ __host__ __forceinline__ bool oa_predicted(const double* conv, int b, int n, double d_tol, double e_pred,
+                                                      double d_pred) {
+  const double r = conv[OA_CONV * b + 3];
+  return conv[OA_CONV * b] <= e_pred * conv[OA_CONV * b + 2] && conv[OA_CONV * b + 1] <= d_pred &&
+         (double)n * r * r <= d_tol;
+}
+__device__ __forceinline__ bool oa_done(const double* conv, int b, int n, double e_tol, double d_tol, double e_pred,
                                         double d_pred) {
-  return oa_converged(conv, b, e_tol, d_tol) || (conv[3 * b] <= e_pred * conv[3 * b + 2] && conv[3 * b + 1] <= d_pred);
+  return oa_converged(conv, b, e_tol, d_tol) || oa_predicted(conv, b, n, d_tol, e_pred, d_pred);
 }
 
 __global__ void __launch_bounds__(256) k_oa_mask(const EigBlock* __restrict__ blocks, const int4* __restrict__ tiles,
@@ -956,7 +967,7 @@ __global__ void __launch_bounds__(256) k_oa_sym(const EigBlock* __restrict__ blo
     dm = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
     // non-negative doubles order like their bit patterns
     atomicMax((unsigned long long*)&offmax[t.x], (unsigned long long)__double_as_longlong(mx));
-    atomicMax((unsigned long long*)&conv[3 * t.x + 1], (unsigned long long)__double_as_longlong(dm));
+    atomicMax((unsigned long long*)&conv[OA_CONV * t.x + 1], (unsigned long long)__double_as_longlong(dm));
   }
 }
 
@@ -1020,7 +1031,7 @@ __global__ void __launch_bounds__(1024) k_oa_prep(const EigBlock* __restrict__ b
   const double eps = 2.220446049250313e-16;
   const double noise = sqrt((double)n) * eps * bq;
   const double delta = fmax(32.0 * offmax[blockIdx.x] + 64.0 * eps * bq, 1e5 * noise);
-  if (tid == 0) conv[3 * blockIdx.x + 2] = noise;
+  if (tid == 0) conv[OA_CONV * blockIdx.x + 2] = noise;
   for (int s = tid; s < n; s += blockDim.x) head[s] = (s == 0) || (key[s] - key[s - 1] > delta);
   __syncthreads();
   // cluster id = sorted position of the cluster's first member (ids are only compared), and the
@@ -1047,7 +1058,7 @@ __global__ void __launch_bounds__(256) k_oa_entries(const EigBlock* __restrict__
                                                     const double* __restrict__ S, const double* __restrict__ Rp,
                                                     const double* __restrict__ lam, const int* __restrict__ cid_g,
                                                     double* __restrict__ E, double* __restrict__ conv) {
-  __shared__ double red[4];
+  __shared__ double red[8];
   const int4 t = tiles[blockIdx.x];
   const EigBlock blk = blocks[t.x];
   const int n = blk.n;
@@ -1056,7 +1067,7 @@ __global__ void __launch_bounds__(256) k_oa_entries(const EigBlock* __restrict__
   double* Eb = E + blk.off;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int j = t.z * OT + tx;
-  double em = 0.0;
+  double em = 0.0, rm = 0.0;  // largest residual and largest rotation of the tile
   if (j < n) {
     const double ltj = lam[blk.e_off + j];
     const int cj = cid_g[blk.e_off + j];
@@ -1075,19 +1086,30 @@ __global__ void __launch_bounds__(256) k_oa_entries(const EigBlock* __restrict__
         // E + E^T = -(R - I) exact in floating point and avoids a quotient of two small numbers
         const double lti = lam[blk.e_off + i];
         const double rho = Sb[e] - 0.5 * (lti + ltj) * Rb[e];
-        v = rho / (ltj - lti) - 0.5 * Rb[e];
+        const double rot = rho / (ltj - lti);
+        v = rot - 0.5 * Rb[e];
         em = fmax(em, fabs(rho));
+        rm = fmax(rm, fabs(rot));
       }
       Eb[e] = v;
     }
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) em = fmax(em, __shfl_xor(em, o, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = em;
+  for (int o = 32; o > 0; o >>= 1) {
+    em = fmax(em, __shfl_xor(em, o, 64));
+    rm = fmax(rm, __shfl_xor(rm, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = em;
+    red[4 + (threadIdx.x >> 6)] = rm;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     em = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    atomicMax((unsigned long long*)&conv[3 * t.x], (unsigned long long)__double_as_longlong(em));
+    rm = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+    // non-negative doubles order as their bit patterns
+    atomicMax((unsigned long long*)&conv[OA_CONV * t.x], (unsigned long long)__double_as_longlong(em));
+    atomicMax((unsigned long long*)&conv[OA_CONV * t.x + 3], (unsigned long long)__double_as_longlong(rm));
   }
 }
 
@@ -1252,8 +1274,8 @@ __global__ void __launch_bounds__(256) k_copy_converged(const EigBlock* __restri
                                                         double e_pred, double d_pred, const double* __restrict__ src,
                                                         double* __restrict__ dst) {
   const int a = blockIdx.x >> 3, part = blockIdx.x & 7;
-  if (!oa_done(conv, a, e_tol, d_tol, e_pred, d_pred)) return;
   const EigBlock blk = blocks[a];
+  if (!oa_done(conv, a, blk.n, e_tol, d_tol, e_pred, d_pred)) return;
   const long long nn = (long long)blk.n * blk.n;
   for (long long e = part * 256LL + threadIdx.x; e < nn; e += 8 * 256LL) dst[blk.off + e] = src[blk.off + e];
 }

@@ -1,0 +1,97 @@
+"""The batched eigensolver on its own (nrgk_eigh_batched: qmatrix::diag, nrgcore/include/utils/
+qmatrix.hpp:375-404, for a list of blocks) on spectra chosen to be hard for it.
+
+Inputs: H = Q diag(lam) Q^T with Q Haar-distributed (QR of a seeded Gaussian matrix, column signs
+fixed) and lam chosen per block -- exact degeneracies of high multiplicity, clusters with gaps of
+1e-9, 1e-11 and 1e-13, values graded over eight orders of magnitude, uniform values -- for block
+sizes 1 to 800 in one batch (the widest goes through the crossover to the tridiagonalising solver).
+
+Pins, by construction and independent of LAPACK: the eigenvalues are lam (to the rounding of forming
+H, ~n eps |lam|_max); U is orthogonal; H U = U diag(w).  Tolerances (the eigen-error bar of the
+chain tests, relative to the block's norm): |w - lam| <= 1e-10 |lam|_max, |U^T U - I|_max <= 1e-10,
+|H U - U diag(w)|_max <= 1e-10 |lam|_max.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 16, 17, 31, 33, 64, 100, 257, 619, 800]
+
+
+def _haar(rng, n):
+    Q, R = np.linalg.qr(rng.standard_normal((n, n)))
+    return Q * np.sign(np.diag(R))
+
+
+def _spectrum(rng, n, kind):
+    if kind == "degenerate":
+        return np.sort(rng.choice([-1.0, -0.3, 0.0, 0.2, 0.7, 1.5], n))
+    if kind == "clustered":
+        base = np.sort(rng.uniform(-1, 1, max(1, n // 4)))
+        lam = []
+        for k, b in enumerate(base):
+            gap = (1e-9, 1e-11, 1e-13)[k % 3]
+            lam += [b + gap * j for j in range(4)]
+        return np.sort(np.array(lam[:n] + list(rng.uniform(-1, 1, n - len(lam[:n])))))
+    if kind == "graded":
+        return np.sort(rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-8, 0, n))
+    return np.sort(rng.uniform(-1, 1, n))
+
+
+def _case(seed):
+    rng = np.random.default_rng(seed)
+    kinds = ["degenerate", "clustered", "graded", "uniform"]
+    blocks, lams = [], []
+    for i, n in enumerate(SIZES):
+        lam = _spectrum(rng, n, kinds[i % 4])
+        Q = _haar(rng, n)
+        H = (Q * lam) @ Q.T
+        blocks.append(0.5 * (H + H.T))
+        lams.append(lam)
+    return blocks, lams
+
+
+def _check(blocks, lams, out, tol=1e-10):
+    for H, lam, (w, U) in zip(blocks, lams, out):
+        n = len(lam)
+        scale = max(1.0e-300, np.abs(lam).max()) if n else 1.0
+        assert np.all(np.diff(w) >= 0), n
+        assert np.abs(w - lam).max() <= tol * scale, (n, np.abs(w - lam).max())
+        assert np.abs(U.T @ U - np.eye(n)).max() <= tol, n
+        assert np.abs(H @ U - U * w).max() <= tol * scale, (n, np.abs(H @ U - U * w).max())
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_constructed_spectra_mixed(nk, seed):
+    blocks, lams = _case(seed)
+    _check(blocks, lams, nk.eigh_batched(blocks))
+
+
+def test_constructed_spectra_plain_and_modes_agree(nk):
+    blocks, lams = _case(3)
+    plain = nk.eigh_batched(blocks, mode=nk.NRGK_MODE_PLAIN64)
+    mixed = nk.eigh_batched(blocks)
+    _check(blocks, lams, plain)
+    for (wp, _), (wm, _), lam in zip(plain, mixed, lams):
+        assert np.abs(wp - wm).max() <= 2e-10 * max(1.0, np.abs(lam).max())
+
+
+def test_special_blocks(nk):
+    """zero blocks, a diagonal block with repeated entries, the identity, a 2x2 with a tiny
+    off-diagonal, and empty blocks among them"""
+    rng = np.random.default_rng(7)
+    d = np.sort(rng.choice([0.5, -0.25, 2.0], 40))
+    blocks = [np.zeros((0, 0)), np.zeros((5, 5)), np.diag(d), np.eye(24), np.zeros((0, 0)),
+              np.array([[1.0, 1e-14], [1e-14, 1.0 + 1e-13]])]
+    out = nk.eigh_batched(blocks)
+    assert [len(w) for w, _ in out] == [0, 5, 40, 24, 0, 2]
+    np.testing.assert_array_equal(out[1][0], np.zeros(5))
+    np.testing.assert_allclose(out[2][0], d, atol=1e-14)
+    np.testing.assert_allclose(out[3][0], np.ones(24), atol=1e-14)
+    for H, (w, U) in zip(blocks, out):
+        n = len(w)
+        if n:
+            assert np.abs(U.T @ U - np.eye(n)).max() <= 1e-12
+            assert np.abs(H @ U - U * w).max() <= 1e-12 * max(1.0, np.abs(w).max())
+    assert nk.eigh_batched([]) == []

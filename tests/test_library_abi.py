@@ -235,3 +235,14 @@ def test_chains_step_argument_checks(nk):
     assert lib.nrgk_chains_step(nulls, 1, nk._dp(one), 2, nk._dp(r), None, 0, None, 0, None) != 0
     assert b"null chain" in lib.nrgk_last_error()
     assert lib.nrgk_chains_step(nulls, -1, nk._dp(one), 2, nk._dp(r), None, 0, None, 0, None) != 0
+
+
+def test_eigh_batched_argument_checks(nk):
+    """nrgk_eigh_batched rejects a null context, negative sizes and an unknown mode before touching a
+    device"""
+    lib = nk.lib()
+    dims = np.array([2], np.int32)
+    H = np.eye(2)
+    w, U = np.zeros(2), np.zeros(4)
+    assert lib.nrgk_eigh_batched(None, 1, nk._ip(dims), nk._dp(H), nk._dp(w), nk._dp(U), nk.NRGK_MODE_MIXED) != 0
+    assert lib.nrgk_eigh_batched(None, -1, None, None, None, None, nk.NRGK_MODE_MIXED) != 0
