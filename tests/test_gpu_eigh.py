@@ -3,7 +3,8 @@ qmatrix.hpp:375-404, for a list of blocks) on spectra chosen to be hard for it.
 
 Inputs: H = Q diag(lam) Q^T with Q Haar-distributed (QR of a seeded Gaussian matrix, column signs
 fixed) and lam chosen per block -- exact degeneracies of high multiplicity, clusters with gaps of
-1e-9, 1e-11 and 1e-13, values graded over eight orders of magnitude, uniform values -- for block
+1e-9, 1e-11 and 1e-13, gaps spread from 1e-11 to 1e-8 (across the refinement's cluster threshold),
+values graded over eight orders of magnitude, uniform values -- for block
 sizes 1 to 800 in one batch (the widest goes through the crossover to the tridiagonalising solver).
 
 Pins, by construction and independent of LAPACK: the eigenvalues are lam (to the rounding of forming
@@ -34,6 +35,8 @@ def _spectrum(rng, n, kind):
             gap = (1e-9, 1e-11, 1e-13)[k % 3]
             lam += [b + gap * j for j in range(4)]
         return np.sort(np.array(lam[:n] + list(rng.uniform(-1, 1, n - len(lam[:n])))))
+    if kind == "threshold":  # gaps from 1e-11 to 1e-8, across the refinement's cluster threshold
+        return np.sort(np.cumsum(np.concatenate([[rng.uniform(-1, 0)], 10.0 ** rng.uniform(-11, -8, n - 1)])))
     if kind == "graded":
         return np.sort(rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-8, 0, n))
     return np.sort(rng.uniform(-1, 1, n))
@@ -41,10 +44,10 @@ def _spectrum(rng, n, kind):
 
 def _case(seed):
     rng = np.random.default_rng(seed)
-    kinds = ["degenerate", "clustered", "graded", "uniform"]
+    kinds = ["degenerate", "clustered", "graded", "uniform", "threshold"]
     blocks, lams = [], []
     for i, n in enumerate(SIZES):
-        lam = _spectrum(rng, n, kinds[i % 4])
+        lam = _spectrum(rng, n, kinds[(i + seed) % 5])
         Q = _haar(rng, n)
         H = (Q * lam) @ Q.T
         blocks.append(0.5 * (H + H.T))
@@ -62,7 +65,7 @@ def _check(blocks, lams, out, tol=1e-10):
         assert np.abs(H @ U - U * w).max() <= tol * scale, (n, np.abs(H @ U - U * w).max())
 
 
-@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
 def test_constructed_spectra_mixed(nk, seed):
     blocks, lams = _case(seed)
     _check(blocks, lams, nk.eigh_batched(blocks))
@@ -95,3 +98,19 @@ def test_special_blocks(nk):
             assert np.abs(U.T @ U - np.eye(n)).max() <= 1e-12
             assert np.abs(H @ U - U * w).max() <= 1e-12 * max(1.0, np.abs(w).max())
     assert nk.eigh_batched([]) == []
+
+
+def test_wide_clusters(nk):
+    """clusters wider than the in-LDS Rayleigh-Ritz (64 members): an exactly 100-fold eigenvalue, an
+    80-member cluster spread over 1e-12, and two 70-fold degenerate levels 1e-10 apart"""
+    rng = np.random.default_rng(11)
+    lam1 = np.sort(np.concatenate([np.full(100, 0.25), rng.uniform(-1, 1, 200)]))
+    lam2 = np.sort(np.concatenate([0.1 + 1e-12 * np.linspace(0, 1, 80), rng.uniform(-1, 1, 120)]))
+    lam3 = np.sort(np.concatenate([np.full(70, -0.5), np.full(70, -0.5 + 1e-10), rng.uniform(-1, 1, 60)]))
+    blocks, lams = [], []
+    for lam in (lam1, lam2, lam3):
+        Q = _haar(rng, len(lam))
+        H = (Q * lam) @ Q.T
+        blocks.append(0.5 * (H + H.T))
+        lams.append(lam)
+    _check(blocks, lams, nk.eigh_batched(blocks))
