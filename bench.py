@@ -18,6 +18,7 @@ Beside the timed value the line carries:
   spectra       the FDM backward pass over all recorded shells (spectra/s)
   z_average     16 z-shifted chains of the z-averaged thermodynamics workload, one after another and
                 stepped together (chain iterations/s)
+  rgflow        the RG-flow workload (K=1024, 40 sites) in the mixed and the plain mode
   serving       a Poisson stream of SIAM requests (64 of the 512-request workload), one chain per
                 request on 8 host workers: p50/p99 request latency and solves/s
   roofline      the dominant kernel family, from the library's per-launch HIP-event timers
@@ -276,6 +277,40 @@ def serving(nk):
             "total_weight_range": [min(o[1] for o in out), max(o[1] for o in out)], "all_finite": all(o[2] for o in out)}
 
 
+def rgflow(nk):
+    """The RG-flow workload (examples/rgflowSIAM form): spinhalf impurity eps=-0.25, U=0.5, Gamma=0.02,
+    Lambda=2.5, K=1024, first site {V, V} then 40 Wilson sites; all_eigenvalue read after every
+    iteration.  The whole flow in the mixed and in the plain FP64 mode (each after one untimed flow),
+    and the largest difference of the final all_eigenvalue between them."""
+    L, K, ns = 2.5, 1024, 40
+    V = np.sqrt(2 * 0.02 / np.pi)
+
+    def flow(mode):
+        g = nk.NrgCore(nk.Site.spinhalf(-0.25, 0.5), nk.Site.spinhalf(0, 0), K, mode=mode)
+        t0 = time.perf_counter()
+        g.add_bath_site([V, V], 1.0)
+        g.update_internal_state()
+        e = g.all_eigenvalue
+        for n in range(ns):
+            t = nk.hopping(n, L)
+            g.add_bath_site([t, t], 1.0 if n == 0 else np.sqrt(L))
+            g.update_internal_state()
+            e = g.all_eigenvalue
+        return time.perf_counter() - t0, e
+
+    out = {"workload": "SIAM RG flow: U=0.5 eps=-0.25 Gamma=0.02 Lambda=2.5 K=1024, 40 sites"}
+    res = {}
+    for name, mode in (("mixed", nk.NRGK_MODE_MIXED), ("plain_fp64", nk.NRGK_MODE_PLAIN64)):
+        flow(mode)
+        secs, e = flow(mode)
+        res[name] = e
+        out[name] = {"seconds": secs, "iterations_per_s": (ns + 1) / secs}
+    out["speedup"] = out["plain_fp64"]["seconds"] / out["mixed"]["seconds"]
+    a, b = res["mixed"], res["plain_fp64"]
+    out["final_all_eigenvalue_max_diff"] = float(np.abs(a - b).max()) if len(a) == len(b) else None
+    return out
+
+
 def two_channel(nk, n_steps):
     """The two-channel SIAM chain (impurity spinhalf eps=-0.2, U=0.4 with its f-dagger duplicated to 4
     operators, two-channel bath, Gamma1 = Gamma2 = 0.01, Lambda = 3, K = 4096): the first n_steps
@@ -309,6 +344,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=0, help="0: the timed iterations")
     ap.add_argument("--no-zavg", action="store_true", help="skip the z-averaged multi-chain sample")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-serving sample")
+    ap.add_argument("--no-rgflow", action="store_true", help="skip the RG-flow sample")
     ap.add_argument("--two-channel-steps", type=int, default=0,
                     help="also time this many iterations of the two-channel K=4096 chain (mixed and plain)")
     ap.add_argument("--stop-after-timed", action="store_true",
@@ -374,6 +410,7 @@ def main():
 
     zavg = None if args.no_zavg else z_average(nk)
     serve = None if args.no_serving else serving(nk)
+    rgf = None if args.no_rgflow else rgflow(nk)
     tc = two_channel(nk, args.two_channel_steps) if args.two_channel_steps > 0 else None
 
     cpu = None
@@ -402,6 +439,7 @@ def main():
         "spectra": spectra,
         "z_average": zavg,
         "serving": serve,
+        "rgflow": rgf,
         "two_channel": tc,
         "step_ms_p50": float(np.percentile(step_ms, 50)),
         "step_ms_p99": float(np.percentile(step_ms, 99)),
