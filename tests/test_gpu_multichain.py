@@ -134,3 +134,29 @@ def test_chains_step_errors(nk):
         nk.chains_step([a, c], [[0.1, 0.1]] * 2, 1.0)
     assert a.nrg_iterations_cnt == a0 + 1 and c.nrg_iterations_cnt == c0 and b.nrg_iterations_cnt == b0
     assert len(nk.chains_step([], np.zeros((0, 2)), [])) == 0
+
+
+def test_chains_with_wide_blocks_stepped_together(nk, monkeypatch):
+    """Three two-channel chains whose wide blocks (limit lowered to 40 rows) go to their own
+    rocSOLVER side streams, stepped together: each bit-identical to the chain stepped alone."""
+    monkeypatch.setenv("NRGK_MIXED_MAX", "40")
+    K, Lw, ns = 128, 3.0, 4
+    params = [(-0.2, 0.4), (-0.25, 0.5), (-0.15, 0.3)]
+
+    def make(e, u):
+        return nk.NrgCore(nk.Site.spinhalf(e, u).duplicate_ops(), nk.Site.two_channel(), K)
+
+    V = np.sqrt(2 * 0.01 / np.pi)
+    ts = [V] + [nk.hopping(n, Lw) for n in range(ns)]
+    rs = [1.0] + [1.0 if n == 0 else np.sqrt(Lw) for n in range(ns)]
+    gs = [make(e, u) for e, u in params]
+    for t, r in zip(ts, rs):
+        nk.chains_step(gs, [[t] * 4] * 3, r)
+    assert max(int(g.sectors()[1].max()) for g in gs) > 40
+    for k, (e, u) in enumerate(params):
+        g = make(e, u)
+        for t, r in zip(ts, rs):
+            g.add_bath_site([t] * 4, r)
+            g.update_internal_state()
+        assert np.array_equal(gs[k].all_eigenvalue, g.all_eigenvalue), k
+        assert np.array_equal(gs[k].kept_counts(), g.kept_counts()), k
