@@ -15,6 +15,7 @@ Beside the timed value the line carries:
   plain_gpu     the same iterations in the plain FP64 GPU form (rocSOLVER dsyevd + rocBLAS dgemm,
                 one call per block), a bounded sample; speedup = value / plain
   eig_err       max |lambda_gpu - lambda_LAPACK| over every block of every iteration after the timed ones
+  solve_ms_p50/p99  the eigensolve alone (sector eigenpairs of a step), device time per timed step
   spectra       the FDM backward pass over all recorded shells (spectra/s)
   z_average     16 z-shifted chains of the z-averaged thermodynamics workload, one after another and
                 stepped together (chain iterations/s)
@@ -370,10 +371,14 @@ def main():
     s.g.profile(True, only="jacobi_round")
     step_ms = []
     t0 = time.perf_counter()
+    solve_ms, unconverged = [], 0
     for _ in range(K):
         ta = time.perf_counter()
         s.step()
         step_ms.append((time.perf_counter() - ta) * 1e3)
+        st = s.g.step_stats()  # device time of this step's eigensolve (HIP events around it)
+        solve_ms.append(float(st["ms_eig"]))
+        unconverged += st["oa_unconverged"]
     t1 = time.perf_counter()
     kstats = s.g.kernel_stats()
     s.g.profile(False)
@@ -445,6 +450,9 @@ def main():
         "two_channel": tc,
         "step_ms_p50": float(np.percentile(step_ms, 50)),
         "step_ms_p99": float(np.percentile(step_ms, 99)),
+        "solve_ms_p50": float(np.percentile(solve_ms, 50)),
+        "solve_ms_p99": float(np.percentile(solve_ms, 99)),
+        "refinement_unconverged_blocks": unconverged,
         "kernels": {k: dict(ms_per_step=v["ms"] / K, launches_per_step=v["launches"] / K,
                             tflops=(v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else None,
                             gbs=(v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None)
