@@ -13,6 +13,7 @@
 //   * errors are std::runtime_error carrying nrgk_last_error(), as the reference throws
 //     std::runtime_error (nrgcore.hpp:117-152).
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -408,6 +409,34 @@ inline std::pair<std::vector<double>, std::vector<double>> z_chain(double z, dou
   check(nrgk_zdisc_chain(z, lambda, D, x.data(), rho.data(), (int)x.size(), quad, n_intervals, normalize ? 1 : 0,
                          n_sites, t.data(), e.data()));
   return {t, e};
+}
+
+// qmatrix::diag (nrgcore/include/utils/qmatrix.hpp:375-404) for many blocks in one call
+// (nrgk_eigh_batched): each block's data is replaced by its eigenvectors (in the columns, row-major,
+// as dsyevd leaves A) and its ascending eigenvalues are returned.
+inline std::vector<std::vector<double>> diag_blocks(std::vector<block>& blocks, context& ctx = context::default_context(),
+                                                    int mode = NRGK_MODE_MIXED) {
+  std::vector<int> dims;
+  std::vector<double> H;
+  for (const auto& b : blocks) {
+    if (b.rows != b.cols || b.data.size() != b.rows * b.cols) throw std::runtime_error("diag_blocks: square blocks only");
+    dims.push_back((int)b.rows);
+    H.insert(H.end(), b.data.begin(), b.data.end());
+  }
+  size_t ne = 0;
+  for (int n : dims) ne += (size_t)n;
+  std::vector<double> w(ne), U(H.size());
+  check(nrgk_eigh_batched(ctx.get(), (int)dims.size(), dims.data(), H.data(), w.data(), U.data(), mode));
+  std::vector<std::vector<double>> out;
+  size_t eo = 0, uo = 0;
+  for (auto& b : blocks) {
+    const size_t n = b.rows;
+    out.emplace_back(w.begin() + eo, w.begin() + eo + n);
+    std::copy(U.begin() + uo, U.begin() + uo + n * n, b.data.begin());
+    eo += n;
+    uo += n * n;
+  }
+  return out;
 }
 
 // One Wilson site on each of several independent chains, run concurrently (nrgk_chains_step; the
