@@ -66,6 +66,17 @@ int main(int argc, char** argv) {
       for (size_t i = 0; i < e.size() && i < n_print; i++) std::printf(" %.17g", e[i]);
       std::printf(" %.17g\n", e.empty() ? 0.0 : e.back());
     }
+    // qmatrix::diag through the batched eigensolver (nrgk::diag_blocks): [[2,1,0],[1,2,0],[0,0,5]] has
+    // eigenvalues 1, 3, 5; a 1 x 1 block is its own eigenvalue
+    std::vector<nrgk::block> blocks(2);
+    blocks[0] = nrgk::block{3, 3, {2, 1, 0, 1, 2, 0, 0, 0, 5}};
+    blocks[1] = nrgk::block{1, 1, {-0.5}};
+    const auto w = nrgk::diag_blocks(blocks);
+    const double ev[3] = {1.0, 3.0, 5.0};
+    bool ok = w.size() == 2 && w[0].size() == 3 && w[1].size() == 1 && std::fabs(w[1][0] + 0.5) < 1e-14;
+    for (int i = 0; ok && i < 3; i++) ok = std::fabs(w[0][i] - ev[i]) < 1e-12;
+    std::fprintf(stderr, "diag_blocks %s\n", ok ? "ok" : "wrong");
+    if (!ok) return 4;
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "%s\n", ex.what());
     return 1;

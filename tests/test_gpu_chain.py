@@ -523,6 +523,7 @@ def test_cpp_nrgcore_api_rgflow_matches_oracle(oracle_mod, n_copies):
     r = subprocess.run([exe, str(eps), str(U), str(G), str(L), str(K), str(ns), "100000", str(n_copies)],
                        capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stderr
+    assert "diag_blocks ok" in r.stderr  # nrgk::diag_blocks on blocks with known eigenvalues
     lines = [list(map(float, ln.split())) for ln in r.stdout.strip().splitlines()]
     assert len(lines) == ns + 1
     ref = oracle_mod.run_rgflow(oracle_mod.driver(imp_kind=0, eps=eps, U=U, bath_kind=0, Lambda=L,
