@@ -114,3 +114,18 @@ def test_wide_clusters(nk):
         blocks.append(0.5 * (H + H.T))
         lams.append(lam)
     _check(blocks, lams, nk.eigh_batched(blocks))
+
+
+def test_random_batches(nk):
+    """eight batches of ten blocks of random sizes 1..899, each with a spectrum of a random kind"""
+    rng = np.random.default_rng(123)
+    kinds = ["degenerate", "clustered", "graded", "uniform", "threshold"]
+    for _ in range(8):
+        blocks, lams = [], []
+        for n in rng.integers(1, 900, 10):
+            lam = _spectrum(rng, int(n), kinds[rng.integers(5)])
+            Q = _haar(rng, int(n))
+            H = (Q * lam) @ Q.T
+            blocks.append(0.5 * (H + H.T))
+            lams.append(lam)
+        _check(blocks, lams, nk.eigh_batched(blocks))
