@@ -146,7 +146,8 @@ int nrgk_chain_step(nrgk_chain* chain, const double* t_hopping, int n_hopping, d
  * packed one after another in H (sum dims[b]^2 doubles, only the blocks' full storage is read).
  * Ascending eigenvalues go to w (sum dims[b]), the eigenvectors as the columns of each block to U
  * (packed like H).  Host arrays; mode NRGK_MODE_MIXED or NRGK_MODE_PLAIN64.  Allocates its
- * device buffers per call. */
+ * device buffers per call.  NRGK_ERR_NOCONV when a block does not converge (the FP32 Jacobi, the
+ * FP64 refinement within its step limit, or dsyevd); the reference only prints dsyevd's info. */
 int nrgk_eigh_batched(nrgk_ctx* ctx, int n_blocks, const int* dims, const double* H, double* w, double* U,
                       int mode);
 /* Several independent chains (z-shifted discretisations, parameter points, requests) advanced by one

@@ -128,6 +128,11 @@ void Chain::solve_blocks(const std::vector<int>& dims, const double* H, double* 
   NRGK_HIP(hipMemcpyAsync(dH.p, H, (size_t)tot_elems * sizeof(double), hipMemcpyHostToDevice, st));
   if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
   else eigensolve_mixed();
+  // a chain reports blocks that the refinement left short of its stopping point in its statistics;
+  // the stand-alone solver fails instead, as dsyevd reports a nonzero info
+  NRGK_REQUIRE(mode == NRGK_MODE_PLAIN64 || oa_unconverged == 0, NRGK_ERR_NOCONV,
+               "nrgk_eigh_batched: the FP64 refinement left " + std::to_string(oa_unconverged) +
+                   " block(s) unconverged");
   NRGK_HIP(hipMemcpyAsync(w, dlam_raw.p, (size_t)N_tot * sizeof(double), hipMemcpyDeviceToHost, st));
   NRGK_HIP(hipMemcpyAsync(U, dU.p, (size_t)tot_elems * sizeof(double), hipMemcpyDeviceToHost, st));
   sync();

@@ -129,3 +129,14 @@ def test_random_batches(nk):
             blocks.append(0.5 * (H + H.T))
             lams.append(lam)
         _check(blocks, lams, nk.eigh_batched(blocks))
+
+
+def test_unconverged_refinement_is_an_error(nk, monkeypatch):
+    """with the refinement limited to one step (NRGK_OA_ITERS=1) a 200-wide block cannot reach its
+    stopping point: the stand-alone solver reports that instead of returning the vectors"""
+    monkeypatch.setenv("NRGK_OA_ITERS", "1")
+    rng = np.random.default_rng(5)
+    lam = np.sort(rng.uniform(-1, 1, 200))
+    Q = _haar(rng, 200)
+    with pytest.raises(nk.NrgkError, match="unconverged"):
+        nk.eigh_batched([(Q * lam) @ Q.T])
