@@ -198,6 +198,10 @@ int nrgk_chain_fdag_blocks(nrgk_chain* chain, int op, int* n_blocks, int* bi, in
 int nrgk_chain_set_state(nrgk_chain* chain, const nrgk_site_desc* state, int cnt);
 /* per-step device timings (ms): assemble, eigensolve, truncate, rotate; and eigensolver stats */
 int nrgk_chain_step_stats(nrgk_chain* chain, double* ms4, int* jacobi_sweeps, int* max_cluster);
+/* kernel launches (a rocBLAS/rocSOLVER call counts once) and host stream synchronisations of the
+ * last step, from add_bath_site to the end of update_internal_state (observable updates and shell
+ * records made between them included); no reference counterpart */
+int nrgk_chain_step_counts(nrgk_chain* chain, long long* launches, long long* syncs);
 /* refinement statistics of the last eigensolve: Ogita-Aishima steps taken and blocks that did not
  * meet the convergence test within the step budget */
 int nrgk_chain_refine_stats(nrgk_chain* chain, int* oa_steps, int* unconverged);
@@ -256,6 +260,12 @@ int nrgk_fdm_backward(nrgk_fdm* fdm, double lambda, int scale_offset, int first_
 int nrgk_fdm_save(nrgk_fdm* rec, const char* path, const char* op_group);
 /* a one-dimensional dataset of an HDF5 file as doubles (n: its length; out may be NULL) */
 int nrgk_h5_read(const char* path, const char* dataset, double* out, size_t cap, size_t* n);
+/* Per operator, the signed weights w+ + w- of the last nrgk_fdm_backward summed over its shells,
+ * taken before the |.|, the 1e-20 cut and the bin-range test of rhoDotOperators
+ * (fdmSpectrum.hpp:231-263).  Over a complete basis of discarded states this is <{B, B+}> (1 for
+ * d+ chi): the FDM sum rule, which the reference does not compute.  n_ops receives the operator
+ * count; out[cap] may be NULL. */
+int nrgk_fdm_signed_weights(nrgk_fdm* fdm, double* out, int cap, int* n_ops);
 /* bin centres e_min*exp(i/delE) (fdmSpectrum.hpp:323-327) */
 int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out);
 

@@ -111,6 +111,7 @@ def lib():
             "nrgk_chain_set_state": (i, [vp, sd, i]),
             "nrgk_chain_step_stats": (i, [vp, dp, ip, ip]),
             "nrgk_chain_refine_stats": (i, [vp, ip, ip]),
+            "nrgk_chain_step_counts": (i, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
             "nrgk_chain_profile": (i, [vp, i]),
             "nrgk_chain_profile_only": (i, [vp, C.c_char_p]),
             "nrgk_chain_kernel_stats": (i, [vp, vp, i, ip]),
@@ -125,6 +126,7 @@ def lib():
             "nrgk_fdm_record": (i, [vp]),
             "nrgk_fdm_backward": (i, [vp, d, i, i, dp, dp, dp, dp, i, ip]),
             "nrgk_fdm_energy_points": (i, [vp, dp]),
+            "nrgk_fdm_signed_weights": (i, [vp, dp, i, ip]),
             "nrgk_thermo": (i, [vp, d, i, i, dp, i, dp, dp]),
             "nrgk_fdm_save": (i, [vp, C.c_char_p, C.c_char_p]),
             "nrgk_h5_read": (i, [C.c_char_p, C.c_char_p, dp, sz, szp]),
@@ -527,8 +529,11 @@ class NrgCore:
         _check(lib().nrgk_chain_step_stats(self._h, _dp(ms), C.byref(sw), C.byref(mc)))
         oa, unc = C.c_int(), C.c_int()
         _check(lib().nrgk_chain_refine_stats(self._h, C.byref(oa), C.byref(unc)))
+        nl, ns = C.c_longlong(), C.c_longlong()
+        _check(lib().nrgk_chain_step_counts(self._h, C.byref(nl), C.byref(ns)))
         return dict(ms_assemble=ms[0], ms_eig=ms[1], ms_truncate=ms[2], ms_rotate=ms[3], jacobi_sweeps=sw.value,
-                    max_cluster=mc.value, oa_steps=oa.value, oa_unconverged=unc.value)
+                    max_cluster=mc.value, oa_steps=oa.value, oa_unconverged=unc.value, launches=nl.value,
+                    host_syncs=ns.value)
 
     def profile(self, enable=True, only=None):
         """Start (clearing the totals) or stop per-kernel-family device timing; `only` restricts it to
@@ -629,6 +634,15 @@ class FdmRecorder:
         _check(lib().nrgk_fdm_backward(self._h, lam, scale_offset, first_cnt, _dp(pos), _dp(neg), _dp(tr), _dp(sm),
                                        max_shells, C.byref(ns)))
         return pos.reshape(n_ops, self.n_bins), neg.reshape(n_ops, self.n_bins), tr[: ns.value], sm[: ns.value]
+
+    def signed_weights(self):
+        """per operator, the signed weights of the last backward pass summed over its shells (the
+        complete-basis sum rule: <{B, B+}>)"""
+        n = C.c_int(0)
+        _check(lib().nrgk_fdm_signed_weights(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        _check(lib().nrgk_fdm_signed_weights(self._h, _dp(out), n.value, C.byref(n)))
+        return out
 
     def thermodynamics(self, lam, T, scale_offset=-1, first_cnt=0):
         T = np.ascontiguousarray(T, np.float64)

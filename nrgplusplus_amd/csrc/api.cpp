@@ -214,7 +214,7 @@ int nrgk_chain_eigenvalues_q(nrgk_chain* ch, double* out, size_t cap, size_t* n)
       NRGK_REQUIRE(cap >= (size_t)c.N_tot, NRGK_ERR_INVALID, "nrgk: output capacity too small");
       const double* src = c.phase == Chain::ASSEMBLED ? c.dlam_raw.as<double>() : c.dlam_shift[c.lcur].as<double>();
       d2h(out, src, c.N_tot, c.st);
-      NRGK_HIP(hipStreamSynchronize(c.st));
+      NRGK_HIP(stream_sync(c.st));
     }
   });
 }
@@ -238,7 +238,7 @@ int nrgk_chain_block_u(nrgk_chain* ch, int s, double* out, size_t cap) {
     const size_t sz = (size_t)c.cur_n[s] * c.cur_n[s];
     NRGK_REQUIRE(cap >= sz, NRGK_ERR_INVALID, "nrgk: output capacity too small");
     d2h(out, c.dU.as<double>() + c.cur_off[s], sz, c.st);
-    NRGK_HIP(hipStreamSynchronize(c.st));
+    NRGK_HIP(stream_sync(c.st));
   });
 }
 int nrgk_chain_keep_hamiltonians(nrgk_chain* ch, int on) {
@@ -281,7 +281,7 @@ int nrgk_chain_fdag_blocks(nrgk_chain* ch, int op, int* n_blocks, int* bi, int* 
         d2h(data + o, c.fbuf[c.fcur].as<double>() + ref.off, (size_t)ref.rows * ref.cols, c.st);
         o += (size_t)ref.rows * ref.cols;
       }
-      NRGK_HIP(hipStreamSynchronize(c.st));
+      NRGK_HIP(stream_sync(c.st));
     }
   });
 }
@@ -347,7 +347,7 @@ int nrgk_obs_blocks(nrgk_obs* obs, int op, int* n_blocks, int* bi, int* bj, int*
         d2h(data + off, o.buf[o.cur].as<double>() + ref.off, (size_t)ref.rows * ref.cols, o.ch->st);
         off += (size_t)ref.rows * ref.cols;
       }
-      NRGK_HIP(hipStreamSynchronize(o.ch->st));
+      NRGK_HIP(stream_sync(o.ch->st));
     }
   });
 }
@@ -386,6 +386,16 @@ int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out) {
     NRGK_REQUIRE(fdm && out, NRGK_ERR_INVALID, "null argument");
     const Fdm& f = *fdm->f;
     for (int i = 0; i < f.nbins; i++) out[i] = f.e_min * std::exp(i * 1. / f.delE);
+  });
+}
+int nrgk_fdm_signed_weights(nrgk_fdm* fdm, double* out, int cap, int* n_ops) {
+  return guard([&] {
+    NRGK_REQUIRE(fdm, NRGK_ERR_INVALID, "null argument");
+    const Fdm& f = *fdm->f;
+    NRGK_REQUIRE(!f.signed_tot.empty(), NRGK_ERR_STATE, "nrgk_fdm_signed_weights: no backward pass yet");
+    if (n_ops) *n_ops = (int)f.signed_tot.size();
+    if (out)
+      for (int p = 0; p < std::min<int>(cap, (int)f.signed_tot.size()); p++) out[p] = f.signed_tot[p];
   });
 }
 int nrgk_thermo(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S,
@@ -446,7 +456,7 @@ int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* w, int n, const d
     h2d(dom, omega, n_omega, nullptr);
     launch_broaden(dx, dw, n, dom, n_omega, b, dout, nullptr);
     d2h(out, dout, n_omega, nullptr);
-    NRGK_HIP(hipStreamSynchronize(nullptr));
+    NRGK_HIP(stream_sync(nullptr));
   });
 }
 
@@ -498,6 +508,14 @@ int nrgk_chain_step_stats(nrgk_chain* ch, double* ms4, int* sweeps, int* max_clu
       for (int i = 0; i < 4; i++) ms4[i] = c.ms[i];
     if (sweeps) *sweeps = c.sweeps;
     if (max_cluster) *max_cluster = c.unresolved ? -c.maxcluster : c.maxcluster;
+  });
+}
+
+int nrgk_chain_step_counts(nrgk_chain* ch, long long* launches, long long* syncs) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    if (launches) *launches = ch->c->last_launches;
+    if (syncs) *syncs = ch->c->last_syncs;
   });
 }
 

@@ -14,6 +14,11 @@
 
 namespace nrgk {
 
+ThreadCounts& thread_counts() {
+  static thread_local ThreadCounts tc;
+  return tc;
+}
+
 // ------------------------------------------------------------------ DevBuf
 void DevBuf::ensure(size_t bytes) {
   if (bytes <= cap) return;
@@ -124,12 +129,21 @@ void Chain::solve_blocks(const std::vector<int>& dims, const double* H, double* 
   }
   if (N_tot == 0) return;
   NRGK_HIP(hipSetDevice(ctx->device));
+  // LAPACKE_dsyevd(ROW_MAJOR, 'U') reads only the upper triangle of each block (qmatrix.hpp:390-391):
+  // it is mirrored into the lower one, which the batched solver also reads
+  std::vector<double> Hs(H, H + tot_elems);
+  for (int s = 0; s < S; s++) {
+    double* A = Hs.data() + cur_off[s];
+    const int n = dims[s];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < i; j++) A[(size_t)i * n + j] = A[(size_t)j * n + i];
+  }
   dH.ensure((size_t)tot_elems * sizeof(double));
-  NRGK_HIP(hipMemcpyAsync(dH.p, H, (size_t)tot_elems * sizeof(double), hipMemcpyHostToDevice, st));
-  if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
-  else eigensolve_mixed();
-  // a chain reports blocks that the refinement left short of its stopping point in its statistics;
-  // the stand-alone solver fails instead, as dsyevd reports a nonzero info
+  NRGK_HIP(hipMemcpyAsync(dH.p, Hs.data(), (size_t)tot_elems * sizeof(double), hipMemcpyHostToDevice, st));
+  fail_on_unconverged = true;
+  eigensolve();
+  // a chain re-solves blocks that the refinement left short of its stopping point; the stand-alone
+  // solver fails instead, as dsyevd reports a nonzero info
   NRGK_REQUIRE(mode == NRGK_MODE_PLAIN64 || oa_unconverged == 0, NRGK_ERR_NOCONV,
                "nrgk_eigh_batched: the FP64 refinement left " + std::to_string(oa_unconverged) +
                    " block(s) unconverged");
@@ -139,7 +153,7 @@ void Chain::solve_blocks(const std::vector<int>& dims, const double* H, double* 
 }
 
 void Chain::sync() {
-  NRGK_HIP(hipStreamSynchronize(st));
+  NRGK_HIP(stream_sync(st));
   stg.reset();
   prof_resolve();
 }
@@ -403,8 +417,23 @@ void Chain::assemble(const std::vector<double>& t, double rescale) {
   const AsmCon* d_cons = stg.place(rg, cons);
   stg.commit(rg, st);
   dH.ensure(std::max<long long>(1, tot_elems) * sizeof(double));
+  // algorithmic bytes: every element of the n x n blocks written once (8 B), every kept x kept
+  // f-dagger block and bath block a connection reads once
+  double by = 0;
+  if (prof_on) {
+    by = 8.0 * (double)tot_elems;
+    for (const auto& sec : secs)
+      for (int k = sec.con_begin; k < sec.con_end; k++) {
+        const AsmCon& c = cons[k];
+        const AsmSub& sk = subs[sec.sub_begin + c.k];
+        const AsmSub& skp = subs[sec.sub_begin + c.kp];
+        by += 8.0 * ((double)sk.kii * skp.kii + (double)sk.dbb * skp.dbb);
+      }
+  }
+  prof_begin("assemble");
   launch_assemble(d_secs, (int)secs.size(), row0, d_subs, d_cons, d_eprev, d_bathE.as<double>(),
                   fbuf[fcur].as<double>(), d_bathF.as<double>(), rescale, dH.as<double>(), st);
+  prof_end("assemble", 1, 0.0, by);
 }
 
 void Chain::run_gemms(std::vector<GemmProb>& probs) {
@@ -657,11 +686,12 @@ void Chain::eigensolve_mixed() {
   launch_normalize_sort(dblk, nb, d_tiles_blk, (int)n_tiles_blk, d_colg, n_colg, dW.as<float>(), d_sigma.as<float>(),
                         dX.as<double>(), dlam_tmp.as<double>(), d_cid.as<int>(), d_rrv.as<double>(),
                         d_rrv.as<double>() + std::max<long long>(1, N_tot), normalize_sort_lds(nmax_m), st);
-  // NRGK_DEBUG_FINITE=1: report the first stage and block that produces a non-finite value
+  // NRGK_DEBUG_FINITE=1: the first stage and block that produces a non-finite value fail the call
+  // with NRGK_ERR_NOCONV (the stage and block are in the message)
   const bool dbg = knob("NRGK_DEBUG_FINITE", 0) != 0;
   auto check = [&](const char* stage, const DevBuf& buf, bool is_float, bool per_block_vec) {
     if (!dbg) return;
-    NRGK_HIP(hipStreamSynchronize(st));
+    NRGK_HIP(stream_sync(st));
     for (auto& b : blks) {
       const size_t cnt = per_block_vec ? (size_t)b.n : (size_t)b.n * b.n;
       const size_t off = per_block_vec ? (size_t)b.e_off : (size_t)b.off;
@@ -669,37 +699,16 @@ void Chain::eigensolve_mixed() {
       if (is_float) {
         std::vector<float> f(cnt);
         d2h(f.data(), buf.as<float>() + off, cnt, st);
-        NRGK_HIP(hipStreamSynchronize(st));
+        NRGK_HIP(stream_sync(st));
         for (size_t i = 0; i < cnt; i++) v[i] = f[i];
       } else {
         d2h(v.data(), buf.as<double>() + off, cnt, st);
-        NRGK_HIP(hipStreamSynchronize(st));
+        NRGK_HIP(stream_sync(st));
       }
       for (double x : v)
-        if (!std::isfinite(x)) {
-          std::fprintf(stderr, "nrgk debug: non-finite after %s in block n=%d off=%lld\n", stage, b.n, b.off);
-          if (const char* path = std::getenv("NRGK_DEBUG_DUMP")) {
-            // raw dump: n, then S, R, E (n*n doubles each) and lam (n) of the block
-            FILE* f = std::fopen(path, "wb");
-            if (f) {
-              std::vector<double> S(b.n * b.n), R(b.n * b.n), E(b.n * b.n), lam(b.n);
-              d2h(S.data(), dS.as<double>() + b.off, S.size(), st);
-              d2h(R.data(), dR.as<double>() + b.off, R.size(), st);
-              d2h(E.data(), dE.as<double>() + b.off, E.size(), st);
-              d2h(lam.data(), dlam_tmp.as<double>() + b.e_off, lam.size(), st);
-              NRGK_HIP(hipStreamSynchronize(st));
-              const double nn = b.n;
-              std::fwrite(&nn, 8, 1, f);
-              std::fwrite(S.data(), 8, S.size(), f);
-              std::fwrite(R.data(), 8, R.size(), f);
-              std::fwrite(E.data(), 8, E.size(), f);
-              std::fwrite(lam.data(), 8, lam.size(), f);
-              std::fclose(f);
-            }
-            std::exit(3);
-          }
-          return;
-        }
+        if (!std::isfinite(x))
+          throw Error(NRGK_ERR_NOCONV, std::string("nrgk: non-finite value after ") + stage + " in block n=" +
+                                           std::to_string(b.n) + " off=" + std::to_string(b.off));
     }
   };
   check("shift_convert+jacobi (W)", dW, true, false);
@@ -811,6 +820,35 @@ void Chain::eigensolve_mixed() {
   maxcluster = hs[0];
   unresolved = hs[1];
   large_blocks_check(*this);
+  // A block the refinement left short of its stopping point is solved again in FP64 by the direct
+  // solver (the next Hamiltonian is built from these vectors, so they must meet the same bar as every
+  // other block's); nrgk_chain_refine_stats reports how many were.  The stand-alone nrgk_eigh_batched
+  // fails with NRGK_ERR_NOCONV instead (solve_blocks).
+  if (oa_unconverged && !fail_on_unconverged) {
+    std::vector<int> secs;
+    for (auto& a : act)
+      for (int s = 0; s < S; s++)
+        if (cur_off[s] == a.off && cur_n[s] == a.n) {
+          secs.push_back(s);
+          break;
+        }
+    solve_dense_fp64(*this, secs);
+  }
+}
+
+// The eigensolver of the current blocks in the chain's mode.  Whatever way it leaves (a status
+// thrown from any stage), the side streams that may still write the chain's buffers have drained.
+void Chain::eigensolve() {
+  try {
+    if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
+    else eigensolve_mixed();
+  } catch (...) {
+    for (int k = 0; k < N_SIDE; k++)
+      if (st_big[k]) (void)hipStreamSynchronize(st_big[k]);
+    (void)hipStreamSynchronize(st);
+    stg.reset();
+    throw;
+  }
 }
 
 void Chain::add_bath_site(const double* t, int nt, double rescale) {
@@ -818,6 +856,8 @@ void Chain::add_bath_site(const double* t, int nt, double rescale) {
                "nrgk: add_bath_site called twice without update_internal_state");
   NRGK_REQUIRE(nt == (int)imp.nops, NRGK_ERR_INVALID,
                std::string("t_hopping ") + std::to_string(nt) + " =! pre_fdag_oparator " + std::to_string(imp.nops));
+  step_l0 = thread_counts().launches;
+  step_s0 = thread_counts().syncs;
   if (cnt == 0 && phase == FRESH) import_impurity();
   // all_eigenvalue keeps the previous truncation's list until the next update_internal_state
   // (nrgcore.hpp:105-109, 269-327): when it has not been read yet, the raw eigenvalues it is sorted
@@ -842,8 +882,7 @@ void Chain::add_bath_site(const double* t, int nt, double rescale) {
     }
     sync();
   }
-  if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
-  else eigensolve_mixed();
+  eigensolve();
   NRGK_HIP(hipEventRecord(ev[2], st));
   sync();
   NRGK_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
@@ -890,6 +929,8 @@ void Chain::update_internal_state() {
   d_eprev = dlam_shift[lcur].as<double>();
   h_last_shift.clear();
   phase = TRUNCATED;
+  last_launches = thread_counts().launches - step_l0;
+  last_syncs = thread_counts().syncs - step_s0;
 }
 
 // f-dagger of the new site in the kept basis (nrgcore::set_current_fdag_operator, nrgcore.hpp:401-477)

@@ -124,6 +124,7 @@ struct Chain {
   DevBuf d_asm, d_eig, d_misc, d_rot, d_gemm;
   DevBuf d_flags, d_sigma_bits, d_sigma, d_stats, d_kept, d_trunc, d_conv, d_eig_act, d_cid, d_sidx, d_offmax, d_dgram, d_dgv;
   int oa_steps = 0, oa_unconverged = 0;
+  bool fail_on_unconverged = false;  // nrgk_eigh_batched: an unconverged block is an error
 
   // host mirrors
   std::vector<double> all_eig;
@@ -134,6 +135,9 @@ struct Chain {
 
   // diagnostics
   float ms[4] = {0, 0, 0, 0};
+  // launches and host synchronisations of the last step (add_bath_site to the end of
+  // update_internal_state, observable updates and records between them included)
+  long long step_l0 = 0, step_s0 = 0, last_launches = 0, last_syncs = 0;
   int sweeps = 0, maxcluster = 0, unresolved = 0;
   hipEvent_t ev[6];
   std::vector<std::vector<double>> h_keep;  // assembled blocks (keep_h)
@@ -201,6 +205,7 @@ struct Chain {
   void import_impurity();
   void create_next_basis();
   void assemble(const std::vector<double>& t, double rescale);
+  void eigensolve();
   void eigensolve_mixed();
   void eigensolve_plain();
   void add_bath_site(const double* t, int nt, double rescale);
@@ -222,5 +227,7 @@ void plain_destroy(Chain& c);
 void large_blocks_begin(Chain& c, const std::vector<int>& sectors);
 void large_blocks_join(Chain& c);
 void large_blocks_check(Chain& c);  // after a synchronisation: the dsyevd infos
+// FP64 dsyevd of the given sectors on the chain's stream (blocks the refinement left unconverged)
+void solve_dense_fp64(Chain& c, const std::vector<int>& sectors);
 
 }  // namespace nrgk

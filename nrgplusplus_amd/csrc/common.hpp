@@ -38,6 +38,24 @@ struct Error : std::runtime_error {
 
 void set_last_error(const std::string& m);
 
+// Kernel launches (the library's own kernels; a rocBLAS or rocSOLVER call counts once) and host
+// synchronisations made by the calling thread.  A chain is driven by one thread at a time, so the
+// difference of two readings around a step is that step's count.
+struct ThreadCounts {
+  long long launches = 0, syncs = 0;
+};
+ThreadCounts& thread_counts();
+#define NRGK_LAUNCH(...)                    \
+  do {                                      \
+    ::nrgk::thread_counts().launches++;     \
+    hipLaunchKernelGGL(__VA_ARGS__);        \
+  } while (0)
+// hipStreamSynchronize, counted
+inline hipError_t stream_sync(hipStream_t s) {
+  thread_counts().syncs++;
+  return hipStreamSynchronize(s);
+}
+
 // ------------------------------------------------------------------ device buffers
 // A device allocation that only grows; per-iteration scratch reuses it without hipMalloc in the
 // steady state.  Every allocation is charged to the owning context's capacity (HBM admission).

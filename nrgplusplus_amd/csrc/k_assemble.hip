@@ -187,7 +187,7 @@ void launch_assemble(const AsmSector* sectors, int n_sectors, int n_rows, const 
                      const double* e_prev, const double* e_bath, const double* f_prev, const double* f_bath,
                      double rescale, double* H, hipStream_t s) {
   if (n_rows <= 0) return;
-  hipLaunchKernelGGL(k_assemble, dim3(n_rows), dim3(256), 0, s, sectors, n_sectors, subs, cons, e_prev, e_bath,
+  NRGK_LAUNCH(k_assemble, dim3(n_rows), dim3(256), 0, s, sectors, n_sectors, subs, cons, e_prev, e_bath,
                      f_prev, f_bath, rescale, H);
   NRGK_HIP(hipGetLastError());
 }
@@ -195,10 +195,10 @@ void launch_assemble(const AsmSector* sectors, int n_sectors, int n_rows, const 
 void launch_shift_convert(const EigBlock* blocks, const int* row_block, const int* row_local, int n_rows,
                           const double* H, unsigned int* sigma_bits, float* sigma_out, float* W, hipStream_t s) {
   if (n_rows <= 0) return;
-  hipLaunchKernelGGL(k_gershgorin, dim3((n_rows + GG_ROWS - 1) / GG_ROWS), dim3(256), 0, s, blocks, row_block, row_local,
+  NRGK_LAUNCH(k_gershgorin, dim3((n_rows + GG_ROWS - 1) / GG_ROWS), dim3(256), 0, s, blocks, row_block, row_local,
                      n_rows, H, sigma_bits);
   NRGK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_to_fp32_shift, dim3(n_rows), dim3(256), 0, s, blocks, row_block, row_local, H, sigma_bits,
+  NRGK_LAUNCH(k_to_fp32_shift, dim3(n_rows), dim3(256), 0, s, blocks, row_block, row_local, H, sigma_bits,
                      sigma_out, W);
   NRGK_HIP(hipGetLastError());
 }

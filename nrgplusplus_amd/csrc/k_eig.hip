@@ -1283,7 +1283,7 @@ __global__ void __launch_bounds__(256) k_copy_converged(const EigBlock* __restri
 void launch_copy_converged(const EigBlock* blocks, int nb, const double* conv, double e_tol, double d_tol,
                            double e_pred, double d_pred, const double* src, double* dst, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_copy_converged, dim3(nb * 8), dim3(256), 0, s, blocks, conv, e_tol, d_tol, e_pred, d_pred, src,
+  NRGK_LAUNCH(k_copy_converged, dim3(nb * 8), dim3(256), 0, s, blocks, conv, e_tol, d_tol, e_pred, d_pred, src,
                      dst);
   NRGK_HIP(hipGetLastError());
 }
@@ -1371,7 +1371,7 @@ size_t final_sort_lds(int nmax) {
 void launch_jacobi_round(const JacTask* tasks, int n_tasks, float* W, int round, int* flags, double* dgram, int* dgv,
                          float tol, float ang_tol, int max_inner, double jac_far, hipStream_t s) {
   if (n_tasks <= 0) return;
-  hipLaunchKernelGGL(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CS * sizeof(float), s, tasks, W,
+  NRGK_LAUNCH(k_jacobi_round, dim3(n_tasks), dim3(JAC_THREADS), JAC_W * JAC_CS * sizeof(float), s, tasks, W,
                      round, flags, dgram, dgv, tol, ang_tol, max_inner, jac_far * jac_far);
   NRGK_HIP(hipGetLastError());
 }
@@ -1379,12 +1379,12 @@ void launch_normalize_sort(const EigBlock* blocks, int nb, const int4* tiles, in
                            int n_groups, const float* W, const float* sigma, double* X, double* lam, int* perm,
                            double* nrm_s, double* nrm_g, size_t lds, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_col_norms, dim3(n_groups), dim3(256), 0, s, blocks, groups, W, nrm_g);
+  NRGK_LAUNCH(k_col_norms, dim3(n_groups), dim3(256), 0, s, blocks, groups, W, nrm_g);
   NRGK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, nrm_g, sigma, lam, perm, nrm_s);
+  NRGK_LAUNCH(k_normalize_sort, dim3(nb), dim3(1024), lds, s, blocks, nrm_g, sigma, lam, perm, nrm_s);
   NRGK_HIP(hipGetLastError());
   if (n_tiles <= 0) return;
-  hipLaunchKernelGGL(k_gather_x, dim3(n_tiles), dim3(256), 0, s, blocks, tiles, W, perm, nrm_s, X);
+  NRGK_LAUNCH(k_gather_x, dim3(n_tiles), dim3(256), 0, s, blocks, tiles, W, perm, nrm_s, X);
   NRGK_HIP(hipGetLastError());
 }
 void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym, const int4* all_tiles, int n_all,
@@ -1392,26 +1392,26 @@ void launch_oa(const EigBlock* blocks, int nb, const int4* sym_tiles, int n_sym,
                double* conv, double* scrW, double* scrG, double* scrT, double* rrv, int nmax, double e_tol,
                double d_tol, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_oa_sym, dim3(n_sym), dim3(256), 0, s, blocks, sym_tiles, S, Rp, offmax, conv);
+  NRGK_LAUNCH(k_oa_sym, dim3(n_sym), dim3(256), 0, s, blocks, sym_tiles, S, Rp, offmax, conv);
   NRGK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_oa_prep, dim3(nb), dim3(1024), oa_prep_lds(nmax), s, blocks, S, Rp, offmax, lam, cid, sidx,
+  NRGK_LAUNCH(k_oa_prep, dim3(nb), dim3(1024), oa_prep_lds(nmax), s, blocks, S, Rp, offmax, lam, cid, sidx,
                      stats, conv);
   NRGK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_oa_entries, dim3(n_all), dim3(256), 0, s, blocks, all_tiles, S, Rp, lam, cid, E, conv);
+  NRGK_LAUNCH(k_oa_entries, dim3(n_all), dim3(256), 0, s, blocks, all_tiles, S, Rp, lam, cid, E, conv);
   NRGK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_oa_rr, dim3(nb), dim3(1024), oa_rr_lds(nmax), s, blocks, S, Rp, E, lam, cid, sidx, scrW, scrG,
+  NRGK_LAUNCH(k_oa_rr, dim3(nb), dim3(1024), oa_rr_lds(nmax), s, blocks, S, Rp, E, lam, cid, sidx, scrW, scrG,
                      scrT, rrv, conv, e_tol, d_tol);
   NRGK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_oa_mask, dim3(n_all), dim3(256), 0, s, blocks, all_tiles, cid, conv, e_tol, d_tol, E);
+  NRGK_LAUNCH(k_oa_mask, dim3(n_all), dim3(256), 0, s, blocks, all_tiles, cid, conv, e_tol, d_tol, E);
   NRGK_HIP(hipGetLastError());
 }
 void launch_final_sort(const EigBlock* blocks, int nb, const int4* tiles, int n_tiles, const double* X,
                        const double* lam_in, double* U, double* lam_out, int* perm, size_t lds, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_final_sort, dim3(nb), dim3(1024), lds, s, blocks, lam_in, lam_out, perm);
+  NRGK_LAUNCH(k_final_sort, dim3(nb), dim3(1024), lds, s, blocks, lam_in, lam_out, perm);
   NRGK_HIP(hipGetLastError());
   if (n_tiles <= 0) return;
-  hipLaunchKernelGGL(k_gather_u, dim3(n_tiles), dim3(256), 0, s, blocks, tiles, X, perm, U);
+  NRGK_LAUNCH(k_gather_u, dim3(n_tiles), dim3(256), 0, s, blocks, tiles, X, perm, U);
   NRGK_HIP(hipGetLastError());
 }
 void set_kernel_lds_limits(int bytes) {

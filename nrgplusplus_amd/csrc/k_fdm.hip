@@ -20,7 +20,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_fdm_bin(const FdmBinItem* __res
                                                          const double* __restrict__ E, const double* __restrict__ bf,
                                                          double scale, double e_min, double delE, int nbins,
                                                          double* __restrict__ pos, double* __restrict__ neg,
-                                                         double* __restrict__ spec_sum) {
+                                                         double* __restrict__ spec_sum, double* __restrict__ signed_sum) {
   extern __shared__ double hist[];  // [2*nbins] when nbins is small enough, else unused
   const FdmBinItem it = items[blockIdx.x];
   const bool lds = nbins > 0 && (size_t)nbins * 2 * sizeof(double) <= 65536;
@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_fdm_bin(const FdmBinItem* __res
   __syncthreads();
   double* gp = pos + (size_t)it.op * nbins;
   double* gn = neg + (size_t)it.op * nbins;
-  double ssum = 0.0;
+  double ssum = 0.0, sgn = 0.0;  // |w| added to the bins; signed w+ + w- of every pair (sum rule)
   const long long tot = (long long)it.rows * it.nj;
   for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
     const int v = it.row0 + (int)(e / it.nj), vp = (int)(e % it.nj);
@@ -45,6 +45,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_fdm_bin(const FdmBinItem* __res
       ar = v < it.kci ? AR[it.ar_off + (long long)v * it.nj + vp] : 0.0;
     }
     const double aa = ra * b, bbv = ar * b;
+    sgn += aa + bbv;
     const int k = (int)(log(1.0 + (fabs(scale * (E[it.ei + v] - E[it.ej + vp])) / e_min)) * delE);
     if (k < nbins && k >= 0) {
       if (fabs(aa) > 1e-20) {
@@ -65,8 +66,12 @@ __global__ void __launch_bounds__(BIN_THREADS) k_fdm_bin(const FdmBinItem* __res
       if (hist[b] != 0.0) atomicAdd(&gp[b], hist[b]);
       if (hist[nbins + b] != 0.0) atomicAdd(&gn[b], hist[nbins + b]);
     }
-  for (int o = 32; o > 0; o >>= 1) ssum += __shfl_xor(ssum, o, 64);
+  for (int o = 32; o > 0; o >>= 1) {
+    ssum += __shfl_xor(ssum, o, 64);
+    sgn += __shfl_xor(sgn, o, 64);
+  }
   if ((threadIdx.x & 63) == 0 && ssum != 0.0) atomicAdd(spec_sum, ssum);
+  if ((threadIdx.x & 63) == 0 && sgn != 0.0) atomicAdd(&signed_sum[it.op], sgn);
 }
 
 // reducedRho[ii](it,itp) += sum_il rho_i[kidx + it + kii*il, kidx + itp + kii*il], contributions taken
@@ -192,46 +197,46 @@ __global__ void __launch_bounds__(256) k_transpose(const double* __restrict__ in
 
 void launch_fdm_bin(const void* items, int n_items, const double* B, const double* RA, const double* AR,
                     const double* E, const double* bf, double scale, double e_min, double delE, int nbins, double* pos,
-                    double* neg, double* spec_sum, hipStream_t s) {
+                    double* neg, double* spec_sum, double* signed_sum, hipStream_t s) {
   if (n_items <= 0) return;
   const size_t lds = (size_t)nbins * 2 * sizeof(double) <= 65536 ? (size_t)nbins * 2 * sizeof(double) : 0;
-  hipLaunchKernelGGL(k_fdm_bin, dim3(n_items), dim3(BIN_THREADS), lds, s, (const FdmBinItem*)items, B, RA, AR, E, bf,
-                     scale, e_min, delE, nbins, pos, neg, spec_sum);
+  NRGK_LAUNCH(k_fdm_bin, dim3(n_items), dim3(BIN_THREADS), lds, s, (const FdmBinItem*)items, B, RA, AR, E, bf,
+                     scale, e_min, delE, nbins, pos, neg, spec_sum, signed_sum);
   NRGK_HIP(hipGetLastError());
 }
 void launch_partial_trace(const void* items, int n_items, const void* cons, const double* rho, double* red,
                           hipStream_t s) {
   if (n_items <= 0) return;
-  hipLaunchKernelGGL(k_partial_trace, dim3(n_items), dim3(256), 0, s, (const PtItem*)items, (const PtContrib*)cons, rho,
+  NRGK_LAUNCH(k_partial_trace, dim3(n_items), dim3(256), 0, s, (const PtItem*)items, (const PtContrib*)cons, rho,
                      red);
   NRGK_HIP(hipGetLastError());
 }
 void launch_thermo(const double* E, const int* shell_off, const double* gs, int n_shells, const double* T, int nT,
                    double* z, double* m1, double* m2, hipStream_t s) {
   if (nT <= 0 || n_shells <= 0) return;
-  hipLaunchKernelGGL(k_thermo, dim3(nT), dim3(256), 0, s, E, shell_off, gs, n_shells, T, z, m1, m2);
+  NRGK_LAUNCH(k_thermo, dim3(nT), dim3(256), 0, s, E, shell_off, gs, n_shells, T, z, m1, m2);
   NRGK_HIP(hipGetLastError());
 }
 void launch_thermo_ops(const double* E, const double* Q, long long q_stride, int nq, const int* shell_off,
                        const double* gs, int n_shells, const double* T, int nT, double* out, hipStream_t s) {
   if (nT <= 0 || n_shells <= 0 || nq <= 0) return;
-  hipLaunchKernelGGL(k_thermo_ops, dim3(nT, nq), dim3(256), 0, s, E, Q, q_stride, shell_off, gs, n_shells, T, out);
+  NRGK_LAUNCH(k_thermo_ops, dim3(nT, nq), dim3(256), 0, s, E, Q, q_stride, shell_off, gs, n_shells, T, out);
   NRGK_HIP(hipGetLastError());
 }
 void launch_diag_gather(const DiagItem* items, int n_items, const double* O, double* out, hipStream_t s) {
   if (n_items <= 0) return;
-  hipLaunchKernelGGL(k_diag_gather, dim3(n_items), dim3(256), 0, s, items, O, out);
+  NRGK_LAUNCH(k_diag_gather, dim3(n_items), dim3(256), 0, s, items, O, out);
   NRGK_HIP(hipGetLastError());
 }
 void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,
                     hipStream_t s) {
   if (no <= 0) return;
-  hipLaunchKernelGGL(k_broaden, dim3((no + 255) / 256), dim3(256), 0, s, x, w, n, omega, no, b, out);
+  NRGK_LAUNCH(k_broaden, dim3((no + 255) / 256), dim3(256), 0, s, x, w, n, omega, no, b, out);
   NRGK_HIP(hipGetLastError());
 }
 void launch_transpose(const double* in, double* out, int n, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, s, in, out, n);
+  NRGK_LAUNCH(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, s, in, out, n);
   NRGK_HIP(hipGetLastError());
 }
 size_t fdm_bin_item_size() { return sizeof(FdmBinItem); }

@@ -168,7 +168,7 @@ int prepare_gemm(std::vector<GemmProb>& probs) {
 
 void launch_gemm(const GemmProb* d_probs, int n_probs, int tiles, hipStream_t s) {
   if (tiles <= 0 || n_probs <= 0) return;
-  hipLaunchKernelGGL(k_gemm64, dim3(tiles), dim3(256), 0, s, d_probs, n_probs);
+  NRGK_LAUNCH(k_gemm64, dim3(tiles), dim3(256), 0, s, d_probs, n_probs);
   NRGK_HIP(hipGetLastError());
 }
 

@@ -57,12 +57,12 @@ __global__ void __launch_bounds__(256) k_obs_scatter(const ObsSeg* __restrict__ 
 void launch_fdag_gather(const RotGroup* g, int ng, const RotSegment* s, const RotTri* t, const double* U,
                         const double* fbath, double* Y, hipStream_t st) {
   if (ng <= 0) return;
-  hipLaunchKernelGGL(k_fdag_gather, dim3(ng), dim3(256), 0, st, g, s, t, U, fbath, Y);
+  NRGK_LAUNCH(k_fdag_gather, dim3(ng), dim3(256), 0, st, g, s, t, U, fbath, Y);
   NRGK_HIP(hipGetLastError());
 }
 void launch_obs_scatter(const ObsSeg* s, int ns, const double* Oprev, double* C, hipStream_t st) {
   if (ns <= 0) return;
-  hipLaunchKernelGGL(k_obs_scatter, dim3(ns), dim3(256), 0, st, s, Oprev, C);
+  NRGK_LAUNCH(k_obs_scatter, dim3(ns), dim3(256), 0, st, s, Oprev, C);
   NRGK_HIP(hipGetLastError());
 }
 

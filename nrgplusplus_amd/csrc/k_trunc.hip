@@ -143,7 +143,7 @@ void set_trunc_lds_limit() {
 void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
                      double errbar, int* kept, TruncOut* out, double* lam_shift, hipStream_t s) {
   const size_t lds = N_tot <= TRUNC_LDS ? (size_t)std::max<long long>(1, N_tot) * sizeof(double) : 0;
-  hipLaunchKernelGGL(k_truncate, dim3(1), dim3(1024), lds, s, bn, beoff, S, lam, N_tot, K, errbar, kept, out, lam_shift);
+  NRGK_LAUNCH(k_truncate, dim3(1), dim3(1024), lds, s, bn, beoff, S, lam, N_tot, K, errbar, kept, out, lam_shift);
   NRGK_HIP(hipGetLastError());
 }
 

@@ -41,7 +41,7 @@ void launch_obs_scatter(const ObsSeg* s, int ns, const double* Oprev, double* C,
 // k_fdm.hip
 void launch_fdm_bin(const void* items, int n_items, const double* B, const double* RA, const double* AR,
                     const double* E, const double* bf, double scale, double e_min, double delE, int nbins, double* pos,
-                    double* neg, double* spec_sum, hipStream_t s);
+                    double* neg, double* spec_sum, double* signed_sum, hipStream_t s);
 void launch_partial_trace(const void* items, int n_items, const void* cons, const double* rho, double* red,
                           hipStream_t s);
 void launch_thermo(const double* E, const int* shell_off, const double* gs, int n_shells, const double* T, int nT,

@@ -36,7 +36,7 @@ Obs::Obs(Chain* c, int n_ops, const int* nblocks, const int* bi, const int* bj, 
     }
   buf[cur].ensure(std::max<size_t>(1, f.size()) * sizeof(double));
   h2d(buf[cur].as<double>(), f.data(), f.size(), ch->st);
-  NRGK_HIP(hipStreamSynchronize(ch->st));
+  NRGK_HIP(stream_sync(ch->st));
 }
 
 void Obs::update() {
@@ -229,7 +229,7 @@ void Fdm::save(const std::string& path, const std::string& op_group) const {
       hO.resize(std::max<long long>(1, tot));
       d2h(hO.data(), sh.dO.as<double>(), tot, c.st);
     }
-    NRGK_HIP(hipStreamSynchronize(c.st));
+    NRGK_HIP(stream_sync(c.st));
     for (int i = 0; i < S; i++) {
       const std::string si = std::to_string(i);
       if (sh.dU.p)
@@ -280,7 +280,9 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
   const int nops = obs->nops;
   dpos.ensure((size_t)nops * nbins * sizeof(double));
   dneg.ensure((size_t)nops * nbins * sizeof(double));
-  dsum.ensure(sizeof(double) * 2);
+  // [0]: this shell's sum of |w|; [1 + p]: operator p's signed weights over all shells
+  dsum.ensure(sizeof(double) * (1 + nops));
+  NRGK_HIP(hipMemsetAsync(dsum.p, 0, sizeof(double) * (1 + nops), st));
   NRGK_HIP(hipMemsetAsync(dpos.p, 0, (size_t)nops * nbins * sizeof(double), st));
   NRGK_HIP(hipMemsetAsync(dneg.p, 0, (size_t)nops * nbins * sizeof(double), st));
   std::vector<Shell*> use;
@@ -324,7 +326,7 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
       for (int i = 0; i < S; i++) tot += (long long)kc[i] * kc[i];
       std::vector<double> hred(tot);
       d2h(hred.data(), dred[rcur].as<double>(), tot, st);
-      NRGK_HIP(hipStreamSynchronize(st));
+      NRGK_HIP(stream_sync(st));
       for (int i = 0; i < S; i++)
         for (int a = 0; a < kc[i]; a++) trace += hred[red_off[i] + (long long)a * kc[i] + a];
     }
@@ -387,9 +389,15 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
     NRGK_HIP(hipMemsetAsync(dsum.p, 0, sizeof(double), st));
     ditems.ensure(std::max<size_t>(1, items.size()) * sizeof(FdmBinItem));
     h2d(ditems.as<FdmBinItem>(), items.data(), items.size(), st);
+    // algorithmic bytes (the binning's roofline): B, rhoA and ARho read once per (v, v') pair
+    double bin_bytes = 0;
+    if (c.prof_on)
+      for (const auto& it : items) bin_bytes += 24.0 * (double)it.rows * it.nj;
+    c.prof_begin("fdm_bin");
     launch_fdm_bin(ditems.p, (int)items.size(), sh.dO.as<double>(), dRA.as<double>(), dAR.as<double>(),
                    sh.dE.as<double>(), last ? dbf.as<double>() : nullptr, scale, e_min, delE, nbins, dpos.as<double>(),
-                   dneg.as<double>(), dsum.as<double>(), st);
+                   dneg.as<double>(), dsum.as<double>(), dsum.as<double>() + 1, st);
+    c.prof_end("fdm_bin", 1, 0.0, bin_bytes);
     double ssum = 0;
     d2h(&ssum, dsum.as<double>(), 1, st);
     // reduced density matrix (setReduceDensityMatrix, fdmSpectrum.hpp:71-114):
@@ -472,7 +480,7 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
     h2d((PtItem*)dpt.p, pit.data(), pit.size(), st);
     h2d((PtContrib*)(dpt.as<char>() + o1), pcs.data(), pcs.size(), st);
     launch_partial_trace(dpt.p, (int)pit.size(), dpt.as<char>() + o1, drho.as<double>(), dred[rnext].as<double>(), st);
-    NRGK_HIP(hipStreamSynchronize(st));
+    NRGK_HIP(stream_sync(st));
     red_off = new_off;
     rcur = rnext;
     if (ns < cap) {
@@ -483,7 +491,9 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
   }
   d2h(pos, dpos.as<double>(), (size_t)nops * nbins, st);
   d2h(neg, dneg.as<double>(), (size_t)nops * nbins, st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  signed_tot.assign(nops, 0.0);
+  d2h(signed_tot.data(), dsum.as<double>() + 1, nops, st);
+  NRGK_HIP(stream_sync(st));
   if (n_shells) *n_shells = ns;
 }
 
@@ -554,7 +564,7 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
   d2h(z.data(), dz, nz, st);
   d2h(m1.data(), dm1, nz, st);
   d2h(m2.data(), dm2, nz, st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  NRGK_HIP(stream_sync(st));
   const int nrgMax = c.cnt;
   for (int t = 0; t < nT; t++) {
     const double kT = T[t];

@@ -52,6 +52,8 @@ struct Fdm {
   int nbins = 0;
   double e_min = 1e-10, e_max = 10, delE = 0;
   std::vector<std::unique_ptr<Shell>> shells;
+  // per operator, the signed weights w+ + w- of the last backward pass summed over its shells
+  std::vector<double> signed_tot;
   // shells live in large chunks, filled in order: a record allocates nothing once a chunk has room
   std::vector<std::unique_ptr<DevBuf>> chunks;
   size_t chunk_used = 0;

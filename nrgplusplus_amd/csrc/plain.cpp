@@ -24,15 +24,15 @@ static rocblas_handle plain_handle(Chain& c) {
 }
 
 void plain_destroy(Chain& c) {
+  if (c.st) (void)hipStreamSynchronize(c.st);
   if (c.rocblas) rocblas_destroy_handle((rocblas_handle)c.rocblas);
   c.rocblas = nullptr;
   for (int k = 0; k < Chain::N_SIDE; k++) {
+    // the side stream drains before its handle goes
+    if (c.st_big[k]) (void)hipStreamSynchronize(c.st_big[k]);
     if (c.rocblas_big[k]) rocblas_destroy_handle((rocblas_handle)c.rocblas_big[k]);
     c.rocblas_big[k] = nullptr;
-    if (c.st_big[k]) {
-      (void)hipStreamSynchronize(c.st_big[k]);
-      (void)hipStreamDestroy(c.st_big[k]);
-    }
+    if (c.st_big[k]) (void)hipStreamDestroy(c.st_big[k]);
     c.st_big[k] = nullptr;
     if (c.ev_big_done[k]) (void)hipEventDestroy(c.ev_big_done[k]);
     c.ev_big_done[k] = nullptr;
@@ -50,12 +50,13 @@ void plain_gemms(Chain& c, const std::vector<GemmProb>& probs) {
       NRGK_HIP(hipMemcpy2DAsync(p.C, (size_t)p.ldc * sizeof(double), p.C0, (size_t)p.ldc0 * sizeof(double),
                                 (size_t)p.n * sizeof(double), p.m, hipMemcpyDeviceToDevice, c.st));
     const double alpha = p.alpha, beta = p.beta;
+    thread_counts().launches++;
     rocblas_status s = rocblas_dgemm(h, p.transB ? rocblas_operation_transpose : rocblas_operation_none,
                                      p.transA ? rocblas_operation_transpose : rocblas_operation_none, p.n, p.m, p.k,
                                      &alpha, p.B, p.ldb, p.A, p.lda, &beta, p.C, p.ldc);
     NRGK_REQUIRE(s == rocblas_status_success, NRGK_ERR_HIP, "rocblas_dgemm failed");
   }
-  NRGK_HIP(hipStreamSynchronize(c.st));
+  NRGK_HIP(stream_sync(c.st));
 }
 
 void Chain::eigensolve_plain() {
@@ -73,16 +74,18 @@ void Chain::eigensolve_plain() {
     if (n == 0) continue;
     double* A = dX.as<double>() + cur_off[s];
     NRGK_HIP(hipMemcpyAsync(A, dH.as<double>() + cur_off[s], (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    // symmetric input: the column-major view is the same matrix; eigenvectors come back as columns
-    // of the column-major result, i.e. rows of the row-major buffer
-    rocblas_status r = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n,
+    // the column-major view of the row-major block is its transpose: its lower triangle is the
+    // row-major upper triangle that LAPACKE_dsyevd(ROW_MAJOR, 'U') reads (qmatrix.hpp:390-391).
+    // Eigenvectors come back as columns of the column-major result, i.e. rows of the row-major buffer
+    thread_counts().launches++;
+    rocblas_status r = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, A, n,
                                         dlam_raw.as<double>() + cur_eoff[s], dlam_tmp.as<double>(), dinfo + s);
     NRGK_REQUIRE(r == rocblas_status_success, NRGK_ERR_HIP, "rocsolver_dsyevd failed");
     launch_transpose(A, dU.as<double>() + cur_off[s], n, st);
   }
   std::vector<int> info(cur_qn.size());
   d2h(info.data(), dinfo, info.size(), st);
-  NRGK_HIP(hipStreamSynchronize(st));
+  NRGK_HIP(stream_sync(st));
   for (int v : info) NRGK_REQUIRE(v == 0, NRGK_ERR_NOCONV, "rocsolver_dsyevd: no convergence");
   sweeps = 0;
   maxcluster = 0;
@@ -139,9 +142,11 @@ void large_blocks_begin(Chain& c, const std::vector<int>& sectors) {
       double* A = c.dX.as<double>() + c.cur_off[s];
       NRGK_HIP(hipMemcpyAsync(A, c.dH.as<double>() + c.cur_off[s], (size_t)n * n * sizeof(double),
                               hipMemcpyDeviceToDevice, c.st_big[k]));
-      // symmetric input: the column-major view is the same matrix; the eigenvectors come back in
-      // the rows of the row-major buffer and are transposed into U's columns
-      rocblas_status r = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n,
+      // the row-major upper triangle (rocBLAS's column-major lower one) is read, as
+      // LAPACKE_dsyevd(ROW_MAJOR, 'U') does; the eigenvectors come back in the rows of the
+      // row-major buffer and are transposed into U's columns
+      thread_counts().launches++;
+    rocblas_status r = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, A, n,
                                           c.dlam_raw.as<double>() + c.cur_eoff[s], c.d_big_work[k].as<double>(),
                                           c.d_big_info.as<int>() + slot++);
       NRGK_REQUIRE(r == rocblas_status_success, NRGK_ERR_HIP, "rocsolver_dsyevd failed");
@@ -149,6 +154,35 @@ void large_blocks_begin(Chain& c, const std::vector<int>& sectors) {
     }
     NRGK_HIP(hipEventRecord(c.ev_big_done[k], c.st_big[k]));
   }
+}
+
+// FP64 dsyevd of the given sectors on the chain's own stream (the blocks the batched refinement left
+// unconverged): eigenvalues into dlam_raw, eigenvectors into U's columns, as the batched path
+// writes them.  Returns after the stream has drained and the infos are checked.
+void solve_dense_fp64(Chain& c, const std::vector<int>& sectors) {
+  if (sectors.empty()) return;
+  rocblas_handle h = plain_handle(c);
+  int nmax = 1;
+  for (int s : sectors) nmax = std::max(nmax, c.cur_n[s]);
+  c.dlam_tmp.ensure(((size_t)std::max<long long>(1, c.N_tot) + nmax + 64) * sizeof(double));
+  c.d_stats.ensure(std::max<size_t>(4, sectors.size()) * sizeof(int));
+  int* dinfo = c.d_stats.as<int>();
+  NRGK_HIP(hipMemsetAsync(dinfo, 0, sectors.size() * sizeof(int), c.st));
+  for (size_t k = 0; k < sectors.size(); k++) {
+    const int s = sectors[k], n = c.cur_n[s];
+    double* A = c.dX.as<double>() + c.cur_off[s];
+    NRGK_HIP(hipMemcpyAsync(A, c.dH.as<double>() + c.cur_off[s], (size_t)n * n * sizeof(double),
+                            hipMemcpyDeviceToDevice, c.st));
+    thread_counts().launches++;
+    rocblas_status r = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, n, A, n,
+                                        c.dlam_raw.as<double>() + c.cur_eoff[s], c.dlam_tmp.as<double>(), dinfo + k);
+    NRGK_REQUIRE(r == rocblas_status_success, NRGK_ERR_HIP, "rocsolver_dsyevd failed");
+    launch_transpose(A, c.dU.as<double>() + c.cur_off[s], n, c.st);
+  }
+  std::vector<int> info(sectors.size());
+  d2h(info.data(), dinfo, info.size(), c.st);
+  NRGK_HIP(stream_sync(c.st));
+  for (int v : info) NRGK_REQUIRE(v == 0, NRGK_ERR_NOCONV, "rocsolver_dsyevd: no convergence");
 }
 
 void large_blocks_join(Chain& c) {
@@ -161,7 +195,7 @@ void large_blocks_check(Chain& c) {
   if (!c.n_big) return;
   std::vector<int> info(c.n_big);
   d2h(info.data(), c.d_big_info.as<int>(), info.size(), c.st);
-  NRGK_HIP(hipStreamSynchronize(c.st));
+  NRGK_HIP(stream_sync(c.st));
   for (int v : info) NRGK_REQUIRE(v == 0, NRGK_ERR_NOCONV, "rocsolver_dsyevd: no convergence");
 }
 

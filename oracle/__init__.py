@@ -90,8 +90,11 @@ def lib():
             "orc_run_thermo_t": (i, [C.POINTER(OrcDriver), dp, i, dp, i, dp, dp]),
             "orc_run_thermo_ops": (i, [C.POINTER(OrcDriver), i, dp, i, i, dp, dp, dp, ip]),
             "orc_run_fdm": (i, [C.POINTER(OrcDriver), i, dp, dp, dp, dp, ip]),
+            "orc_run_fdm_signed": (i, [C.POINTER(OrcDriver), i, dp, dp, dp, dp, ip, dp]),
+            "orc_set_fdm_plant": (None, [i]),
             "orc_broaden": (None, [dp, dp, i, dp, i, d, dp]),
             "orc_time_forward": (i, [C.POINTER(OrcDriver), i, i, dp]),
+            "orc_time_rgflow": (i, [C.POINTER(OrcDriver), i, i, dp]),
             "orc_time_forward2": (i, [C.POINTER(OrcDriver), i, i, i, dp, dp]),
             "orc_hopping": (d, [i, d]),
         }
@@ -324,16 +327,35 @@ def run_thermo_ops(drv, T, min_iter0, op_kind):
     return S, Cv, V[: nq.value * len(T)].reshape(nq.value, len(T))
 
 
-def run_fdm(drv, nbins):
+def run_fdm(drv, nbins, signed=False):
+    """(pos, neg, traces, sums) per operator / shell; with signed=True also the per-operator signed
+    weights summed over all shells (before |.|, the 1e-20 cut and the bin range)"""
     nops = 2
     pos = np.zeros(nops * nbins)
     neg = np.zeros(nops * nbins)
     tr = np.zeros(drv.nsites + 2)
     sums = np.zeros(drv.nsites + 2)
+    sg = np.zeros(nops)
     ns = C.c_int(0)
-    _check(lib().orc_run_fdm(C.byref(drv), nbins, _dp(pos), _dp(neg), _dp(tr), _dp(sums), C.byref(ns)))
+    _check(lib().orc_run_fdm_signed(C.byref(drv), nbins, _dp(pos), _dp(neg), _dp(tr), _dp(sums), C.byref(ns),
+                                    _dp(sg)))
     n = ns.value
-    return pos.reshape(nops, nbins), neg.reshape(nops, nbins), tr[:n], sums[:n]
+    out = (pos.reshape(nops, nbins), neg.reshape(nops, nbins), tr[:n], sums[:n])
+    return out + (sg,) if signed else out
+
+
+def set_fdm_plant(k):
+    """plant a mistake in the oracle's FDM backward pass (tests of the tests only): 0 none, 1 rho
+    rotated as U^T rho U, 2 the last bath state dropped from the partial trace"""
+    lib().orc_set_fdm_plant(int(k))
+
+
+def time_rgflow(drv, n_skip, n_time):
+    """Seconds spent in loop sites [n_skip, n_skip + n_time) of the RG-flow driver form (first site
+    {V..V}, then add_bath_site + update_internal_state per site)."""
+    s = C.c_double(0)
+    _check(lib().orc_time_rgflow(C.byref(drv), n_skip, n_time, C.byref(s)))
+    return s.value
 
 
 def time_forward(drv, n_skip, n_time):

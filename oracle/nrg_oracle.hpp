@@ -686,6 +686,13 @@ struct FdmSpectrum {
   std::vector<Mat> rhoZero, reducedRho;
   std::vector<std::vector<double>> positiveWeight, negativeWeight;
   std::vector<double> rhoTraces, specSums;
+  // sum over all shells of the signed weights w+ + w- of each operator, taken before the |.|, the
+  // 1e-20 cut and the bin-range test of rhoDotOperators: over the complete basis of discarded states
+  // it is <{B, B+}> (the FDM sum rule), which pins the backward pass (not in the reference)
+  std::vector<double> signedWeight;
+  // planted mistakes for the oracle's own tests (0: none; 1: rho rotated as U^T rho U; 2: the last
+  // bath state dropped from the partial trace); never set outside tests
+  int plant = 0;
   size_t nops = 0;
 
   explicit FdmSpectrum(size_t n_ops, int pts = 100000) : energyPts(pts) {
@@ -693,6 +700,7 @@ struct FdmSpectrum {
     nops = n_ops;
     positiveWeight.assign(n_ops, std::vector<double>(energyPts, 0.0));
     negativeWeight.assign(n_ops, std::vector<double>(energyPts, 0.0));
+    signedWeight.assign(n_ops, 0.0);
   }
 
   // calcSpectrum (fdmSpectrum.hpp:55-63)
@@ -762,6 +770,7 @@ struct FdmSpectrum {
             for (size_t vp = 0; vp < nj; vp++) {
               double aa = rhoA(v, vp) * A(vp, v);
               double bbv = ARho(v, vp) * A(vp, v);
+              signedWeight[ip] += aa + bbv;
               int k = int(std::log(1 + (std::fabs(energyRescale * (sh.eigenvaluesQ[i][v] - sh.eigenvaluesQ[j][vp])) /
                                          minEnergy)) * delE);
               if (k < energyPts && k >= 0) {
@@ -786,7 +795,8 @@ struct FdmSpectrum {
     for (size_t i = 0; i < sh.pre_sysmQ.size(); i++)
       reducedRho.emplace_back(sh.kept_indices[i].size(), sh.kept_indices[i].size());
     for (size_t i = 0; i < sh.current_sysmQ.size(); i++)
-      rhoZero[i] = dot(sh.U[i], dot(rhoZero[i], transpose(sh.U[i])));
+      rhoZero[i] = plant == 1 ? dot(transpose(sh.U[i]), dot(rhoZero[i], sh.U[i]))
+                              : dot(sh.U[i], dot(rhoZero[i], transpose(sh.U[i])));
     for (size_t i = 0; i < sh.current_sysmQ.size(); i++) {
       size_t kidx = 0;
       for (auto k : sh.coupled_nQ_index[i]) {
@@ -795,7 +805,8 @@ struct FdmSpectrum {
         for (size_t it : sh.kept_indices[ii])
           for (size_t itp : sh.kept_indices[ii]) {
             double aa = 0;
-            for (size_t il = 0; il < bathE[bb].size(); il++) aa += rhoZero[i](kidx + it + kii * il, kidx + itp + kii * il);
+            const size_t nl = plant == 2 ? bathE[bb].size() - 1 : bathE[bb].size();
+            for (size_t il = 0; il < nl; il++) aa += rhoZero[i](kidx + it + kii * il, kidx + itp + kii * il);
             reducedRho[ii](it, itp) += aa;
           }
         kidx += kii * bathE[bb].size();

@@ -345,8 +345,19 @@ int orc_run_thermo_ops(const OrcDriver* d, int min_iter0, const double* T, int n
 
 // T = 0 FDM spectral function of f^dagger chi (examples/fdmSpectrumSiam/main.cpp:61-187, with the
 // hopping list sized to the operator count).  pos/neg: [n_ops][nbins]; traces/sums per shell.
+// planted mistake for the FDM backward pass (FdmSpectrum::plant), used by the oracle's own tests
+static int g_fdm_plant = 0;
+void orc_set_fdm_plant(int p) { g_fdm_plant = p; }
+
+int orc_run_fdm_signed(const OrcDriver* d, int nbins, double* pos, double* neg, double* traces, double* sums,
+                       int* n_shells, double* signed_tot);
 int orc_run_fdm(const OrcDriver* d, int nbins, double* pos, double* neg, double* traces, double* sums,
                 int* n_shells) {
+  return orc_run_fdm_signed(d, nbins, pos, neg, traces, sums, n_shells, nullptr);
+}
+// orc_run_fdm, also returning per operator the signed weights summed over all shells
+int orc_run_fdm_signed(const OrcDriver* d, int nbins, double* pos, double* neg, double* traces, double* sums,
+                       int* n_shells, double* signed_tot) {
   return guard([&] {
     SiteModel imp = driver_imp(*d), bath = driver_bath(*d);
     Chain c(imp, bath);
@@ -372,6 +383,7 @@ int orc_run_fdm(const OrcDriver* d, int nbins, double* pos, double* neg, double*
       c.update_internal_state();
     }
     FdmSpectrum f(ops.size(), nbins);
+    f.plant = g_fdm_plant;
     int ns = 0;
     for (size_t in = (size_t)d->nsites; in > minIterations; in--) {
       c.nrg_iterations_cnt--;
@@ -389,7 +401,34 @@ int orc_run_fdm(const OrcDriver* d, int nbins, double* pos, double* neg, double*
       traces[i] = f.rhoTraces[i];
       sums[i] = f.specSums[i];
     }
+    if (signed_tot)
+      for (size_t p = 0; p < ops.size(); p++) signed_tot[p] = f.signedWeight[p];
     *n_shells = ns;
+  });
+}
+
+// CPU baseline of the RG-flow drivers (examples/rgflowSIAM/main.cpp:29-42, twoChannelSiam): first site
+// {V..V} with rescale 1, then loop sites in with rescale in > 0 ? sqrt(Lambda) : 1; each step is
+// add_bath_site + update_internal_state and the all_eigenvalue read, timed with std::chrono over loop
+// sites [n_skip, n_skip + n_time).
+int orc_time_rgflow(const OrcDriver* d, int n_skip, int n_time, double* seconds) {
+  return guard([&] {
+    SiteModel imp = driver_imp(*d), bath = driver_bath(*d);
+    Chain c(imp, bath);
+    c.set_parameters((size_t)d->K);
+    size_t nop = imp.f_dag_operator.size();
+    c.add_bath_site(std::vector<double>(nop, d->first_V), 1.0);
+    c.update_internal_state();
+    double tot = 0, guard_sum = 0;
+    for (int in = 0; in < n_skip + n_time; in++) {
+      auto t0 = std::chrono::steady_clock::now();
+      c.add_bath_site(std::vector<double>(nop, hopping(in, d->Lambda)), in > 0 ? std::sqrt(d->Lambda) : 1.0);
+      c.update_internal_state();
+      guard_sum += c.all_eigenvalue.empty() ? 0.0 : c.all_eigenvalue[0];
+      auto t1 = std::chrono::steady_clock::now();
+      if (in >= n_skip) tot += std::chrono::duration<double>(t1 - t0).count();
+    }
+    *seconds = guard_sum == guard_sum ? tot : -1.0;
   });
 }
 

@@ -152,7 +152,7 @@ def test_reference_thermodynamics_on_gpu(nk, golden):
     assert rel.max() < 2e-6
 
 
-def _gpu_fdm(nk, eps, U, G, L, K, nsites, nbins):
+def _gpu_fdm(nk, eps, U, G, L, K, nsites, nbins, signed=False):
     g = nk.NrgCore(nk.Site.spinhalf(eps, U), nk.Site.spinhalf(0, 0), K)
     op = nk.SystemOperator(g, nk.Site.fdag_chi_spinhalf())
     rec = nk.FdmRecorder(g, op, nbins, 1e-10, 10.0)
@@ -170,7 +170,8 @@ def _gpu_fdm(nk, eps, U, G, L, K, nsites, nbins):
         else:
             min_it += 1
         g.update_internal_state()
-    return rec.backward(L, scale_offset=-1, first_cnt=min_it + 1), rec.energy_points()
+    out = rec.backward(L, scale_offset=-1, first_cnt=min_it + 1), rec.energy_points()
+    return out + (rec.signed_weights(),) if signed else out
 
 
 def test_fdm_spectrum_matches_oracle(nk, oracle_mod):

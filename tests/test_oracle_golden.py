@@ -137,6 +137,55 @@ def test_siam_chain_equals_exact_diagonalisation(oracle_mod):
         np.testing.assert_allclose(np.sort(expect), specs[k], atol=1e-12)
 
 
+def _ed_two_channel_spectra(eps, U, t0, ts, rescales):
+    """Spectra of the two-channel Anderson chain by dense diagonalisation of its Jordan-Wigner Fock
+    space, built afresh for every length (no idle orbitals).  Orbitals: d_up = 0, d_dn = 1, then four
+    per Wilson site, p = 0..3 = (channel 1 up, channel 1 down, channel 2 up, channel 2 down), the
+    order of the twoChannel site's four f-dagger operators (spin = p mod 2).
+    H_0 = eps n_d + U n_up n_dn + sum_p t0[p] (d+_{p mod 2} c_{0,p} + h.c.);
+    H_{k+1} = r_k H_k + t_k sum_p (c+_{k,p} c_{k+1,p} + h.c.)."""
+    specs = []
+    for n_sites in range(1, len(ts) + 2):
+        n_orb = 2 + 4 * n_sites
+        c = _jw_ops(n_orb)
+        a = [x.T for x in c]
+        nd = [c[s] @ a[s] for s in (0, 1)]
+        H = eps * (nd[0] + nd[1]) + U * (nd[0] @ nd[1])
+        for p in range(4):
+            hop = c[p % 2] @ a[2 + p]
+            H = H + t0[p] * (hop + hop.T)
+        for k in range(n_sites - 1):
+            H = rescales[k] * H
+            for p in range(4):
+                hop = c[2 + 4 * k + p] @ a[2 + 4 * (k + 1) + p]
+                H = H + ts[k] * (hop + hop.T)
+        specs.append(np.sort(np.linalg.eigvalsh(H)))
+    return specs
+
+
+@pytest.mark.parametrize("t0", [(0.09, 0.09, 0.09, 0.09), (0.11, 0.11, 0.05, 0.05)])
+def test_two_channel_chain_equals_exact_diagonalisation(oracle_mod, t0):
+    """The two-channel chain (examples/twoChannelSiam/main.cpp: spinhalf impurity with its two f-dagger
+    operators duplicated to four, twoChannel bath of 16 states) untruncated at iterations 0 and 1 (64
+    and 1024 states) against exact diagonalisation; also with unequal channel couplings
+    {V_L, V_L, V_R, V_R} (examples/fdmSpectrumTwoChannel/main.cpp:71), which distinguishes the channels."""
+    eps, U, L = -0.2, 0.4, 3.0
+    ts = [oracle_mod.hopping(0, L)]
+    rs = [1.0]
+    specs = _ed_two_channel_spectra(eps, U, t0, ts, rs)
+    ch = oracle_mod.Chain(oracle_mod.Model("spinhalf", eps=eps, U=U, dup_ops=True),
+                          oracle_mod.Model("two_channel"), 100000)
+    ch.add_bath_site(list(t0), 1.0)
+    ch.update_internal_state()
+    got = [ch.all_eigenvalue()]
+    ch.add_bath_site([ts[0]] * 4, rs[0])
+    ch.update_internal_state()
+    got.append(ch.all_eigenvalue())
+    assert [len(g) for g in got] == [64, 1024]
+    for k in range(2):
+        np.testing.assert_allclose(got[k], specs[k], atol=1e-12)
+
+
 def _ed_siam_final(eps, U, V, ts, rescales):
     """Final Hamiltonian of the chain of _ed_siam and the impurity creation operators d+_up, d+_dn."""
     n_orb = 2 * (len(ts) + 2)
@@ -220,6 +269,46 @@ def test_fdm_reduced_density_matrices_have_unit_trace(oracle_mod):
     assert len(tr) >= 8
     np.testing.assert_allclose(tr, 1.0, atol=1e-12)
     assert np.all(pos >= 0) and np.all(neg >= 0)
+
+
+@pytest.mark.parametrize("U,L,K,nsites", [(0.3, 2.0, 256, 18), (0.6, 2.5, 256, 14), (0.3, 2.0, 1024, 12)])
+def test_fdm_complete_basis_sum_rule(oracle_mod, U, L, K, nsites):
+    """The multi-shell backward pass against the complete-basis sum rule: the signed weights
+    w+ + w- of rhoDotOperators (fdmSpectrum.hpp:231-243), summed over every shell before the |.|, the
+    1e-20 cut and the bin-range test, add up to <{B, B+}> = 1 for B = d+_sigma chi.  Every discarded
+    state of every recorded shell and every state of the last shell enter once, so a wrong density
+    matrix rotation (U^T rho U instead of U rho U^T, fdmSpectrum.hpp:87-91) or a term dropped from the
+    partial trace (:95-113) breaks it, while the trace of each shell's rho would still be 1 after the
+    first.  K is a power of the bath dimension 4: the reference records a shell only when the previous
+    iteration kept at least K states (checkHigherEnergyDiscarded, nrgcore.hpp:384-386), so otherwise
+    the first truncating iteration is never recorded and its discarded weight is missing from the sum
+    (test_fdm_first_truncating_shell_is_not_recorded)."""
+    drv = oracle_mod.driver(imp_kind=0, eps=-U / 2, U=U, bath_kind=0, Lambda=L,
+                            first_V=np.sqrt(2 * 0.01 / np.pi), nsites=nsites, K=K)
+    _, _, tr, _, sg = oracle_mod.run_fdm(drv, 200, signed=True)
+    assert len(tr) >= 6
+    np.testing.assert_allclose(sg, 1.0, atol=1e-12)
+    try:
+        for plant in (1, 2):
+            oracle_mod.set_fdm_plant(plant)
+            _, _, _, _, bad = oracle_mod.run_fdm(drv, 200, signed=True)
+            assert np.abs(bad - 1.0).max() > 1e-3, plant
+    finally:
+        oracle_mod.set_fdm_plant(0)
+
+
+def test_fdm_first_truncating_shell_is_not_recorded(oracle_mod):
+    """With K = 96 the first iteration that truncates (256 states) follows one that kept 64 < K, so
+    checkHigherEnergyDiscarded is false there and the shell is not recorded (nrgcore.hpp:384-386,
+    examples/fdmSpectrumSiam/main.cpp:130-140): the signed total falls short of 1 by the same amount
+    whatever the chain length, the weight of the discarded states of that one iteration."""
+    short = []
+    for nsites in (10, 14, 20):
+        drv = oracle_mod.driver(imp_kind=0, eps=-0.15, U=0.3, bath_kind=0, Lambda=2.0,
+                                first_V=np.sqrt(2 * 0.01 / np.pi), nsites=nsites, K=96)
+        short.append(1.0 - oracle_mod.run_fdm(drv, 200, signed=True)[4][0])
+    assert 1e-5 < short[0] < 1e-2
+    assert abs(short[2] - short[1]) < 0.02 * short[0]
 
 
 # ---------------------------------------------------------------- static observables
