@@ -160,7 +160,7 @@ def two_channel_headline(nk, W, K, with_plain=True, with_check=True):
     for _ in range(W):
         s.step()
     s.g.profile(True)
-    step_ms, solve_ms, feig, unconv, launches, syncs, eigs = [], [], 0.0, 0, [], [], []
+    step_ms, solve_ms, feig, unconv, launches, syncs, eigs, ties = [], [], 0.0, 0, [], [], [], []
     t0 = time.perf_counter()
     for _ in range(K):
         ta = time.perf_counter()
@@ -172,6 +172,7 @@ def two_channel_headline(nk, W, K, with_plain=True, with_check=True):
         unconv += st["oa_unconverged"]
         launches.append(st["launches"])
         syncs.append(st["host_syncs"])
+        ties.append(st["near_ties"])
     t1 = time.perf_counter()
     kstats = s.g.kernel_stats()
     s.g.profile(False)
@@ -179,7 +180,7 @@ def two_channel_headline(nk, W, K, with_plain=True, with_check=True):
     progress("two-channel mixed: %d timed iterations, %.1f ms/step" % (K, (t1 - t0) * 1e3 / K))
     out = {"value": value, "ms_per_step": (t1 - t0) * 1e3 / K, "step_ms": step_ms, "solve_ms": solve_ms,
            "refinement_unconverged_blocks": unconv, "launches_per_step": float(np.mean(launches)),
-           "host_syncs_per_step": float(np.mean(syncs)), "kstats": kstats}
+           "host_syncs_per_step": float(np.mean(syncs)), "near_ties": int(np.sum(ties)), "kstats": kstats}
     # F_eig of the timed iterations (block dims and kept counts, from a replay with the Hamiltonians kept)
     if with_check:
         r = TwoChannel(nk, nk.NRGK_MODE_MIXED)
@@ -398,22 +399,20 @@ def z_average(nk):
             "identical": bool(np.array_equal(S_seq, S_tog))}
 
 
-SCFG = dict(requests=64, rate=5.0, workers=8, seed=20261202, nbins=2000, n_omega=1000, b=0.6)
+SCFG = dict(requests=512, seed=20261202, nbins=2000, n_omega=1000, b=0.6, slots=8, probe=48, load=0.8)
 
 
 def serving(nk):
-    """A bounded sample of the request workload: SIAM requests with U ~ Unif[0.1, 1], eps = -U/2 + d,
-    d ~ Unif[-0.1, 0.1], Gamma log-uniform in [0.005, 0.05], Lambda in {2, 2.5, 3}, K in {512, 1024,
-    2048}, 40-60 Wilson sites, each returning A(omega) (f-dagger chi, T = 0 FDM, 2000 bins,
-    log-Gaussian b = 0.6) on 1000 positive and 1000 negative frequencies.  64 requests (of the
-    workload's 512) arrive as a Poisson stream at 5 requests/s (seeded numpy generator); 8 host
-    workers each run one request at a time on its own chain and stream.  Latency = completion time -
-    arrival time."""
-    from concurrent.futures import ThreadPoolExecutor
+    """The request workload through the library's request loop (nrgk_server_*): SIAM requests with
+    U ~ Unif[0.1, 1], eps = -U/2 + d, d ~ Unif[-0.1, 0.1], Gamma log-uniform in [0.005, 0.05], Lambda in
+    {2, 2.5, 3}, K in {512, 1024, 2048}, 40-60 Wilson sites, each returning A(omega) (f-dagger chi, T = 0
+    FDM, 2000 bins, log-Gaussian b = 0.6) at 1000 positive and 1000 negative frequencies.  8 slots,
+    admission by device footprint.  First the capacity: a burst of 48 requests submitted at once
+    (solves/s while saturated).  Then all 512 requests as a Poisson stream (seeded numpy generator) at
+    0.8 of that capacity; latency = completion - arrival."""
     c = SCFG
     rng = np.random.default_rng(c["seed"])
     n = c["requests"]
-    arrive = np.cumsum(rng.exponential(1.0 / c["rate"], n))
     reqs = []
     for _ in range(n):
         U = rng.uniform(0.1, 1.0)
@@ -422,47 +421,52 @@ def serving(nk):
                          nit=int(rng.integers(40, 61))))
     omega = np.logspace(-10, 1, c["n_omega"])
 
-    def solve(r):
-        g = nk.NrgCore(nk.Site.spinhalf(r["eps"], r["U"]), nk.Site.spinhalf(0, 0), r["K"])
-        op = nk.SystemOperator(g, nk.Site.fdag_chi_spinhalf())
-        rec = nk.FdmRecorder(g, op, c["nbins"], 1e-10, 10.0)
-        L, V = r["L"], np.sqrt(2 * r["G"] / np.pi)
-        g.add_bath_site([V, V], 1.0)
-        op.update()
-        g.update_internal_state()
-        m = 0
-        for k in range(r["nit"]):
-            t = nk.hopping(k, L)
-            g.add_bath_site([t, t], 1.0 if k == 0 else np.sqrt(L))
-            op.update()
-            if g.checkHigherEnergyDiscarded():
-                rec.record()
-            else:
-                m += 1
-            g.update_internal_state()
-        pos, neg, _, _ = rec.backward(L, scale_offset=-1, first_cnt=m + 1)
-        x = rec.energy_points()
-        A = np.concatenate([nk.broaden(x, neg.sum(0), omega, c["b"])[::-1], nk.broaden(x, pos.sum(0), omega, c["b"])])
-        return time.perf_counter(), float(pos.sum() + neg.sum()), bool(np.all(np.isfinite(A)))
+    def submit(srv, r):
+        return srv.submit(nk.Site.spinhalf(r["eps"], r["U"]), nk.Site.spinhalf(0, 0), nk.Site.fdag_chi_spinhalf(),
+                          r["K"], r["L"], float(np.sqrt(2 * r["G"] / np.pi)), r["nit"], c["nbins"], omega, c["b"])
 
-    solve(dict(U=0.3, eps=-0.15, G=0.01, L=2.0, K=512, nit=12))  # warm-up
+    srv = nk.Server(n_slots=c["slots"])
+    srv.wait(submit(srv, dict(U=0.3, eps=-0.15, G=0.01, L=2.0, K=512, nit=12)))  # warm-up
+    # capacity: a saturated burst
     t0 = time.perf_counter()
-    futs = []
-    with ThreadPoolExecutor(c["workers"]) as ex:
-        for k in range(n):
-            wait = t0 + arrive[k] - time.perf_counter()
-            if wait > 0:
-                time.sleep(wait)
-            futs.append(ex.submit(solve, reqs[k]))
-        out = [f.result() for f in futs]
-    done = np.array([o[0] - t0 for o in out])
-    lat = (done - arrive) * 1e3
+    ids = [submit(srv, reqs[k % n]) for k in range(c["probe"])]
+    for i in ids:
+        srv.wait(i)
+    cap = c["probe"] / (time.perf_counter() - t0)
+    progress("serving capacity %.2f solves/s" % cap)
+    rate = c["load"] * cap
+    arrive = np.cumsum(np.random.default_rng(c["seed"] + 1).exponential(1.0 / rate, n))
+    t0 = time.perf_counter()
+    ids, sub_t = [], []
+    for k in range(n):
+        wait = t0 + arrive[k] - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        sub_t.append(time.perf_counter() - t0)
+        ids.append(submit(srv, reqs[k]))
+        if k % 64 == 63:
+            progress("serving: %d of %d requests submitted" % (k + 1, n))
+    lat, qms, tw, fin = [], [], [], []
+    for i in ids:
+        ap, an, info = srv.wait(i)
+        lat.append(info["latency_ms"])
+        qms.append(info["queue_ms"])
+        tw.append(info["total_weight"])
+        fin.append(bool(np.all(np.isfinite(ap)) and np.all(np.isfinite(an))))
+    t1 = time.perf_counter()
+    st = srv.stats()
+    srv.close()
+    # latency from the scheduled arrival: submission lag of the host loop included
+    lat = np.array(lat) + 1e3 * (np.array(sub_t) - arrive)
     return {"workload": "SIAM requests (U, eps, Gamma, Lambda, K, 40-60 sites drawn per request), each returning "
                         "A(omega) on 2000 frequencies; Poisson arrivals",
-            "sample": "64 of 512 requests at 5 requests/s, seed %d, 8 workers" % c["seed"],
+            "server": "nrgk_server: %d slots, admission by device footprint, one chain and stream per request" % c["slots"],
+            "capacity_solves_per_s": cap,
+            "sample": "all %d requests at %.2f requests/s (%.1f x capacity), seed %d" % (n, rate, c["load"], c["seed"]),
             "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
-            "solves_per_s": n / float(done.max() - arrive[0]),
-            "total_weight_range": [min(o[1] for o in out), max(o[1] for o in out)], "all_finite": all(o[2] for o in out)}
+            "queue_ms_p50": float(np.percentile(qms, 50)), "queue_ms_p99": float(np.percentile(qms, 99)),
+            "solves_per_s": n / (t1 - t0), "peak_running": st["peak_running"],
+            "total_weight_range": [float(min(tw)), float(max(tw))], "all_finite": all(fin)}
 
 
 def rgflow(nk):
@@ -555,6 +559,7 @@ def main():
         "launches_per_step": h["launches_per_step"],
         "host_syncs_per_step": h["host_syncs_per_step"],
         "refinement_unconverged_blocks": h["refinement_unconverged_blocks"],
+        "truncation_near_ties": h["near_ties"],
         "kernels": kernel_table(kst, K),
         "roofline": roofline(kst),
     }

@@ -178,6 +178,13 @@ int nrgk_chain_relative_gs(nrgk_chain* chain, double* out, size_t cap, size_t* n
 /* current_sysmQ (nrgcore.hpp:528) / eigenvaluesQ sizes: qn[n_sectors*n_qn], dims[n_sectors];
  * both may be NULL to query n_sectors */
 int nrgk_chain_sectors(nrgk_chain* chain, int* n_sectors, int* qn, int* dims);
+/* pre_sysmQ (nrgcore.hpp:530): the sectors of the previous (kept) basis, qn[n_sectors*n_qn] (may be NULL);
+ * after update_internal_state they are the current ones, as in the reference (nrgcore.hpp:105-109) */
+int nrgk_chain_pre_sectors(nrgk_chain* chain, int* n_sectors, int* qn);
+/* coupled_nQ_index (nrgcore.hpp:534, create_next_basis 238-261): for each current sector, the list of
+ * i * n_bath_sectors + j over (previous sector i, bath sector j), first-seen order.  counts[n_sectors]
+ * (may be NULL), flat[cap] (may be NULL), total: the summed length */
+int nrgk_chain_coupled_index(nrgk_chain* chain, int* counts, int* flat, size_t cap, size_t* total);
 /* eigenvaluesQ (nrgcore.hpp:532), concatenated over sectors (shifted after truncation) */
 int nrgk_chain_eigenvalues_q(nrgk_chain* chain, double* out, size_t cap, size_t* n);
 /* eigenvaluesQ_kept_indices sizes (nrgcore.hpp:524); kept indices are always a prefix */
@@ -198,6 +205,10 @@ int nrgk_chain_fdag_blocks(nrgk_chain* chain, int op, int* n_blocks, int* bi, in
 int nrgk_chain_set_state(nrgk_chain* chain, const nrgk_site_desc* state, int cnt);
 /* per-step device timings (ms): assemble, eigensolve, truncate, rotate; and eigensolver stats */
 int nrgk_chain_step_stats(nrgk_chain* chain, double* ms4, int* jacobi_sweeps, int* max_cluster);
+/* the last truncation: E_cut = sorted[K] (discard_higher_energies, nrgcore.hpp:285) and the number of
+ * eigenvalues within 2e-10 of E_cut + 1e-8 (either side), whose keep/discard decision an eigenvalue
+ * error of that size could flip (0 when nothing was truncated); no reference counterpart */
+int nrgk_chain_truncation_stats(nrgk_chain* chain, double* e_cut, int* near_ties);
 /* kernel launches (a rocBLAS/rocSOLVER call counts once) and host stream synchronisations of the
  * last step, from add_bath_site to the end of update_internal_state (observable updates and shell
  * records made between them included); no reference counterpart */
@@ -258,6 +269,12 @@ int nrgk_fdm_backward(nrgk_fdm* fdm, double lambda, int scale_offset, int first_
  * op_group is not NULL, {op_group}{cnt-1} with attribute operatorSize and at{p}/qmat{k},
  * idVector{k}, colVector, rowVector.  The HDF5 library is loaded when first needed. */
 int nrgk_fdm_save(nrgk_fdm* rec, const char* path, const char* op_group);
+/* h5stream::h5stream(name) (nrgcore/include/utils/h5stream.hpp:244-478): creates (truncates) an HDF5
+ * file; nrgk_h5_write adds a one-dimensional dataset of n doubles to it (its groups created as
+ * needed), as h5stream::write(std::vector<double>, name) does.  The HDF5 library is loaded when first
+ * needed. */
+int nrgk_h5_create(const char* path);
+int nrgk_h5_write(const char* path, const char* name, const double* data, size_t n);
 /* a one-dimensional dataset of an HDF5 file as doubles (n: its length; out may be NULL) */
 int nrgk_h5_read(const char* path, const char* dataset, double* out, size_t cap, size_t* n);
 /* Per operator, the signed weights w+ + w- of the last nrgk_fdm_backward summed over its shells,
@@ -266,6 +283,20 @@ int nrgk_h5_read(const char* path, const char* dataset, double* out, size_t cap,
  * d+ chi): the FDM sum rule, which the reference does not compute.  n_ops receives the operator
  * count; out[cap] may be NULL. */
 int nrgk_fdm_signed_weights(nrgk_fdm* fdm, double* out, int cap, int* n_ops);
+/* The reference's backward loop one shell at a time (examples/fdmSpectrumSiam/main.cpp:158-180,
+ * examples/entropySIAM/main.cpp:80-91): after the driver decrements nrg_iterations_cnt and
+ * NrgData::loadCurrentData restores the shell, fdmSpectrum::calcSpectrum(scale) or
+ * fdmThermodynamics::calcThermodynamics(scale) processes it.  nrgk_fdm_add_shell registers the shell
+ * recorded at iteration counter cnt with its energy scale; the registered shells are processed in
+ * registration order (the last shell of the chain first) by nrgk_fdm_run_spectrum (the FDM pass,
+ * outputs as nrgk_fdm_backward) or nrgk_thermo_run (fdmThermodynamics::saveFinalData: entropy S,
+ * specific heat C and, when a system operator was set, its expectation values ops[n_ops][nT], may be
+ * NULL), which then clear the registration.  nrg_max: nrg_iterations_cnt when the reference's
+ * fdmThermodynamics object was constructed (nrgMaxIterations, fdmThermodynamics.hpp:26-29). */
+int nrgk_fdm_add_shell(nrgk_fdm* fdm, int cnt, double scale);
+int nrgk_fdm_run_spectrum(nrgk_fdm* fdm, double* pos, double* neg, double* traces, double* sums, int cap_shells,
+                          int* n_shells);
+int nrgk_thermo_run(nrgk_fdm* fdm, int nrg_max, const double* T, int nT, double* S, double* C, double* ops);
 /* bin centres e_min*exp(i/delE) (fdmSpectrum.hpp:323-327) */
 int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out);
 
@@ -305,6 +336,54 @@ int nrgk_thermo_expectation(nrgk_fdm* rec, double lambda, int scale_offset, int 
  * zAverage.hpp:33-43); entries past the star's m - 1 hoppings are 0.  No GPU work. */
 int nrgk_zdisc_chain(double z, double lambda, double D, const double* x, const double* rho, int n_x, int quad,
                      int n_intervals, int normalize, int n_sites, double* t_out, double* eps_out);
+
+/* ---------------------------------------------------------------- request loop (serving) */
+/* Impurity solves served concurrently: no reference counterpart (the reference's drivers run one
+ * chain per process).  A request is the FDM spectral-function driver of
+ * examples/fdmSpectrumSiam/main.cpp:86-180 for its own models and parameters: the first site with
+ * hopping first_hopping (rescale 1), then n_sites Wilson sites hopping(n, lambda) (rescale sqrt(lambda)
+ * from the second), the observable's operators rotated each step, a shell recorded whenever higher
+ * energies were discarded, the T = 0 FDM backward pass on n_bins bins over [1e-10, 10], and the weights
+ * summed over the operators and broadened (log-Gaussian, width broadening; spec.py:27-44) on the n_omega
+ * positive frequencies omega: A_pos(omega) from the positive-frequency weights, A_neg(omega) (i.e.
+ * A(-omega)) from the negative ones.  fail_at_site >= 0 makes that site's hopping NaN (failure
+ * injection: the request fails with NRGK_ERR_INVALID, the others are not affected); -1 otherwise.
+ * The model descriptions and omega are copied at submission. */
+typedef struct {
+  const nrgk_site_desc* impurity;
+  const nrgk_site_desc* bath;
+  const nrgk_site_desc* observable; /* operators on the impurity sectors (op_nblocks/op_bi/op_bj/op_data) */
+  size_t max_kept;
+  double lambda;
+  double first_hopping;
+  int n_sites;
+  int n_bins;
+  const double* omega;
+  int n_omega;
+  double broadening;
+  int fail_at_site;
+} nrgk_request;
+typedef struct nrgk_server nrgk_server;
+/* A server on a context with n_slots requests running at once, each admitted only when its estimated
+ * device footprint (chain arenas + one FDM shell per site) fits beside the running ones within
+ * hbm_budget bytes (0: no budget of its own; the context's capacity still fails allocations with
+ * NRGK_ERR_NOMEM).  Admission is first come, first served; every running request has its own chain and
+ * stream, so their iterations interleave on the device, and a finished request frees its slot and
+ * memory for the next at once. */
+int nrgk_server_create(nrgk_ctx* ctx, int n_slots, size_t hbm_budget, nrgk_server** out);
+/* waits for the requests already submitted, then stops the workers */
+int nrgk_server_destroy(nrgk_server* srv);
+/* queues a request (returns at once); NRGK_ERR_NOMEM if its footprint exceeds the budget */
+int nrgk_server_submit(nrgk_server* srv, const nrgk_request* req, long long* id);
+/* *done = 1 once request id has finished (successfully or not) */
+int nrgk_server_poll(nrgk_server* srv, long long id, int* done);
+/* Waits for request id and returns its status (NRGK_OK or the failure of its solve, message in
+ * nrgk_last_error()); A_pos and A_neg receive n_omega values each (may be NULL); info (may be NULL)
+ * receives 6 doubles: latency ms (submission to completion), queue ms (submission to admission),
+ * total FDM weight, iterations, recorded shells, admitted footprint in bytes.  The request is released. */
+int nrgk_server_wait(nrgk_server* srv, long long id, double* A_pos, double* A_neg, double* info);
+/* running and queued requests, bytes reserved by the running ones, the most that ran at once */
+int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reserved, int* peak_running);
 
 /* ---------------------------------------------------------------- broadening */
 /* log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44): out[o] = (1/w_o) sum_k

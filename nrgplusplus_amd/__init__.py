@@ -43,6 +43,23 @@ class SiteDesc(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [
+        ("impurity", C.POINTER(SiteDesc)),
+        ("bath", C.POINTER(SiteDesc)),
+        ("observable", C.POINTER(SiteDesc)),
+        ("max_kept", C.c_size_t),
+        ("lam", C.c_double),
+        ("first_hopping", C.c_double),
+        ("n_sites", C.c_int),
+        ("n_bins", C.c_int),
+        ("omega", C.POINTER(C.c_double)),
+        ("n_omega", C.c_int),
+        ("broadening", C.c_double),
+        ("fail_at_site", C.c_int),
+    ]
+
+
 class NrgkError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("nrgk status %d: %s" % (status, msg))
@@ -112,6 +129,7 @@ def lib():
             "nrgk_chain_step_stats": (i, [vp, dp, ip, ip]),
             "nrgk_chain_refine_stats": (i, [vp, ip, ip]),
             "nrgk_chain_step_counts": (i, [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+            "nrgk_chain_truncation_stats": (i, [vp, dp, ip]),
             "nrgk_chain_profile": (i, [vp, i]),
             "nrgk_chain_profile_only": (i, [vp, C.c_char_p]),
             "nrgk_chain_kernel_stats": (i, [vp, vp, i, ip]),
@@ -133,6 +151,12 @@ def lib():
             "nrgk_fdm_set_system_operator": (i, [vp, vp]),
             "nrgk_thermo_expectation": (i, [vp, d, i, i, dp, i, dp]),
             "nrgk_broaden": (i, [vp, dp, dp, i, dp, i, d, dp]),
+            "nrgk_server_create": (i, [vp, i, sz, C.POINTER(vp)]),
+            "nrgk_server_destroy": (i, [vp]),
+            "nrgk_server_submit": (i, [vp, C.POINTER(Request), C.POINTER(C.c_longlong)]),
+            "nrgk_server_poll": (i, [vp, C.c_longlong, ip]),
+            "nrgk_server_wait": (i, [vp, C.c_longlong, dp, dp, dp]),
+            "nrgk_server_stats": (i, [vp, ip, ip, szp, ip]),
             "nrgk_zdisc_chain": (i, [d, d, d, dp, dp, i, i, i, i, i, dp, dp]),
         }
         for name, (res, args) in sig.items():
@@ -531,9 +555,11 @@ class NrgCore:
         _check(lib().nrgk_chain_refine_stats(self._h, C.byref(oa), C.byref(unc)))
         nl, ns = C.c_longlong(), C.c_longlong()
         _check(lib().nrgk_chain_step_counts(self._h, C.byref(nl), C.byref(ns)))
+        cut, ties = C.c_double(), C.c_int()
+        _check(lib().nrgk_chain_truncation_stats(self._h, C.byref(cut), C.byref(ties)))
         return dict(ms_assemble=ms[0], ms_eig=ms[1], ms_truncate=ms[2], ms_rotate=ms[3], jacobi_sweeps=sw.value,
                     max_cluster=mc.value, oa_steps=oa.value, oa_unconverged=unc.value, launches=nl.value,
-                    host_syncs=ns.value)
+                    host_syncs=ns.value, e_cut=cut.value, near_ties=ties.value)
 
     def profile(self, enable=True, only=None):
         """Start (clearing the totals) or stop per-kernel-family device timing; `only` restricts it to
@@ -773,3 +799,59 @@ def hopping(n, lam):
     """Wilson-chain hopping (examples/rgflowSIAM/main.cpp:5-10)."""
     return 0.5 * (1.0 + 1.0 / lam) * (1.0 - lam ** (-n - 1)) / np.sqrt(
         (1.0 - lam ** (-2.0 * n - 1)) * (1.0 - lam ** (-2.0 * n - 3)))
+
+
+# ---------------------------------------------------------------- request loop
+class Server:
+    """Spectral-function requests served concurrently (nrgk_server_*): admission by device-memory
+    footprint and slots, one chain and stream per running request, first come first served."""
+
+    def __init__(self, n_slots=8, hbm_budget=0, ctx=None):
+        self.ctx = ctx or default_context()
+        p = C.c_void_p()
+        _check(lib().nrgk_server_create(self.ctx._h, n_slots, hbm_budget, C.byref(p)))
+        self._h = p
+        self._n_omega = {}
+
+    def submit(self, impurity, bath, observable, max_kept, lam, first_hopping, n_sites, n_bins, omega,
+               broadening=0.6, fail_at_site=-1):
+        """queues one request; returns its id.  impurity, bath: Site; observable: Site whose operators are
+        the observable blocks (e.g. Site.fdag_chi_spinhalf())"""
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        di, db, do = impurity.desc(), bath.desc(), observable.desc()
+        r = Request(C.pointer(di), C.pointer(db), C.pointer(do), max_kept, lam, first_hopping, n_sites, n_bins,
+                    _dp(om), len(om), broadening, fail_at_site)
+        rid = C.c_longlong()
+        _check(lib().nrgk_server_submit(self._h, C.byref(r), C.byref(rid)))
+        self._n_omega[rid.value] = len(om)
+        return rid.value
+
+    def done(self, rid):
+        d = C.c_int()
+        _check(lib().nrgk_server_poll(self._h, rid, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid):
+        """(A_pos, A_neg, info) of request rid: info = dict(latency_ms, queue_ms, total_weight, iterations,
+        shells, bytes); raises NrgkError if the request failed"""
+        n = self._n_omega.pop(rid)
+        ap, an, info = np.zeros(n), np.zeros(n), np.zeros(6)
+        _check(lib().nrgk_server_wait(self._h, rid, _dp(ap), _dp(an), _dp(info)))
+        return ap, an, dict(latency_ms=info[0], queue_ms=info[1], total_weight=info[2], iterations=int(info[3]),
+                            shells=int(info[4]), bytes=int(info[5]))
+
+    def stats(self):
+        r, q, pk = C.c_int(), C.c_int(), C.c_int()
+        res = C.c_size_t()
+        _check(lib().nrgk_server_stats(self._h, C.byref(r), C.byref(q), C.byref(res), C.byref(pk)))
+        return dict(running=r.value, queued=q.value, reserved=res.value, peak_running=pk.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _check(lib().nrgk_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.nrgk_server_destroy(self._h)
+            self._h = None

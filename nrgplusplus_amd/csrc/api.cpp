@@ -12,6 +12,7 @@
 #include "h5io.hpp"
 #include "kernels.hpp"
 #include "observables.hpp"
+#include "server.hpp"
 
 namespace nrgk {
 static thread_local std::string g_last_error;
@@ -40,6 +41,9 @@ struct nrgk_ctx {
 struct nrgk_chain {
   std::unique_ptr<Chain> c;
 };
+struct nrgk_server {
+  std::unique_ptr<Server> s;
+};
 
 extern "C" {
 
@@ -64,6 +68,7 @@ int nrgk_ctx_create(int device, nrgk_ctx** out) {
       lds = optin;
     h->c.lds_max = lds - 1024;  // room for the kernels' static LDS
     set_kernel_lds_limits(h->c.lds_max);
+    set_ht_lds_limits(h->c.lds_max);
     size_t fr = 0, tot = 0;
     NRGK_HIP(hipMemGetInfo(&fr, &tot));
     h->c.capacity = (size_t)(0.9 * (double)fr);
@@ -204,6 +209,33 @@ int nrgk_chain_sectors(nrgk_chain* ch, int* n_sectors, int* qn, int* dims) {
     }
   });
 }
+int nrgk_chain_pre_sectors(nrgk_chain* ch, int* n_sectors, int* qn) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    if (n_sectors) *n_sectors = (int)c.pre_qn.size();
+    if (qn)
+      for (size_t s = 0; s < c.pre_qn.size(); s++)
+        for (int q = 0; q < c.imp.nqn; q++) qn[s * c.imp.nqn + q] = c.pre_qn[s][q];
+  });
+}
+int nrgk_chain_coupled_index(nrgk_chain* ch, int* counts, int* flat, size_t cap, size_t* total) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    Chain& c = *ch->c;
+    size_t t = 0;
+    for (auto& v : c.coupled) t += v.size();
+    if (total) *total = t;
+    if (counts)
+      for (size_t s = 0; s < c.coupled.size(); s++) counts[s] = (int)c.coupled[s].size();
+    if (flat) {
+      NRGK_REQUIRE(cap >= t, NRGK_ERR_INVALID, "nrgk: output capacity too small");
+      size_t o = 0;
+      for (auto& v : c.coupled)
+        for (auto [ii, bb] : v) flat[o++] = ii * c.bath.S + bb;  // coupled_nQ_index (nrgcore.hpp:238-261)
+    }
+  });
+}
 int nrgk_chain_eigenvalues_q(nrgk_chain* ch, double* out, size_t cap, size_t* n) {
   return guard([&] {
     NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
@@ -237,6 +269,7 @@ int nrgk_chain_block_u(nrgk_chain* ch, int s, double* out, size_t cap) {
     NRGK_REQUIRE(c.phase != Chain::FRESH && s >= 0 && s < (int)c.cur_n.size(), NRGK_ERR_INVALID, "nrgk: bad sector");
     const size_t sz = (size_t)c.cur_n[s] * c.cur_n[s];
     NRGK_REQUIRE(cap >= sz, NRGK_ERR_INVALID, "nrgk: output capacity too small");
+    if (c.is_wide(s)) c.wide_vectors(true);  // all columns of the wide blocks, when first asked for
     d2h(out, c.dU.as<double>() + c.cur_off[s], sz, c.st);
     NRGK_HIP(stream_sync(c.st));
   });
@@ -388,6 +421,52 @@ int nrgk_fdm_energy_points(nrgk_fdm* fdm, double* out) {
     for (int i = 0; i < f.nbins; i++) out[i] = f.e_min * std::exp(i * 1. / f.delE);
   });
 }
+int nrgk_fdm_add_shell(nrgk_fdm* fdm, int cnt, double scale) {
+  return guard([&] {
+    NRGK_REQUIRE(fdm && std::isfinite(scale), NRGK_ERR_INVALID, "nrgk_fdm_add_shell: bad argument");
+    fdm->f->shell_at(cnt);  // must exist
+    fdm->f->registered.push_back({cnt, scale});
+  });
+}
+int nrgk_fdm_run_spectrum(nrgk_fdm* fdm, double* pos, double* neg, double* traces, double* sums, int cap_shells,
+                          int* n_shells) {
+  return guard([&] {
+    NRGK_REQUIRE(fdm && pos && neg, NRGK_ERR_INVALID, "nrgk_fdm_run_spectrum: null argument");
+    Fdm& f = *fdm->f;
+    NRGK_REQUIRE(!f.registered.empty(), NRGK_ERR_STATE, "nrgk_fdm_run_spectrum: no shell registered");
+    std::vector<std::pair<Shell*, double>> use;
+    for (auto& [cnt, sc] : f.registered) use.push_back({f.shell_at(cnt), sc});
+    f.registered.clear();
+    f.backward_shells(use, pos, neg, traces, sums, cap_shells, n_shells);
+  });
+}
+int nrgk_thermo_run(nrgk_fdm* fdm, int nrg_max, const double* T, int nT, double* S, double* Cv, double* ops) {
+  return guard([&] {
+    NRGK_REQUIRE(fdm && T && S && Cv && nT > 0, NRGK_ERR_INVALID, "nrgk_thermo_run: null argument");
+    Fdm& f = *fdm->f;
+    NRGK_REQUIRE(!f.registered.empty(), NRGK_ERR_STATE, "nrgk_thermo_run: no shell registered");
+    std::vector<std::pair<Shell*, double>> use;
+    for (auto& [cnt, sc] : f.registered) use.push_back({f.shell_at(cnt), sc});
+    f.registered.clear();
+    f.thermo_shells(use, nrg_max, T, nT, S, Cv, ops);
+  });
+}
+int nrgk_h5_create(const char* path) {
+  return guard([&] {
+    NRGK_REQUIRE(path, NRGK_ERR_INVALID, "nrgk_h5_create: null path");
+    h5::File f(path, true);
+  });
+}
+int nrgk_h5_write(const char* path, const char* name, const double* data, size_t n) {
+  return guard([&] {
+    NRGK_REQUIRE(path && name && (data || n == 0), NRGK_ERR_INVALID, "nrgk_h5_write: null argument");
+    h5::File f(path, h5::File::Mode::ReadWrite);
+    const std::string nm(name);
+    const size_t sl = nm.find_last_of('/');
+    if (sl != std::string::npos && sl > 0) f.group(nm.substr(0, sl));
+    f.write(nm, h5::Type::F64, data, n);
+  });
+}
 int nrgk_fdm_signed_weights(nrgk_fdm* fdm, double* out, int cap, int* n_ops) {
   return guard([&] {
     NRGK_REQUIRE(fdm, NRGK_ERR_INVALID, "null argument");
@@ -439,6 +518,97 @@ int nrgk_thermo_expectation(nrgk_fdm* rec, double lambda, int scale_offset, int 
     rec->f->thermo(lambda, scale_offset, first_cnt, T, nT, S.data(), C.data(), out);
   });
 }
+int nrgk_server_create(nrgk_ctx* ctx, int n_slots, size_t hbm_budget, nrgk_server** out) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx && out, NRGK_ERR_INVALID, "nrgk_server_create: null argument");
+    auto* h = new nrgk_server;
+    try {
+      h->s = std::make_unique<Server>(&ctx->c, n_slots, hbm_budget);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int nrgk_server_destroy(nrgk_server* srv) {
+  return guard([&] {
+    if (!srv) return;
+    {
+      std::unique_lock<std::mutex> lk(srv->s->m);
+      srv->s->cv.wait(lk, [&] { return srv->s->pending.empty() && srv->s->running == 0; });
+    }
+    delete srv;
+  });
+}
+int nrgk_server_submit(nrgk_server* srv, const nrgk_request* q, long long* id) {
+  return guard([&] {
+    NRGK_REQUIRE(srv && q && id && q->impurity && q->bath && q->observable, NRGK_ERR_INVALID,
+                 "nrgk_server_submit: null argument");
+    NRGK_REQUIRE(q->n_omega == 0 || q->omega, NRGK_ERR_INVALID, "nrgk_server_submit: null omega");
+    Request r;
+    r.impurity = OwnedSite::copy(q->impurity);
+    r.bath = OwnedSite::copy(q->bath);
+    r.observable = OwnedSite::copy(q->observable);
+    NRGK_REQUIRE(r.observable.n_ops > 0, NRGK_ERR_INVALID, "nrgk_server_submit: the observable has no operator");
+    r.max_kept = q->max_kept;
+    r.lambda = q->lambda;
+    r.first_hopping = q->first_hopping;
+    r.n_sites = q->n_sites;
+    r.n_bins = q->n_bins;
+    r.n_omega = q->n_omega;
+    r.omega.assign(q->omega, q->omega + std::max(0, q->n_omega));
+    r.broadening = q->broadening;
+    r.fail_at_site = q->fail_at_site;
+    *id = srv->s->submit(std::move(r));
+  });
+}
+int nrgk_server_poll(nrgk_server* srv, long long id, int* done) {
+  return guard([&] {
+    NRGK_REQUIRE(srv && done, NRGK_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(srv->s->m);
+    auto it = srv->s->all.find(id);
+    NRGK_REQUIRE(it != srv->s->all.end(), NRGK_ERR_INVALID, "nrgk_server_poll: unknown request");
+    *done = it->second->done ? 1 : 0;
+  });
+}
+int nrgk_server_wait(nrgk_server* srv, long long id, double* A_pos, double* A_neg, double* info) {
+  std::shared_ptr<Request> r;
+  const int rc = guard([&] {
+    NRGK_REQUIRE(srv, NRGK_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(srv->s->m);
+    auto it = srv->s->all.find(id);
+    NRGK_REQUIRE(it != srv->s->all.end(), NRGK_ERR_INVALID, "nrgk_server_wait: unknown request");
+    r = it->second;
+    srv->s->cv.wait(lk, [&] { return r->done; });
+    srv->s->all.erase(it);
+  });
+  if (rc != NRGK_OK) return rc;
+  using ms = std::chrono::duration<double, std::milli>;
+  if (A_pos) std::copy(r->A_pos.begin(), r->A_pos.end(), A_pos);
+  if (A_neg) std::copy(r->A_neg.begin(), r->A_neg.end(), A_neg);
+  if (info) {
+    info[0] = ms(r->t_done - r->t_submit).count();
+    info[1] = ms(r->t_start - r->t_submit).count();
+    info[2] = r->total_weight;
+    info[3] = r->iterations;
+    info[4] = r->shells;
+    info[5] = (double)r->bytes;
+  }
+  if (r->status != NRGK_OK) set_last_error(r->message);
+  return r->status;
+}
+int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reserved, int* peak_running) {
+  return guard([&] {
+    NRGK_REQUIRE(srv, NRGK_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(srv->s->m);
+    if (running) *running = srv->s->running;
+    if (queued) *queued = (int)srv->s->pending.size();
+    if (reserved) *reserved = srv->s->reserved;
+    if (peak_running) *peak_running = srv->s->peak_running;
+  });
+}
+
 int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* w, int n, const double* omega, int n_omega, double b,
                  double* out) {
   return guard([&] {
@@ -511,6 +681,13 @@ int nrgk_chain_step_stats(nrgk_chain* ch, double* ms4, int* sweeps, int* max_clu
   });
 }
 
+int nrgk_chain_truncation_stats(nrgk_chain* ch, double* e_cut, int* near_ties) {
+  return guard([&] {
+    NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");
+    if (e_cut) *e_cut = ch->c->last_cut;
+    if (near_ties) *near_ties = ch->c->near_ties;
+  });
+}
 int nrgk_chain_step_counts(nrgk_chain* ch, long long* launches, long long* syncs) {
   return guard([&] {
     NRGK_REQUIRE(ch, NRGK_ERR_INVALID, "null chain");

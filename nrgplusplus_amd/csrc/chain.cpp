@@ -100,7 +100,7 @@ void Chain::init_device() {
   for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
                     &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_prev, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
                     &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act,
-                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv})
+                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv, &d_ht, &d_wide_ws, &d_wide_s, &d_wide_q})
     b->ctx = ctx;
   stg.ctx = ctx;
   NRGK_HIP(hipSetDevice(ctx->device));
@@ -168,15 +168,15 @@ hipEvent_t Chain::take_event() {
   ev_pool.pop_back();
   return e;
 }
-void Chain::prof_begin(const char* family) {
+void Chain::prof_begin(const char* family, hipStream_t s) {
   if (!prof_on || (!prof_only.empty() && prof_only != family)) return;
   prof_open = take_event();
-  NRGK_HIP(hipEventRecord(prof_open, st));
+  NRGK_HIP(hipEventRecord(prof_open, s ? s : st));
 }
-void Chain::prof_end(const char* family, long long launches, double flops, double bytes) {
+void Chain::prof_end(const char* family, long long launches, double flops, double bytes, hipStream_t s) {
   if (!prof_on || !prof_open) return;
   hipEvent_t e1 = take_event();
-  NRGK_HIP(hipEventRecord(e1, st));
+  NRGK_HIP(hipEventRecord(e1, s ? s : st));
   prof_pending.push_back(ProfRun{family, prof_open, e1, launches, flops, bytes});
   prof_open = nullptr;
 }
@@ -489,9 +489,22 @@ void Chain::eigensolve_mixed() {
     if (cur_n[s] > mixed_max) big.push_back(s);
   const size_t bytes64_all = std::max<long long>(1, tot_elems) * sizeof(double);
   dX.ensure(bytes64_all);  // every buffer the side stream touches is sized before it starts
+  dX2.ensure(bytes64_all);
   dU.ensure(bytes64_all);
   dlam_raw.ensure(std::max<long long>(1, N_tot) * sizeof(double));
-  large_blocks_begin(*this, big);
+  // the wide blocks: the FP64 Householder path (default) or, with NRGK_WIDE_SOLVER=rocsolver,
+  // rocSOLVER dsyevd per block on the side streams
+  const char* wsv = std::getenv("NRGK_WIDE_SOLVER");
+  const bool wide_rocsolver = wsv && std::string(wsv) == "rocsolver";
+  ht.clear();
+  wide_sectors.clear();
+  wide_state = WIDE_NONE;
+  if (wide_rocsolver) {
+    large_blocks_begin(*this, big);
+  } else {
+    large_blocks_begin(*this, {});
+    wide_begin(big);
+  }
   std::vector<EigBlock> blks;
   std::vector<int> row_block, row_local;
   std::vector<int2> tasks;
@@ -522,9 +535,14 @@ void Chain::eigensolve_mixed() {
   const int nb = (int)blks.size();
   if (nb == 0) {  // every block went to the side streams: no batched launch (empty grids)
     large_blocks_join(*this);
+    wide_join();
     sync();
     large_blocks_check(*this);
     sweeps = maxcluster = unresolved = oa_steps = oa_unconverged = 0;
+    if (need_full_u || fail_on_unconverged) {
+      wide_vectors(true);
+      sync();
+    }
     return;
   }
   const int nrows = (int)row_block.size();
@@ -816,10 +834,15 @@ void Chain::eigensolve_mixed() {
                     dU.as<double>(), dlam_raw.as<double>(), d_sidx.as<int>(), final_sort_lds(nmax_m), st);
   const int* hs = stg.get(d_stats.as<int>(), 4, st);
   large_blocks_join(*this);
+  wide_join();
   sync();
   maxcluster = hs[0];
   unresolved = hs[1];
   large_blocks_check(*this);
+  if (need_full_u || fail_on_unconverged) {
+    wide_vectors(true);
+    sync();
+  }
   // A block the refinement left short of its stopping point is solved again in FP64 by the direct
   // solver (the next Hamiltonian is built from these vectors, so they must meet the same bar as every
   // other block's); nrgk_chain_refine_stats reports how many were.  The stand-alone nrgk_eigh_batched
@@ -856,6 +879,9 @@ void Chain::add_bath_site(const double* t, int nt, double rescale) {
                "nrgk: add_bath_site called twice without update_internal_state");
   NRGK_REQUIRE(nt == (int)imp.nops, NRGK_ERR_INVALID,
                std::string("t_hopping ") + std::to_string(nt) + " =! pre_fdag_oparator " + std::to_string(imp.nops));
+  for (int p = 0; p < nt; p++)
+    NRGK_REQUIRE(std::isfinite(t[p]), NRGK_ERR_INVALID, "nrgk: add_bath_site: non-finite hopping");
+  NRGK_REQUIRE(std::isfinite(rescale), NRGK_ERR_INVALID, "nrgk: add_bath_site: non-finite rescale");
   step_l0 = thread_counts().launches;
   step_s0 = thread_counts().syncs;
   if (cnt == 0 && phase == FRESH) import_impurity();
@@ -905,13 +931,19 @@ void Chain::update_internal_state() {
   const int lnext = lcur ^ 1;
   dlam_shift[lnext].ensure(std::max<long long>(1, N_tot) * sizeof(double));
   NRGK_HIP(hipEventRecord(ev[3], st));
-  launch_truncate(d_bn, d_beo, S, dlam_raw.as<double>(), N_tot, (long long)K, errbar, d_kept.as<int>(), dout,
+  launch_truncate(d_bn, d_beo, S, dlam_raw.as<double>(), N_tot, (long long)K, errbar, tie_window, d_kept.as<int>(), dout,
                   dlam_shift[lnext].as<double>(), st);
   const int* h_kept = stg.get(d_kept.as<int>(), S, st);
   const TruncOut* h_out = stg.get(dout, 1, st);
   sync();
   kept.assign(h_kept, h_kept + S);
   const TruncOut out = *h_out;
+  last_cut = out.e_cut;
+  near_ties = out.near_ties;
+  NRGK_REQUIRE(out.nonfinite == 0, NRGK_ERR_NOCONV,
+               "nrgk: " + std::to_string(out.nonfinite) + " non-finite eigenvalue(s) in iteration " + std::to_string(cnt));
+  // the wide blocks' kept eigenvectors, before the f-dagger rotation reads them
+  if (wide_state == WIDE_EIGENVALUES) wide_vectors(false);
   NRGK_HIP(hipEventRecord(ev[4], st));
   all_eig.clear();  // sorted lazily from dlam_raw
   no_kept = (size_t)out.n_kept;

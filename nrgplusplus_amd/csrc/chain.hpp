@@ -89,6 +89,11 @@ struct Chain {
   int cnt = 0, cnt_min = 0;
   size_t no_kept = 0;
   double errbar = 1e-8;
+  // a state within tie_window of the cut (an eigenvalue error of that size could flip its keep or
+  // discard decision) is counted in near_ties by every truncation
+  double tie_window = 2e-10;
+  double last_cut = 0.0;
+  int near_ties = 0;
   bool keep_h = false;
   enum Phase { FRESH, ASSEMBLED, TRUNCATED } phase = FRESH;
 
@@ -142,6 +147,24 @@ struct Chain {
   hipEvent_t ev[6];
   std::vector<std::vector<double>> h_keep;  // assembled blocks (keep_h)
 
+  // wide blocks: FP64 Householder tridiagonalisation on side stream 0, eigenvectors when needed
+  // (wide.cpp)
+  std::vector<HtBlock> ht;          // widest first
+  std::vector<int> wide_sectors;    // sector of each ht entry
+  DevBuf d_ht, d_wide_ws, d_wide_s, d_wide_q;
+  double* wide_A = nullptr;  // the working matrices (then V_p T_p) and Householder vectors
+  double* wide_V = nullptr;
+  enum WideState { WIDE_NONE, WIDE_EIGENVALUES, WIDE_KEPT, WIDE_ALL } wide_state = WIDE_NONE;
+  bool wide_joined = true;
+  // every eigenvector of every block is formed in add_bath_site (observables and FDM shells read
+  // full U blocks); otherwise the wide blocks' kept columns only, after the truncation
+  bool need_full_u = false;
+  void ensure_side_streams();
+  void wide_begin(const std::vector<int>& sectors);
+  void wide_join();
+  void wide_vectors(bool all);
+  bool is_wide(int sector) const;
+
   // plain-mode handles (rocSOLVER / rocBLAS), created on first use
   void* rocblas = nullptr;
   // blocks wider than the batched solver's LDS panel: rocSOLVER on side streams (plain.cpp)
@@ -189,8 +212,8 @@ struct Chain {
   hipEvent_t prof_open = nullptr;
   hipEvent_t take_event();
   std::string prof_only;  // a single family to time (empty: all)
-  void prof_begin(const char* family);
-  void prof_end(const char* family, long long launches, double flops, double bytes);
+  void prof_begin(const char* family, hipStream_t s = nullptr);
+  void prof_end(const char* family, long long launches, double flops, double bytes, hipStream_t s = nullptr);
   void prof_resolve();
 
   Chain(Ctx* c, const nrgk_site_desc* imp, const nrgk_site_desc* bath, size_t K, int mode);

@@ -367,10 +367,63 @@ struct DiagItem {     // diagonal of block (i, i) of one system operator
   int n;
 };
 
+// Wide-block eigensolver (k_tridiag.hip): blocked Householder tridiagonalisation in FP64, bisection
+// for the eigenvalues, inverse iteration for the eigenvectors, WY back-transformation.  All n x n
+// arenas of a block share the block's element offset a_off (row-major, leading dimension n):
+//   A    working copy of H; the lower triangle is reduced in place; later the V T products
+//   Vall Householder vectors v_k in column k (v_k[k+1] = 1, rows above zero)
+//   U    the eigenvectors of T, then (in place) those of H, in columns 0 .. m-1
+// The per-block workspace (w_off into the workspace arena) holds the panel's W (n x 32), the symv
+// partials, the column under reduction and the tridiagonal (d, e, tau); see k_tridiag.hip.
+constexpr int HT_TB = 64;  // tile edge of the symmetric matrix-vector product and the rank-2k update
+constexpr int HT_NB = 32;  // panel width (reflectors per rank-2k update and per WY block)
+struct HtBlock {
+  long long a_off;  // element offset of the block's n x n arenas
+  long long w_off;  // element offset of its workspace
+  long long s_off;  // element offset of its inverse-iteration scratch (3 n x mpad)
+  int n;
+  int T;            // HT_TB tiles per dimension
+  int e_off;        // offset of the block's eigenvalues (prefix of n)
+  int m;            // eigenvectors to compute (columns 0 .. m-1)
+  int mpad;         // threads of the inverse-iteration launch for this block (multiple of 256)
+  int pad;
+};
+
+// The workspace of one block: W (n x HT_NB), the symv partials of the NT = T(T+1)/2 lower tiles
+// (row sums rp and column sums cp, 64 each, and the v^T A v share sp), the per-tile-row partials of
+// V^T v and W^T v (qv, qw: HT_NB each) and of the squared column norm (ss), the column under
+// reduction (u) and the norm partials (ss) in two buffers each by column parity -- a launch reads
+// column k's while it writes column k + 1's -- and the tridiagonal d, e with the reflector scales tau.
+struct HtWs {
+  double *W, *rp, *cp, *sp, *qv, *qw, *ss, *u, *d, *e, *tau;
+};
+__host__ __device__ inline long long ht_ws_size(int n, int T) {
+  const long long NT = (long long)T * (T + 1) / 2;
+  return (long long)n * HT_NB + NT * 129 + (long long)T * (2 * HT_NB + 2) + 5LL * n;
+}
+__host__ __device__ inline HtWs ht_ws(double* base, int n, int T) {
+  const long long NT = (long long)T * (T + 1) / 2;
+  HtWs w;
+  w.W = base;
+  w.rp = w.W + (long long)n * HT_NB;
+  w.cp = w.rp + NT * 64;
+  w.sp = w.cp + NT * 64;
+  w.qv = w.sp + NT;
+  w.qw = w.qv + (long long)T * HT_NB;
+  w.ss = w.qw + (long long)T * HT_NB;
+  w.u = w.ss + 2LL * T;
+  w.d = w.u + 2LL * n;
+  w.e = w.d + n;
+  w.tau = w.e + n;
+  return w;
+}
+
 struct TruncOut {
   double e_cut, e_min, e_max;
   int truncated;
   int n_kept;
+  int near_ties;  // eigenvalues within the tie window of the cut e_cut + errbar (either side)
+  int nonfinite;  // eigenvalues that are not finite (the step fails)
 };
 
 }  // namespace nrgk

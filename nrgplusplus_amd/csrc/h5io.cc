@@ -24,7 +24,7 @@ using hsize_t = unsigned long long;
 using hssize_t = long long;
 
 namespace {
-constexpr unsigned ACC_RDONLY = 0x0000u, ACC_TRUNC = 0x0002u;
+constexpr unsigned ACC_RDONLY = 0x0000u, ACC_RDWR = 0x0001u, ACC_TRUNC = 0x0002u;
 constexpr hid_t P_DEFAULT = 0, S_ALL = 0, E_DEFAULT = 0;
 constexpr int S_SCALAR = 0;
 
@@ -117,6 +117,11 @@ File::File(const std::string& path, bool write) : path_(path) {
   const Api& a = api();
   id_ = write ? a.Fcreate(path.c_str(), ACC_TRUNC, P_DEFAULT, P_DEFAULT) : a.Fopen(path.c_str(), ACC_RDONLY, P_DEFAULT);
   if (id_ < 0) throw std::runtime_error("nrgk: cannot " + std::string(write ? "create" : "open") + " HDF5 file " + path);
+}
+File::File(const std::string& path, Mode) : path_(path) {
+  const Api& a = api();
+  id_ = a.Fopen(path.c_str(), ACC_RDWR, P_DEFAULT);
+  if (id_ < 0) throw std::runtime_error("nrgk: cannot open HDF5 file " + path + " for writing");
 }
 File::~File() {
   if (id_ >= 0) api().Fclose(id_);

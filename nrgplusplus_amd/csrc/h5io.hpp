@@ -11,7 +11,9 @@ enum class Type { F64, I32, U64 };
 
 class File {
  public:
+  enum class Mode { ReadWrite };
   File(const std::string& path, bool write);
+  File(const std::string& path, Mode);  // an existing file, for further datasets
   ~File();
   File(const File&) = delete;
   File& operator=(const File&) = delete;

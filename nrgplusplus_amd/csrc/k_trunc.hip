@@ -47,8 +47,8 @@ constexpr long long TRUNC_LDS = 16384;  // eigenvalues staged in LDS up to this 
 
 __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, const int* __restrict__ beoff, int S,
                                                    const double* __restrict__ lam, long long N_tot, long long K,
-                                                   double errbar, int* __restrict__ kept, TruncOut* __restrict__ out,
-                                                   double* __restrict__ lam_shift) {
+                                                   double errbar, double tie_window, int* __restrict__ kept,
+                                                   TruncOut* __restrict__ out, double* __restrict__ lam_shift) {
   __shared__ double shd[16];
   __shared__ long long shl[16];
   extern __shared__ double lds_lam[];  // all eigenvalues, when they fit (the searches then run in LDS)
@@ -123,6 +123,14 @@ __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, c
     nk += k;
   }
   nk = block_reduce(nk, shl, [](long long a, long long d) { return a + d; });
+  // near-ties: states whose keep/discard decision an eigenvalue error of tie_window could flip
+  long long nt = 0, nf = 0;
+  for (long long i = tid; i < N_tot; i += blockDim.x) {
+    if (trunc) nt += fabs(L[i] - x) <= tie_window ? 1 : 0;
+    nf += isfinite(L[i]) ? 0 : 1;
+  }
+  nt = block_reduce(nt, shl, [](long long a, long long d) { return a + d; });
+  nf = block_reduce(nf, shl, [](long long a, long long d) { return a + d; });
   // shift every eigenvalue (not only the kept ones) when truncating (nrgcore.hpp:320-325); the
   // per-block eigenvalue ranges are contiguous prefix offsets covering [0, N_tot)
   for (long long i = tid; i < N_tot; i += blockDim.x) lam_shift[i] = trunc ? lam[i] - mn : lam[i];
@@ -132,6 +140,8 @@ __global__ void __launch_bounds__(1024) k_truncate(const int* __restrict__ bn, c
     out->e_max = mx;
     out->truncated = trunc ? 1 : 0;
     out->n_kept = (int)nk;
+    out->near_ties = (int)nt;
+    out->nonfinite = (int)nf;
   }
 }
 
@@ -141,9 +151,10 @@ void set_trunc_lds_limit() {
 }
 
 void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
-                     double errbar, int* kept, TruncOut* out, double* lam_shift, hipStream_t s) {
+                     double errbar, double tie_window, int* kept, TruncOut* out, double* lam_shift, hipStream_t s) {
   const size_t lds = N_tot <= TRUNC_LDS ? (size_t)std::max<long long>(1, N_tot) * sizeof(double) : 0;
-  NRGK_LAUNCH(k_truncate, dim3(1), dim3(1024), lds, s, bn, beoff, S, lam, N_tot, K, errbar, kept, out, lam_shift);
+  NRGK_LAUNCH(k_truncate, dim3(1), dim3(1024), lds, s, bn, beoff, S, lam, N_tot, K, errbar, tie_window, kept, out,
+              lam_shift);
   NRGK_HIP(hipGetLastError());
 }
 

@@ -30,10 +30,26 @@ void set_kernel_lds_limits(int bytes);
 // k_gemm.hip
 int prepare_gemm(std::vector<GemmProb>& probs);
 void launch_gemm(const GemmProb* d_probs, int n_probs, int tiles, hipStream_t s);
+// k_tridiag.hip (the wide blocks)
+int ht_grid(const std::vector<HtBlock>& bl, int k, int kind);
+void launch_ht_first(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* A, double* W, hipStream_t s);
+int launch_ht_symv(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, const double* V, double* W,
+                   hipStream_t s);
+int launch_ht_update(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, double* V, double* W,
+                     hipStream_t s);
+int launch_ht_syr2k(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, double* A, const double* V, double* W,
+                    hipStream_t s);
+void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, double* lam, hipStream_t s);
+// queue: 1 + 2 * (eigenvalues of the blocks) ints of device scratch
+void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
+                     double* S, double ctol_rel, int* queue, hipStream_t s);
+void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* V, const double* W, double* A,
+                     hipStream_t s);
+void set_ht_lds_limits(int bytes);
 // k_trunc.hip
 void set_trunc_lds_limit();
 void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
-                     double errbar, int* kept, TruncOut* out, double* lam_shift, hipStream_t s);
+                     double errbar, double tie_window, int* kept, TruncOut* out, double* lam_shift, hipStream_t s);
 // k_rotate.hip
 void launch_fdag_gather(const RotGroup* g, int ng, const RotSegment* s, const RotTri* t, const double* U,
                         const double* fbath, double* Y, hipStream_t st);

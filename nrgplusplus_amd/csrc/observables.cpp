@@ -21,6 +21,7 @@ namespace nrgk {
 Obs::Obs(Chain* c, int n_ops, const int* nblocks, const int* bi, const int* bj, const double* data) : ch(c), nops(n_ops) {
   for (DevBuf* b : {&buf[0], &buf[1], &dseg, &dC, &dY}) b->ctx = ch->ctx;
   NRGK_REQUIRE(ch->phase == Chain::FRESH, NRGK_ERR_STATE, "nrgk: observables are created before the first site");
+  ch->need_full_u = true;  // the rotation reads every column of U
   blocks.assign(nops, OpMap());
   std::vector<double> f;
   size_t bo = 0, dof = 0;
@@ -272,8 +273,26 @@ void Fdm::save(const std::string& path, const std::string& op_group) const {
   }
 }
 
+// the recorded shell with iteration counter cnt
+Shell* Fdm::shell_at(int cnt) const {
+  for (auto& s : shells)
+    if (s->cnt == cnt) return s.get();
+  throw Error(NRGK_ERR_STATE, "nrgk: no shell was recorded at iteration " + std::to_string(cnt));
+}
+
 void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, double* neg, double* traces,
                    double* sums, int cap, int* n_shells) {
+  std::vector<std::pair<Shell*, double>> use;
+  for (auto it = shells.rbegin(); it != shells.rend(); ++it)
+    if ((*it)->cnt >= first_cnt) use.push_back({it->get(), std::pow(lambda, -((double)(*it)->cnt + scale_offset) / 2.0)});
+  backward_shells(use, pos, neg, traces, sums, cap, n_shells);
+}
+
+// The backward pass over the given shells in the given order (the last shell of the chain first),
+// each with its energy scale: the loop of calcSpectrum(scale) calls (fdmSpectrum.hpp:55-63) of the
+// reference's drivers (examples/fdmSpectrumSiam/main.cpp:158-180).
+void Fdm::backward_shells(const std::vector<std::pair<Shell*, double>>& list, double* pos, double* neg, double* traces,
+                          double* sums, int cap, int* n_shells) {
   NRGK_REQUIRE(obs, NRGK_ERR_STATE, "nrgk: the FDM spectrum needs an observable");
   Chain& c = *ch;
   hipStream_t st = c.st;
@@ -286,8 +305,7 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
   NRGK_HIP(hipMemsetAsync(dpos.p, 0, (size_t)nops * nbins * sizeof(double), st));
   NRGK_HIP(hipMemsetAsync(dneg.p, 0, (size_t)nops * nbins * sizeof(double), st));
   std::vector<Shell*> use;
-  for (auto it = shells.rbegin(); it != shells.rend(); ++it)
-    if ((*it)->cnt >= first_cnt) use.push_back(it->get());
+  for (auto& u : list) use.push_back(u.first);
   int ns = 0;
   std::vector<int> kc;               // kept counts of the shell being processed (currentKeptIndex)
   std::vector<long long> red_off;    // reducedRho offsets (indexed by this shell's sectors)
@@ -296,7 +314,7 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
     Shell& sh = *use[u];
     const int S = (int)sh.n.size();
     const bool last = (u == 0);
-    const double scale = std::pow(lambda, -((double)sh.cnt + scale_offset) / 2.0);
+    const double scale = list[u].second;
     // currentKeptIndex (fdmSpectrum.hpp:289-298): empty at the last shell, else the kept set saved
     // with the later shell
     if (last) kc.assign(S, 0);
@@ -500,11 +518,21 @@ void Fdm::backward(double lambda, int scale_offset, int first_cnt, double* pos, 
 // fdmThermodynamics (fdmThermodynamics.hpp:42-236) over the recorded shells.
 void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S_out, double* C_out,
                  double* ops_out) {
+  std::vector<std::pair<Shell*, double>> use;
+  for (auto it = shells.rbegin(); it != shells.rend(); ++it)
+    if ((*it)->cnt >= first_cnt) use.push_back({it->get(), std::pow(lambda, -((double)(*it)->cnt + scale_offset) / 2.0)});
+  thermo_shells(use, ch->cnt, T, nT, S_out, C_out, ops_out);
+}
+
+// calcThermodynamics(scale) over the given shells in order (the last shell first) and saveFinalData
+// (fdmThermodynamics.hpp:42-236); nrg_max: nrg_iterations_cnt when the reference's fdmThermodynamics
+// was constructed (nrgMaxIterations, the shell prefactor's exponent base, :26-29, :162)
+void Fdm::thermo_shells(const std::vector<std::pair<Shell*, double>>& list, int nrg_max, const double* T, int nT,
+                        double* S_out, double* C_out, double* ops_out) {
   Chain& c = *ch;
   hipStream_t st = c.st;
   std::vector<Shell*> use;
-  for (auto it = shells.rbegin(); it != shells.rend(); ++it)
-    if ((*it)->cnt >= first_cnt) use.push_back(it->get());
+  for (auto& u : list) use.push_back(u.first);
   const int ns = (int)use.size();
   NRGK_REQUIRE(ns > 0, NRGK_ERR_STATE, "nrgk: no recorded shells");
   size_t bathDim = 0;
@@ -519,7 +547,7 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
     Shell& sh = *use[m];
     NRGK_REQUIRE(sh.q_ops >= nq && (nq == 0 || !sh.Q.empty()), NRGK_ERR_STATE,
                  "nrgk: a shell was recorded without the system operator");
-    const double scale = std::pow(lambda, -((double)sh.cnt + scale_offset) / 2.0);
+    const double scale = list[m].second;
     const std::vector<int> kc = m == 0 ? std::vector<int>(sh.n.size(), 0) : use[m - 1]->pre_kept;
     NRGK_REQUIRE(kc.size() == sh.n.size(), NRGK_ERR_STATE, "nrgk: recorded shells are not consecutive");
     double mn = 1e308;
@@ -565,7 +593,7 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
   d2h(m1.data(), dm1, nz, st);
   d2h(m2.data(), dm2, nz, st);
   NRGK_HIP(stream_sync(st));
-  const int nrgMax = c.cnt;
+  const int nrgMax = nrg_max;
   for (int t = 0; t < nT; t++) {
     const double kT = T[t];
     std::vector<double> wm(ns);

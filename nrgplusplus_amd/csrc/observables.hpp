@@ -65,6 +65,14 @@ struct Fdm {
   void save(const std::string& path, const std::string& op_group) const;
   void backward(double lambda, int scale_offset, int first_cnt, double* pos, double* neg, double* traces,
                 double* sums, int cap, int* n_shells);
+  void backward_shells(const std::vector<std::pair<Shell*, double>>& list, double* pos, double* neg, double* traces,
+                       double* sums, int cap, int* n_shells);
+  void thermo_shells(const std::vector<std::pair<Shell*, double>>& list, int nrg_max, const double* T, int nT,
+                     double* S, double* C, double* ops_out);
+  Shell* shell_at(int cnt) const;
+  // shells registered one at a time (calcSpectrum / calcThermodynamics of the reference's driver loop):
+  // (iteration counter, energy scale) in call order
+  std::vector<std::pair<int, double>> registered;
   // ops_out ([sysop ops][nT], may be null): the system operators' thermal expectation values
   void thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S, double* C,
               double* ops_out = nullptr);

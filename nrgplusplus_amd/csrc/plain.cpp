@@ -104,19 +104,16 @@ void large_blocks_begin(Chain& c, const std::vector<int>& sectors) {
   c.n_big = (int)sectors.size();
   c.big_used = 0;
   if (sectors.empty()) return;
-  if (!c.ev_big_ready) {
-    NRGK_HIP(hipEventCreateWithFlags(&c.ev_big_ready, hipEventDisableTiming));
-    for (int k = 0; k < Chain::N_SIDE; k++) {
-      NRGK_HIP(hipStreamCreateWithFlags(&c.st_big[k], hipStreamNonBlocking));
-      NRGK_HIP(hipEventCreateWithFlags(&c.ev_big_done[k], hipEventDisableTiming));
+  c.ensure_side_streams();
+  for (int k = 0; k < Chain::N_SIDE; k++)
+    if (!c.rocblas_big[k]) {
       rocblas_handle h;
       NRGK_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, NRGK_ERR_HIP, "rocblas_create_handle failed");
       rocblas_set_stream(h, c.st_big[k]);
       c.rocblas_big[k] = h;
       c.d_big_work[k].ctx = c.ctx;
     }
-    c.d_big_info.ctx = c.ctx;
-  }
+  c.d_big_info.ctx = c.ctx;
   std::vector<int> order(sectors);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return c.cur_n[x] > c.cur_n[y]; });
   double load[Chain::N_SIDE] = {0};

@@ -1,0 +1,176 @@
+// The wide sector blocks of an iteration (wider than the batched Jacobi's crossover): their
+// eigenvalues from a blocked FP64 Householder tridiagonalisation and bisection, on a side stream of
+// the chain beside the batched solve of the other blocks; their eigenvectors by inverse iteration and
+// WY back-transformation, only for the columns that are read (k_tridiag.hip).
+//
+// The reference diagonalises every sector with LAPACKE_dsyevd (qmatrix.hpp:375-404, called per
+// sector at nrgcore.hpp:85-89), which always forms every eigenvector.  Here a chain without
+// observables, FDM recording or an explicit request for U needs only the kept columns of the wide
+// blocks (the f-dagger rotation reads U_i(:, 0:k_i), nrgcore.hpp:466-467), so their eigenvectors are
+// formed after the truncation has fixed k_i; a later request for U (current_hamiltonQ) forms the rest.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "chain.hpp"
+#include "kernels.hpp"
+
+namespace nrgk {
+
+static double env_knob(const char* name, double dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atof(v) : dflt;
+}
+
+void Chain::ensure_side_streams() {
+  if (ev_big_ready) return;
+  NRGK_HIP(hipEventCreateWithFlags(&ev_big_ready, hipEventDisableTiming));
+  for (int k = 0; k < N_SIDE; k++) {
+    NRGK_HIP(hipStreamCreateWithFlags(&st_big[k], hipStreamNonBlocking));
+    NRGK_HIP(hipEventCreateWithFlags(&ev_big_done[k], hipEventDisableTiming));
+  }
+}
+
+// Tridiagonalisation and eigenvalues of the given sectors on side stream 0, started once the main
+// stream has assembled H.  Writes the eigenvalues into dlam_raw; wide_join() makes the main stream
+// wait for them.  Each block's working copy of H is in dX, its Householder vectors in dX2 (both n x n
+// at the block's offset), the workspace in d_wide_ws.
+void Chain::wide_begin(const std::vector<int>& sectors) {
+  wide_sectors = sectors;
+  std::stable_sort(wide_sectors.begin(), wide_sectors.end(), [&](int x, int y) { return cur_n[x] > cur_n[y]; });
+  ht.clear();
+  long long ws = 0;
+  int nmax = 1;
+  for (int s : wide_sectors) {
+    HtBlock b{};
+    b.n = cur_n[s];
+    b.T = (b.n + HT_TB - 1) / HT_TB;
+    b.a_off = cur_off[s];
+    b.w_off = ws;
+    b.e_off = cur_eoff[s];
+    ws += (ht_ws_size(b.n, b.T) + 31) / 32 * 32;
+    nmax = std::max(nmax, b.n);
+    ht.push_back(b);
+  }
+  wide_state = WIDE_NONE;
+  if (ht.empty()) return;
+  ensure_side_streams();
+  hipStream_t s2 = st_big[0];
+  d_wide_ws.ensure((size_t)ws * sizeof(double));
+  d_ht.ensure(ht.size() * sizeof(HtBlock));
+  // the descriptors go through the main stream's staging (the side stream starts after it)
+  const HtBlock* dbl = stg.put(ht, st);
+  NRGK_HIP(hipMemcpyAsync(d_ht.p, dbl, ht.size() * sizeof(HtBlock), hipMemcpyDeviceToDevice, st));
+  NRGK_HIP(hipEventRecord(ev_big_ready, st));
+  NRGK_HIP(hipStreamWaitEvent(s2, ev_big_ready, 0));
+  const HtBlock* d = d_ht.as<HtBlock>();
+  // the refinement of the other blocks swaps dX and dX2 between its steps: the wide blocks keep the
+  // buffers they started in
+  wide_A = dX.as<double>();
+  wide_V = dX2.as<double>();
+  double* A = wide_A;
+  double* V = wide_V;
+  double* W = d_wide_ws.as<double>();
+  for (const auto& b : ht) {
+    const size_t bytes = (size_t)b.n * b.n * sizeof(double);
+    NRGK_HIP(hipMemcpyAsync(A + b.a_off, dH.as<double>() + b.a_off, bytes, hipMemcpyDeviceToDevice, s2));
+    NRGK_HIP(hipMemsetAsync(V + b.a_off, 0, bytes, s2));
+  }
+  launch_ht_first(d, ht, A, W, s2);
+  // algorithmic bytes of the symmetric products (the lower triangle of the trailing matrix read once
+  // per column) and of the rank-2k updates (that triangle read and written once per panel)
+  prof_begin("tridiagonalise", s2);
+  double by = 0.0;
+  long long launches = 0;
+  for (int k = 0; k + 2 <= ht[0].n; k++) {
+    if (prof_on)
+      for (const auto& b : ht) {
+        if (k > b.n - 2) break;
+        const double m = b.n - k - 1;
+        by += 8.0 * m * (m + 1) / 2;
+        if (k % HT_NB == HT_NB - 1 || k == b.n - 2) by += 16.0 * m * (m + 1) / 2;
+      }
+    launches += launch_ht_symv(d, ht, k, A, V, W, s2) > 0;
+    launches += launch_ht_update(d, ht, k, A, V, W, s2) > 0;
+    launches += launch_ht_syr2k(d, ht, k, A, V, W, s2) > 0;
+  }
+  prof_end("tridiagonalise", launches, 0.0, by, s2);
+  prof_begin("tridiag_eigenvalues", s2);
+  launch_ht_stebz(d, ht, W, dlam_raw.as<double>(), s2);
+  prof_end("tridiag_eigenvalues", 1, 0.0, 0.0, s2);
+  NRGK_HIP(hipEventRecord(ev_big_done[0], s2));
+  wide_state = WIDE_EIGENVALUES;
+  wide_joined = false;
+}
+
+void Chain::wide_join() {
+  if (ht.empty() || wide_joined) return;
+  NRGK_HIP(hipStreamWaitEvent(st, ev_big_done[0], 0));
+  wide_joined = true;
+}
+
+// Eigenvectors of the wide blocks on the main stream: all columns, or columns 0 .. kept - 1 (the
+// truncation has run).  Inverse iteration on T into U's columns, then U = Q Z by the WY products.
+void Chain::wide_vectors(bool all) {
+  if (ht.empty() || wide_state == WIDE_NONE) return;
+  if (wide_state == WIDE_ALL || (!all && wide_state == WIDE_KEPT)) return;
+  wide_join();
+  std::vector<HtBlock> bl;
+  long long so = 0;
+  for (size_t i = 0; i < ht.size(); i++) {
+    HtBlock b = ht[i];
+    const int s = wide_sectors[i];
+    b.m = all ? b.n : std::min(b.n, kept.empty() ? b.n : kept[s]);
+    if (b.m <= 0) continue;
+    b.mpad = (b.m + 255) / 256 * 256;
+    b.s_off = so;
+    so += 3LL * b.n * b.mpad;
+    bl.push_back(b);
+  }
+  if (bl.empty()) {
+    wide_state = all ? WIDE_ALL : WIDE_KEPT;
+    return;
+  }
+  d_wide_s.ensure((size_t)so * sizeof(double));
+  long long nq = 0;
+  for (const auto& b : bl) nq += b.m;
+  d_wide_q.ensure((size_t)(1 + 2 * nq) * sizeof(int));
+  const HtBlock* d = stg.put(bl, st);
+  double* W = d_wide_ws.as<double>();
+  double* U = dU.as<double>();
+  // clusters: eigenvalues within 1e-5 ||T|| of their neighbour.  Inverse iteration leaves vectors of
+  // eigenvalues g apart orthogonal to about eps ||T|| / g, so that bound keeps the rest at ~2e-11
+  const double ctol = env_knob("NRGK_WIDE_CLUSTER", 1e-5);
+  prof_begin("tridiag_vectors");
+  launch_ht_stein(d, bl, W, dlam_raw.as<double>(), U, d_wide_s.as<double>(), ctol, d_wide_q.as<int>(), st);
+  prof_end("tridiag_vectors", 1, 0.0, 0.0);
+  // V_p T_p over each block's working matrix (no longer needed), then per panel, last first:
+  // Y = V_p^T Z, Z -= (V_p T_p) Y
+  launch_ht_larft(d, bl, wide_V, W, wide_A, st);
+  int pmax = 0;
+  for (const auto& b : bl) pmax = std::max(pmax, (b.n - 1 + HT_NB - 1) / HT_NB);
+  double* Ys = d_wide_s.as<double>();  // the inverse-iteration scratch is free again
+  for (int step = 0; step < pmax; step++) {
+    std::vector<GemmProb> g1, g2;
+    for (const auto& b : bl) {
+      const int P = (b.n - 1 + HT_NB - 1) / HT_NB;
+      if (step >= P) continue;
+      const int p = P - 1 - step, j0 = p * HT_NB, nbp = std::min(HT_NB, b.n - 1 - j0), R = b.n - j0 - 1;
+      const double* Vp = wide_V + b.a_off + (long long)(j0 + 1) * b.n + j0;
+      const double* VTp = wide_A + b.a_off + (long long)(j0 + 1) * b.n + j0;
+      double* Z = U + b.a_off + (long long)(j0 + 1) * b.n;
+      double* Y = Ys + b.s_off;
+      g1.push_back(GemmProb{Vp, Z, nullptr, Y, nbp, b.m, R, b.n, b.n, b.m, b.m, 1, 0, 1.0, 0.0, 0, 0});
+      g2.push_back(GemmProb{VTp, Y, Z, Z, R, b.m, nbp, b.n, b.m, b.n, b.n, 0, 0, -1.0, 1.0, 0, 0});
+    }
+    run_gemms(g1);
+    run_gemms(g2);
+  }
+  wide_state = all ? WIDE_ALL : WIDE_KEPT;
+}
+
+bool Chain::is_wide(int sector) const {
+  return std::find(wide_sectors.begin(), wide_sectors.end(), sector) != wide_sectors.end() && !ht.empty();
+}
+
+}  // namespace nrgk

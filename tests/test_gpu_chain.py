@@ -535,3 +535,30 @@ def test_cpp_nrgcore_api_rgflow_matches_oracle(oracle_mod, n_copies):
         vals = np.array(got[2:-1])
         assert len(vals) == len(ref[n])
         assert np.abs(vals - ref[n]).max() < 1e-9, (n, np.abs(vals - ref[n]).max())
+
+
+@pytest.mark.gpu
+def test_cpp_reference_form_entropy_driver(golden, tmp_path):
+    """examples/entropySIAM/main.cpp written against include/nrgk/nrgcore.hpp with the reference's own
+    calls in their order (NrgData saveCurrentData / loadCurrentData, a backward loop decrementing
+    nrg_iterations_cnt, fdmThermodynamics(&siam, T).calcThermodynamics(scale) / saveFinalData,
+    h5stream writes): its HDF5 output against the reference's resultSIAMU0.200000.h5 (entropy to 1e-8,
+    specific heat to 2e-6 relative, where the exp clamp does not dominate)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "nrgplusplus_amd", "nrgk_entropy_siam")
+    if not os.path.exists(exe):
+        pytest.fail("nrgk_entropy_siam is not built (run __graft_entry__.build())")
+    out = str(tmp_path / "resultSIAM.h5")
+    r = subprocess.run([exe, out], capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stderr
+    import nrgplusplus_amd as nk
+    gd = golden("siam_U02_L25_K512")
+    T = nk.h5_read(out, "temperatureArray")
+    S, Cv = nk.h5_read(out, "entropy"), nk.h5_read(out, "specificHeat")
+    np.testing.assert_allclose(T, gd["temperatureArray"], rtol=1e-14)
+    m = T >= 3e-7
+    assert np.abs(S - gd["entropy"])[m].max() < 1e-8
+    rel = np.abs(Cv - gd["specificHeat"])[m] / np.maximum(np.abs(gd["specificHeat"][m]), 1e-3)
+    assert rel.max() < 2e-6
+    assert len(nk.h5_read(out, "Eigenvalues40")) > 0

@@ -104,11 +104,9 @@ def test_two_channel_workload_at_full_shape(nk, oracle_mod):
     its real shape: iteration 2 (16 384 states, blocks up to 1 225) in lockstep with the oracle (sector
     labels and dims, assembled Hamiltonians and kept counts bit-exact), then iterations 3-4 (66 048
     states, 28 blocks wider than 768, up to ~4 200) free-running.  Every step: each block's eigenvalues
-    within 1e-10 of LAPACK on the same H, the widest blocks' eigenvectors orthonormal and their residual
-    ||H U - U diag(E)|| within 1e-10, kept counts equal to the truncation rule applied to the LAPACK
-    eigenvalues."""
-    import torch
-    dev = torch.device("cuda:0")
+    within 1e-10 of LAPACK on the same H, the two widest blocks' eigenvectors orthonormal and their
+    residual ||H U - U diag(E)|| within 1e-10, kept counts equal to the truncation rule applied to the
+    LAPACK eigenvalues."""
     eps, U, L, K = -0.2, 0.4, 3.0, 4096
     V = np.sqrt(2 * 0.01 / np.pi)
     g = nk.NrgCore(nk.Site.spinhalf(eps, U).duplicate_ops(), nk.Site.two_channel(), K)
@@ -129,7 +127,7 @@ def test_two_channel_workload_at_full_shape(nk, oracle_mod):
             assert np.array_equal(qn, o.cur_qn()) and np.array_equal(dims, o.sector_dims())
         eq = g.eigenvaluesQ
         lap = []
-        wide = sorted(range(len(dims)), key=lambda s: -dims[s])[:6]
+        wide = sorted(range(len(dims)), key=lambda s: -dims[s])[:2]
         for s, n in enumerate(dims):
             if n == 0:
                 lap.append(np.zeros(0))
@@ -141,11 +139,9 @@ def test_two_channel_workload_at_full_shape(nk, oracle_mod):
             lap.append(w)
             assert np.abs(w - eq[s]).max() <= 1e-10, (it, s, n, np.abs(w - eq[s]).max())
             if s in wide and n > 1:
-                Ht = torch.from_numpy(H).to(dev)
-                Ut = torch.from_numpy(g.current_hamiltonQ(s)).to(dev)
-                Et = torch.from_numpy(np.ascontiguousarray(eq[s])).to(dev)
-                orth = (Ut.T @ Ut - torch.eye(n, dtype=torch.float64, device=dev)).abs().max().item()
-                res = (Ht @ Ut - Ut * Et).abs().max().item()
+                Us = g.current_hamiltonQ(s)
+                orth = np.abs(Us.T @ Us - np.eye(n)).max()
+                res = np.abs(H @ Us - Us * eq[s]).max()
                 assert orth <= 1e-10 and res <= 1e-10, (it, s, n, orth, res)
         g.update_internal_state()
         kept = g.kept_counts()

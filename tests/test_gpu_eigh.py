@@ -140,3 +140,69 @@ def test_unconverged_refinement_is_an_error(nk, monkeypatch):
     Q = _haar(rng, 200)
     with pytest.raises(nk.NrgkError, match="unconverged"):
         nk.eigh_batched([(Q * lam) @ Q.T])
+
+
+# ---------------------------------------------------------------- wide blocks: Householder path
+def _case_sizes(seed, sizes, kinds):
+    rng = np.random.default_rng(seed)
+    blocks, lams = [], []
+    for i, n in enumerate(sizes):
+        lam = _spectrum(rng, n, kinds[i % len(kinds)])
+        Q = _haar(rng, n)
+        H = (Q * lam) @ Q.T
+        blocks.append(0.5 * (H + H.T))
+        lams.append(lam)
+    return blocks, lams
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_wide_blocks_householder(nk, seed):
+    """blocks above the crossover (769 .. 1600 rows) through the FP64 Householder tridiagonalisation,
+    bisection, inverse iteration and WY back-transformation, beside narrow blocks in the same batch:
+    degenerate levels (clusters of ~200 members), clusters at 1e-9 .. 1e-13, graded and uniform
+    spectra"""
+    blocks, lams = _case_sizes(seed, [1600, 40, 1025, 769, 300, 1200],
+                               ["degenerate", "uniform", "clustered", "graded", "uniform", "uniform"])
+    _check(blocks, lams, nk.eigh_batched(blocks))
+
+
+@pytest.mark.parametrize("mixed_max", ["0", "40"])
+def test_householder_path_on_small_blocks(nk, monkeypatch, mixed_max):
+    """the crossover moved down (NRGK_MIXED_MAX): blocks of 1 .. 300 rows, below and above one 64-row
+    tile, and the special blocks (zero, identity, repeated diagonal, tiny off-diagonal) through the
+    Householder path"""
+    monkeypatch.setenv("NRGK_MIXED_MAX", mixed_max)
+    blocks, lams = _case_sizes(31, [1, 2, 3, 41, 63, 64, 65, 127, 129, 300],
+                               ["degenerate", "clustered", "graded", "uniform", "threshold"])
+    _check(blocks, lams, nk.eigh_batched(blocks))
+    rng = np.random.default_rng(7)
+    d = np.sort(rng.choice([0.5, -0.25, 2.0], 40))
+    special = [np.zeros((5, 5)), np.diag(d), np.eye(24), np.array([[1.0, 1e-14], [1e-14, 1.0 + 1e-13]])]
+    out = nk.eigh_batched(special)
+    np.testing.assert_array_equal(out[0][0], np.zeros(5))
+    np.testing.assert_allclose(out[1][0], d, atol=1e-14)
+    np.testing.assert_allclose(out[2][0], np.ones(24), atol=1e-14)
+    for H, (w, U) in zip(special, out):
+        n = len(w)
+        assert np.abs(U.T @ U - np.eye(n)).max() <= 1e-12
+        assert np.abs(H @ U - U * w).max() <= 1e-12 * max(1.0, np.abs(w).max())
+
+
+def test_upper_triangle_only_input(nk, monkeypatch):
+    """LAPACKE_dsyevd(ROW_MAJOR, 'U') reads the upper triangle only (qmatrix.hpp:390-391): a block
+    given with its strict lower triangle zeroed has the eigenpairs of its symmetric completion, below
+    and above the crossover and in the plain FP64 form"""
+    rng = np.random.default_rng(41)
+    blocks, full = [], []
+    for n in (50, 800):
+        lam = np.sort(rng.uniform(-1, 1, n))
+        Q = _haar(rng, n)
+        H = (Q * lam) @ Q.T
+        H = 0.5 * (H + H.T)
+        full.append((H, lam))
+        blocks.append(np.triu(H))
+    for mode in (nk.NRGK_MODE_MIXED, nk.NRGK_MODE_PLAIN64):
+        out = nk.eigh_batched(blocks, mode=mode)
+        for (H, lam), (w, U) in zip(full, out):
+            assert np.abs(w - lam).max() <= 1e-10
+            assert np.abs(H @ U - U * w).max() <= 1e-10
