@@ -120,6 +120,17 @@ int nrgk_ctx_destroy(nrgk_ctx* ctx);
 int nrgk_ctx_set_capacity(nrgk_ctx* ctx, size_t bytes);
 int nrgk_ctx_used_bytes(nrgk_ctx* ctx, size_t* used, size_t* capacity);
 
+/* Autotuner (no reference counterpart): measures, on batches of random symmetric blocks of widths
+ * 192..768, the batched Jacobi path (all eigenvectors) against the Householder path (eigenvalues and
+ * the first quarter of the eigenvectors) and sets the context's crossover to the widest width at which
+ * the Jacobi path is not slower.  The result and the measured times go to a cache file (cache_path, or
+ * NULL: $NRGK_TUNE_CACHE, else $HOME/.cache/nrgk/tune-<arch>-<CUs>cu.txt) that every context created
+ * later on the same GPU model reads.  table[n_rows][4]: width, blocks, Jacobi ms, Householder ms (may
+ * be NULL).  NRGK_MIXED_MAX, when set, overrides the crossover per call. */
+int nrgk_tune(nrgk_ctx* ctx, const char* cache_path, int* crossover, double* table, int cap_rows, int* n_rows);
+/* the context's crossover and whether it came from the autotuner (its cache or nrgk_tune) */
+int nrgk_ctx_crossover(nrgk_ctx* ctx, int* crossover, int* tuned);
+
 /* ---------------------------------------------------------------- chain (nrgcore) */
 /* nrgcore(im_type&, bath_type&) (nrgcore.hpp:44-53) + set_parameters(max_kept) (nrgcore.hpp:336).
  * The model-consistency checks of nrgcore::test (nrgcore.hpp:117-134) return NRGK_ERR_INVALID. */
@@ -275,6 +286,12 @@ int nrgk_fdm_save(nrgk_fdm* rec, const char* path, const char* op_group);
  * needed. */
 int nrgk_h5_create(const char* path);
 int nrgk_h5_write(const char* path, const char* name, const double* data, size_t n);
+/* NrgData::loadCurrentData + loadqOperator (nrgcore/include/nrgcore/nrgData.hpp:214-293) for every
+ * shell of a file nrgk_fdm_save wrote: the recorder's shells are replaced by the file's, rebuilt on the
+ * device (U blocks, eigenvalues, sectors, coupled index, kept counts and, when the recorder has an
+ * observable, its blocks from {op_group}{cnt-1}), so that the backward pass or the thermodynamics runs
+ * in a later process than the forward pass.  The recorder's chain must have the file's bath model. */
+int nrgk_fdm_load(nrgk_fdm* rec, const char* path, const char* op_group);
 /* a one-dimensional dataset of an HDF5 file as doubles (n: its length; out may be NULL) */
 int nrgk_h5_read(const char* path, const char* dataset, double* out, size_t cap, size_t* n);
 /* Per operator, the signed weights w+ + w- of the last nrgk_fdm_backward summed over its shells,
@@ -382,8 +399,10 @@ int nrgk_server_poll(nrgk_server* srv, long long id, int* done);
  * receives 6 doubles: latency ms (submission to completion), queue ms (submission to admission),
  * total FDM weight, iterations, recorded shells, admitted footprint in bytes.  The request is released. */
 int nrgk_server_wait(nrgk_server* srv, long long id, double* A_pos, double* A_neg, double* info);
-/* running and queued requests, bytes reserved by the running ones, the most that ran at once */
-int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reserved, int* peak_running);
+/* running and queued requests, bytes reserved by the running ones, the most that ran at once and the
+ * most bytes they reserved at once (every pointer may be NULL) */
+int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reserved, int* peak_running,
+                      size_t* peak_reserved);
 
 /* ---------------------------------------------------------------- broadening */
 /* log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44): out[o] = (1/w_o) sum_k

@@ -147,16 +147,19 @@ def lib():
             "nrgk_fdm_signed_weights": (i, [vp, dp, i, ip]),
             "nrgk_thermo": (i, [vp, d, i, i, dp, i, dp, dp]),
             "nrgk_fdm_save": (i, [vp, C.c_char_p, C.c_char_p]),
+            "nrgk_fdm_load": (i, [vp, C.c_char_p, C.c_char_p]),
             "nrgk_h5_read": (i, [C.c_char_p, C.c_char_p, dp, sz, szp]),
             "nrgk_fdm_set_system_operator": (i, [vp, vp]),
             "nrgk_thermo_expectation": (i, [vp, d, i, i, dp, i, dp]),
             "nrgk_broaden": (i, [vp, dp, dp, i, dp, i, d, dp]),
+            "nrgk_tune": (i, [vp, C.c_char_p, ip, dp, i, ip]),
+            "nrgk_ctx_crossover": (i, [vp, ip, ip]),
             "nrgk_server_create": (i, [vp, i, sz, C.POINTER(vp)]),
             "nrgk_server_destroy": (i, [vp]),
             "nrgk_server_submit": (i, [vp, C.POINTER(Request), C.POINTER(C.c_longlong)]),
             "nrgk_server_poll": (i, [vp, C.c_longlong, ip]),
             "nrgk_server_wait": (i, [vp, C.c_longlong, dp, dp, dp]),
-            "nrgk_server_stats": (i, [vp, ip, ip, szp, ip]),
+            "nrgk_server_stats": (i, [vp, ip, ip, szp, ip, szp]),
             "nrgk_zdisc_chain": (i, [d, d, d, dp, dp, i, i, i, i, i, dp, dp]),
         }
         for name, (res, args) in sig.items():
@@ -376,6 +379,21 @@ class Context:
 
     def set_capacity(self, nbytes):
         _check(lib().nrgk_ctx_set_capacity(self._h, nbytes))
+
+    def tune(self, cache_path=None):
+        """autotune the eigensolver crossover (nrgk_tune): (crossover, rows of (width, blocks, jacobi ms,
+        householder ms))"""
+        cross, nr = C.c_int(), C.c_int()
+        tab = np.zeros(4 * 32)
+        _check(lib().nrgk_tune(self._h, os.fsencode(cache_path) if cache_path else None, C.byref(cross), _dp(tab),
+                               32, C.byref(nr)))
+        return cross.value, tab[: 4 * nr.value].reshape(nr.value, 4)
+
+    def crossover(self):
+        """(crossover, tuned): the block width above which the Householder path solves a block"""
+        c, t = C.c_int(), C.c_int()
+        _check(lib().nrgk_ctx_crossover(self._h, C.byref(c), C.byref(t)))
+        return c.value, bool(t.value)
 
     def used_bytes(self):
         u, c = C.c_size_t(), C.c_size_t()
@@ -682,6 +700,11 @@ class FdmRecorder:
         the reference's HDF5 layout; op_group=None leaves the observable out"""
         _check(lib().nrgk_fdm_save(self._h, os.fsencode(path), op_group.encode() if op_group else None))
 
+    def load(self, path, op_group="/nparticle"):
+        """NrgData::loadCurrentData + loadqOperator (nrgData.hpp:214-293) of every shell of a file save()
+        wrote: the recorder's shells are replaced by the file's"""
+        _check(lib().nrgk_fdm_load(self._h, os.fsencode(path), op_group.encode() if op_group else None))
+
     def setSystemOperator(self, op):
         """fdmThermodynamics::setSystemOperator (static/fdmThermodynamics.hpp:39-41): shells recorded
         from now on keep the diagonals of ``op`` (a SystemOperator updated before each record)."""
@@ -842,9 +865,10 @@ class Server:
 
     def stats(self):
         r, q, pk = C.c_int(), C.c_int(), C.c_int()
-        res = C.c_size_t()
-        _check(lib().nrgk_server_stats(self._h, C.byref(r), C.byref(q), C.byref(res), C.byref(pk)))
-        return dict(running=r.value, queued=q.value, reserved=res.value, peak_running=pk.value)
+        res, pres = C.c_size_t(), C.c_size_t()
+        _check(lib().nrgk_server_stats(self._h, C.byref(r), C.byref(q), C.byref(res), C.byref(pk), C.byref(pres)))
+        return dict(running=r.value, queued=q.value, reserved=res.value, peak_running=pk.value,
+                    peak_reserved=pres.value)
 
     def close(self):
         if getattr(self, "_h", None) is not None:

@@ -67,12 +67,37 @@ int nrgk_ctx_create(int device, nrgk_ctx** out) {
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess && optin > lds)
       lds = optin;
     h->c.lds_max = lds - 1024;  // room for the kernels' static LDS
+    h->c.arch = prop.gcnArchName;
+    const size_t colon = h->c.arch.find(':');
+    if (colon != std::string::npos) h->c.arch.resize(colon);
     set_kernel_lds_limits(h->c.lds_max);
     set_ht_lds_limits(h->c.lds_max);
+    tune_load(h->c);
     size_t fr = 0, tot = 0;
     NRGK_HIP(hipMemGetInfo(&fr, &tot));
     h->c.capacity = (size_t)(0.9 * (double)fr);
     *out = h;
+  });
+}
+int nrgk_tune(nrgk_ctx* ctx, const char* cache_path, int* crossover, double* table, int cap_rows, int* n_rows) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx, NRGK_ERR_INVALID, "null context");
+    NRGK_HIP(hipSetDevice(ctx->c.device));
+    std::vector<std::vector<double>> t;
+    const std::string path = cache_path ? std::string(cache_path) : tune_cache_path(ctx->c);
+    const int c = tune_crossover(&ctx->c, path, t);
+    if (crossover) *crossover = c;
+    if (n_rows) *n_rows = (int)t.size();
+    if (table)
+      for (int r = 0; r < std::min(cap_rows, (int)t.size()); r++)
+        for (int k = 0; k < 4; k++) table[4 * r + k] = t[r][k];
+  });
+}
+int nrgk_ctx_crossover(nrgk_ctx* ctx, int* crossover, int* tuned) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx, NRGK_ERR_INVALID, "null context");
+    if (crossover) *crossover = ctx->c.mixed_max;
+    if (tuned) *tuned = ctx->c.tuned ? 1 : 0;
   });
 }
 int nrgk_ctx_destroy(nrgk_ctx* ctx) {
@@ -484,6 +509,12 @@ int nrgk_thermo(nrgk_fdm* rec, double lambda, int scale_offset, int first_cnt, c
     rec->f->thermo(lambda, scale_offset, first_cnt, T, nT, S, C);
   });
 }
+int nrgk_fdm_load(nrgk_fdm* rec, const char* path, const char* op_group) {
+  return guard([&] {
+    NRGK_REQUIRE(rec && path, NRGK_ERR_INVALID, "nrgk_fdm_load: null argument");
+    rec->f->load(path, op_group ? op_group : "");
+  });
+}
 int nrgk_fdm_save(nrgk_fdm* rec, const char* path, const char* op_group) {
   return guard([&] {
     NRGK_REQUIRE(rec && path, NRGK_ERR_INVALID, "nrgk_fdm_save: null argument");
@@ -598,7 +629,8 @@ int nrgk_server_wait(nrgk_server* srv, long long id, double* A_pos, double* A_ne
   if (r->status != NRGK_OK) set_last_error(r->message);
   return r->status;
 }
-int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reserved, int* peak_running) {
+int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reserved, int* peak_running,
+                      size_t* peak_reserved) {
   return guard([&] {
     NRGK_REQUIRE(srv, NRGK_ERR_INVALID, "null server");
     std::lock_guard<std::mutex> lk(srv->s->m);
@@ -606,6 +638,7 @@ int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reser
     if (queued) *queued = (int)srv->s->pending.size();
     if (reserved) *reserved = srv->s->reserved;
     if (peak_running) *peak_running = srv->s->peak_running;
+    if (peak_reserved) *peak_reserved = srv->s->peak_reserved;
   });
 }
 
@@ -616,15 +649,17 @@ int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* w, int n, const d
                  "nrgk_broaden: bad argument");
     NRGK_HIP(hipSetDevice(ctx->c.device));
     DevBuf d(&ctx->c);
-    d.ensure(((size_t)2 * n + 2 * n_omega + 4) * sizeof(double));
+    const size_t ns = broaden_scratch_doubles(n, n_omega);
+    d.ensure(((size_t)2 * n + 2 * n_omega + ns + 4) * sizeof(double));
     double* dx = d.as<double>();
     double* dw = dx + n;
     double* dom = dw + n;
     double* dout = dom + n_omega;
+    double* dscr = dout + n_omega;
     h2d(dx, x, n, nullptr);
     h2d(dw, w, n, nullptr);
     h2d(dom, omega, n_omega, nullptr);
-    launch_broaden(dx, dw, n, dom, n_omega, b, dout, nullptr);
+    launch_broaden(dx, dw, n, dom, n_omega, b, dout, dscr, nullptr);
     d2h(out, dout, n_omega, nullptr);
     NRGK_HIP(stream_sync(nullptr));
   });

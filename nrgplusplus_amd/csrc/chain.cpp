@@ -100,7 +100,7 @@ void Chain::init_device() {
   for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
                     &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_prev, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
                     &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act,
-                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv, &d_ht, &d_wide_ws, &d_wide_s, &d_wide_q})
+                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv, &d_ht, &d_wide_ws, &d_wide_s})
     b->ctx = ctx;
   stg.ctx = ctx;
   NRGK_HIP(hipSetDevice(ctx->device));
@@ -111,6 +111,21 @@ void Chain::init_device() {
 // qmatrix::diag (qmatrix.hpp:375-404) for a list of blocks: the blocks are laid out as an
 // iteration's sector blocks are (packed, offsets n^2 apart) and go through the same eigensolver.
 void Chain::solve_blocks(const std::vector<int>& dims, const double* H, double* w, double* U) {
+  prepare_blocks(dims, H);
+  if (N_tot == 0) return;
+  fail_on_unconverged = true;
+  eigensolve();
+  // a chain re-solves blocks that the refinement left short of its stopping point; the stand-alone
+  // solver fails instead, as dsyevd reports a nonzero info
+  NRGK_REQUIRE(mode == NRGK_MODE_PLAIN64 || oa_unconverged == 0, NRGK_ERR_NOCONV,
+               "nrgk_eigh_batched: the FP64 refinement left " + std::to_string(oa_unconverged) +
+                   " block(s) unconverged");
+  NRGK_HIP(hipMemcpyAsync(w, dlam_raw.p, (size_t)N_tot * sizeof(double), hipMemcpyDeviceToHost, st));
+  NRGK_HIP(hipMemcpyAsync(U, dU.p, (size_t)tot_elems * sizeof(double), hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+void Chain::prepare_blocks(const std::vector<int>& dims, const double* H) {
   const int S = (int)dims.size();
   cur_qn.assign(S, std::vector<int>(1, 0));
   cur_n = dims;
@@ -140,16 +155,8 @@ void Chain::solve_blocks(const std::vector<int>& dims, const double* H, double* 
   }
   dH.ensure((size_t)tot_elems * sizeof(double));
   NRGK_HIP(hipMemcpyAsync(dH.p, Hs.data(), (size_t)tot_elems * sizeof(double), hipMemcpyHostToDevice, st));
-  fail_on_unconverged = true;
-  eigensolve();
-  // a chain re-solves blocks that the refinement left short of its stopping point; the stand-alone
-  // solver fails instead, as dsyevd reports a nonzero info
-  NRGK_REQUIRE(mode == NRGK_MODE_PLAIN64 || oa_unconverged == 0, NRGK_ERR_NOCONV,
-               "nrgk_eigh_batched: the FP64 refinement left " + std::to_string(oa_unconverged) +
-                   " block(s) unconverged");
-  NRGK_HIP(hipMemcpyAsync(w, dlam_raw.p, (size_t)N_tot * sizeof(double), hipMemcpyDeviceToHost, st));
-  NRGK_HIP(hipMemcpyAsync(U, dU.p, (size_t)tot_elems * sizeof(double), hipMemcpyDeviceToHost, st));
-  sync();
+  // the host copy must outlive the asynchronous copy
+  NRGK_HIP(stream_sync(st));
 }
 
 void Chain::sync() {
@@ -482,8 +489,9 @@ static int choose_bw(int n) { return n <= 32 ? std::max(1, (n + 1) / 2) : 16; }
 // K2 + K6: FP32 block Jacobi, then two Ogita-Aishima refinement steps in FP64.
 void Chain::eigensolve_mixed() {
   const int S = (int)cur_qn.size();
-  // blocks wider than the LDS-resident Jacobi panel go to rocSOLVER on the side stream
-  const int mixed_max = (int)knob("NRGK_MIXED_MAX", 768);
+  // blocks wider than the crossover (the context's, tuned or 768; NRGK_MIXED_MAX overrides it) go to
+  // the wide-block path on the side stream
+  const int mixed_max = mixed_max_override >= 0 ? mixed_max_override : (int)knob("NRGK_MIXED_MAX", ctx->mixed_max);
   std::vector<int> big;
   for (int s = 0; s < S; s++)
     if (cur_n[s] > mixed_max) big.push_back(s);
@@ -495,7 +503,9 @@ void Chain::eigensolve_mixed() {
   // the wide blocks: the FP64 Householder path (default) or, with NRGK_WIDE_SOLVER=rocsolver,
   // rocSOLVER dsyevd per block on the side streams
   const char* wsv = std::getenv("NRGK_WIDE_SOLVER");
-  const bool wide_rocsolver = wsv && std::string(wsv) == "rocsolver";
+  bool wide_rocsolver = wsv && std::string(wsv) == "rocsolver";
+  // blocks wider than the tridiagonal kernels' LDS (over 9 000 rows) take rocSOLVER
+  for (int s : big) wide_rocsolver = wide_rocsolver || cur_n[s] > ht_max_n(ctx->lds_max);
   ht.clear();
   wide_sectors.clear();
   wide_state = WIDE_NONE;

@@ -151,7 +151,7 @@ struct Chain {
   // (wide.cpp)
   std::vector<HtBlock> ht;          // widest first
   std::vector<int> wide_sectors;    // sector of each ht entry
-  DevBuf d_ht, d_wide_ws, d_wide_s, d_wide_q;
+  DevBuf d_ht, d_wide_ws, d_wide_s;
   double* wide_A = nullptr;  // the working matrices (then V_p T_p) and Householder vectors
   double* wide_V = nullptr;
   enum WideState { WIDE_NONE, WIDE_EIGENVALUES, WIDE_KEPT, WIDE_ALL } wide_state = WIDE_NONE;
@@ -222,6 +222,9 @@ struct Chain {
   // eigenpairs of the given row-major symmetric blocks (host arrays packed one after another):
   // ascending eigenvalues into w, eigenvectors as the columns of each block into U
   void solve_blocks(const std::vector<int>& dims, const double* H, double* w, double* U);
+  // lays out blocks as an iteration's sectors and uploads H (mirrored from its upper triangle)
+  void prepare_blocks(const std::vector<int>& dims, const double* H);
+  int mixed_max_override = -1;  // the autotuner's crossover under test (-1: the context's)
   ~Chain();
   void set_bath(const nrgk_site_desc* b);
   void upload_bath();

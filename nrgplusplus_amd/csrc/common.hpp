@@ -98,6 +98,11 @@ struct Ctx {
   std::atomic<size_t> used{0};  // chains of one context may be stepped from several host threads
   int lds_max = 65536;  // bytes of LDS one workgroup may allocate
   int n_cu = 0;
+  std::string arch;     // gcnArchName of the device
+  // the eigensolver crossover: blocks wider than this leave the batched Jacobi for the Householder
+  // path (768, the widest LDS-resident Jacobi panel, unless the autotuner's cache says otherwise)
+  int mixed_max = 768;
+  bool tuned = false;
   void charge(size_t bytes) {
     size_t u = used.load();
     do {
@@ -385,7 +390,7 @@ struct HtBlock {
   int T;            // HT_TB tiles per dimension
   int e_off;        // offset of the block's eigenvalues (prefix of n)
   int m;            // eigenvectors to compute (columns 0 .. m-1)
-  int mpad;         // threads of the inverse-iteration launch for this block (multiple of 256)
+  int mpad;         // scratch slots of the inverse iteration for this block (ht_stein_slots(m))
   int pad;
 };
 

@@ -168,19 +168,40 @@ __global__ void __launch_bounds__(256) k_diag_gather(const DiagItem* __restrict_
     out[it.dst + j] = it.src >= 0 ? O[it.src + (long long)j * (it.n + 1)] : 0.0;
 }
 
-__global__ void __launch_bounds__(256) k_broaden(const double* __restrict__ x, const double* __restrict__ w, int n,
-                                                 const double* __restrict__ omega, int no, double b,
-                                                 double* __restrict__ out) {
+// Log-Gaussian broadening (examples/fdmSpectrumSiam/spec.py:27-44): out(w) = (1/w) sum_k weight_k
+// exp(-(ln(w / x_k) / b - b / 4)^2) / (b sqrt(pi)).  The peaks are split into chunks of BROADEN_CHUNK:
+// workgroup (frequency block, chunk) writes the chunk's partial sums to scratch[chunk][frequency], and
+// k_broaden_sum adds the chunks in order (the same result whatever the launch shape).
+constexpr int BROADEN_CHUNK = 512;
+__global__ void __launch_bounds__(256) k_broaden_part(const double* __restrict__ x, const double* __restrict__ w, int n,
+                                                      const double* __restrict__ omega, int no, double b,
+                                                      double* __restrict__ part) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k0 = blockIdx.y * BROADEN_CHUNK, k1 = min(n, k0 + BROADEN_CHUNK);
+  __shared__ double sx[BROADEN_CHUNK], sw[BROADEN_CHUNK];
+  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+    sx[k - k0] = x[k];
+    sw[k - k0] = w[k];
+  }
+  __syncthreads();
+  if (o >= no) return;
+  const double lom = log(omega[o]);
+  double s = 0.0;
+  for (int k = 0; k < k1 - k0; k++) {
+    if (sw[k] == 0.0) continue;
+    const double y = -b * 0.25 + (lom - log(sx[k])) / b;
+    s += sw[k] * exp(-y * y);
+  }
+  part[(long long)blockIdx.y * no + o] = s;
+}
+__global__ void __launch_bounds__(256) k_broaden_sum(const double* __restrict__ part, int n_chunks,
+                                                     const double* __restrict__ omega, int no, double b,
+                                                     double* __restrict__ out) {
   const int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= no) return;
-  const double om = omega[o];
   double s = 0.0;
-  for (int k = 0; k < n; k++) {
-    if (w[k] == 0.0) continue;
-    const double y = -b * 0.25 + log(om / x[k]) / b;
-    s += w[k] * exp(-y * y);
-  }
-  out[o] = s / om / (b * sqrt(M_PI));
+  for (int c = 0; c < n_chunks; c++) s += part[(long long)c * no + o];
+  out[o] = s / omega[o] / (b * sqrt(M_PI));
 }
 
 // dense transpose of square blocks (plain FP64 mode: rocSOLVER returns column-major eigenvectors)
@@ -228,10 +249,15 @@ void launch_diag_gather(const DiagItem* items, int n_items, const double* O, dou
   NRGK_LAUNCH(k_diag_gather, dim3(n_items), dim3(256), 0, s, items, O, out);
   NRGK_HIP(hipGetLastError());
 }
+size_t broaden_scratch_doubles(int n, int no) {
+  return (size_t)((std::max(n, 1) + BROADEN_CHUNK - 1) / BROADEN_CHUNK) * (size_t)std::max(no, 1);
+}
 void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,
-                    hipStream_t s) {
+                    double* scratch, hipStream_t s) {
   if (no <= 0) return;
-  NRGK_LAUNCH(k_broaden, dim3((no + 255) / 256), dim3(256), 0, s, x, w, n, omega, no, b, out);
+  const int nc = (std::max(n, 1) + BROADEN_CHUNK - 1) / BROADEN_CHUNK;
+  NRGK_LAUNCH(k_broaden_part, dim3((no + 255) / 256, nc), dim3(256), 0, s, x, w, n, omega, no, b, scratch);
+  NRGK_LAUNCH(k_broaden_sum, dim3((no + 255) / 256), dim3(256), 0, s, scratch, nc, omega, no, b, out);
   NRGK_HIP(hipGetLastError());
 }
 void launch_transpose(const double* in, double* out, int n, hipStream_t s) {

@@ -20,9 +20,10 @@
 //    and no launch waits on a single-workgroup reduction.
 // 2. Eigenvalues of T by bisection on Sturm counts (one thread per eigenvalue, T in LDS), ascending
 //    as dsyevd returns them.
-// 3. Eigenvectors of T by inverse iteration with partial-pivoting LU of T - lambda I (one thread per
-//    cluster of eigenvalues closer than ctol, the cluster's members orthogonalised against each other
-//    by modified Gram-Schmidt), for the columns that are needed: all of them, or only the kept ones.
+// 3. Eigenvectors of T by inverse iteration with partial-pivoting LU of T - lambda I (one lane per
+//    eigenvector, all iterated at once; the members of a cluster of eigenvalues closer than ctol
+//    made orthonormal afterwards by modified Gram-Schmidt across the wavefront), for the columns
+//    that are needed: all of them, or only the kept ones.
 // 4. Back-transformation U = Q Z in WY form, Q_p = I - V_p T_p V_p^T per panel (k_ht_larft forms
 //    V_p T_p), as two batched FP64 GEMMs per panel on the matrix cores (k_gemm64).
 #include <cmath>
@@ -62,7 +63,8 @@ __device__ __forceinline__ void ht_tile_of(int t, int& Ip, int& Jp) {
 
 // Workgroup g of a column-k launch over the blocks [0, nblk) (sorted by n, widest first; a block takes
 // part while k <= n - 2): its block b and task t.  KIND 0: tiles, 1: tile-rows, 2: tiles of the blocks
-// whose panel ends at k.
+// whose panel ends at k.  Each wavefront reads 64 blocks' sizes at once, scans their task counts
+// and finds g's block by a ballot (all 64 lanes must take part).
 template <int KIND>
 __device__ __forceinline__ bool ht_find(const HtBlock* bl, int nblk, int k, int g, int& b, int& t) {
   for (int i = 0; i < nblk; i++) {
@@ -91,31 +93,39 @@ int ht_grid(const std::vector<HtBlock>& bl, int k, int kind) {
 }
 
 // The reflector of column k (dlarfg's form): from alpha = A(k+1, k) and the squared norm of the
-// entries below it (the tile-rows' partials, summed in tile-row order).
+// entries below it, the sum of the tile-rows' partials.  One wavefront sums them (lane l takes
+// tile-rows t0 + l, t0 + l + 64, ..., then a butterfly) and lane 0 writes the reflector: every
+// workgroup that needs it runs the same reduction in the same order, so all agree bit for bit.
 struct HtRefl {
   double alpha, beta, tau, sc;
 };
-__device__ HtRefl ht_reflector(const HtWs& w, int n, int T, int k) {
+__device__ void ht_reflector_wave(const HtWs& w, int n, int T, int k, int lane, HtRefl* out) {
   const int s = k + 1;
-  double xn2 = 0.0;
   const double* ss = w.ss + (long long)(k & 1) * T;
-  for (int I = s / HT_TB; I < T; I++) xn2 += ss[I];
-  const double* u = w.u + (long long)(k & 1) * n;
-  HtRefl r;
-  r.alpha = u[s];
-  if (xn2 == 0.0) {
-    r.beta = r.alpha;
-    r.tau = 0.0;
-    r.sc = 0.0;
-  } else {
-    const double nrm = sqrt(r.alpha * r.alpha + xn2);
-    r.beta = r.alpha >= 0.0 ? -nrm : nrm;
-    r.tau = (r.beta - r.alpha) / r.beta;
-    r.sc = 1.0 / (r.alpha - r.beta);
+  const double alpha = lane == 0 ? w.u[(long long)(k & 1) * n + s] : 0.0;
+  double x = 0.0;
+  for (int I = s / HT_TB + lane; I < T; I += 64) x += ss[I];
+  x = ht_wave_sum(x);
+  if (lane == 0) {
+    HtRefl r;
+    r.alpha = alpha;
+    if (x == 0.0) {
+      r.beta = alpha;
+      r.tau = 0.0;
+      r.sc = 0.0;
+    } else {
+      const double nrm = sqrt(alpha * alpha + x);
+      r.beta = alpha >= 0.0 ? -nrm : nrm;
+      r.tau = (r.beta - alpha) / r.beta;
+      r.sc = 1.0 / (alpha - r.beta);
+    }
+    *out = r;
   }
-  return r;
 }
-// entry r of v_k (v[k+1] = 1, zero above)
+// entry r of v_k (v[k+1] = 1, zero above) from the column's entry ur = u[r]
+__device__ __forceinline__ double ht_v_of(double ur, const HtRefl& R, int s, int n, int r) {
+  return (r < s || r >= n) ? 0.0 : (r == s ? 1.0 : ur * R.sc);
+}
 __device__ __forceinline__ double ht_v(const double* u, const HtRefl& R, int s, int n, int r) {
   return (r < s || r >= n) ? 0.0 : (r == s ? 1.0 : u[r] * R.sc);
 }
@@ -168,12 +178,9 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   __shared__ double cred[4][HT_TB];
   __shared__ double sp2[2];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  if (tid == 0) R = ht_reflector(w, n, T, k);
-  __syncthreads();
-  if (tid < 64) vr[tid] = ht_v(u, R, s, n, I * HT_TB + tid);
-  else if (tid < 128) vc[tid - 64] = ht_v(u, R, s, n, J * HT_TB + tid - 64);
-  __syncthreads();
-  // wave wv: rows 16 wv .. 16 wv + 15 of the tile, lane: column (one 512-byte row segment per load)
+  // everything the tile reads is requested at once: its entries (wave wv: rows 16 wv .. 16 wv + 15,
+  // lane: column, one 512-byte row segment per load), the column under reduction at its rows (wave
+  // 0) and columns (wave 1), and the reflector's norm partials (wave 2)
   const int c = J * HT_TB + lane;
   const bool diag = I == J;
   double a[16];
@@ -183,6 +190,13 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
     const bool ok = r < n && c < n && r >= s && c >= s && (!diag || c <= r);
     a[i] = ok ? A[(long long)r * n + c] : 0.0;
   }
+  const int rv = (wv == 0 ? I : J) * HT_TB + lane;
+  const double uv = wv < 2 && rv < n ? u[rv] : 0.0;
+  if (wv == 2) ht_reflector_wave(w, n, T, k, lane, &R);
+  __syncthreads();
+  if (wv == 0) vr[lane] = ht_v_of(uv, R, s, n, rv);
+  else if (wv == 1) vc[lane] = ht_v_of(uv, R, s, n, rv);
+  __syncthreads();
   double colacc = 0.0;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
@@ -214,6 +228,7 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
     const int j0 = (k / HT_NB) * HT_NB, cc = k - j0;
     if (tid < cc) {
       double acc = 0.0;
+#pragma unroll 16
       for (int l = 0; l < HT_TB; l++) {
         const int r = I * HT_TB + l;
         if (r >= s && r < n) acc += V[(long long)r * n + j0 + tid] * vr[l];
@@ -222,6 +237,7 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
     } else if (tid >= 64 && tid - 64 < cc) {
       const int j = tid - 64;
       double acc = 0.0;
+#pragma unroll 16
       for (int l = 0; l < HT_TB; l++) {
         const int r = I * HT_TB + l;
         if (r >= s && r < n) acc += w.W[(long long)r * HT_NB + j] * vr[l];
@@ -231,20 +247,38 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   }
 }
 
-// w(r) = tau (y(r) - V(r,:) W^T v - W(r,:) V^T v) - (tau/2) (w^T v) v(r), y from the tile partials
-__device__ __forceinline__ double ht_wrow(const HtWs& w, const double* V, int n, int T, int k, int j0, int cc,
-                                          const HtRefl& R, const double* u, const double* Vv, const double* Wv,
-                                          double wvs, int r) {
-  const int s = k + 1, t0 = s / HT_TB, I = r / HT_TB, q = r - I * HT_TB;
+// Quarter h (0..3) of y(r) = (A v)(r) from the tile partials: the list of the row sums of tiles
+// (I, t0..I) followed by the column sums of tiles (I..T-1, I), entries h, h + 4, ... of it.
+__device__ __forceinline__ double ht_ypart(const HtWs& w, int T, int t0, int r, int h) {
+  const int I = r / HT_TB, q = r - I * HT_TB, nr = I - t0 + 1, L = nr + T - I;
   double y = 0.0;
-  for (int J = t0; J <= I; J++) y += w.rp[ht_tidx(I, J) * 64 + q];
-  for (int Ir = I; Ir < T; Ir++) y += w.cp[ht_tidx(Ir, I) * 64 + q];
+  for (int e0 = h; e0 < L; e0 += 4 * 8) {
+    double x[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int e = e0 + 4 * j;
+      x[j] = e >= L ? 0.0 : (e < nr ? w.rp[ht_tidx(I, t0 + e) * 64 + q] : w.cp[ht_tidx(I + e - nr, I) * 64 + q]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) y += x[j];
+  }
+  return y;
+}
+// V(r,:) W^T v + W(r,:) V^T v over the panel's cc earlier columns
+__device__ __forceinline__ double ht_corr(const HtWs& w, const double* V, int n, int j0, int cc, const double* Vv,
+                                          const double* Wv, int r) {
   double corr = 0.0;
+#pragma unroll 8
   for (int j = 0; j < cc; j++) corr += V[(long long)r * n + j0 + j] * Wv[j] + w.W[(long long)r * HT_NB + j] * Vv[j];
-  return R.tau * (y - corr) - 0.5 * R.tau * wvs * ht_v(u, R, s, n, r);
+  return corr;
 }
 
-__global__ void __launch_bounds__(256) k_ht_update(const HtBlock* __restrict__ bl, int nblk, int k,
+// One workgroup per 64-row tile-row of the trailing matrix: w(r) = tau (y(r) - V(r,:) W^T v -
+// W(r,:) V^T v) - (tau/2) (w^T v) v(r) for its rows, v and w stored, then the next column with the
+// panel corrections and its norm partial.  Threads 0..255 split each row's partial sums in quarters
+// (thread 4 q + h: row q, quarter h); threads 256..259 do the same for row k + 1, whose w every
+// workgroup needs, so its value is the same bits as the one its owner stores.
+__global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ bl, int nblk, int k,
                                                    const double* __restrict__ Aa, double* __restrict__ Va,
                                                    double* __restrict__ Wa) {
   int b, t;
@@ -259,32 +293,67 @@ __global__ void __launch_bounds__(256) k_ht_update(const HtBlock* __restrict__ b
   const double* u = w.u + (long long)(k & 1) * n;
   __shared__ HtRefl R;
   __shared__ double Vv[HT_NB], Wv[HT_NB];
+  __shared__ double qpart[4][2][HT_NB];
   __shared__ double sred[256];
+  __shared__ double ypart[4][HT_TB], yspart[4];
   __shared__ double sh_wv, sh_w1;
-  const int tid = threadIdx.x;
-  if (tid == 0) R = ht_reflector(w, n, T, k);
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  if (wv == 4) ht_reflector_wave(w, n, T, k, lane, &R);
   // v^T A v over the active tiles: a fixed partition and tree, the same in every workgroup
   const int m = T - t0, NTa = m * (m + 1) / 2;
-  double acc = 0.0;
-  for (int q = tid; q < NTa; q += 256) {
-    int Ip, Jp;
-    ht_tile_of(q, Ip, Jp);
-    acc += w.sp[ht_tidx(t0 + Ip, t0 + Jp)];
+  if (tid < 256) {
+    double acc = 0.0;
+    for (int q0 = tid; q0 < NTa; q0 += 256 * 8) {
+      double x[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int q = q0 + 256 * j;
+        x[j] = 0.0;
+        if (q < NTa) {
+          int Ip, Jp;
+          ht_tile_of(q, Ip, Jp);
+          x[j] = w.sp[ht_tidx(t0 + Ip, t0 + Jp)];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) acc += x[j];
+    }
+    sred[tid] = acc;
+    // the panel's V^T v and W^T v: thread 4 p + h sums quarter h of the tile-rows of pair p
+    const int pr = tid >> 2, h = tid & 3, arr = pr >> 5, j = pr & 31;
+    if (j < cc) {
+      const double* src = arr == 0 ? w.qv : w.qw;
+      double a = 0.0;
+      for (int i0 = t0 + h; i0 < T; i0 += 4 * 8) {
+        double x[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) {
+          const int Ir = i0 + 4 * jj;
+          x[jj] = Ir < T ? src[Ir * HT_NB + j] : 0.0;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) a += x[jj];
+      }
+      qpart[h][arr][j] = a;
+    }
+    // the partial sums of this workgroup's rows
+    const int r = I * HT_TB + pr;
+    ypart[h][pr] = r < n && r >= s ? ht_ypart(w, T, t0, r, h) : 0.0;
+  } else if (tid < 260 && !pend) {
+    yspart[tid - 256] = ht_ypart(w, T, t0, s, tid - 256);
   }
-  sred[tid] = acc;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (tid < o) sred[tid] += sred[tid + o];
     __syncthreads();
   }
-  if (tid < cc) {
-    double a = 0.0, bq = 0.0;
-    for (int Ir = t0; Ir < T; Ir++) {
-      a += w.qv[Ir * HT_NB + tid];
-      bq += w.qw[Ir * HT_NB + tid];
+  if (tid < 2 * HT_NB) {
+    const int arr = tid >> 5, j = tid & 31;
+    if (j < cc) {
+      const double a = (qpart[0][arr][j] + qpart[1][arr][j]) + (qpart[2][arr][j] + qpart[3][arr][j]);
+      if (arr == 0) Vv[j] = a;
+      else Wv[j] = a;
     }
-    Vv[tid] = a;
-    Wv[tid] = bq;
   }
   __syncthreads();
   if (tid == 0) {
@@ -294,15 +363,18 @@ __global__ void __launch_bounds__(256) k_ht_update(const HtBlock* __restrict__ b
   }
   __syncthreads();
   const double wvs = sh_wv;
-  if (tid == 0 && !pend) sh_w1 = ht_wrow(w, V, n, T, k, j0, cc, R, u, Vv, Wv, wvs, s);
   const int r = I * HT_TB + tid;
   double wr = 0.0, vrr = 0.0;
   const bool own = tid < 64 && r < n && r >= s;
   if (own) {
-    wr = ht_wrow(w, V, n, T, k, j0, cc, R, u, Vv, Wv, wvs, r);
+    const double y = (ypart[0][tid] + ypart[1][tid]) + (ypart[2][tid] + ypart[3][tid]);
     vrr = ht_v(u, R, s, n, r);
+    wr = R.tau * (y - ht_corr(w, V, n, j0, cc, Vv, Wv, r)) - 0.5 * R.tau * wvs * vrr;
     w.W[(long long)r * HT_NB + cc] = wr;
     V[(long long)r * n + k] = vrr;
+  } else if (tid == 256 && !pend) {
+    const double y = (yspart[0] + yspart[1]) + (yspart[2] + yspart[3]);
+    sh_w1 = R.tau * (y - ht_corr(w, V, n, j0, cc, Vv, Wv, s)) - 0.5 * R.tau * wvs * 1.0;
   }
   if (t == 0 && tid == 0) {
     w.tau[k] = R.tau;
@@ -315,6 +387,7 @@ __global__ void __launch_bounds__(256) k_ht_update(const HtBlock* __restrict__ b
   double sq = 0.0;
   if (own) {
     double un = A[(long long)r * n + s];
+#pragma unroll 8
     for (int j = 0; j < cc; j++)
       un -= V[(long long)r * n + j0 + j] * w.W[(long long)s * HT_NB + j] +
             w.W[(long long)r * HT_NB + j] * V[(long long)s * n + j0 + j];
@@ -467,6 +540,10 @@ __global__ void __launch_bounds__(256) k_ht_stebz(const HtBlock* __restrict__ bl
   const double atol = 0.25 * eps * tnorm;
   const int j = g * 256 + tid;
   if (j >= n) return;
+  if (tnorm == 0.0) {  // T = 0: every eigenvalue is exactly zero
+    lam[B.e_off + j] = 0.0;
+    return;
+  }
   double a = rlo[0] - 2.0 * eps * tnorm - pivmin, c = rhi[0] + 2.0 * eps * tnorm + pivmin;
   for (int it = 0; it < 200; it++) {
     const double mid = 0.5 * (a + c);
@@ -497,76 +574,225 @@ __device__ __forceinline__ double ht_start(unsigned p, unsigned i) {
   return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
+// Rows per register tile of the column loops below: each tile's loads are issued together, so a
+// lane waits for memory once per tile rather than once per row of its serial recurrences.
+constexpr int HT_RT = 16;
+
 // One inverse-iteration step for eigenvalue lp: the iterate in column p of U (stride n) is replaced by
 // (T - lp I)^-1 times it.  T - lp I = P L U with partial pivoting; the factorisation and the
 // elimination of the iterate share one pass (the L factor is not kept), the upper factor's three
-// diagonals go to the scratch f0, f1, f2 at entry i * mp + slot, then one backward pass.
-__device__ void ht_inverse_step(const double* d, const double* e, int n, double lp, double piv0, double* U, int p,
-                                double* f0, double* f1, double* f2, long long mp, int slot) {
+// diagonals go to the scratch f0 (as the pivot's reciprocal), f1, f2 at entry i * mp + slot, then one
+// backward pass.  A pivot smaller than piv0 in magnitude is moved out to +-piv0 (dlagtf/dlagts'
+// perturbation), which bounds the growth of one step by about 1 / piv0.
+__device__ void ht_inverse_step(const double* __restrict__ d, const double* __restrict__ e, int n, double lp,
+                                double piv0, double* __restrict__ U, int p, double* __restrict__ f0,
+                                double* __restrict__ f1, double* __restrict__ f2, long long mp, int slot) {
   double cd = d[0] - lp, cu = n > 1 ? e[0] : 0.0;
   double xi = U[p];
-  for (int i = 0; i < n - 1; i++) {
-    const double sub = e[i], nd = d[i + 1] - lp, nu = i + 1 < n - 1 ? e[i + 1] : 0.0;
-    const double xn = U[(long long)(i + 1) * n + p];
-    const long long o = (long long)i * mp + slot;
-    if (fabs(cd) >= fabs(sub)) {
-      if (cd == 0.0) cd = piv0;
-      const double ml = sub / cd;
-      f0[o] = cd;
-      f1[o] = cu;
-      f2[o] = 0.0;
-      U[(long long)i * n + p] = xi;
-      xi = xn - ml * xi;
-      cd = nd - ml * cu;
-      cu = nu;
-    } else {  // rows i and i + 1 exchanged
-      const double ml = cd / sub;
-      f0[o] = sub;
-      f1[o] = nd;
-      f2[o] = nu;
-      U[(long long)i * n + p] = xn;
-      xi = xi - ml * xn;
-      cd = cu - ml * nd;
-      cu = -ml * nu;
+  for (int i0 = 0; i0 < n - 1; i0 += HT_RT) {
+    double xn[HT_RT], sb[HT_RT], dn[HT_RT], en[HT_RT];
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) {
+      const int i = i0 + j;
+      const bool ok = i < n - 1;
+      xn[j] = ok ? U[(long long)(i + 1) * n + p] : 0.0;
+      sb[j] = ok ? e[i] : 0.0;
+      dn[j] = ok ? d[i + 1] : 0.0;
+      en[j] = i + 1 < n - 1 ? e[i + 1] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) {
+      const int i = i0 + j;
+      if (i >= n - 1) continue;
+      const double sub = sb[j], nd = dn[j] - lp, nu = en[j];
+      const long long o = (long long)i * mp + slot;
+      if (fabs(cd) >= fabs(sub)) {
+        if (fabs(cd) < piv0) cd = cd < 0.0 ? -piv0 : piv0;
+        const double r = 1.0 / cd, ml = sub * r;
+        f0[o] = r;
+        f1[o] = cu;
+        f2[o] = 0.0;
+        U[(long long)i * n + p] = xi;
+        xi = xn[j] - ml * xi;
+        cd = nd - ml * cu;
+        cu = nu;
+      } else {  // rows i and i + 1 exchanged
+        const double r = 1.0 / sub, ml = cd * r;
+        f0[o] = r;
+        f1[o] = nd;
+        f2[o] = nu;
+        U[(long long)i * n + p] = xn[j];
+        xi = xi - ml * xn[j];
+        cd = cu - ml * nd;
+        cu = -ml * nu;
+      }
     }
   }
-  if (cd == 0.0) cd = piv0;
-  f0[(long long)(n - 1) * mp + slot] = cd;
+  if (fabs(cd) < piv0) cd = cd < 0.0 ? -piv0 : piv0;
+  const long long ol = (long long)(n - 1) * mp + slot;
+  f0[ol] = 1.0 / cd;
+  f1[ol] = 0.0;
+  f2[ol] = 0.0;
   U[(long long)(n - 1) * n + p] = xi;
   double x1 = 0.0, x2 = 0.0;
-  for (int i = n - 1; i >= 0; i--) {
-    const long long o = (long long)i * mp + slot;
-    double v = U[(long long)i * n + p];
-    if (i + 1 < n) v -= f1[o] * x1;
-    if (i + 2 < n) v -= f2[o] * x2;
-    v /= f0[o];
-    U[(long long)i * n + p] = v;
-    x2 = x1;
-    x1 = v;
+  for (int i1 = n - 1; i1 >= 0; i1 -= HT_RT) {
+    double uu[HT_RT], a0[HT_RT], a1[HT_RT], a2[HT_RT];
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) {
+      const int i = i1 - j;
+      const bool ok = i >= 0;
+      const long long o = (long long)i * mp + slot;
+      uu[j] = ok ? U[(long long)i * n + p] : 0.0;
+      a0[j] = ok ? f0[o] : 0.0;
+      a1[j] = ok ? f1[o] : 0.0;
+      a2[j] = ok ? f2[o] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) {
+      const int i = i1 - j;
+      if (i < 0) continue;
+      const double v = (uu[j] - a1[j] * x1 - a2[j] * x2) * a0[j];
+      U[(long long)i * n + p] = v;
+      x2 = x1;
+      x1 = v;
+    }
   }
 }
 
-constexpr int HT_SMALL_CLUSTER = 8;  // clusters up to this size: one thread each
-constexpr int HT_ITERS = 3;          // inverse-iteration steps per eigenvector
+constexpr int HT_ITERS = 3;  // inverse-iteration steps per eigenvector
 
 // the infinity norm of T (d, e in memory p)
 __device__ __forceinline__ double ht_tnorm_entry(const double* d, const double* e, int n, int i) {
   return fabs(d[i]) + (i < n - 1 ? fabs(e[i]) : 0.0) + (i > 0 ? fabs(e[i - 1]) : 0.0);
 }
 
+// first index >= x that starts a cluster (lambda more than ctol above its predecessor), or n
+__device__ __forceinline__ int ht_cluster_start_at(const double* L, int n, int x, double ctol) {
+  while (x > 0 && x < n && L[x] - L[x - 1] <= ctol) x++;
+  return x;
+}
+
+// column p of U (stride n) dotted with column q, summed in row order
+__device__ double ht_col_dot(const double* __restrict__ U, int n, int p, int q) {
+  double s = 0.0;
+  for (int i0 = 0; i0 < n; i0 += HT_RT) {
+    double a[HT_RT], b[HT_RT];
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) {
+      const int i = i0 + j;
+      a[j] = i < n ? U[(long long)i * n + p] : 0.0;
+      b[j] = i < n ? U[(long long)i * n + q] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) s += a[j] * b[j];
+  }
+  return s;
+}
+// column p += c * column q
+__device__ void ht_col_axpy(double* __restrict__ U, int n, int p, int q, double c) {
+  for (int i0 = 0; i0 < n; i0 += HT_RT) {
+    double a[HT_RT], b[HT_RT];
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) {
+      const int i = i0 + j;
+      a[j] = i < n ? U[(long long)i * n + p] : 0.0;
+      b[j] = i < n ? U[(long long)i * n + q] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++)
+      if (i0 + j < n) U[(long long)(i0 + j) * n + p] = a[j] + c * b[j];
+  }
+}
+// column p normalised, scaled by its largest entry first so that the squares of an iterate of any
+// size stay finite
+__device__ void ht_col_normalise(double* __restrict__ U, int n, int p) {
+  double mx = 0.0;
+  for (int i0 = 0; i0 < n; i0 += HT_RT) {
+    double a[HT_RT];
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) a[j] = i0 + j < n ? U[(long long)(i0 + j) * n + p] : 0.0;
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) mx = fmax(mx, fabs(a[j]));
+  }
+  if (mx == 0.0) return;
+  const double sc = 1.0 / mx;
+  double nr = 0.0;
+  for (int i0 = 0; i0 < n; i0 += HT_RT) {
+    double a[HT_RT];
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) a[j] = i0 + j < n ? U[(long long)(i0 + j) * n + p] * sc : 0.0;
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) nr += a[j] * a[j];
+  }
+  const double inv = sc / sqrt(nr);
+  for (int i0 = 0; i0 < n; i0 += HT_RT) {
+    double a[HT_RT];
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++) a[j] = i0 + j < n ? U[(long long)(i0 + j) * n + p] : 0.0;
+#pragma unroll
+    for (int j = 0; j < HT_RT; j++)
+      if (i0 + j < n) U[(long long)(i0 + j) * n + p] = a[j] * inv;
+  }
+}
+
+// Modified Gram-Schmidt over the c adjacent columns cs .. cs + c - 1 of U (stride n) by the whole
+// wavefront, rows split over the lanes (lane l: rows l, l + 64, ...; each lane only ever touches its
+// own rows).  Round r, for the later columns sixteen at a time: one sweep forms their dot products
+// d_s with column cs + r (each lane's rows in order, a butterfly, lane 0's sum broadcast, so every
+// lane holds the same d), a second takes (d_s / d_r) column r out of them; column r is then scaled
+// by 1 / sqrt(d_r).
+constexpr int HT_MG = 16;
+__device__ void ht_mgs_wave(double* __restrict__ U, int n, int cs, int c, int lane) {
+  for (int r = 0; r < c; r++) {
+    double drr = 0.0;
+    for (int s0 = r; s0 < c; s0 += HT_MG) {
+      double acc[HT_MG];
+#pragma unroll
+      for (int j = 0; j < HT_MG; j++) acc[j] = 0.0;
+      for (int i = lane; i < n; i += 64) {
+        const double* row = U + (long long)i * n + cs;
+        const double xr = row[r];
+#pragma unroll
+        for (int j = 0; j < HT_MG; j++)
+          if (s0 + j < c) acc[j] += xr * row[s0 + j];
+      }
+#pragma unroll
+      for (int j = 0; j < HT_MG; j++) acc[j] = __shfl(ht_wave_sum(acc[j]), 0, 64);
+      if (s0 == r) drr = acc[0];
+      double coef[HT_MG];
+#pragma unroll
+      for (int j = 0; j < HT_MG; j++) coef[j] = s0 + j > r && s0 + j < c && drr > 0.0 ? acc[j] / drr : 0.0;
+      for (int i = lane; i < n; i += 64) {
+        double* row = U + (long long)i * n + cs;
+        const double xr = row[r];
+#pragma unroll
+        for (int j = 0; j < HT_MG; j++)
+          if (s0 + j > r && s0 + j < c) row[s0 + j] -= coef[j] * xr;
+      }
+    }
+    const double inv = drr > 0.0 ? 1.0 / sqrt(drr) : 0.0;
+    for (int i = lane; i < n; i += 64) U[(long long)i * n + cs + r] *= inv;
+  }
+}
+
 // Eigenvectors of T for the eigenvalues 0 .. m-1 of each block (and the rest of a cluster that
-// starts below m), into the columns of U.  Thread j starts a cluster when lambda_j is more than ctol
-// above lambda_{j-1}.  A cluster of at most HT_SMALL_CLUSTER members is done by its thread: three
-// inverse-iteration steps per member in turn, each followed by orthogonalisation against the members
-// before it (modified Gram-Schmidt) and normalisation.  A larger cluster is queued for
-// k_ht_stein_wide.  The LU factors live in interleaved scratch (entry i of slot j at i * mpad + j).
-__global__ void __launch_bounds__(256) k_ht_stein(const HtBlock* __restrict__ bl, int nblk, const double* __restrict__ Wa,
-                                                  const double* __restrict__ lam, double* __restrict__ Ua,
-                                                  double* __restrict__ Sa, double ctol_rel, int* __restrict__ queue) {
-  extern __shared__ double lds[];
+// starts below m), into the columns of U.  Clusters are runs of eigenvalues each within ctol of the
+// one before.  Workgroup (one wavefront) g of a block owns the clusters that start in
+// [HT_CHUNK g, HT_CHUNK (g + 1)) below m; lane t iterates the members a + t, a + t + 64, ... of that
+// range (a = its first cluster start) in its scratch slot g * 64 + t: HT_ITERS inverse-iteration
+// steps from a fixed pseudo-random start, each normalised, all members at once.  After every step
+// each cluster of more than one member is made orthonormal by the wavefront (ht_mgs_wave, one
+// cluster after another), and once more after the last: the next step then amplifies each
+// member's own direction again, so the components outside the cluster stay at eps ||T|| / gap
+// however close to dependent the members were before the orthogonalisation.  The scratch holds
+// the LU factors of one member per lane (slot g * 64 + t of mpad = ht_stein_slots(m)).
+constexpr int HT_CHUNK = 48;
+__global__ void __launch_bounds__(64) k_ht_stein(const HtBlock* __restrict__ bl, int nblk, const double* __restrict__ Wa,
+                                                 const double* __restrict__ lam, double* __restrict__ Ua,
+                                                 double* __restrict__ Sa, double ctol_rel) {
   int g = blockIdx.x, b = -1;
   for (int i = 0; i < nblk; i++) {
-    const int cw = bl[i].mpad / 256;
+    const int cw = (bl[i].m + HT_CHUNK - 1) / HT_CHUNK;
     if (g < cw) {
       b = i;
       break;
@@ -578,139 +804,46 @@ __global__ void __launch_bounds__(256) k_ht_stein(const HtBlock* __restrict__ bl
   const int n = B.n;
   const long long mp = B.mpad;
   const HtWs w = ht_ws(const_cast<double*>(Wa) + B.w_off, n, B.T);
-  double* d = lds;
-  double* e = lds + n;
-  __shared__ double rnm[256];
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   double nm = 0.0;
-  for (int i = tid; i < n; i += 256) {
-    d[i] = w.d[i];
-    e[i] = i < n - 1 ? w.e[i] : 0.0;
-    nm = fmax(nm, ht_tnorm_entry(w.d, w.e, n, i));
-  }
-  rnm[tid] = nm;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o) rnm[tid] = fmax(rnm[tid], rnm[tid + o]);
-    __syncthreads();
-  }
-  const double tnorm = rnm[0];
-  const int j = g * 256 + tid;
-  if (j >= B.m) return;
+  for (int i = lane; i < n; i += 64) nm = fmax(nm, ht_tnorm_entry(w.d, w.e, n, i));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nm = fmax(nm, __shfl_xor(nm, o, 64));
+  const double tnorm = nm;
   const double* L = lam + B.e_off;
   const double ctol = ctol_rel * tnorm;
-  if (j > 0 && L[j] - L[j - 1] <= ctol) return;  // inside a cluster started below
-  int j1 = j + 1;
-  while (j1 < n && L[j1] - L[j1 - 1] <= ctol) j1++;
-  if (j1 - j > HT_SMALL_CLUSTER) {
-    const int q = atomicAdd(queue, 1);
-    queue[1 + 2 * q] = b;
-    queue[2 + 2 * q] = j;
-    return;
-  }
+  const int a = ht_cluster_start_at(L, n, g * HT_CHUNK, ctol);
+  const int e_nom = min((g + 1) * HT_CHUNK, B.m);
+  const int bend = ht_cluster_start_at(L, n, e_nom, ctol);
   double* U = Ua + B.a_off;
+  const int slot = g * 64 + lane;
   double* f0 = Sa + B.s_off;
   double* f1 = f0 + (long long)n * mp;
   double* f2 = f1 + (long long)n * mp;
-  // a pivot that vanishes is replaced by eps ||T|| (by 1 for T = 0, where any basis is one)
+  // pivots are kept at least eps ||T|| in magnitude (1 for T = 0, where any basis is one)
   const double piv0 = tnorm > 0.0 ? 2.220446049250313e-16 * tnorm : 1.0;
-  for (int p = j; p < j1; p++) {
-    const double lp = L[p];
+  bool multi = false;  // a cluster of more than one member starts in this chunk
+  for (int p = a + 1; p < bend; p++) multi = multi || L[p] - L[p - 1] <= ctol;
+  for (int p = a + lane; p < bend; p += 64)
     for (int i = 0; i < n; i++) U[(long long)i * n + p] = ht_start((unsigned)p, (unsigned)i);
-    for (int it = 0; it < HT_ITERS; it++) {
-      ht_inverse_step(d, e, n, lp, piv0, U, p, f0, f1, f2, mp, j);
-      for (int q = j; q < p; q++) {
-        double dt = 0.0;
-        for (int i = 0; i < n; i++) dt += U[(long long)i * n + q] * U[(long long)i * n + p];
-        for (int i = 0; i < n; i++) U[(long long)i * n + p] -= dt * U[(long long)i * n + q];
+  for (int it = 0; it <= HT_ITERS; it++) {
+    if (it < HT_ITERS)
+      for (int p = a + lane; p < bend; p += 64) {
+        ht_inverse_step(w.d, w.e, n, L[p], piv0, U, p, f0, f1, f2, mp, slot);
+        ht_col_normalise(U, n, p);
       }
-      double nr = 0.0;
-      for (int i = 0; i < n; i++) {
-        const double v = U[(long long)i * n + p];
-        nr += v * v;
-      }
-      const double inv = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
-      for (int i = 0; i < n; i++) U[(long long)i * n + p] *= inv;
+    if (!multi) {
+      if (it == HT_ITERS - 1) break;  // singletons only: no orthogonalisation
+      continue;
     }
-  }
-}
-
-// The queued clusters (more than HT_SMALL_CLUSTER members): one 64-thread workgroup per cluster, taken
-// from the queue in strides of the grid.  Thread 0 runs the (sequential) inverse-iteration steps; the
-// orthogonalisation and normalisation are spread over the workgroup, with a barrier between the
-// phases.  Dot products are summed in a fixed order (per-thread strided sums, then a tree).
-__global__ void __launch_bounds__(64) k_ht_stein_wide(const HtBlock* __restrict__ bl, const double* __restrict__ Wa,
-                                                      const double* __restrict__ lam, double* __restrict__ Ua,
-                                                      double* __restrict__ Sa, double ctol_rel,
-                                                      const int* __restrict__ queue) {
-  __shared__ double red[64];
-  __shared__ double sh_tn;
-  const int nq = queue[0];
-  const int tid = threadIdx.x;
-  for (int q = blockIdx.x; q < nq; q += gridDim.x) {
-    const int b = queue[1 + 2 * q], j = queue[2 + 2 * q];
-    const HtBlock B = bl[b];
-    const int n = B.n;
-    const long long mp = B.mpad;
-    const HtWs w = ht_ws(const_cast<double*>(Wa) + B.w_off, n, B.T);
-    double nm = 0.0;
-    for (int i = tid; i < n; i += 64) nm = fmax(nm, ht_tnorm_entry(w.d, w.e, n, i));
-    red[tid] = nm;
     __syncthreads();
-    for (int o = 32; o > 0; o >>= 1) {
-      if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
-      __syncthreads();
+    for (int cs = a; cs < bend;) {
+      int ce = cs + 1;
+      while (ce < bend && L[ce] - L[ce - 1] <= ctol) ce++;
+      if (ce - cs > 1) ht_mgs_wave(U, n, cs, ce - cs, lane);
+      cs = ce;
     }
-    if (tid == 0) sh_tn = red[0];
     __syncthreads();
-    const double tnorm = sh_tn;
-    const double* L = lam + B.e_off;
-    const double ctol = ctol_rel * tnorm;
-    int j1 = j + 1;
-    while (j1 < n && L[j1] - L[j1 - 1] <= ctol) j1++;
-    double* U = Ua + B.a_off;
-    double* f0 = Sa + B.s_off;
-    double* f1 = f0 + (long long)n * mp;
-    double* f2 = f1 + (long long)n * mp;
-    // a pivot that vanishes is replaced by eps ||T|| (by 1 for T = 0, where any basis is one)
-  const double piv0 = tnorm > 0.0 ? 2.220446049250313e-16 * tnorm : 1.0;
-    for (int p = j; p < j1; p++) {
-      for (int i = tid; i < n; i += 64) U[(long long)i * n + p] = ht_start((unsigned)p, (unsigned)i);
-      __syncthreads();
-      for (int it = 0; it < HT_ITERS; it++) {
-        if (tid == 0) ht_inverse_step(w.d, w.e, n, L[p], piv0, U, p, f0, f1, f2, mp, j);
-        __syncthreads();
-        for (int q2 = j; q2 < p; q2++) {
-          double dt = 0.0;
-          for (int i = tid; i < n; i += 64) dt += U[(long long)i * n + q2] * U[(long long)i * n + p];
-          red[tid] = dt;
-          __syncthreads();
-          for (int o = 32; o > 0; o >>= 1) {
-            if (tid < o) red[tid] += red[tid + o];
-            __syncthreads();
-          }
-          dt = red[0];
-          __syncthreads();
-          for (int i = tid; i < n; i += 64) U[(long long)i * n + p] -= dt * U[(long long)i * n + q2];
-          __syncthreads();
-        }
-        double nr = 0.0;
-        for (int i = tid; i < n; i += 64) {
-          const double v = U[(long long)i * n + p];
-          nr += v * v;
-        }
-        red[tid] = nr;
-        __syncthreads();
-        for (int o = 32; o > 0; o >>= 1) {
-          if (tid < o) red[tid] += red[tid + o];
-          __syncthreads();
-        }
-        const double inv = red[0] > 0.0 ? 1.0 / sqrt(red[0]) : 0.0;
-        __syncthreads();
-        for (int i = tid; i < n; i += 64) U[(long long)i * n + p] *= inv;
-        __syncthreads();
-      }
-    }
   }
 }
 
@@ -792,7 +925,7 @@ int launch_ht_symv(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, c
 int launch_ht_update(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, double* V, double* W,
                      hipStream_t s) {
   const int g = ht_grid(bl, k, 1);
-  if (g) NRGK_LAUNCH(k_ht_update, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), k, A, V, W);
+  if (g) NRGK_LAUNCH(k_ht_update, dim3(g), dim3(320), 0, s, d_bl, (int)bl.size(), k, A, V, W);
   NRGK_HIP(hipGetLastError());
   return g;
 }
@@ -813,21 +946,14 @@ void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
   NRGK_HIP(hipGetLastError());
 }
 void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
-                     double* S, double ctol_rel, int* queue, hipStream_t s) {
-  int g = 0, nmax = 1;
-  for (const auto& b : bl) {
-    g += b.mpad / 256;
-    nmax = std::max(nmax, b.n);
-  }
+                     double* S, double ctol_rel, hipStream_t s) {
+  int g = 0;
+  for (const auto& b : bl) g += (b.m + HT_CHUNK - 1) / HT_CHUNK;
   if (!g) return;
-  NRGK_HIP(hipMemsetAsync(queue, 0, sizeof(int), s));
-  NRGK_LAUNCH(k_ht_stein, dim3(g), dim3(256), (size_t)nmax * 2 * sizeof(double), s, d_bl, (int)bl.size(), W, lam, U, S,
-              ctol_rel, queue);
-  NRGK_HIP(hipGetLastError());
-  // the large clusters: a fixed grid that walks the queue (its length is read on the device)
-  NRGK_LAUNCH(k_ht_stein_wide, dim3(1024), dim3(64), 0, s, d_bl, W, lam, U, S, ctol_rel, (const int*)queue);
+  NRGK_LAUNCH(k_ht_stein, dim3(g), dim3(64), 0, s, d_bl, (int)bl.size(), W, lam, U, S, ctol_rel);
   NRGK_HIP(hipGetLastError());
 }
+int ht_stein_slots(int m) { return (m + HT_CHUNK - 1) / HT_CHUNK * 64; }
 void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* V, const double* W, double* A,
                      hipStream_t s) {
   int g = 0;
@@ -835,10 +961,13 @@ void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
   if (g) NRGK_LAUNCH(k_ht_larft, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), V, W, A);
   NRGK_HIP(hipGetLastError());
 }
-size_t ht_lds_bytes(int nmax) { return (size_t)nmax * 2 * sizeof(double); }
+// dynamic LDS of the bisection kernel (d and e^2 of the widest block); its static LDS (below 8 KB)
+// comes on top of it
+constexpr int HT_STATIC_LDS = 8192;
+int ht_max_n(int lds_bytes) { return (lds_bytes - HT_STATIC_LDS) / (int)(2 * sizeof(double)); }
 void set_ht_lds_limits(int bytes) {
-  NRGK_HIP(hipFuncSetAttribute((const void*)k_ht_stebz, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-  NRGK_HIP(hipFuncSetAttribute((const void*)k_ht_stein, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  const int dyn = bytes - HT_STATIC_LDS;
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_ht_stebz, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
 }
 
 }  // namespace nrgk

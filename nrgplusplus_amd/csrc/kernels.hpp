@@ -42,10 +42,16 @@ int launch_ht_syr2k(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, 
 void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, double* lam, hipStream_t s);
 // queue: 1 + 2 * (eigenvalues of the blocks) ints of device scratch
 void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
-                     double* S, double ctol_rel, int* queue, hipStream_t s);
+                     double* S, double ctol_rel, hipStream_t s);
+int ht_stein_slots(int m);  // scratch slots (HtBlock::mpad) of the inverse iteration for m columns
 void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* V, const double* W, double* A,
                      hipStream_t s);
 void set_ht_lds_limits(int bytes);
+int ht_max_n(int lds_bytes);  // the widest block the tridiagonal kernels hold in LDS
+// tune.cpp
+std::string tune_cache_path(const Ctx& c);
+void tune_load(Ctx& c);
+int tune_crossover(Ctx* ctx, const std::string& path, std::vector<std::vector<double>>& table);
 // k_trunc.hip
 void set_trunc_lds_limit();
 void launch_truncate(const int* bn, const int* beoff, int S, const double* lam, long long N_tot, long long K,
@@ -65,8 +71,10 @@ void launch_thermo(const double* E, const int* shell_off, const double* gs, int 
 void launch_thermo_ops(const double* E, const double* Q, long long q_stride, int nq, const int* shell_off,
                        const double* gs, int n_shells, const double* T, int nT, double* out, hipStream_t s);
 void launch_diag_gather(const DiagItem* items, int n_items, const double* O, double* out, hipStream_t s);
+// scratch: broaden_scratch_doubles(n, no) doubles of device memory
+size_t broaden_scratch_doubles(int n, int no);
 void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,
-                    hipStream_t s);
+                    double* scratch, hipStream_t s);
 void launch_transpose(const double* in, double* out, int n, hipStream_t s);
 
 }  // namespace nrgk

@@ -273,6 +273,106 @@ void Fdm::save(const std::string& path, const std::string& op_group) const {
   }
 }
 
+// NrgData::loadCurrentData + loadqOperator (nrgData.hpp:214-293) for every shell of a file that
+// Fdm::save wrote: the shells are rebuilt on the device (U blocks, eigenvalues and, when op_group is
+// given and the recorder has an observable, the operator blocks), so that a backward pass or the
+// thermodynamics can run in another process than the forward pass.  The chain must have the bath the
+// file was written with (its sector dimensions place the sub-blocks).  Shells already recorded are
+// replaced.
+void Fdm::load(const std::string& path, const std::string& op_group) {
+  Chain& c = *ch;
+  h5::File f(path, false);
+  int first = -1;
+  for (int k = 0; k < 100000 && first < 0; k++)
+    if (f.exists("/NrgItr" + std::to_string(k))) first = k;
+  NRGK_REQUIRE(first >= 0, NRGK_ERR_INVALID, "nrgk_fdm_load: no /NrgItr group in " + path);
+  shells.clear();
+  chunks.clear();
+  chunk_used = 0;
+  const int nb = c.bath.S;
+  for (int cnt = first; f.exists("/NrgItr" + std::to_string(cnt)); cnt++) {
+    const std::string g = "/NrgItr" + std::to_string(cnt) + "/";
+    auto sh = std::make_unique<Shell>();
+    sh->cnt = cnt;
+    sh->n_bath = nb;
+    for (int i = 0; f.exists(g + "pre_sysmQ" + std::to_string(i)); i++) {
+      const std::string si = std::to_string(i);
+      std::vector<int> q(f.size(g + "pre_sysmQ" + si));
+      f.read(g + "pre_sysmQ" + si, h5::Type::I32, q.data(), q.size());
+      sh->pre_qn.push_back(q);
+      sh->pre_kept.push_back((int)f.size(g + "eigenvaluesQ_kept_indices" + si));
+    }
+    long long uo = 0;
+    for (int i = 0; f.exists(g + "current_sysmQ" + std::to_string(i)); i++) {
+      const std::string si = std::to_string(i);
+      std::vector<int> q(f.size(g + "current_sysmQ" + si));
+      f.read(g + "current_sysmQ" + si, h5::Type::I32, q.data(), q.size());
+      sh->qn.push_back(q);
+      const int n = (int)f.size(g + "eigenvaluesQ" + si);
+      sh->eoff.push_back((int)sh->E.size());
+      sh->n.push_back(n);
+      sh->uoff.push_back(uo);
+      uo += (long long)n * n;
+      std::vector<double> e(n);
+      f.read(g + "eigenvaluesQ" + si, h5::Type::F64, e.data(), n);
+      sh->E.insert(sh->E.end(), e.begin(), e.end());
+      std::vector<unsigned long> cq(f.size(g + "coupled_nQ_index" + si));
+      f.read(g + "coupled_nQ_index" + si, h5::Type::U64, cq.data(), cq.size());
+      std::vector<std::pair<int, int>> cp;
+      std::vector<int> kidx;
+      int k0 = 0;
+      for (unsigned long v : cq) {
+        const int ii = (int)(v / nb), bb = (int)(v % nb);
+        NRGK_REQUIRE(ii < (int)sh->pre_kept.size(), NRGK_ERR_INVALID, "nrgk_fdm_load: coupled index out of range");
+        cp.push_back({ii, bb});
+        kidx.push_back(k0);
+        k0 += sh->pre_kept[ii] * c.bath.dim(bb);
+      }
+      NRGK_REQUIRE(k0 == n, NRGK_ERR_INVALID, "nrgk_fdm_load: the chain's bath does not match the file");
+      sh->coupled.push_back(cp);
+      sh->sub_kidx.push_back(kidx);
+    }
+    const int S = (int)sh->n.size();
+    if (obs && f.exists(g + "current_hamiltonQ0")) {
+      std::vector<double> hU(std::max<long long>(1, uo));
+      for (int i = 0; i < S; i++)
+        f.read(g + "current_hamiltonQ" + std::to_string(i), h5::Type::F64, hU.data() + sh->uoff[i],
+               (size_t)sh->n[i] * sh->n[i]);
+      sh->dU.p = arena_alloc(hU.size() * sizeof(double));
+      h2d(sh->dU.as<double>(), hU.data(), hU.size(), c.st);
+      sh->dE.p = arena_alloc(std::max<size_t>(1, sh->E.size()) * sizeof(double));
+      h2d(sh->dE.as<double>(), sh->E.data(), sh->E.size(), c.st);
+      NRGK_REQUIRE(!op_group.empty(), NRGK_ERR_INVALID, "nrgk_fdm_load: the recorder's observable needs op_group");
+      const std::string og = op_group + std::to_string(cnt - 1);
+      sh->ops.assign(obs->nops, OpMap());
+      std::vector<double> hO;
+      for (int p = 0; p < obs->nops; p++) {
+        const std::string lg = og + "/at" + std::to_string(p) + "/";
+        NRGK_REQUIRE(f.exists(lg + "rowVector"), NRGK_ERR_INVALID, "nrgk_fdm_load: no operator " + lg + " in " + path);
+        std::vector<unsigned long> rows(f.size(lg + "rowVector")), cols(f.size(lg + "colVector"));
+        f.read(lg + "rowVector", h5::Type::U64, rows.data(), rows.size());
+        f.read(lg + "colVector", h5::Type::U64, cols.data(), cols.size());
+        for (size_t k = 0; k < rows.size(); k++) {
+          unsigned long id[2];
+          f.read(lg + "idVector" + std::to_string(k), h5::Type::U64, id, 2);
+          const size_t sz = (size_t)rows[k] * cols[k];
+          const long long off = (long long)hO.size();
+          hO.resize(hO.size() + sz);
+          f.read(lg + "qmat" + std::to_string(k), h5::Type::F64, hO.data() + off, sz);
+          sh->ops[p][{(int)id[0], (int)id[1]}] = BlockRef{off, (int)rows[k], (int)cols[k]};
+        }
+      }
+      sh->dO.p = arena_alloc(std::max<size_t>(1, hO.size()) * sizeof(double));
+      h2d(sh->dO.as<double>(), hO.data(), hO.size(), c.st);
+      NRGK_HIP(stream_sync(c.st));
+    }
+    shells.push_back(std::move(sh));
+  }
+  // the count the forward pass ended at (after the last shell's update_internal_state): the shell
+  // prefactors of the thermodynamics count from it
+  loaded_nrg_max = shells.back()->cnt + 1;
+}
+
 // the recorded shell with iteration counter cnt
 Shell* Fdm::shell_at(int cnt) const {
   for (auto& s : shells)
@@ -521,7 +621,7 @@ void Fdm::thermo(double lambda, int scale_offset, int first_cnt, const double* T
   std::vector<std::pair<Shell*, double>> use;
   for (auto it = shells.rbegin(); it != shells.rend(); ++it)
     if ((*it)->cnt >= first_cnt) use.push_back({it->get(), std::pow(lambda, -((double)(*it)->cnt + scale_offset) / 2.0)});
-  thermo_shells(use, ch->cnt, T, nT, S_out, C_out, ops_out);
+  thermo_shells(use, loaded_nrg_max > 0 ? loaded_nrg_max : ch->cnt, T, nT, S_out, C_out, ops_out);
 }
 
 // calcThermodynamics(scale) over the given shells in order (the last shell first) and saveFinalData

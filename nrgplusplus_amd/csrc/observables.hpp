@@ -63,6 +63,8 @@ struct Fdm {
   void record();
   // NrgData::saveCurrentData + saveqOperator (nrgData.hpp:134-205) of every recorded shell
   void save(const std::string& path, const std::string& op_group) const;
+  // NrgData::loadCurrentData + loadqOperator (nrgData.hpp:214-293) of every shell of a saved file
+  void load(const std::string& path, const std::string& op_group);
   void backward(double lambda, int scale_offset, int first_cnt, double* pos, double* neg, double* traces,
                 double* sums, int cap, int* n_shells);
   void backward_shells(const std::vector<std::pair<Shell*, double>>& list, double* pos, double* neg, double* traces,
@@ -73,6 +75,7 @@ struct Fdm {
   // shells registered one at a time (calcSpectrum / calcThermodynamics of the reference's driver loop):
   // (iteration counter, energy scale) in call order
   std::vector<std::pair<int, double>> registered;
+  int loaded_nrg_max = 0;  // set by load(): the forward pass's final iteration count
   // ops_out ([sysop ops][nT], may be null): the system operators' thermal expectation values
   void thermo(double lambda, int scale_offset, int first_cnt, const double* T, int nT, double* S, double* C,
               double* ops_out = nullptr);

@@ -79,9 +79,9 @@ nrgk_site_desc OwnedSite::desc() const {
 
 // Device bytes a request is admitted against: the chain's per-iteration arenas (about a dozen n x n
 // double arenas over the sector blocks) and one FDM shell per site (U blocks, eigenvalues, observable
-// blocks).  The sector blocks of a spin-1/2 chain at K kept states hold N = d_bath (K + 32) states and
-// sum_b n_b^2 ~ N^2 / 40 elements (measured on the SIAM workloads: 0.02-0.03 N^2); the context's
-// capacity remains the hard limit.
+// blocks).  The sector blocks of a chain at K kept states hold N = d_bath (K + 32) states and at most
+// about sum_b n_b^2 = N^2 / 32 elements (the SIAM workloads have 0.02-0.03 N^2); the estimate admits
+// against that bound, and the context's capacity remains the hard limit.
 size_t Server::footprint(const Request& r) const {
   size_t dbath = 0;
   for (int v : r.bath.dims) dbath += (size_t)v;
@@ -143,6 +143,7 @@ void Server::worker() {
       reserved += r->bytes;
       running++;
       peak_running = std::max(peak_running, running);
+      peak_reserved = std::max(peak_reserved, reserved);
       r->t_start = std::chrono::steady_clock::now();
     }
     int status = NRGK_OK;
@@ -212,19 +213,21 @@ void Server::solve(Request& r) {
   if (r.n_omega > 0) {
     DevBuf d(ctx);
     const int nb = r.n_bins, no = r.n_omega;
-    d.ensure(((size_t)3 * nb + 3 * no + 8) * sizeof(double));
+    const size_t ns = broaden_scratch_doubles(nb, no);
+    d.ensure(((size_t)3 * nb + 3 * no + ns + 8) * sizeof(double));
     double* dx = d.as<double>();
     double* dwp = dx + nb;
     double* dwn = dwp + nb;
     double* dom = dwn + nb;
     double* dap = dom + no;
     double* dan = dap + no;
+    double* dscr = dan + no;
     h2d(dx, x.data(), nb, ch.st);
     h2d(dwp, wp.data(), nb, ch.st);
     h2d(dwn, wn.data(), nb, ch.st);
     h2d(dom, r.omega.data(), no, ch.st);
-    launch_broaden(dx, dwp, nb, dom, no, r.broadening, dap, ch.st);
-    launch_broaden(dx, dwn, nb, dom, no, r.broadening, dan, ch.st);
+    launch_broaden(dx, dwp, nb, dom, no, r.broadening, dap, dscr, ch.st);
+    launch_broaden(dx, dwn, nb, dom, no, r.broadening, dan, dscr, ch.st);
     d2h(r.A_pos.data(), dap, no, ch.st);
     d2h(r.A_neg.data(), dan, no, ch.st);
     NRGK_HIP(stream_sync(ch.st));

@@ -52,7 +52,7 @@ struct Server {
   std::vector<std::thread> workers;
   bool stop = false;
   long long next_id = 1;
-  size_t reserved = 0;
+  size_t reserved = 0, peak_reserved = 0;
   int running = 0, peak_running = 0;
   long long completed = 0;
 

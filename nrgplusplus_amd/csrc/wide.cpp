@@ -122,7 +122,7 @@ void Chain::wide_vectors(bool all) {
     const int s = wide_sectors[i];
     b.m = all ? b.n : std::min(b.n, kept.empty() ? b.n : kept[s]);
     if (b.m <= 0) continue;
-    b.mpad = (b.m + 255) / 256 * 256;
+    b.mpad = ht_stein_slots(b.m);
     b.s_off = so;
     so += 3LL * b.n * b.mpad;
     bl.push_back(b);
@@ -132,9 +132,6 @@ void Chain::wide_vectors(bool all) {
     return;
   }
   d_wide_s.ensure((size_t)so * sizeof(double));
-  long long nq = 0;
-  for (const auto& b : bl) nq += b.m;
-  d_wide_q.ensure((size_t)(1 + 2 * nq) * sizeof(int));
   const HtBlock* d = stg.put(bl, st);
   double* W = d_wide_ws.as<double>();
   double* U = dU.as<double>();
@@ -142,7 +139,7 @@ void Chain::wide_vectors(bool all) {
   // eigenvalues g apart orthogonal to about eps ||T|| / g, so that bound keeps the rest at ~2e-11
   const double ctol = env_knob("NRGK_WIDE_CLUSTER", 1e-5);
   prof_begin("tridiag_vectors");
-  launch_ht_stein(d, bl, W, dlam_raw.as<double>(), U, d_wide_s.as<double>(), ctol, d_wide_q.as<int>(), st);
+  launch_ht_stein(d, bl, W, dlam_raw.as<double>(), U, d_wide_s.as<double>(), ctol, st);
   prof_end("tridiag_vectors", 1, 0.0, 0.0);
   // V_p T_p over each block's working matrix (no longer needed), then per panel, last first:
   // Y = V_p^T Z, Z -= (V_p T_p) Y

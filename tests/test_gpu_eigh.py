@@ -206,3 +206,31 @@ def test_upper_triangle_only_input(nk, monkeypatch):
         for (H, lam), (w, U) in zip(full, out):
             assert np.abs(w - lam).max() <= 1e-10
             assert np.abs(H @ U - U * w).max() <= 1e-10
+
+
+def test_autotuner_sets_and_caches_the_crossover(nk, tmp_path, monkeypatch):
+    """nrgk_tune measures both paths on widths 192..768, picks the widest at which the batched Jacobi is
+    not slower, writes the choice and the measured times to its cache, and a context created later
+    reads it"""
+    path = str(tmp_path / "tune.txt")
+    ctx = nk.Context(0)
+    cross, table = ctx.tune(path)
+    assert cross in [int(w) for w in table[:, 0]]
+    assert np.all(table[:, 2] > 0) and np.all(table[:, 3] > 0)
+    faster = [int(w) for w, _, tj, th in table if tj <= th]
+    assert cross == (max(faster) if faster else int(table[0, 0]))
+    assert ctx.crossover() == (cross, True)
+    text = open(path).read()
+    assert "crossover %d" % cross in text
+    monkeypatch.setenv("NRGK_TUNE_CACHE", path)
+    assert nk.Context(0).crossover() == (cross, True)
+    # a block just above and one just below the tuned crossover, solved with it
+    rng = np.random.default_rng(3)
+    blocks, lams = [], []
+    for n in (cross, cross + 17):
+        lam = np.sort(rng.uniform(-1, 1, n))
+        Q = _haar(rng, n)
+        H = (Q * lam) @ Q.T
+        blocks.append(0.5 * (H + H.T))
+        lams.append(lam)
+    _check(blocks, lams, nk.eigh_batched(blocks, ctx=nk.Context(0)))
