@@ -409,6 +409,14 @@ int nrgk_server_stats(nrgk_server* srv, int* running, int* queued, size_t* reser
  * weight_k exp(-(ln(w_o/x_k)/b - b/4)^2)/(b sqrt(pi)), on the device */
 int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* weight, int n, const double* omega,
                  int n_omega, double b, double* out);
+/* No counterpart in the reference: nrgk_broaden for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  x, weight, omega and out are DEVICE buffers of the
+ * caller; the kernels are enqueued on `stream` (a hipStream_t, NULL for the default stream) and the
+ * call allocates nothing: the partial sums use a fixed 4 MiB device array per GPU, so
+ * ceil(n / 512) * n_omega must not exceed 524288 (else NRGK_ERR_INVALID).  Results equal
+ * nrgk_broaden's bit for bit.  Calls that overlap on one GPU race on that array. */
+int nrgk_broaden_noalloc(nrgk_ctx* ctx, const double* x, const double* weight, int n, const double* omega,
+                         int n_omega, double b, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -72,6 +72,7 @@ int nrgk_ctx_create(int device, nrgk_ctx** out) {
     if (colon != std::string::npos) h->c.arch.resize(colon);
     set_kernel_lds_limits(h->c.lds_max);
     set_ht_lds_limits(h->c.lds_max);
+    noalloc_scratch_setup(device);
     tune_load(h->c);
     size_t fr = 0, tot = 0;
     NRGK_HIP(hipMemGetInfo(&fr, &tot));
@@ -662,6 +663,24 @@ int nrgk_broaden(nrgk_ctx* ctx, const double* x, const double* w, int n, const d
     launch_broaden(dx, dw, n, dom, n_omega, b, dout, dscr, nullptr);
     d2h(out, dout, n_omega, nullptr);
     NRGK_HIP(stream_sync(nullptr));
+  });
+}
+// nrgk_broaden's kernels on device buffers the caller owns, with the fixed scratch of noalloc.hip:
+// enqueues on the caller's stream and allocates nothing (no hipMalloc, no library allocation)
+int nrgk_broaden_noalloc(nrgk_ctx* ctx, const double* x, const double* w, int n, const double* omega, int n_omega,
+                         double b, double* out, void* stream) {
+  return guard([&] {
+    NRGK_REQUIRE(ctx && x && w && omega && out && n >= 0 && n_omega >= 0 && b > 0, NRGK_ERR_INVALID,
+                 "nrgk_broaden_noalloc: bad argument");
+    const size_t need = broaden_scratch_doubles(n, n_omega);
+    NRGK_REQUIRE(need <= NOALLOC_SCRATCH_DOUBLES, NRGK_ERR_INVALID,
+                 "nrgk_broaden_noalloc: ceil(n / 512) * n_omega = " + std::to_string(need) +
+                     " partial sums exceed the fixed scratch of " + std::to_string(NOALLOC_SCRATCH_DOUBLES) +
+                     " doubles");
+    double* scr = noalloc_scratch(ctx->c.device);
+    NRGK_REQUIRE(scr, NRGK_ERR_STATE, "nrgk_broaden_noalloc: the context's device has no scratch address");
+    NRGK_HIP(hipSetDevice(ctx->c.device));
+    launch_broaden(x, w, n, omega, n_omega, b, out, scr, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -73,6 +73,10 @@ void launch_thermo_ops(const double* E, const double* Q, long long q_stride, int
 void launch_diag_gather(const DiagItem* items, int n_items, const double* O, double* out, hipStream_t s);
 // scratch: broaden_scratch_doubles(n, no) doubles of device memory
 size_t broaden_scratch_doubles(int n, int no);
+// the allocation-free entry point's fixed device scratch (noalloc.hip)
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 19;
+double* noalloc_scratch_setup(int device);  // with the device current: looks its address up once
+double* noalloc_scratch(int device);        // that address (nullptr before the set-up)
 void launch_broaden(const double* x, const double* w, int n, const double* omega, int no, double b, double* out,
                     double* scratch, hipStream_t s);
 void launch_transpose(const double* in, double* out, int n, hipStream_t s);
