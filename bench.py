@@ -22,8 +22,12 @@ Beside the timed value the line carries:
   solve_ms_p50/p99  the eigensolve alone (sector eigenpairs of a step), device time per timed step
   eigensolve_roofline  F_eig = sum_b (4/3) n_b^3 + 2 n_b^2 k_b of the timed iterations over the
                 eigensolve's device time, against the FP32 matrix peak
-  roofline      the dominant kernel family of the timed steps, from the library's HIP-event timers
-                (algorithmic bytes or flops per launch over the average launch duration)
+  roofline      the dominant kernel, k_ht_symv (the symmetric products of the wide blocks'
+                tridiagonalisation), from HIP events on its stream around one launch in eight over the
+                timed steps: algorithmic bytes (the trailing lower triangles read once) per launch over
+                the average launch duration, against HBM; traffic from the committed rocprof counters
+  kernels       per kernel family device time, launches and rates, from a replay of the timed
+                iterations
   siam_fdm      the SIAM FDM spectral-function workload (K=2048, 60 sites, 2000 bins): mixed vs plain,
                 eigenvalue error, backward pass, HBM fractions of assembly and binning
   z_average     16 z-shifted chains of the z-averaged thermodynamics workload, one after another and
@@ -106,7 +110,7 @@ def kernel_table(kstats, K):
 
 
 # compute families (flops counted) and HBM families (bytes counted) of the library's timers
-HBM_FAMILIES = ("oa_update", "assemble", "fdm_bin", "sytrd_symv", "sytrd_update", "tridiag_vectors")
+HBM_FAMILIES = ("oa_update", "assemble", "fdm_bin", "ht_symv", "tridiagonalise", "tridiag_vectors")
 F32_FAMILIES = ("jacobi_round",)
 
 
@@ -123,7 +127,7 @@ def measured_traffic(kernel):
     return None, None
 
 
-FAMILY_KERNEL = {"jacobi_round": "k_jacobi_round", "gemm_f64": "k_gemm64", "sytrd_symv": "k_sytrd_symv",
+FAMILY_KERNEL = {"jacobi_round": "k_jacobi_round", "gemm_f64": "k_gemm64", "ht_symv": "k_ht_symv",
                  "assemble": "k_assemble", "fdm_bin": "k_fdm_bin"}
 
 
@@ -157,15 +161,18 @@ def two_channel_headline(nk, W, K, with_plain=True, with_check=True):
     """The headline: W untimed iterations, K timed, then the plain FP64 form over the same iterations
     and the eigenvalue check on a replay of the timed iterations."""
     s = TwoChannel(nk, nk.NRGK_MODE_MIXED)
-    for _ in range(W):
+    for i in range(W):
         s.step()
-    s.g.profile(True)
+        progress("two-channel warm-up iteration %d done" % i)
+    # the dominant kernel, k_ht_symv, timed by HIP events on its stream around one launch in eight
+    s.g.profile(True, only="ht_symv")
     step_ms, solve_ms, feig, unconv, launches, syncs, eigs, ties = [], [], 0.0, 0, [], [], [], []
     t0 = time.perf_counter()
     for _ in range(K):
         ta = time.perf_counter()
         e = s.step()
         step_ms.append((time.perf_counter() - ta) * 1e3)
+        progress("two-channel timed iteration %d: %.1f ms" % (W + len(step_ms) - 1, step_ms[-1]))
         eigs.append(e)
         st = s.g.step_stats()
         solve_ms.append(float(st["ms_eig"]))
@@ -186,6 +193,7 @@ def two_channel_headline(nk, W, K, with_plain=True, with_check=True):
         r = TwoChannel(nk, nk.NRGK_MODE_MIXED)
         for _ in range(W):
             r.step()
+        r.g.profile(True)  # every kernel family, on the replay
         err, same, widest = 0.0, True, 0
         for k in range(K):
             r.g.keep_hamiltonians(True)
@@ -199,7 +207,9 @@ def two_channel_headline(nk, W, K, with_plain=True, with_check=True):
             feig += f_eig(dims, r.g.kept_counts())
             a = r.g.all_eigenvalue
             same = same and len(a) == len(eigs[k]) and bool(np.array_equal(a, eigs[k]))
-        out.update(eig_err=err, replay_bit_identical=same, widest_block=widest, F_eig=feig)
+        out.update(eig_err=err, replay_bit_identical=same, widest_block=widest, F_eig=feig,
+                   kstats_replay=r.g.kernel_stats())
+        r.g.profile(False)
         progress("two-channel check: eig_err %.3g, replay identical %s" % (err, same))
     if with_plain:
         p = TwoChannel(nk, nk.NRGK_MODE_PLAIN64)
@@ -531,6 +541,7 @@ def main():
     quick = args.stop_after_timed
     h = two_channel_headline(nk, W, K, with_plain=not (args.no_plain or quick), with_check=not (args.no_check or quick))
     kst = h.pop("kstats")
+    kst_all = h.pop("kstats_replay", None)
     line = {
         "metric": METRIC,
         "value": h["value"],
@@ -560,9 +571,11 @@ def main():
         "host_syncs_per_step": h["host_syncs_per_step"],
         "refinement_unconverged_blocks": h["refinement_unconverged_blocks"],
         "truncation_near_ties": h["near_ties"],
-        "kernels": kernel_table(kst, K),
-        "roofline": roofline(kst),
+        "kernels": kernel_table(kst_all, K) if kst_all else None,
+        "roofline": roofline_of("ht_symv", kst["ht_symv"]) if "ht_symv" in kst else None,
     }
+    if line["roofline"] is not None:
+        line["roofline"]["sampled"] = "k_ht_symv launches of one column in 8, timed region"
     if "F_eig" in h:
         sec = sum(h["solve_ms"]) * 1e-3
         line["eigensolve_roofline"] = {"F_eig_per_step": h["F_eig"] / K, "eigensolve_s_per_step": sec / K,

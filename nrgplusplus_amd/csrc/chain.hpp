@@ -152,7 +152,8 @@ struct Chain {
   std::vector<HtBlock> ht;          // widest first
   std::vector<int> wide_sectors;    // sector of each ht entry
   DevBuf d_ht, d_wide_ws, d_wide_s;
-  double* wide_A = nullptr;  // the working matrices (then V_p T_p) and Householder vectors
+  DevBuf d_wide_A, d_wide_V;  // the working matrices (then V_p T_p) and Householder vectors, row stride ld
+  double* wide_A = nullptr;
   double* wide_V = nullptr;
   enum WideState { WIDE_NONE, WIDE_EIGENVALUES, WIDE_KEPT, WIDE_ALL } wide_state = WIDE_NONE;
   bool wide_joined = true;

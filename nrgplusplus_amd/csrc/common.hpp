@@ -383,7 +383,8 @@ struct DiagItem {     // diagonal of block (i, i) of one system operator
 constexpr int HT_TB = 64;  // tile edge of the symmetric matrix-vector product and the rank-2k update
 constexpr int HT_NB = 32;  // panel width (reflectors per rank-2k update and per WY block)
 struct HtBlock {
-  long long a_off;  // element offset of the block's n x n arenas
+  long long a_off;  // element offset of the block's n x n arenas (H, U: row stride n)
+  long long p_off;  // element offset of its working matrix and Householder vectors (row stride ld)
   long long w_off;  // element offset of its workspace
   long long s_off;  // element offset of its inverse-iteration scratch (3 n x mpad)
   int n;
@@ -391,7 +392,7 @@ struct HtBlock {
   int e_off;        // offset of the block's eigenvalues (prefix of n)
   int m;            // eigenvectors to compute (columns 0 .. m-1)
   int mpad;         // scratch slots of the inverse iteration for this block (ht_stein_slots(m))
-  int pad;
+  int ld;           // row stride of the working matrix: n rounded up to 16 (128-byte rows)
 };
 
 // The workspace of one block: W (n x HT_NB), the symv partials of the NT = T(T+1)/2 lower tiles

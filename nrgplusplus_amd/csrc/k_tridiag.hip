@@ -145,11 +145,11 @@ __global__ void __launch_bounds__(64) k_ht_first(const HtBlock* __restrict__ bl,
   const HtBlock B = bl[b];
   const int n = B.n;
   const HtWs w = ht_ws(const_cast<double*>(Wa) + B.w_off, n, B.T);
-  const double* A = Aa + B.a_off;
+  const double* A = Aa + B.p_off;
   const int r = g * HT_TB + threadIdx.x;
   double sq = 0.0;
   if (r < n) {
-    const double a = A[(long long)r * n];
+    const double a = A[(long long)r * B.ld];
     if (r == 0) w.d[0] = a;
     else w.u[r] = a;
     if (r >= 2) sq = a * a;
@@ -166,8 +166,9 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   const HtBlock B = bl[b];
   const int n = B.n, T = B.T, s = k + 1, t0 = s / HT_TB;
   const HtWs w = ht_ws(Wa + B.w_off, n, T);
-  const double* A = Aa + B.a_off;
-  const double* V = Va + B.a_off;
+  const double* A = Aa + B.p_off;
+  const double* V = Va + B.p_off;
+  const long long ld = B.ld;
   const double* u = w.u + (long long)(k & 1) * n;
   int Ip, Jp;
   ht_tile_of(t, Ip, Jp);
@@ -188,7 +189,7 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   for (int i = 0; i < 16; i++) {
     const int r = I * HT_TB + wv * 16 + i;
     const bool ok = r < n && c < n && r >= s && c >= s && (!diag || c <= r);
-    a[i] = ok ? A[(long long)r * n + c] : 0.0;
+    a[i] = ok ? A[r * ld + c] : 0.0;
   }
   const int rv = (wv == 0 ? I : J) * HT_TB + lane;
   const double uv = wv < 2 && rv < n ? u[rv] : 0.0;
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
 #pragma unroll 16
       for (int l = 0; l < HT_TB; l++) {
         const int r = I * HT_TB + l;
-        if (r >= s && r < n) acc += V[(long long)r * n + j0 + tid] * vr[l];
+        if (r >= s && r < n) acc += V[r * ld + j0 + tid] * vr[l];
       }
       w.qv[I * HT_NB + tid] = acc;
     } else if (tid >= 64 && tid - 64 < cc) {
@@ -265,11 +266,11 @@ __device__ __forceinline__ double ht_ypart(const HtWs& w, int T, int t0, int r, 
   return y;
 }
 // V(r,:) W^T v + W(r,:) V^T v over the panel's cc earlier columns
-__device__ __forceinline__ double ht_corr(const HtWs& w, const double* V, int n, int j0, int cc, const double* Vv,
-                                          const double* Wv, int r) {
+__device__ __forceinline__ double ht_corr(const HtWs& w, const double* V, long long ld, int j0, int cc,
+                                          const double* Vv, const double* Wv, int r) {
   double corr = 0.0;
 #pragma unroll 8
-  for (int j = 0; j < cc; j++) corr += V[(long long)r * n + j0 + j] * Wv[j] + w.W[(long long)r * HT_NB + j] * Vv[j];
+  for (int j = 0; j < cc; j++) corr += V[r * ld + j0 + j] * Wv[j] + w.W[(long long)r * HT_NB + j] * Vv[j];
   return corr;
 }
 
@@ -288,8 +289,9 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
   const int j0 = (k / HT_NB) * HT_NB, cc = k - j0;
   const bool pend = ht_panel_end(n, k);
   const HtWs w = ht_ws(Wa + B.w_off, n, T);
-  const double* A = Aa + B.a_off;
-  double* V = Va + B.a_off;
+  const double* A = Aa + B.p_off;
+  double* V = Va + B.p_off;
+  const long long ld = B.ld;
   const double* u = w.u + (long long)(k & 1) * n;
   __shared__ HtRefl R;
   __shared__ double Vv[HT_NB], Wv[HT_NB];
@@ -369,12 +371,12 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
   if (own) {
     const double y = (ypart[0][tid] + ypart[1][tid]) + (ypart[2][tid] + ypart[3][tid]);
     vrr = ht_v(u, R, s, n, r);
-    wr = R.tau * (y - ht_corr(w, V, n, j0, cc, Vv, Wv, r)) - 0.5 * R.tau * wvs * vrr;
+    wr = R.tau * (y - ht_corr(w, V, ld, j0, cc, Vv, Wv, r)) - 0.5 * R.tau * wvs * vrr;
     w.W[(long long)r * HT_NB + cc] = wr;
-    V[(long long)r * n + k] = vrr;
+    V[r * ld + k] = vrr;
   } else if (tid == 256 && !pend) {
     const double y = (yspart[0] + yspart[1]) + (yspart[2] + yspart[3]);
-    sh_w1 = R.tau * (y - ht_corr(w, V, n, j0, cc, Vv, Wv, s)) - 0.5 * R.tau * wvs * 1.0;
+    sh_w1 = R.tau * (y - ht_corr(w, V, ld, j0, cc, Vv, Wv, s)) - 0.5 * R.tau * wvs * 1.0;
   }
   if (t == 0 && tid == 0) {
     w.tau[k] = R.tau;
@@ -386,11 +388,10 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
   const double w1 = sh_w1;
   double sq = 0.0;
   if (own) {
-    double un = A[(long long)r * n + s];
+    double un = A[r * ld + s];
 #pragma unroll 8
     for (int j = 0; j < cc; j++)
-      un -= V[(long long)r * n + j0 + j] * w.W[(long long)s * HT_NB + j] +
-            w.W[(long long)r * HT_NB + j] * V[(long long)s * n + j0 + j];
+      un -= V[r * ld + j0 + j] * w.W[(long long)s * HT_NB + j] + w.W[(long long)r * HT_NB + j] * V[s * ld + j0 + j];
     un -= vrr * w1 + wr;  // column k: V(r, k) W(s, k) + W(r, k) V(s, k), V(s, k) = 1
     if (r == s) w.d[s] = un;
     else w.u[(long long)(s & 1) * n + r] = un;
@@ -413,8 +414,9 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
   const int n = B.n, T = B.T, s = k + 1, t0 = s / HT_TB;
   const int j0 = (k / HT_NB) * HT_NB, nbu = k - j0 + 1;
   const HtWs w = ht_ws(Wa + B.w_off, n, T);
-  double* A = Aa + B.a_off;
-  const double* V = Va + B.a_off;
+  double* A = Aa + B.p_off;
+  const double* V = Va + B.p_off;
+  const long long ld = B.ld;
   int Ip, Jp;
   ht_tile_of(t, Ip, Jp);
   const int I = t0 + Ip, J = t0 + Jp;
@@ -433,9 +435,9 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
       const int jj = e >> 6, l = e & 63, j = jb + jj;
       const int ri = I * HT_TB + l, rj = J * HT_TB + l;
       const bool okj = j < nbu;
-      VI[jj][l] = okj && ri < n ? V[(long long)ri * n + j0 + j] : 0.0;
+      VI[jj][l] = okj && ri < n ? V[ri * ld + j0 + j] : 0.0;
       WI[jj][l] = okj && ri < n ? w.W[(long long)ri * HT_NB + j] : 0.0;
-      VJ[jj][l] = okj && rj < n ? V[(long long)rj * n + j0 + j] : 0.0;
+      VJ[jj][l] = okj && rj < n ? V[rj * ld + j0 + j] : 0.0;
       WJ[jj][l] = okj && rj < n ? w.W[(long long)rj * HT_NB + j] : 0.0;
     }
     __syncthreads();
@@ -466,7 +468,7 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
     for (int j = 0; j < 4; j++) {
       const int c = J * HT_TB + 4 * tc + j;
       if (r < n && c < n && r >= s && c >= s && (!diag || c <= r)) {
-        const long long o = (long long)r * n + c;
+        const long long o = r * ld + c;
         const double na = A[o] - acc[i][j];
         A[o] = na;
         if (c == s) {
@@ -782,9 +784,11 @@ __device__ void ht_mgs_wave(double* __restrict__ U, int n, int cs, int c, int la
 // range (a = its first cluster start) in its scratch slot g * 64 + t: HT_ITERS inverse-iteration
 // steps from a fixed pseudo-random start, each normalised, all members at once.  After every step
 // each cluster of more than one member is made orthonormal by the wavefront (ht_mgs_wave, one
-// cluster after another), and once more after the last: the next step then amplifies each
-// member's own direction again, so the components outside the cluster stay at eps ||T|| / gap
-// however close to dependent the members were before the orthogonalisation.  The scratch holds
+// cluster after another), and once more after the last.  Orthogonalising after every step matters:
+// the members of a numerically degenerate cluster (copies of an eigenvalue in different split
+// blocks of T) are all pulled toward the copy their shifts sit closest to, so two steps without it
+// leave them nearly dependent, and their orthogonalisation would then blow up the components outside
+// the cluster; kept apart each step, those stay at eps ||T|| / gap.  The scratch holds
 // the LU factors of one member per lane (slot g * 64 + t of mpad = ht_stein_slots(m)).
 constexpr int HT_CHUNK = 48;
 __global__ void __launch_bounds__(64) k_ht_stein(const HtBlock* __restrict__ bl, int nblk, const double* __restrict__ Wa,
@@ -865,15 +869,16 @@ __global__ void __launch_bounds__(256) k_ht_larft(const HtBlock* __restrict__ bl
   const HtBlock B = bl[b];
   const int n = B.n, j0 = g * HT_NB, nbp = min(HT_NB, n - 1 - j0);
   const HtWs w = ht_ws(const_cast<double*>(Wa) + B.w_off, n, B.T);
-  const double* V = Va + B.a_off;
-  double* Y = Aa + B.a_off;
+  const double* V = Va + B.p_off;
+  double* Y = Aa + B.p_off;
+  const long long ld = B.ld;
   __shared__ double G[HT_NB][HT_NB + 1], Tm[HT_NB][HT_NB + 1], St[64][HT_NB + 1];
   const int tid = threadIdx.x, ga = tid >> 3, gb = tid & 7;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int r0 = j0 + 1; r0 < n; r0 += 64) {
     for (int e2 = tid; e2 < 64 * HT_NB; e2 += 256) {
       const int l = e2 / HT_NB, j = e2 % HT_NB, r = r0 + l;
-      St[l][j] = r < n && j < nbp ? V[(long long)r * n + j0 + j] : 0.0;
+      St[l][j] = r < n && j < nbp ? V[r * ld + j0 + j] : 0.0;
     }
     __syncthreads();
     for (int l = 0; l < 64; l++) {
@@ -903,8 +908,8 @@ __global__ void __launch_bounds__(256) k_ht_larft(const HtBlock* __restrict__ bl
     const int r = j0 + 1 + e2 / HT_NB, jj = e2 % HT_NB;
     if (jj >= nbp) continue;
     double x = 0.0;
-    for (int i = 0; i <= jj; i++) x += V[(long long)r * n + j0 + i] * Tm[i][jj];
-    Y[(long long)r * n + j0 + jj] = x;
+    for (int i = 0; i <= jj; i++) x += V[r * ld + j0 + i] * Tm[i][jj];
+    Y[r * ld + j0 + jj] = x;
   }
 }
 

@@ -40,7 +40,6 @@ int launch_ht_update(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k,
 int launch_ht_syr2k(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, double* A, const double* V, double* W,
                     hipStream_t s);
 void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, double* lam, hipStream_t s);
-// queue: 1 + 2 * (eigenvalues of the blocks) ints of device scratch
 void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
                      double* S, double ctol_rel, hipStream_t s);
 int ht_stein_slots(int m);  // scratch slots (HtBlock::mpad) of the inverse iteration for m columns

@@ -31,21 +31,33 @@ void Chain::ensure_side_streams() {
   }
 }
 
+// Row stride of a wide block's working matrices: n rounded up to 16 doubles, so that every 64-column
+// tile row is four whole 128-byte lines, and moved off multiples of 256 doubles (strides of a large
+// power of two put a tile's rows on few HBM channels).
+static int wide_ld(int n) {
+  int ld = (n + 15) / 16 * 16;
+  if (ld % 256 == 0) ld += 16;
+  return ld;
+}
+
 // Tridiagonalisation and eigenvalues of the given sectors on side stream 0, started once the main
 // stream has assembled H.  Writes the eigenvalues into dlam_raw; wide_join() makes the main stream
-// wait for them.  Each block's working copy of H is in dX, its Householder vectors in dX2 (both n x n
-// at the block's offset), the workspace in d_wide_ws.
+// wait for them.  Each block's working copy of H is in d_wide_A, its Householder vectors in d_wide_V
+// (n x ld at the block's p_off, 128-byte aligned), the workspace in d_wide_ws.
 void Chain::wide_begin(const std::vector<int>& sectors) {
   wide_sectors = sectors;
   std::stable_sort(wide_sectors.begin(), wide_sectors.end(), [&](int x, int y) { return cur_n[x] > cur_n[y]; });
   ht.clear();
-  long long ws = 0;
+  long long ws = 0, po = 0;
   int nmax = 1;
   for (int s : wide_sectors) {
     HtBlock b{};
     b.n = cur_n[s];
     b.T = (b.n + HT_TB - 1) / HT_TB;
     b.a_off = cur_off[s];
+    b.ld = wide_ld(b.n);
+    b.p_off = po;
+    po += ((long long)b.n * b.ld + 31) / 32 * 32;
     b.w_off = ws;
     b.e_off = cur_eoff[s];
     ws += (ht_ws_size(b.n, b.T) + 31) / 32 * 32;
@@ -57,6 +69,8 @@ void Chain::wide_begin(const std::vector<int>& sectors) {
   ensure_side_streams();
   hipStream_t s2 = st_big[0];
   d_wide_ws.ensure((size_t)ws * sizeof(double));
+  d_wide_A.ensure((size_t)po * sizeof(double));
+  d_wide_V.ensure((size_t)po * sizeof(double));
   d_ht.ensure(ht.size() * sizeof(HtBlock));
   // the descriptors go through the main stream's staging (the side stream starts after it)
   const HtBlock* dbl = stg.put(ht, st);
@@ -64,33 +78,42 @@ void Chain::wide_begin(const std::vector<int>& sectors) {
   NRGK_HIP(hipEventRecord(ev_big_ready, st));
   NRGK_HIP(hipStreamWaitEvent(s2, ev_big_ready, 0));
   const HtBlock* d = d_ht.as<HtBlock>();
-  // the refinement of the other blocks swaps dX and dX2 between its steps: the wide blocks keep the
-  // buffers they started in
-  wide_A = dX.as<double>();
-  wide_V = dX2.as<double>();
+  wide_A = d_wide_A.as<double>();
+  wide_V = d_wide_V.as<double>();
   double* A = wide_A;
   double* V = wide_V;
   double* W = d_wide_ws.as<double>();
   for (const auto& b : ht) {
-    const size_t bytes = (size_t)b.n * b.n * sizeof(double);
-    NRGK_HIP(hipMemcpyAsync(A + b.a_off, dH.as<double>() + b.a_off, bytes, hipMemcpyDeviceToDevice, s2));
-    NRGK_HIP(hipMemsetAsync(V + b.a_off, 0, bytes, s2));
+    const size_t row = (size_t)b.n * sizeof(double), pitch = (size_t)b.ld * sizeof(double);
+    NRGK_HIP(hipMemcpy2DAsync(A + b.p_off, pitch, dH.as<double>() + b.a_off, row, row, b.n, hipMemcpyDeviceToDevice, s2));
+    NRGK_HIP(hipMemsetAsync(V + b.p_off, 0, (size_t)b.n * pitch, s2));
   }
   launch_ht_first(d, ht, A, W, s2);
   // algorithmic bytes of the symmetric products (the lower triangle of the trailing matrix read once
-  // per column) and of the rank-2k updates (that triangle read and written once per panel)
+  // per column) and of the rank-2k updates (that triangle read and written once per panel).  With the
+  // profile restricted to "ht_symv", one column in HT_SYMV_SAMPLE has an event pair around its
+  // k_ht_symv launch alone (its duration and algorithmic bytes), so the kernel's own rate is measured
+  // without an event between every pair of launches.
+  constexpr int HT_SYMV_SAMPLE = 8;
+  const bool symv_only = prof_on && prof_only == "ht_symv";
   prof_begin("tridiagonalise", s2);
   double by = 0.0;
   long long launches = 0;
   for (int k = 0; k + 2 <= ht[0].n; k++) {
+    double bsym = 0.0;
     if (prof_on)
       for (const auto& b : ht) {
         if (k > b.n - 2) break;
         const double m = b.n - k - 1;
-        by += 8.0 * m * (m + 1) / 2;
+        bsym += 8.0 * m * (m + 1) / 2;
         if (k % HT_NB == HT_NB - 1 || k == b.n - 2) by += 16.0 * m * (m + 1) / 2;
       }
-    launches += launch_ht_symv(d, ht, k, A, V, W, s2) > 0;
+    by += bsym;
+    const bool sample = symv_only && k % HT_SYMV_SAMPLE == 0;
+    if (sample) prof_begin("ht_symv", s2);
+    const int gs = launch_ht_symv(d, ht, k, A, V, W, s2);
+    if (sample) prof_end("ht_symv", gs > 0, 0.0, bsym, s2);
+    launches += gs > 0;
     launches += launch_ht_update(d, ht, k, A, V, W, s2) > 0;
     launches += launch_ht_syr2k(d, ht, k, A, V, W, s2) > 0;
   }
@@ -153,12 +176,12 @@ void Chain::wide_vectors(bool all) {
       const int P = (b.n - 1 + HT_NB - 1) / HT_NB;
       if (step >= P) continue;
       const int p = P - 1 - step, j0 = p * HT_NB, nbp = std::min(HT_NB, b.n - 1 - j0), R = b.n - j0 - 1;
-      const double* Vp = wide_V + b.a_off + (long long)(j0 + 1) * b.n + j0;
-      const double* VTp = wide_A + b.a_off + (long long)(j0 + 1) * b.n + j0;
+      const double* Vp = wide_V + b.p_off + (long long)(j0 + 1) * b.ld + j0;
+      const double* VTp = wide_A + b.p_off + (long long)(j0 + 1) * b.ld + j0;
       double* Z = U + b.a_off + (long long)(j0 + 1) * b.n;
       double* Y = Ys + b.s_off;
-      g1.push_back(GemmProb{Vp, Z, nullptr, Y, nbp, b.m, R, b.n, b.n, b.m, b.m, 1, 0, 1.0, 0.0, 0, 0});
-      g2.push_back(GemmProb{VTp, Y, Z, Z, R, b.m, nbp, b.n, b.m, b.n, b.n, 0, 0, -1.0, 1.0, 0, 0});
+      g1.push_back(GemmProb{Vp, Z, nullptr, Y, nbp, b.m, R, b.ld, b.n, b.m, b.m, 1, 0, 1.0, 0.0, 0, 0});
+      g2.push_back(GemmProb{VTp, Y, Z, Z, R, b.m, nbp, b.ld, b.m, b.n, b.n, 0, 0, -1.0, 1.0, 0, 0});
     }
     run_gemms(g1);
     run_gemms(g2);
