@@ -802,9 +802,16 @@ int nrgk_chains_step(nrgk_chain* const* chains, int n_chains, const double* t, i
       }
     };
     const int nw = std::max(1, std::min(n_chains, n_threads > 0 ? n_threads : 16));
+    // NRGK_DEBUG_MAX_WORKERS=k: thread creation fails after k - 1 workers (the path below, on demand)
+    const char* mw = std::getenv("NRGK_DEBUG_MAX_WORKERS");
+    const int max_started = mw ? std::max(0, std::atoi(mw) - 1) : nw;
     std::vector<std::thread> pool;
     try {
-      for (int w = 1; w < nw; w++) pool.emplace_back(work);
+      for (int w = 1; w < nw; w++) {
+        if ((int)pool.size() >= max_started)
+          throw std::system_error(std::make_error_code(std::errc::resource_unavailable_try_again));
+        pool.emplace_back(work);
+      }
     } catch (const std::system_error&) {
       // no more threads: the workers that started and this thread take the remaining chains
     }

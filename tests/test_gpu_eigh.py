@@ -166,6 +166,15 @@ def test_wide_blocks_householder(nk, seed):
     _check(blocks, lams, nk.eigh_batched(blocks))
 
 
+def test_wide_blocks_at_the_headline_width(nk):
+    """the widest blocks of the two-channel K=4096 workload (~4 300 rows): a degenerate spectrum
+    (six levels, clusters of ~700) and a clustered one (gaps 1e-9 .. 1e-13), Haar-rotated, through the
+    Householder path, to the same 1e-10 bars"""
+    blocks, lams = _case_sizes(43, [4300, 3100], ["clustered", "degenerate"])
+    print("wide blocks 4300 + 3100 built", flush=True)
+    _check(blocks, lams, nk.eigh_batched(blocks))
+
+
 @pytest.mark.parametrize("mixed_max", ["0", "40"])
 def test_householder_path_on_small_blocks(nk, monkeypatch, mixed_max):
     """the crossover moved down (NRGK_MIXED_MAX): blocks of 1 .. 300 rows, below and above one 64-row

@@ -113,6 +113,32 @@ def test_fdm_chains_with_observables_equal_chains_stepped_alone(nk):
         np.testing.assert_allclose(sm, sm1, rtol=1e-12)
 
 
+@pytest.mark.parametrize("max_workers", ["1", "2"])
+def test_chains_step_when_worker_threads_cannot_start(nk, monkeypatch, max_workers):
+    """Thread creation failing after k - 1 workers (NRGK_DEBUG_MAX_WORKERS=k; 1: none start) leaves
+    the remaining chains to the workers that did start and the calling thread: five chains asked for
+    five workers still all step, bit-identical to the chains stepped alone."""
+    monkeypatch.setenv("NRGK_DEBUG_MAX_WORKERS", max_workers)
+    K, ns = 128, 8
+    zs = [0.2, 0.4, 0.6, 0.8, 1.0]
+    hops = _z_hoppings(nk, zs, ns)
+    gs = [nk.NrgCore(nk.Site.spinhalf(EPS, U), nk.Site.spinhalf(0, 0), K) for _ in zs]
+    V = np.sqrt(2 * G / np.pi)
+    nk.chains_step(gs, [[V, V]] * len(gs), 1.0, n_threads=5)
+    for n in range(ns):
+        nk.chains_step(gs, [[h[n], h[n]] for h in hops], 1.0 if n == 0 else np.sqrt(L), n_threads=5)
+    monkeypatch.delenv("NRGK_DEBUG_MAX_WORKERS")
+    for k, g in enumerate(gs):
+        a = nk.NrgCore(nk.Site.spinhalf(EPS, U), nk.Site.spinhalf(0, 0), K)
+        a.add_bath_site([V, V], 1.0)
+        a.update_internal_state()
+        for n in range(ns):
+            a.add_bath_site([hops[k][n]] * 2, 1.0 if n == 0 else np.sqrt(L))
+            a.update_internal_state()
+        assert g.nrg_iterations_cnt == a.nrg_iterations_cnt
+        assert np.array_equal(g.all_eigenvalue, a.all_eigenvalue) and np.array_equal(g.kept_counts(), a.kept_counts())
+
+
 def test_chains_step_errors(nk):
     """A chain listed twice, an observable or recorder of another chain: rejected before any chain
     moves.  A chain whose own step fails (wrong hopping count) is named; the others complete."""
