@@ -63,10 +63,29 @@ __device__ __forceinline__ void ht_tile_of(int t, int& Ip, int& Jp) {
 
 // Workgroup g of a column-k launch over the blocks [0, nblk) (sorted by n, widest first; a block takes
 // part while k <= n - 2): its block b and task t.  KIND 0: tiles, 1: tile-rows, 2: tiles of the blocks
-// whose panel ends at k.  Each wavefront reads 64 blocks' sizes at once, scans their task counts
-// and finds g's block by a ballot (all 64 lanes must take part).
+// whose panel ends at k.  The launch carries the running task counts of the first HT_LMAX blocks in
+// its arguments (end[i] = tasks of blocks 0..i), so the lookup is a scan over kernel-argument words;
+// with more blocks (nb < 0) the counts are formed from the block table.
+constexpr int HT_LMAX = 64;
+struct HtLaunch {
+  int nb;  // blocks taking part (a prefix of the table), or -1: scan the table
+  int end[HT_LMAX];
+};
 template <int KIND>
-__device__ __forceinline__ bool ht_find(const HtBlock* bl, int nblk, int k, int g, int& b, int& t) {
+__device__ __forceinline__ bool ht_find(const HtBlock* bl, int nblk, const HtLaunch& L, int k, int g, int& b,
+                                        int& t) {
+  if (L.nb >= 0) {
+    int prev = 0;
+    for (int i = 0; i < L.nb; i++) {
+      if (g < L.end[i]) {
+        b = i;
+        t = g - prev;
+        return true;
+      }
+      prev = L.end[i];
+    }
+    return false;
+  }
   for (int i = 0; i < nblk; i++) {
     const int n = bl[i].n;
     if (k > n - 2) break;
@@ -81,7 +100,7 @@ __device__ __forceinline__ bool ht_find(const HtBlock* bl, int nblk, int k, int 
   }
   return false;
 }
-// the same counts on the host (grid sizes)
+// the same counts on the host: the grid size and the launch's table of running counts
 int ht_grid(const std::vector<HtBlock>& bl, int k, int kind) {
   int g = 0;
   for (const auto& b : bl) {
@@ -90,6 +109,21 @@ int ht_grid(const std::vector<HtBlock>& bl, int k, int kind) {
     g += kind == 1 ? ht_rows(b.T, k + 1) : ht_tiles(b.T, k + 1);
   }
   return g;
+}
+static HtLaunch ht_launch_table(const std::vector<HtBlock>& bl, int k, int kind) {
+  HtLaunch L{};
+  L.nb = 0;
+  int g = 0;
+  for (const auto& b : bl) {
+    if (k > b.n - 2) break;
+    if (L.nb == HT_LMAX) {
+      L.nb = -1;
+      return L;
+    }
+    if (!(kind == 2 && !ht_panel_end(b.n, k))) g += kind == 1 ? ht_rows(b.T, k + 1) : ht_tiles(b.T, k + 1);
+    L.end[L.nb++] = g;
+  }
+  return L;
 }
 
 // The reflector of column k (dlarfg's form): from alpha = A(k+1, k) and the squared norm of the
@@ -158,11 +192,11 @@ __global__ void __launch_bounds__(64) k_ht_first(const HtBlock* __restrict__ bl,
   if (threadIdx.x == 0) w.ss[g] = sq;  // column 0: buffer 0
 }
 
-__global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl, int nblk, int k,
+__global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int k,
                                                  const double* __restrict__ Aa, const double* __restrict__ Va,
                                                  double* __restrict__ Wa) {
   int b, t;
-  if (!ht_find<0>(bl, nblk, k, blockIdx.x, b, t)) return;
+  if (!ht_find<0>(bl, nblk, L, k, blockIdx.x, b, t)) return;
   const HtBlock B = bl[b];
   const int n = B.n, T = B.T, s = k + 1, t0 = s / HT_TB;
   const HtWs w = ht_ws(Wa + B.w_off, n, T);
@@ -225,25 +259,27 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   __syncthreads();
   if (tid == 0) w.sp[ix] = sp2[0] + sp2[1];
   if (diag) {
-    // this tile-row's shares of V^T v and W^T v over the panel's earlier columns
+    // this tile-row's shares of V^T v and W^T v over the panel's earlier columns: wave h sums rows
+    // 16 h .. 16 h + 15 (lane: V or W, column j; all sixteen loads at once), then the quarters in order
     const int j0 = (k / HT_NB) * HT_NB, cc = k - j0;
-    if (tid < cc) {
-      double acc = 0.0;
-#pragma unroll 16
-      for (int l = 0; l < HT_TB; l++) {
-        const int r = I * HT_TB + l;
-        if (r >= s && r < n) acc += V[r * ld + j0 + tid] * vr[l];
+    const int arr = lane >> 5, j = lane & 31;
+    double acc = 0.0;
+    if (j < cc) {
+      double x[16];
+#pragma unroll
+      for (int l = 0; l < 16; l++) {
+        const int r = I * HT_TB + wv * 16 + l;
+        x[l] = r >= s && r < n ? (arr == 0 ? V[r * ld + j0 + j] : w.W[(long long)r * HT_NB + j]) : 0.0;
       }
-      w.qv[I * HT_NB + tid] = acc;
-    } else if (tid >= 64 && tid - 64 < cc) {
-      const int j = tid - 64;
-      double acc = 0.0;
-#pragma unroll 16
-      for (int l = 0; l < HT_TB; l++) {
-        const int r = I * HT_TB + l;
-        if (r >= s && r < n) acc += w.W[(long long)r * HT_NB + j] * vr[l];
-      }
-      w.qw[I * HT_NB + j] = acc;
+#pragma unroll
+      for (int l = 0; l < 16; l++) acc += x[l] * vr[wv * 16 + l];
+    }
+    cred[wv][lane] = acc;
+    __syncthreads();
+    if (tid < 64 && j < cc) {
+      const double q = (cred[0][tid] + cred[1][tid]) + (cred[2][tid] + cred[3][tid]);
+      if (arr == 0) w.qv[I * HT_NB + j] = q;
+      else w.qw[I * HT_NB + j] = q;
     }
   }
 }
@@ -279,11 +315,11 @@ __device__ __forceinline__ double ht_corr(const HtWs& w, const double* V, long l
 // panel corrections and its norm partial.  Threads 0..255 split each row's partial sums in quarters
 // (thread 4 q + h: row q, quarter h); threads 256..259 do the same for row k + 1, whose w every
 // workgroup needs, so its value is the same bits as the one its owner stores.
-__global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ bl, int nblk, int k,
+__global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int k,
                                                    const double* __restrict__ Aa, double* __restrict__ Va,
                                                    double* __restrict__ Wa) {
   int b, t;
-  if (!ht_find<1>(bl, nblk, k, blockIdx.x, b, t)) return;
+  if (!ht_find<1>(bl, nblk, L, k, blockIdx.x, b, t)) return;
   const HtBlock B = bl[b];
   const int n = B.n, T = B.T, s = k + 1, t0 = s / HT_TB, I = t0 + t;
   const int j0 = (k / HT_NB) * HT_NB, cc = k - j0;
@@ -299,6 +335,7 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
   __shared__ double sred[256];
   __shared__ double ypart[4][HT_TB], yspart[4];
   __shared__ double sh_wv, sh_w1;
+  __shared__ double sh_ws[HT_NB], sh_vs[HT_NB];  // W(s, :) and V(s, j0 + :), the panel's row k + 1
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   if (wv == 4) ht_reflector_wave(w, n, T, k, lane, &R);
   // v^T A v over the active tiles: a fixed partition and tree, the same in every workgroup
@@ -356,6 +393,12 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
       if (arr == 0) Vv[j] = a;
       else Wv[j] = a;
     }
+  } else if (tid < 2 * HT_NB + HT_NB && !pend) {
+    const int j = tid - 2 * HT_NB;
+    if (j < cc) {
+      sh_ws[j] = w.W[(long long)s * HT_NB + j];
+      sh_vs[j] = V[s * ld + j0 + j];
+    }
   }
   __syncthreads();
   if (tid == 0) {
@@ -390,8 +433,7 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
   if (own) {
     double un = A[r * ld + s];
 #pragma unroll 8
-    for (int j = 0; j < cc; j++)
-      un -= V[r * ld + j0 + j] * w.W[(long long)s * HT_NB + j] + w.W[(long long)r * HT_NB + j] * V[s * ld + j0 + j];
+    for (int j = 0; j < cc; j++) un -= V[r * ld + j0 + j] * sh_ws[j] + w.W[(long long)r * HT_NB + j] * sh_vs[j];
     un -= vrr * w1 + wr;  // column k: V(r, k) W(s, k) + W(r, k) V(s, k), V(s, k) = 1
     if (r == s) w.d[s] = un;
     else w.u[(long long)(s & 1) * n + r] = un;
@@ -405,11 +447,11 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
 
 // Rank-2k update of the trailing lower triangle (rows and columns >= s = k + 1) at the end of a panel:
 // A -= V W^T + W V^T over the panel's columns; the tiles of tile-column s / HT_TB then extract column s.
-__global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl, int nblk, int k,
+__global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int k,
                                                   double* __restrict__ Aa, const double* __restrict__ Va,
                                                   double* __restrict__ Wa) {
   int b, t;
-  if (!ht_find<2>(bl, nblk, k, blockIdx.x, b, t)) return;
+  if (!ht_find<2>(bl, nblk, L, k, blockIdx.x, b, t)) return;
   const HtBlock B = bl[b];
   const int n = B.n, T = B.T, s = k + 1, t0 = s / HT_TB;
   const int j0 = (k / HT_NB) * HT_NB, nbu = k - j0 + 1;
@@ -923,21 +965,21 @@ void launch_ht_first(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
 int launch_ht_symv(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, const double* V, double* W,
                    hipStream_t s) {
   const int g = ht_grid(bl, k, 0);
-  if (g) NRGK_LAUNCH(k_ht_symv, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), k, A, V, W);
+  if (g) NRGK_LAUNCH(k_ht_symv, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), ht_launch_table(bl, k, 0), k, A, V, W);
   NRGK_HIP(hipGetLastError());
   return g;
 }
 int launch_ht_update(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, double* V, double* W,
                      hipStream_t s) {
   const int g = ht_grid(bl, k, 1);
-  if (g) NRGK_LAUNCH(k_ht_update, dim3(g), dim3(320), 0, s, d_bl, (int)bl.size(), k, A, V, W);
+  if (g) NRGK_LAUNCH(k_ht_update, dim3(g), dim3(320), 0, s, d_bl, (int)bl.size(), ht_launch_table(bl, k, 1), k, A, V, W);
   NRGK_HIP(hipGetLastError());
   return g;
 }
 int launch_ht_syr2k(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, double* A, const double* V, double* W,
                     hipStream_t s) {
   const int g = ht_grid(bl, k, 2);
-  if (g) NRGK_LAUNCH(k_ht_syr2k, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), k, A, V, W);
+  if (g) NRGK_LAUNCH(k_ht_syr2k, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), ht_launch_table(bl, k, 2), k, A, V, W);
   NRGK_HIP(hipGetLastError());
   return g;
 }
