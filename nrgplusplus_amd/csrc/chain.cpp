@@ -205,6 +205,7 @@ void Chain::prof_resolve() {
 }
 
 Chain::~Chain() {
+  wide_thread_join();
   if (st) (void)hipStreamSynchronize(st);
   plain_destroy(*this);
   for (auto& e : ev) (void)hipEventDestroy(e);
@@ -877,6 +878,8 @@ void Chain::eigensolve() {
     if (mode == NRGK_MODE_PLAIN64) eigensolve_plain();
     else eigensolve_mixed();
   } catch (...) {
+    wide_thread_join();
+    wide_err = nullptr;
     for (int k = 0; k < N_SIDE; k++)
       if (st_big[k]) (void)hipStreamSynchronize(st_big[k]);
     (void)hipStreamSynchronize(st);

@@ -4,8 +4,10 @@
 #include <algorithm>
 #include <map>
 #include <stdexcept>
+#include <exception>
 #include <string>
 #include <memory>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -157,6 +159,11 @@ struct Chain {
   double* wide_V = nullptr;
   enum WideState { WIDE_NONE, WIDE_EIGENVALUES, WIDE_KEPT, WIDE_ALL } wide_state = WIDE_NONE;
   bool wide_joined = true;
+  // the host worker that enqueues the wide blocks' column loop on side stream 0 while this thread runs
+  // the batched solve; its launches, timing runs and failure are taken over when it is joined
+  std::thread wide_thread;
+  std::exception_ptr wide_err;
+  long long wide_launches = 0;
   // every eigenvector of every block is formed in add_bath_site (observables and FDM shells read
   // full U blocks); otherwise the wide blocks' kept columns only, after the truncation
   bool need_full_u = false;
@@ -210,6 +217,8 @@ struct Chain {
   };
   std::vector<hipEvent_t> ev_pool;
   std::vector<ProfRun> prof_pending;
+  std::vector<ProfRun> wide_prof;  // the worker's timing runs, merged into prof_pending at the join
+  void wide_thread_join();
   hipEvent_t prof_open = nullptr;
   hipEvent_t take_event();
   std::string prof_only;  // a single family to time (empty: all)

@@ -45,6 +45,8 @@ static int wide_ld(int n) {
 // wait for them.  Each block's working copy of H is in d_wide_A, its Householder vectors in d_wide_V
 // (n x ld at the block's p_off, 128-byte aligned), the workspace in d_wide_ws.
 void Chain::wide_begin(const std::vector<int>& sectors) {
+  wide_thread_join();  // a worker of an earlier solve that was not joined (an error path)
+  wide_err = nullptr;
   wide_sectors = sectors;
   std::stable_sort(wide_sectors.begin(), wide_sectors.end(), [&](int x, int y) { return cur_n[x] > cur_n[y]; });
   ht.clear();
@@ -89,44 +91,96 @@ void Chain::wide_begin(const std::vector<int>& sectors) {
     NRGK_HIP(hipMemsetAsync(V + b.p_off, 0, (size_t)b.n * pitch, s2));
   }
   launch_ht_first(d, ht, A, W, s2);
-  // algorithmic bytes of the symmetric products (the lower triangle of the trailing matrix read once
-  // per column) and of the rank-2k updates (that triangle read and written once per panel).  With the
-  // profile restricted to "ht_symv", one column in HT_SYMV_SAMPLE has an event pair around its
-  // k_ht_symv launch alone (its duration and algorithmic bytes), so the kernel's own rate is measured
-  // without an event between every pair of launches.
-  constexpr int HT_SYMV_SAMPLE = 8;
-  const bool symv_only = prof_on && prof_only == "ht_symv";
-  prof_begin("tridiagonalise", s2);
-  double by = 0.0;
-  long long launches = 0;
-  for (int k = 0; k + 2 <= ht[0].n; k++) {
-    double bsym = 0.0;
-    if (prof_on)
-      for (const auto& b : ht) {
-        if (k > b.n - 2) break;
-        const double m = b.n - k - 1;
-        bsym += 8.0 * m * (m + 1) / 2;
-        if (k % HT_NB == HT_NB - 1 || k == b.n - 2) by += 16.0 * m * (m + 1) / 2;
-      }
-    by += bsym;
-    const bool sample = symv_only && k % HT_SYMV_SAMPLE == 0;
-    if (sample) prof_begin("ht_symv", s2);
-    const int gs = launch_ht_symv(d, ht, k, A, V, W, s2);
-    if (sample) prof_end("ht_symv", gs > 0, 0.0, bsym, s2);
-    launches += gs > 0;
-    launches += launch_ht_update(d, ht, k, A, V, W, s2) > 0;
-    launches += launch_ht_syr2k(d, ht, k, A, V, W, s2) > 0;
-  }
-  prof_end("tridiagonalise", launches, 0.0, by, s2);
-  prof_begin("tridiag_eigenvalues", s2);
-  launch_ht_stebz(d, ht, W, dlam_raw.as<double>(), s2);
-  prof_end("tridiag_eigenvalues", 1, 0.0, 0.0, s2);
-  NRGK_HIP(hipEventRecord(ev_big_done[0], s2));
   wide_state = WIDE_EIGENVALUES;
   wide_joined = false;
+  wide_err = nullptr;
+  wide_launches = 0;
+  wide_prof.clear();
+  // The column loop (two or three launches per column of the widest block, ~12 000 for a 4 300-row
+  // block) is enqueued by a host worker, so that this thread reaches the batched solve of the other
+  // blocks at once instead of after the enqueueing.  The worker touches only the side stream, its own
+  // events and the block tables, which stay fixed until the join.
+  wide_thread = std::thread([this, d, A, V, W, s2] {
+    try {
+      NRGK_HIP(hipSetDevice(ctx->device));
+      const long long l0 = thread_counts().launches;
+      auto want = [&](const char* f) { return prof_on && (prof_only.empty() || prof_only == f); };
+      auto mark = [&](hipEvent_t& e) {
+        NRGK_HIP(hipEventCreate(&e));
+        NRGK_HIP(hipEventRecord(e, s2));
+      };
+      // algorithmic bytes of the symmetric products (the lower triangle of the trailing matrix read
+      // once per column) and of the rank-2k updates (that triangle read and written once per panel).
+      // With the profile restricted to "ht_symv", one column in HT_SYMV_SAMPLE has an event pair
+      // around its k_ht_symv launch alone (its duration and algorithmic bytes), so the kernel's own
+      // rate is measured without an event between every pair of launches.
+      constexpr int HT_SYMV_SAMPLE = 8;
+      const bool symv_only = prof_on && prof_only == "ht_symv";
+      const bool fam = want("tridiagonalise");
+      hipEvent_t f0 = nullptr, f1 = nullptr;
+      if (fam) mark(f0);
+      double by = 0.0;
+      long long launches = 0;
+      for (int k = 0; k + 2 <= ht[0].n; k++) {
+        double bsym = 0.0;
+        if (prof_on)
+          for (const auto& b : ht) {
+            if (k > b.n - 2) break;
+            const double m = b.n - k - 1;
+            bsym += 8.0 * m * (m + 1) / 2;
+            if (k % HT_NB == HT_NB - 1 || k == b.n - 2) by += 16.0 * m * (m + 1) / 2;
+          }
+        by += bsym;
+        const bool sample = symv_only && k % HT_SYMV_SAMPLE == 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (sample) mark(e0);
+        const int gs = launch_ht_symv(d, ht, k, A, V, W, s2);
+        if (sample) {
+          mark(e1);
+          wide_prof.push_back(ProfRun{"ht_symv", e0, e1, gs > 0, 0.0, bsym});
+        }
+        launches += gs > 0;
+        launches += launch_ht_update(d, ht, k, A, V, W, s2) > 0;
+        launches += launch_ht_syr2k(d, ht, k, A, V, W, s2) > 0;
+      }
+      if (fam) {
+        mark(f1);
+        wide_prof.push_back(ProfRun{"tridiagonalise", f0, f1, launches, 0.0, by});
+      }
+      hipEvent_t g0 = nullptr, g1 = nullptr;
+      const bool fe = want("tridiag_eigenvalues");
+      if (fe) mark(g0);
+      launch_ht_stebz(d, ht, W, dlam_raw.as<double>(), s2);
+      if (fe) {
+        mark(g1);
+        wide_prof.push_back(ProfRun{"tridiag_eigenvalues", g0, g1, 1, 0.0, 0.0});
+      }
+      NRGK_HIP(hipEventRecord(ev_big_done[0], s2));
+      wide_launches = thread_counts().launches - l0;
+    } catch (...) {
+      wide_err = std::current_exception();
+    }
+  });
+}
+
+// The worker has enqueued everything (its launches and timing runs are taken over); its failure, if
+// any, is rethrown here.
+void Chain::wide_thread_join() {
+  if (!wide_thread.joinable()) return;
+  wide_thread.join();
+  thread_counts().launches += wide_launches;
+  wide_launches = 0;
+  prof_pending.insert(prof_pending.end(), wide_prof.begin(), wide_prof.end());
+  wide_prof.clear();
 }
 
 void Chain::wide_join() {
+  wide_thread_join();
+  if (wide_err) {
+    std::exception_ptr e = wide_err;
+    wide_err = nullptr;
+    std::rethrow_exception(e);
+  }
   if (ht.empty() || wide_joined) return;
   NRGK_HIP(hipStreamWaitEvent(st, ev_big_done[0], 0));
   wide_joined = true;
