@@ -100,7 +100,7 @@ void Chain::init_device() {
   for (DevBuf* b : {&eprev_own, &fbuf[0], &fbuf[1], &d_bathE, &d_bathF, &dH, &dW, &dX, &dX2, &dP, &dS, &dR, &dE, &dU,
                     &dY, &dC, &dlam_tmp, &dlam_raw, &dlam_prev, &dlam_shift[0], &dlam_shift[1], &d_asm, &d_eig, &d_misc, &d_rot,
                     &d_gemm, &d_flags, &d_sigma_bits, &d_sigma, &d_stats, &d_kept, &d_trunc, &d_conv, &d_eig_act,
-                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv, &d_ht, &d_wide_ws, &d_wide_s,
+                    &dQ, &d_rrv, &d_cid, &d_sidx, &d_offmax, &d_dgram, &d_dgv, &d_ht, &d_wide_ws, &d_wide_s, &d_wide_q,
                     &d_wide_A, &d_wide_V})
     b->ctx = ctx;
   stg.ctx = ctx;

@@ -153,7 +153,7 @@ struct Chain {
   // (wide.cpp)
   std::vector<HtBlock> ht;          // widest first
   std::vector<int> wide_sectors;    // sector of each ht entry
-  DevBuf d_ht, d_wide_ws, d_wide_s;
+  DevBuf d_ht, d_wide_ws, d_wide_s, d_wide_q;
   DevBuf d_wide_A, d_wide_V;  // the working matrices (then V_p T_p) and Householder vectors, row stride ld
   double* wide_A = nullptr;
   double* wide_V = nullptr;

@@ -779,63 +779,34 @@ __device__ void ht_col_normalise(double* __restrict__ U, int n, int p) {
   }
 }
 
-// Modified Gram-Schmidt over the c adjacent columns cs .. cs + c - 1 of U (stride n) by the whole
-// wavefront, rows split over the lanes (lane l: rows l, l + 64, ...; each lane only ever touches its
-// own rows).  Round r, for the later columns sixteen at a time: one sweep forms their dot products
-// d_s with column cs + r (each lane's rows in order, a butterfly, lane 0's sum broadcast, so every
-// lane holds the same d), a second takes (d_s / d_r) column r out of them; column r is then scaled
-// by 1 / sqrt(d_r).
-constexpr int HT_MG = 16;
-__device__ void ht_mgs_wave(double* __restrict__ U, int n, int cs, int c, int lane) {
-  for (int r = 0; r < c; r++) {
-    double drr = 0.0;
-    for (int s0 = r; s0 < c; s0 += HT_MG) {
-      double acc[HT_MG];
-#pragma unroll
-      for (int j = 0; j < HT_MG; j++) acc[j] = 0.0;
-      for (int i = lane; i < n; i += 64) {
-        const double* row = U + (long long)i * n + cs;
-        const double xr = row[r];
-#pragma unroll
-        for (int j = 0; j < HT_MG; j++)
-          if (s0 + j < c) acc[j] += xr * row[s0 + j];
-      }
-#pragma unroll
-      for (int j = 0; j < HT_MG; j++) acc[j] = __shfl(ht_wave_sum(acc[j]), 0, 64);
-      if (s0 == r) drr = acc[0];
-      double coef[HT_MG];
-#pragma unroll
-      for (int j = 0; j < HT_MG; j++) coef[j] = s0 + j > r && s0 + j < c && drr > 0.0 ? acc[j] / drr : 0.0;
-      for (int i = lane; i < n; i += 64) {
-        double* row = U + (long long)i * n + cs;
-        const double xr = row[r];
-#pragma unroll
-        for (int j = 0; j < HT_MG; j++)
-          if (s0 + j > r && s0 + j < c) row[s0 + j] -= coef[j] * xr;
-      }
-    }
-    const double inv = drr > 0.0 ? 1.0 / sqrt(drr) : 0.0;
-    for (int i = lane; i < n; i += 64) U[(long long)i * n + cs + r] *= inv;
-  }
-}
-
 // Eigenvectors of T for the eigenvalues 0 .. m-1 of each block (and the rest of a cluster that
 // starts below m), into the columns of U.  Clusters are runs of eigenvalues each within ctol of the
-// one before.  Workgroup (one wavefront) g of a block owns the clusters that start in
+// one before.  The host runs HT_ITERS rounds of
+//   k_ht_stein_step  one inverse-iteration step for every wanted eigenvector (normalised), from a fixed
+//                    pseudo-random start in the first round
+//   k_ht_stein_mgs   every cluster of more than one member made orthonormal
+// and one more k_ht_stein_mgs at the end.  Orthogonalising after every step matters: the members of a
+// numerically degenerate cluster (copies of an eigenvalue in different split blocks of T) are all
+// pulled toward the copy their shifts sit closest to, so steps without it leave them nearly
+// dependent, and their orthogonalisation would then blow up the components outside the cluster;
+// kept apart each step, those stay at eps ||T|| / gap.
+//
+// k_ht_stein_step: workgroup (one wavefront) g of a block takes the clusters that start in
 // [HT_CHUNK g, HT_CHUNK (g + 1)) below m; lane t iterates the members a + t, a + t + 64, ... of that
-// range (a = its first cluster start) in its scratch slot g * 64 + t: HT_ITERS inverse-iteration
-// steps from a fixed pseudo-random start, each normalised, all members at once.  After every step
-// each cluster of more than one member is made orthonormal by the wavefront (ht_mgs_wave, one
-// cluster after another), and once more after the last.  Orthogonalising after every step matters:
-// the members of a numerically degenerate cluster (copies of an eigenvalue in different split
-// blocks of T) are all pulled toward the copy their shifts sit closest to, so two steps without it
-// leave them nearly dependent, and their orthogonalisation would then blow up the components outside
-// the cluster; kept apart each step, those stay at eps ||T|| / gap.  The scratch holds
-// the LU factors of one member per lane (slot g * 64 + t of mpad = ht_stein_slots(m)).
+// range (a = its first cluster start), with the LU factors in its scratch slot g * 64 + t of
+// mpad = ht_stein_slots(m).
 constexpr int HT_CHUNK = 48;
-__global__ void __launch_bounds__(64) k_ht_stein(const HtBlock* __restrict__ bl, int nblk, const double* __restrict__ Wa,
-                                                 const double* __restrict__ lam, double* __restrict__ Ua,
-                                                 double* __restrict__ Sa, double ctol_rel) {
+__device__ __forceinline__ double ht_tnorm_wave(const HtWs& w, int n, int lane) {
+  double nm = 0.0;
+  for (int i = lane; i < n; i += 64) nm = fmax(nm, ht_tnorm_entry(w.d, w.e, n, i));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nm = fmax(nm, __shfl_xor(nm, o, 64));
+  return nm;
+}
+__global__ void __launch_bounds__(64) k_ht_stein_step(const HtBlock* __restrict__ bl, int nblk,
+                                                      const double* __restrict__ Wa, const double* __restrict__ lam,
+                                                      double* __restrict__ Ua, double* __restrict__ Sa, double ctol_rel,
+                                                      int first) {
   int g = blockIdx.x, b = -1;
   for (int i = 0; i < nblk; i++) {
     const int cw = (bl[i].m + HT_CHUNK - 1) / HT_CHUNK;
@@ -851,16 +822,11 @@ __global__ void __launch_bounds__(64) k_ht_stein(const HtBlock* __restrict__ bl,
   const long long mp = B.mpad;
   const HtWs w = ht_ws(const_cast<double*>(Wa) + B.w_off, n, B.T);
   const int lane = threadIdx.x;
-  double nm = 0.0;
-  for (int i = lane; i < n; i += 64) nm = fmax(nm, ht_tnorm_entry(w.d, w.e, n, i));
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) nm = fmax(nm, __shfl_xor(nm, o, 64));
-  const double tnorm = nm;
+  const double tnorm = ht_tnorm_wave(w, n, lane);
   const double* L = lam + B.e_off;
   const double ctol = ctol_rel * tnorm;
   const int a = ht_cluster_start_at(L, n, g * HT_CHUNK, ctol);
-  const int e_nom = min((g + 1) * HT_CHUNK, B.m);
-  const int bend = ht_cluster_start_at(L, n, e_nom, ctol);
+  const int bend = ht_cluster_start_at(L, n, min((g + 1) * HT_CHUNK, B.m), ctol);
   double* U = Ua + B.a_off;
   const int slot = g * 64 + lane;
   double* f0 = Sa + B.s_off;
@@ -868,28 +834,96 @@ __global__ void __launch_bounds__(64) k_ht_stein(const HtBlock* __restrict__ bl,
   double* f2 = f1 + (long long)n * mp;
   // pivots are kept at least eps ||T|| in magnitude (1 for T = 0, where any basis is one)
   const double piv0 = tnorm > 0.0 ? 2.220446049250313e-16 * tnorm : 1.0;
-  bool multi = false;  // a cluster of more than one member starts in this chunk
-  for (int p = a + 1; p < bend; p++) multi = multi || L[p] - L[p - 1] <= ctol;
-  for (int p = a + lane; p < bend; p += 64)
-    for (int i = 0; i < n; i++) U[(long long)i * n + p] = ht_start((unsigned)p, (unsigned)i);
-  for (int it = 0; it <= HT_ITERS; it++) {
-    if (it < HT_ITERS)
-      for (int p = a + lane; p < bend; p += 64) {
-        ht_inverse_step(w.d, w.e, n, L[p], piv0, U, p, f0, f1, f2, mp, slot);
-        ht_col_normalise(U, n, p);
+  for (int p = a + lane; p < bend; p += 64) {
+    if (first)
+      for (int i = 0; i < n; i++) U[(long long)i * n + p] = ht_start((unsigned)p, (unsigned)i);
+    ht_inverse_step(w.d, w.e, n, L[p], piv0, U, p, f0, f1, f2, mp, slot);
+    ht_col_normalise(U, n, p);
+  }
+}
+
+// The clusters of more than one member that start below m, one workgroup per block: (block, first
+// member, members) triples appended to list, their number in count (zeroed beforehand).
+__global__ void __launch_bounds__(256) k_ht_stein_clusters(const HtBlock* __restrict__ bl, const double* __restrict__ Wa,
+                                                           const double* __restrict__ lam, double ctol_rel,
+                                                           int* __restrict__ list, int* __restrict__ count) {
+  const int b = blockIdx.x;
+  const HtBlock B = bl[b];
+  const int n = B.n, tid = threadIdx.x;
+  const HtWs w = ht_ws(const_cast<double*>(Wa) + B.w_off, n, B.T);
+  __shared__ double red[4];
+  const double nm = ht_tnorm_wave(w, n, tid & 63);
+  if ((tid & 63) == 0) red[tid >> 6] = nm;
+  __syncthreads();
+  const double tnorm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const double ctol = ctol_rel * tnorm;
+  const double* L = lam + B.e_off;
+  for (int j = tid; j < B.m; j += 256) {
+    const bool start = j == 0 || L[j] - L[j - 1] > ctol;
+    if (!start || j + 1 >= n || L[j + 1] - L[j] > ctol) continue;
+    int e = j + 1;
+    while (e < n && L[e] - L[e - 1] <= ctol) e++;
+    const int q = atomicAdd(count, 1);
+    list[3 * q] = b;
+    list[3 * q + 1] = j;
+    list[3 * q + 2] = e - j;
+  }
+}
+
+// Modified Gram-Schmidt over each listed cluster (columns cs .. cs + c - 1 of its block's U, stride
+// n) by one 256-thread workgroup, rows split over the threads (thread t: rows t, t + 256, ...; each
+// thread only ever touches its own rows), the workgroups taking the list in strides of the grid.
+// Round r, for the later columns sixteen at a time: one sweep forms their dot products d_s with
+// column cs + r (each thread's rows in order, a butterfly per wavefront, the four wavefronts' sums
+// added in order, so every thread holds the same d), a second takes (d_s / d_r) column r out of
+// them; column r is then scaled by 1 / sqrt(d_r).
+constexpr int HT_MG = 16;
+__global__ void __launch_bounds__(256) k_ht_stein_mgs(const HtBlock* __restrict__ bl, const int* __restrict__ list,
+                                                      const int* __restrict__ count, double* __restrict__ Ua) {
+  __shared__ double red[4][HT_MG];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int nq = *count;
+  for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+    const HtBlock B = bl[list[3 * q]];
+    const int cs = list[3 * q + 1], c = list[3 * q + 2], n = B.n;
+    double* U = Ua + B.a_off;
+    for (int r = 0; r < c; r++) {
+      double drr = 0.0;
+      for (int s0 = r; s0 < c; s0 += HT_MG) {
+        double acc[HT_MG];
+#pragma unroll
+        for (int j = 0; j < HT_MG; j++) acc[j] = 0.0;
+        for (int i = tid; i < n; i += 256) {
+          const double* row = U + (long long)i * n + cs;
+          const double xr = row[r];
+#pragma unroll
+          for (int j = 0; j < HT_MG; j++)
+            if (s0 + j < c) acc[j] += xr * row[s0 + j];
+        }
+#pragma unroll
+        for (int j = 0; j < HT_MG; j++) {
+          const double v = ht_wave_sum(acc[j]);
+          if (lane == 0) red[wv][j] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < HT_MG; j++) acc[j] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+        __syncthreads();
+        if (s0 == r) drr = acc[0];
+        double coef[HT_MG];
+#pragma unroll
+        for (int j = 0; j < HT_MG; j++) coef[j] = s0 + j > r && s0 + j < c && drr > 0.0 ? acc[j] / drr : 0.0;
+        for (int i = tid; i < n; i += 256) {
+          double* row = U + (long long)i * n + cs;
+          const double xr = row[r];
+#pragma unroll
+          for (int j = 0; j < HT_MG; j++)
+            if (s0 + j > r && s0 + j < c) row[s0 + j] -= coef[j] * xr;
+        }
       }
-    if (!multi) {
-      if (it == HT_ITERS - 1) break;  // singletons only: no orthogonalisation
-      continue;
+      const double inv = drr > 0.0 ? 1.0 / sqrt(drr) : 0.0;
+      for (int i = tid; i < n; i += 256) U[(long long)i * n + cs + r] *= inv;
     }
-    __syncthreads();
-    for (int cs = a; cs < bend;) {
-      int ce = cs + 1;
-      while (ce < bend && L[ce] - L[ce - 1] <= ctol) ce++;
-      if (ce - cs > 1) ht_mgs_wave(U, n, cs, ce - cs, lane);
-      cs = ce;
-    }
-    __syncthreads();
   }
 }
 
@@ -993,11 +1027,19 @@ void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
   NRGK_HIP(hipGetLastError());
 }
 void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
-                     double* S, double ctol_rel, hipStream_t s) {
+                     double* S, double ctol_rel, int* list, int* count, hipStream_t s) {
   int g = 0;
   for (const auto& b : bl) g += (b.m + HT_CHUNK - 1) / HT_CHUNK;
   if (!g) return;
-  NRGK_LAUNCH(k_ht_stein, dim3(g), dim3(64), 0, s, d_bl, (int)bl.size(), W, lam, U, S, ctol_rel);
+  const int nb = (int)bl.size();
+  NRGK_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
+  NRGK_LAUNCH(k_ht_stein_clusters, dim3(nb), dim3(256), 0, s, d_bl, W, lam, ctol_rel, list, count);
+  for (int it = 0; it <= HT_ITERS; it++) {
+    if (it < HT_ITERS)
+      NRGK_LAUNCH(k_ht_stein_step, dim3(g), dim3(64), 0, s, d_bl, nb, W, lam, U, S, ctol_rel, it == 0 ? 1 : 0);
+    // a fixed grid that walks the cluster list (its length is read on the device)
+    NRGK_LAUNCH(k_ht_stein_mgs, dim3(1024), dim3(256), 0, s, d_bl, (const int*)list, (const int*)count, U);
+  }
   NRGK_HIP(hipGetLastError());
 }
 int ht_stein_slots(int m) { return (m + HT_CHUNK - 1) / HT_CHUNK * 64; }

@@ -40,8 +40,9 @@ int launch_ht_update(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k,
 int launch_ht_syr2k(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, double* A, const double* V, double* W,
                     hipStream_t s);
 void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, double* lam, hipStream_t s);
+// list: 3 ints per cluster of more than one member (at most the wanted columns), count: one int
 void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
-                     double* S, double ctol_rel, hipStream_t s);
+                     double* S, double ctol_rel, int* list, int* count, hipStream_t s);
 int ht_stein_slots(int m);  // scratch slots (HtBlock::mpad) of the inverse iteration for m columns
 void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* V, const double* W, double* A,
                      hipStream_t s);

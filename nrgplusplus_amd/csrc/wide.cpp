@@ -209,6 +209,9 @@ void Chain::wide_vectors(bool all) {
     return;
   }
   d_wide_s.ensure((size_t)so * sizeof(double));
+  long long nq = 0;
+  for (const auto& b : bl) nq += b.m;
+  d_wide_q.ensure((size_t)(64 + 3 * nq) * sizeof(int));
   const HtBlock* d = stg.put(bl, st);
   double* W = d_wide_ws.as<double>();
   double* U = dU.as<double>();
@@ -216,8 +219,9 @@ void Chain::wide_vectors(bool all) {
   // eigenvalues g apart orthogonal to about eps ||T|| / g, so that bound keeps the rest at ~2e-11
   const double ctol = env_knob("NRGK_WIDE_CLUSTER", 1e-5);
   prof_begin("tridiag_vectors");
-  launch_ht_stein(d, bl, W, dlam_raw.as<double>(), U, d_wide_s.as<double>(), ctol, st);
-  prof_end("tridiag_vectors", 1, 0.0, 0.0);
+  launch_ht_stein(d, bl, W, dlam_raw.as<double>(), U, d_wide_s.as<double>(), ctol, d_wide_q.as<int>() + 64,
+                  d_wide_q.as<int>(), st);
+  prof_end("tridiag_vectors", 8, 0.0, 0.0);  // the cluster list, three steps, four orthogonalisations
   // V_p T_p over each block's working matrix (no longer needed), then per panel, last first:
   // Y = V_p^T Z, Z -= (V_p T_p) Y
   launch_ht_larft(d, bl, wide_V, W, wide_A, st);
