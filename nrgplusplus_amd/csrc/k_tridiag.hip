@@ -989,6 +989,157 @@ __global__ void __launch_bounds__(256) k_ht_larft(const HtBlock* __restrict__ bl
   }
 }
 
+// The back-transformation U = Q Z of the wanted columns, one panel at a time (last first): with
+// V_p (rows j0+1 .. n-1, columns j0 .. j0+nbp-1 of the Householder vectors) and V_p T_p (the same
+// place in the working matrix, k_ht_larft), Y = V_p^T Z and Z -= (V_p T_p) Y.  The wanted columns
+// are few in an RG step (tens per block), so the products run over rows instead of output tiles:
+//   k_ht_wy_y  workgroup (block, 256-row chunk c, 64-column tile): the chunk's share of Y, a 32 x 64
+//              partial, from 64-row pieces of V_p and Z staged in LDS (rows summed in order)
+//   k_ht_wy_z  the same workgroups: Y's tile summed over the chunks in order, then the chunk's rows of
+//              Z updated from 64-row pieces of V_p T_p staged in LDS
+// Each partial lives in the block's scratch at (c * 32 + j) * m + column.
+constexpr int WY_ROWS = 256;
+__device__ __forceinline__ bool ht_wy_find(const HtBlock* bl, int nblk, const HtLaunch& L, int step, int g, int& b,
+                                           int& t) {
+  if (L.nb >= 0) {
+    int prev = 0;
+    for (int i = 0; i < L.nb; i++) {
+      if (g < L.end[i]) {
+        b = i;
+        t = g - prev;
+        return true;
+      }
+      prev = L.end[i];
+    }
+    return false;
+  }
+  for (int i = 0; i < nblk; i++) {
+    const int n = bl[i].n, P = (n - 1 + HT_NB - 1) / HT_NB;
+    if (step >= P) continue;
+    const int R = n - (P - 1 - step) * HT_NB - 1;
+    const int cnt = ((R + WY_ROWS - 1) / WY_ROWS) * ((bl[i].m + 63) / 64);
+    if (g < cnt) {
+      b = i;
+      t = g;
+      return true;
+    }
+    g -= cnt;
+  }
+  return false;
+}
+struct WyPart {
+  int n, m, j0, nbp, R, C, c, ct;
+};
+__device__ __forceinline__ WyPart ht_wy_part(const HtBlock& B, int step, int t) {
+  WyPart q;
+  q.n = B.n;
+  q.m = B.m;
+  const int P = (q.n - 1 + HT_NB - 1) / HT_NB;
+  q.j0 = (P - 1 - step) * HT_NB;
+  q.nbp = min(HT_NB, q.n - 1 - q.j0);
+  q.R = q.n - q.j0 - 1;
+  q.C = (q.R + WY_ROWS - 1) / WY_ROWS;
+  const int CT = (q.m + 63) / 64;
+  q.c = t / CT;
+  q.ct = t % CT;
+  return q;
+}
+__global__ void __launch_bounds__(256) k_ht_wy_y(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int step,
+                                                 const double* __restrict__ Va, const double* __restrict__ Ua,
+                                                 double* __restrict__ Ys) {
+  int b, t;
+  if (!ht_wy_find(bl, nblk, L, step, blockIdx.x, b, t)) return;
+  const HtBlock B = bl[b];
+  const WyPart q = ht_wy_part(B, step, t);
+  const double* V = Va + B.p_off;
+  const double* U = Ua + B.a_off;
+  const long long ld = B.ld;
+  __shared__ double sV[64][HT_NB + 1], sZ[64][65];
+  const int tid = threadIdx.x, jj = tid >> 3, cg = (tid & 7) * 8;
+  double acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc[c] = 0.0;
+  const int row0 = q.j0 + 1 + q.c * WY_ROWS;
+  for (int pc = 0; pc < WY_ROWS / 64; pc++) {
+    const int rb = row0 + pc * 64;
+    if (rb >= q.n) break;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // 64 rows x 32 columns of V_p
+      const int e = tid + 256 * i, rr = e >> 5, cc = e & 31, r = rb + rr;
+      sV[rr][cc] = r < q.n && cc < q.nbp ? V[r * ld + q.j0 + cc] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {  // 64 rows x 64 columns of Z
+      const int e = tid + 256 * i, rr = e >> 6, cc = e & 63, r = rb + rr, col = q.ct * 64 + cc;
+      sZ[rr][cc] = r < q.n && col < q.m ? U[(long long)r * q.n + col] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = 0; rr < 64; rr++) {
+      const double v = sV[rr][jj];
+#pragma unroll
+      for (int c = 0; c < 8; c++) acc[c] += v * sZ[rr][cg + c];
+    }
+    __syncthreads();
+  }
+  double* P = Ys + B.s_off + ((long long)q.c * HT_NB + jj) * q.m;
+  if (jj < q.nbp)
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int col = q.ct * 64 + cg + c;
+      if (col < q.m) P[col] = acc[c];
+    }
+}
+__global__ void __launch_bounds__(256) k_ht_wy_z(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int step,
+                                                 const double* __restrict__ VTa, const double* __restrict__ Ys,
+                                                 double* __restrict__ Ua) {
+  int b, t;
+  if (!ht_wy_find(bl, nblk, L, step, blockIdx.x, b, t)) return;
+  const HtBlock B = bl[b];
+  const WyPart q = ht_wy_part(B, step, t);
+  const double* VT = VTa + B.p_off;
+  double* U = Ua + B.a_off;
+  const long long ld = B.ld;
+  __shared__ double sY[HT_NB][64], sT[64][HT_NB + 1];
+  const int tid = threadIdx.x;
+  const double* P = Ys + B.s_off;
+  for (int e = tid; e < HT_NB * 64; e += 256) {
+    const int j = e >> 6, cl = e & 63, col = q.ct * 64 + cl;
+    double y = 0.0;
+    if (j < q.nbp && col < q.m)
+      for (int c = 0; c < q.C; c++) y += P[((long long)c * HT_NB + j) * q.m + col];
+    sY[j][cl] = y;
+  }
+  // thread: row rr = tid / 4 of each 64-row piece, columns cq .. cq + 15 of the tile
+  const int rr = tid >> 2, cq = (tid & 3) * 16;
+  const int row0 = q.j0 + 1 + q.c * WY_ROWS;
+  for (int pc = 0; pc < WY_ROWS / 64; pc++) {
+    const int rb = row0 + pc * 64;
+    if (rb >= q.n) break;
+    __syncthreads();  // sY ready; the previous piece's sT no longer read
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // 64 rows x 32 columns of V_p T_p
+      const int e = tid + 256 * i, r2 = e >> 5, cc = e & 31, r = rb + r2;
+      sT[r2][cc] = r < q.n && cc < q.nbp ? VT[r * ld + q.j0 + cc] : 0.0;
+    }
+    __syncthreads();
+    const int r = rb + rr;
+    if (r < q.n) {
+      double* zr = U + (long long)r * q.n + q.ct * 64 + cq;
+      double z[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++) z[c] = q.ct * 64 + cq + c < q.m ? zr[c] : 0.0;
+      for (int j = 0; j < q.nbp; j++) {
+        const double v = sT[rr][j];
+#pragma unroll
+        for (int c = 0; c < 16; c++) z[c] -= v * sY[j][cq + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (q.ct * 64 + cq + c < q.m) zr[c] = z[c];
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------- launchers
 void launch_ht_first(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* A, double* W, hipStream_t s) {
   int g = 0;
@@ -1043,6 +1194,27 @@ void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
   NRGK_HIP(hipGetLastError());
 }
 int ht_stein_slots(int m) { return (m + HT_CHUNK - 1) / HT_CHUNK * 64; }
+// the back-transformation step `step` (panel P-1-step of every block that has it): partials, then Z
+int launch_ht_wy(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int step, const double* V, const double* VT,
+                 double* U, double* Ys, hipStream_t s) {
+  HtLaunch L{};
+  int g = 0;
+  L.nb = (int)bl.size() <= HT_LMAX ? 0 : -1;
+  for (const auto& b : bl) {
+    const int P = (b.n - 1 + HT_NB - 1) / HT_NB;
+    if (step < P) {
+      const int R = b.n - (P - 1 - step) * HT_NB - 1;
+      g += ((R + WY_ROWS - 1) / WY_ROWS) * ((b.m + 63) / 64);
+    }
+    if (L.nb >= 0) L.end[L.nb++] = g;
+  }
+  if (!g) return 0;
+  const int nb = (int)bl.size();
+  NRGK_LAUNCH(k_ht_wy_y, dim3(g), dim3(256), 0, s, d_bl, nb, L, step, V, (const double*)U, Ys);
+  NRGK_LAUNCH(k_ht_wy_z, dim3(g), dim3(256), 0, s, d_bl, nb, L, step, VT, (const double*)Ys, U);
+  NRGK_HIP(hipGetLastError());
+  return 2;
+}
 void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* V, const double* W, double* A,
                      hipStream_t s) {
   int g = 0;

@@ -44,6 +44,8 @@ void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
 void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
                      double* S, double ctol_rel, int* list, int* count, hipStream_t s);
 int ht_stein_slots(int m);  // scratch slots (HtBlock::mpad) of the inverse iteration for m columns
+int launch_ht_wy(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int step, const double* V, const double* VT,
+                 double* U, double* Ys, hipStream_t s);
 void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* V, const double* W, double* A,
                      hipStream_t s);
 void set_ht_lds_limits(int bytes);

@@ -223,27 +223,15 @@ void Chain::wide_vectors(bool all) {
                   d_wide_q.as<int>(), st);
   prof_end("tridiag_vectors", 8, 0.0, 0.0);  // the cluster list, three steps, four orthogonalisations
   // V_p T_p over each block's working matrix (no longer needed), then per panel, last first:
-  // Y = V_p^T Z, Z -= (V_p T_p) Y
+  // Y = V_p^T Z, Z -= (V_p T_p) Y, row-parallel (k_ht_wy_y, k_ht_wy_z)
   launch_ht_larft(d, bl, wide_V, W, wide_A, st);
   int pmax = 0;
   for (const auto& b : bl) pmax = std::max(pmax, (b.n - 1 + HT_NB - 1) / HT_NB);
   double* Ys = d_wide_s.as<double>();  // the inverse-iteration scratch is free again
-  for (int step = 0; step < pmax; step++) {
-    std::vector<GemmProb> g1, g2;
-    for (const auto& b : bl) {
-      const int P = (b.n - 1 + HT_NB - 1) / HT_NB;
-      if (step >= P) continue;
-      const int p = P - 1 - step, j0 = p * HT_NB, nbp = std::min(HT_NB, b.n - 1 - j0), R = b.n - j0 - 1;
-      const double* Vp = wide_V + b.p_off + (long long)(j0 + 1) * b.ld + j0;
-      const double* VTp = wide_A + b.p_off + (long long)(j0 + 1) * b.ld + j0;
-      double* Z = U + b.a_off + (long long)(j0 + 1) * b.n;
-      double* Y = Ys + b.s_off;
-      g1.push_back(GemmProb{Vp, Z, nullptr, Y, nbp, b.m, R, b.ld, b.n, b.m, b.m, 1, 0, 1.0, 0.0, 0, 0});
-      g2.push_back(GemmProb{VTp, Y, Z, Z, R, b.m, nbp, b.ld, b.m, b.n, b.n, 0, 0, -1.0, 1.0, 0, 0});
-    }
-    run_gemms(g1);
-    run_gemms(g2);
-  }
+  prof_begin("back_transform");
+  long long nl = 0;
+  for (int step = 0; step < pmax; step++) nl += launch_ht_wy(d, bl, step, wide_V, wide_A, U, Ys, st);
+  prof_end("back_transform", nl, 0.0, 0.0);
   wide_state = all ? WIDE_ALL : WIDE_KEPT;
 }
 
