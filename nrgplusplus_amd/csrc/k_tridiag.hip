@@ -13,8 +13,8 @@
 //                   column, the lower triangle only)
 //      k_ht_update  one workgroup per 64-row tile-row: reduces those partials to w, stores v and w,
 //                   and forms the next column with the panel corrections and its norm
-//      k_ht_syr2k   the panel's rank-2k update of the trailing lower triangle; the tiles of the
-//                   next column also extract it
+//      k_ht_syr2k   the panel's rank-2k update of the trailing lower triangle on the FP64 matrix
+//                   cores; the tiles of the next column also extract it
 //    Every scalar a later launch needs (the reflector of column k from the norm partials) is
 //    recomputed by each workgroup from the same partials in the same order, so all agree bit for bit
 //    and no launch waits on a single-workgroup reduction.
@@ -25,13 +25,16 @@
 //    made orthonormal afterwards by modified Gram-Schmidt across the wavefront), for the columns
 //    that are needed: all of them, or only the kept ones.
 // 4. Back-transformation U = Q Z in WY form, Q_p = I - V_p T_p V_p^T per panel (k_ht_larft forms
-//    V_p T_p), as two batched FP64 GEMMs per panel on the matrix cores (k_gemm64).
+//    V_p T_p): per panel Y = V_p^T Z and Z -= (V_p T_p) Y on the FP64 matrix cores (k_ht_wy_y,
+//    k_ht_wy_z).
 #include <cmath>
 
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace nrgk {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double ht_wave_sum(double v) {
 #pragma unroll
@@ -192,6 +195,19 @@ __global__ void __launch_bounds__(64) k_ht_first(const HtBlock* __restrict__ bl,
   if (threadIdx.x == 0) w.ss[g] = sq;  // column 0: buffer 0
 }
 
+// One reduce-scatter exchange over a[0 .. 2H): the lane keeps a[H ..) if hi, else a[0 .. H), sends the
+// other half to its DPP partner (CTRL) and adds what the partner sends into a[0 .. H).  The partner
+// holds the same rows with hi the other way round, and every lane of the wavefront takes part.
+template <int H, int CTRL>
+__device__ __forceinline__ void ht_rs_step(double* a, bool hi) {
+#pragma unroll
+  for (int i = 0; i < H; i++) {
+    const double send = hi ? a[i] : a[i + H];
+    const double keep = hi ? a[i + H] : a[i];
+    a[i] = keep + __builtin_amdgcn_update_dpp(0.0, send, CTRL, 0xF, 0xF, false);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int k,
                                                  const double* __restrict__ Aa, const double* __restrict__ Va,
                                                  double* __restrict__ Wa) {
@@ -209,7 +225,7 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   const int I = t0 + Ip, J = t0 + Jp;
   __shared__ HtRefl R;
   __shared__ double vr[HT_TB], vc[HT_TB];
-  __shared__ double red[4][16][HT_TB + 1];
+  __shared__ double red[4][4][16];
   __shared__ double cred[4][HT_TB];
   __shared__ double sp2[2];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -219,11 +235,13 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   const int c = J * HT_TB + lane;
   const bool diag = I == J;
   double a[16];
+  const int r0 = I * HT_TB + __builtin_amdgcn_readfirstlane(wv) * 16;  // the same in every lane
+  const double* Ar = A + r0 * ld;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
-    const int r = I * HT_TB + wv * 16 + i;
+    const int r = r0 + i;
     const bool ok = r < n && c < n && r >= s && c >= s && (!diag || c <= r);
-    a[i] = ok ? A[r * ld + c] : 0.0;
+    a[i] = ok ? Ar[i * ld + c] : 0.0;
   }
   const int rv = (wv == 0 ? I : J) * HT_TB + lane;
   const double uv = wv < 2 && rv < n ? u[rv] : 0.0;
@@ -233,19 +251,30 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   else if (wv == 1) vc[lane] = ht_v_of(uv, R, s, n, rv);
   __syncthreads();
   double colacc = 0.0;
+  const double vcl = vc[lane];
 #pragma unroll
   for (int i = 0; i < 16; i++) {
-    const int r = I * HT_TB + wv * 16 + i;
-    red[wv][i][lane] = a[i] * vc[lane];
-    colacc += (!diag || c < r) ? a[i] * vr[wv * 16 + i] : 0.0;
+    const int r = r0 + i;
+    colacc += (!diag || c < r) ? a[i] * vr[r - I * HT_TB] : 0.0;
+    a[i] *= vcl;
   }
   cred[wv][lane] = colacc;
+  // row sums of the wave's 16 x 64 products over each 16-lane row of the wavefront, reduce-scattered
+  // in registers: four exchanges with the partner lane p ^ 15, p ^ 7, p ^ 1, p ^ 2 (DPP row_mirror,
+  // row_half_mirror, quad_perm), each keeping the half of the rows the lane goes on with, leave lane
+  // p with row 8 p3 + 4 p2 + 2 p0 + p1 (p = lane & 15, bits p3..p0) summed over its 16 lanes
+  const int p = lane & 15;
+  ht_rs_step<8, 0x140>(a, (p & 8) != 0);
+  ht_rs_step<4, 0x141>(a, (p & 4) != 0);
+  ht_rs_step<2, 0xB1>(a, (p & 1) != 0);
+  ht_rs_step<1, 0x4E>(a, (p & 2) != 0);
+  red[wv][lane >> 4][((p >> 3) << 3) | (((p >> 2) & 1) << 2) | ((p & 1) << 1) | ((p >> 1) & 1)] = a[0];
   __syncthreads();
   const long long ix = ht_tidx(I, J);
   double myv = 0.0;
   if (tid < 64) {
-    double rs = 0.0;
-    for (int l = 0; l < HT_TB; l++) rs += red[tid >> 4][tid & 15][l];
+    const int h = tid >> 4, i = tid & 15;
+    const double rs = (red[h][0][i] + red[h][1][i]) + (red[h][2][i] + red[h][3][i]);
     w.rp[ix * 64 + tid] = rs;
     myv = rs * vr[tid];
   } else if (tid < 128) {
@@ -257,7 +286,7 @@ __global__ void __launch_bounds__(256) k_ht_symv(const HtBlock* __restrict__ bl,
   myv = ht_wave_sum(myv);
   if (lane == 0 && wv < 2) sp2[wv] = myv;
   __syncthreads();
-  if (tid == 0) w.sp[ix] = sp2[0] + sp2[1];
+  if (tid == 0) w.sp[t] = sp2[0] + sp2[1];  // by tile index within the launch
   if (diag) {
     // this tile-row's shares of V^T v and W^T v over the panel's earlier columns: wave h sums rows
     // 16 h .. 16 h + 15 (lane: V or W, column j; all sixteen loads at once), then the quarters in order
@@ -332,32 +361,26 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
   __shared__ HtRefl R;
   __shared__ double Vv[HT_NB], Wv[HT_NB];
   __shared__ double qpart[4][2][HT_NB];
-  __shared__ double sred[256];
+  __shared__ double sred[4];
   __shared__ double ypart[4][HT_TB], yspart[4];
   __shared__ double sh_wv, sh_w1;
   __shared__ double sh_ws[HT_NB], sh_vs[HT_NB];  // W(s, :) and V(s, j0 + :), the panel's row k + 1
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   if (wv == 4) ht_reflector_wave(w, n, T, k, lane, &R);
-  // v^T A v over the active tiles: a fixed partition and tree, the same in every workgroup
+  // v^T A v over the launch's tiles (partials by tile index): a fixed partition and order, the same in
+  // every workgroup
   const int m = T - t0, NTa = m * (m + 1) / 2;
   if (tid < 256) {
     double acc = 0.0;
     for (int q0 = tid; q0 < NTa; q0 += 256 * 8) {
       double x[8];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int q = q0 + 256 * j;
-        x[j] = 0.0;
-        if (q < NTa) {
-          int Ip, Jp;
-          ht_tile_of(q, Ip, Jp);
-          x[j] = w.sp[ht_tidx(t0 + Ip, t0 + Jp)];
-        }
-      }
+      for (int j = 0; j < 8; j++) x[j] = q0 + 256 * j < NTa ? w.sp[q0 + 256 * j] : 0.0;
 #pragma unroll
       for (int j = 0; j < 8; j++) acc += x[j];
     }
-    sred[tid] = acc;
+    acc = ht_wave_sum(acc);
+    if (lane == 0) sred[wv] = acc;
     // the panel's V^T v and W^T v: thread 4 p + h sums quarter h of the tile-rows of pair p
     const int pr = tid >> 2, h = tid & 3, arr = pr >> 5, j = pr & 31;
     if (j < cc) {
@@ -382,10 +405,6 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
     yspart[tid - 256] = ht_ypart(w, T, t0, s, tid - 256);
   }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o) sred[tid] += sred[tid + o];
-    __syncthreads();
-  }
   if (tid < 2 * HT_NB) {
     const int arr = tid >> 5, j = tid & 31;
     if (j < cc) {
@@ -404,7 +423,7 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
   if (tid == 0) {
     double vw = 0.0;
     for (int j = 0; j < cc; j++) vw += Vv[j] * Wv[j];
-    sh_wv = R.tau * (sred[0] - 2.0 * vw);
+    sh_wv = R.tau * ((sred[0] + sred[1]) + (sred[2] + sred[3]) - 2.0 * vw);
   }
   __syncthreads();
   const double wvs = sh_wv;
@@ -447,6 +466,11 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
 
 // Rank-2k update of the trailing lower triangle (rows and columns >= s = k + 1) at the end of a panel:
 // A -= V W^T + W V^T over the panel's columns; the tiles of tile-column s / HT_TB then extract column s.
+// One workgroup per 64 x 64 tile (I, J) on the FP64 matrix cores: A_IJ -= X Y^T with X = [V_I W_I],
+// Y = [W_J V_J] (2 nbu columns, four a step).  Wave wv: rows 32 (wv / 2) .., columns 32 (wv % 2) ..
+// (2 x 2 blocks of 16 x 16); the accumulators start as the tile's entries and take -X Y^T.  Lane l
+// supplies X(r + l%16, kk + l/16) and Y(c + l%16, kk + l/16) straight from memory; entry (i, j) of a
+// block lands in lane 16 (i%4) + j, register i/4.
 __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int k,
                                                   double* __restrict__ Aa, const double* __restrict__ Va,
                                                   double* __restrict__ Wa) {
@@ -454,7 +478,7 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
   if (!ht_find<2>(bl, nblk, L, k, blockIdx.x, b, t)) return;
   const HtBlock B = bl[b];
   const int n = B.n, T = B.T, s = k + 1, t0 = s / HT_TB;
-  const int j0 = (k / HT_NB) * HT_NB, nbu = k - j0 + 1;
+  const int j0 = (k / HT_NB) * HT_NB, nbu = k - j0 + 1, K2 = 2 * nbu;
   const HtWs w = ht_ws(Wa + B.w_off, n, T);
   double* A = Aa + B.p_off;
   const double* V = Va + B.p_off;
@@ -462,74 +486,67 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
   int Ip, Jp;
   ht_tile_of(t, Ip, Jp);
   const int I = t0 + Ip, J = t0 + Jp;
-  // thread: rows 4 tr .. 4 tr + 3, columns 4 tc .. 4 tc + 3 of the tile
-  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
-  __shared__ double VI[16][HT_TB], WI[16][HT_TB], VJ[16][HT_TB], WJ[16][HT_TB];
-  __shared__ double sqp[16];
-  double acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-  for (int jb = 0; jb < nbu; jb += 16) {
-    // stage 16 panel columns of V and W for the tile's rows (I) and columns (J)
-    for (int e = tid; e < 16 * HT_TB; e += 256) {
-      const int jj = e >> 6, l = e & 63, j = jb + jj;
-      const int ri = I * HT_TB + l, rj = J * HT_TB + l;
-      const bool okj = j < nbu;
-      VI[jj][l] = okj && ri < n ? V[ri * ld + j0 + j] : 0.0;
-      WI[jj][l] = okj && ri < n ? w.W[(long long)ri * HT_NB + j] : 0.0;
-      VJ[jj][l] = okj && rj < n ? V[rj * ld + j0 + j] : 0.0;
-      WJ[jj][l] = okj && rj < n ? w.W[(long long)rj * HT_NB + j] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int jj = 0; jj < 16; jj++) {
-      double vi[4], wi[4], vj[4], wj[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        vi[q] = VI[jj][4 * tr + q];
-        wi[q] = WI[jj][4 * tr + q];
-        vj[q] = VJ[jj][4 * tc + q];
-        wj[q] = WJ[jj][4 * tc + q];
-      }
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] += vi[i] * wj[j] + wi[i] * vj[j];
-    }
-    __syncthreads();
-  }
   const bool diag = I == J;
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
+  const int rb = I * HT_TB + (wv >> 1) * 32, cb = J * HT_TB + (wv & 1) * 32;
+  __shared__ double sqp[4];
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = rb + 16 * bi + 4 * r + lk, col = cb + 16 * bj + li;
+        const bool ok = row < n && col < n && row >= s && col >= s && (!diag || col <= row);
+        acc[bi][bj][r] = ok ? A[row * ld + col] : 0.0;
+      }
+  int xr[2], yc[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    xr[i] = rb + 16 * i + li;
+    yc[i] = cb + 16 * i + li;
+  }
+#pragma unroll
+  for (int s4 = 0; s4 < 2 * HT_NB / 4; s4++) {
+    const int kk = 4 * s4 + lk;
+    if (4 * s4 >= K2) break;
+    double x[2], y[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const bool okx = xr[i] < n && kk < K2, oky = yc[i] < n && kk < K2;
+      x[i] = !okx ? 0.0 : (kk < nbu ? -V[xr[i] * ld + j0 + kk] : -w.W[(long long)xr[i] * HT_NB + kk - nbu]);
+      y[i] = !oky ? 0.0 : (kk < nbu ? w.W[(long long)yc[i] * HT_NB + kk] : V[yc[i] * ld + j0 + kk - nbu]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[bi], y[bj], acc[bi][bj], 0, 0, 0);
+  }
   double sq = 0.0;
-  const int cs = s - J * HT_TB;  // column s inside this tile (when J == t0)
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int r = I * HT_TB + 4 * tr + i;
+  for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int c = J * HT_TB + 4 * tc + j;
-      if (r < n && c < n && r >= s && c >= s && (!diag || c <= r)) {
-        const long long o = r * ld + c;
-        const double na = A[o] - acc[i][j];
-        A[o] = na;
-        if (c == s) {
-          if (r == s) w.d[s] = na;
-          else w.u[(long long)(s & 1) * n + r] = na;
-          if (r >= s + 2) sq += na * na;
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = rb + 16 * bi + 4 * r + lk, col = cb + 16 * bj + li;
+        if (row < n && col < n && row >= s && col >= s && (!diag || col <= row)) {
+          const double na = acc[bi][bj][r];
+          A[row * ld + col] = na;
+          if (col == s) {
+            if (row == s) w.d[s] = na;
+            else w.u[(long long)(s & 1) * n + row] = na;
+            if (row >= s + 2) sq += na * na;
+          }
         }
       }
-    }
-  }
   if (J == t0) {
-    // the threads holding column s: tc == cs / 4 (one per 4 rows); the tile-row's norm partial
-    if (tc == (cs >> 2)) sqp[tr] = sq;
+    // the tile-row's norm partial: wave sums, then the four waves in order
+    sq = ht_wave_sum(sq);
+    if (l == 0) sqp[wv] = sq;
     __syncthreads();
-    if (tid == 0) {
-      double a = 0.0;
-      for (int q = 0; q < 16; q++) a += sqp[q];
-      w.ss[(long long)(s & 1) * T + I] = a;
-    }
+    if (tid == 0) w.ss[(long long)(s & 1) * T + I] = (sqp[0] + sqp[1]) + (sqp[2] + sqp[3]);
   }
 }
 
@@ -994,9 +1011,9 @@ __global__ void __launch_bounds__(256) k_ht_larft(const HtBlock* __restrict__ bl
 // place in the working matrix, k_ht_larft), Y = V_p^T Z and Z -= (V_p T_p) Y.  The wanted columns
 // are few in an RG step (tens per block), so the products run over rows instead of output tiles:
 //   k_ht_wy_y  workgroup (block, 256-row chunk c, 64-column tile): the chunk's share of Y, a 32 x 64
-//              partial, from 64-row pieces of V_p and Z staged in LDS (rows summed in order)
-//   k_ht_wy_z  the same workgroups: Y's tile summed over the chunks in order, then the chunk's rows of
-//              Z updated from 64-row pieces of V_p T_p staged in LDS
+//              partial, on the matrix cores with operands read straight from memory
+//   k_ht_wy_z  the same workgroups: Y's tile summed over the chunks in order (into LDS), then the
+//              chunk's 256 x 64 piece of Z updated on the matrix cores
 // Each partial lives in the block's scratch at (c * 32 + j) * m + column.
 constexpr int WY_ROWS = 256;
 __device__ __forceinline__ bool ht_wy_find(const HtBlock* bl, int nblk, const HtLaunch& L, int step, int g, int& b,
@@ -1054,40 +1071,41 @@ __global__ void __launch_bounds__(256) k_ht_wy_y(const HtBlock* __restrict__ bl,
   const double* V = Va + B.p_off;
   const double* U = Ua + B.a_off;
   const long long ld = B.ld;
-  __shared__ double sV[64][HT_NB + 1], sZ[64][65];
-  const int tid = threadIdx.x, jj = tid >> 3, cg = (tid & 7) * 8;
-  double acc[8];
+  // wave wv: panel columns jb .. jb + 15 of Y, tile columns cb .. cb + 31 (two 16 x 16 blocks), summed
+  // over the chunk's rows four at a time on the FP64 matrix cores.  Lane l supplies V(r, j0 + jb + l%16)
+  // and Z(r, c + l%16) for row r = r0 + l/16 of a 4-row step, straight from memory (16 lanes read 128
+  // consecutive bytes of a row); entry (j, c) of a block lands in lane 16 (j%4) + c, register j/4.
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+  const int jb = (wv & 1) * 16, c0 = q.ct * 64 + (wv >> 1) * 32 + li, c1 = c0 + 16;
+  const int row0 = q.j0 + 1 + q.c * WY_ROWS, rend = min(row0 + WY_ROWS, q.n);
+  const bool jok = jb + li < q.nbp, ok0 = c0 < q.m, ok1 = c1 < q.m;
+  const double* vp = V + q.j0 + jb + li;
+  dbl4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = row0; r0 < rend; r0 += 32) {
+    double a[8], b0[8], b1[8];
 #pragma unroll
-  for (int c = 0; c < 8; c++) acc[c] = 0.0;
-  const int row0 = q.j0 + 1 + q.c * WY_ROWS;
-  for (int pc = 0; pc < WY_ROWS / 64; pc++) {
-    const int rb = row0 + pc * 64;
-    if (rb >= q.n) break;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {  // 64 rows x 32 columns of V_p
-      const int e = tid + 256 * i, rr = e >> 5, cc = e & 31, r = rb + rr;
-      sV[rr][cc] = r < q.n && cc < q.nbp ? V[r * ld + q.j0 + cc] : 0.0;
+    for (int s = 0; s < 8; s++) {
+      const int r = r0 + 4 * s + lk;
+      const bool rok = r < rend;
+      a[s] = rok && jok ? vp[r * ld] : 0.0;
+      b0[s] = rok && ok0 ? U[(long long)r * q.n + c0] : 0.0;
+      b1[s] = rok && ok1 ? U[(long long)r * q.n + c1] : 0.0;
     }
 #pragma unroll
-    for (int i = 0; i < 16; i++) {  // 64 rows x 64 columns of Z
-      const int e = tid + 256 * i, rr = e >> 6, cc = e & 63, r = rb + rr, col = q.ct * 64 + cc;
-      sZ[rr][cc] = r < q.n && col < q.m ? U[(long long)r * q.n + col] : 0.0;
+    for (int s = 0; s < 8; s++) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b0[s], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b1[s], acc1, 0, 0, 0);
     }
-    __syncthreads();
-    for (int rr = 0; rr < 64; rr++) {
-      const double v = sV[rr][jj];
-#pragma unroll
-      for (int c = 0; c < 8; c++) acc[c] += v * sZ[rr][cg + c];
-    }
-    __syncthreads();
   }
-  double* P = Ys + B.s_off + ((long long)q.c * HT_NB + jj) * q.m;
-  if (jj < q.nbp)
+  double* P = Ys + B.s_off + (long long)q.c * HT_NB * q.m;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int col = q.ct * 64 + cg + c;
-      if (col < q.m) P[col] = acc[c];
+  for (int r = 0; r < 4; r++) {
+    const int j = jb + 4 * r + lk;
+    if (j < q.nbp) {
+      if (ok0) P[(long long)j * q.m + c0] = acc0[r];
+      if (ok1) P[(long long)j * q.m + c1] = acc1[r];
     }
+  }
 }
 __global__ void __launch_bounds__(256) k_ht_wy_z(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int step,
                                                  const double* __restrict__ VTa, const double* __restrict__ Ys,
@@ -1099,45 +1117,68 @@ __global__ void __launch_bounds__(256) k_ht_wy_z(const HtBlock* __restrict__ bl,
   const double* VT = VTa + B.p_off;
   double* U = Ua + B.a_off;
   const long long ld = B.ld;
-  __shared__ double sY[HT_NB][64], sT[64][HT_NB + 1];
+  __shared__ double sY[HT_NB][64];
   const int tid = threadIdx.x;
-  const double* P = Ys + B.s_off;
-  for (int e = tid; e < HT_NB * 64; e += 256) {
-    const int j = e >> 6, cl = e & 63, col = q.ct * 64 + cl;
-    double y = 0.0;
-    if (j < q.nbp && col < q.m)
-      for (int c = 0; c < q.C; c++) y += P[((long long)c * HT_NB + j) * q.m + col];
-    sY[j][cl] = y;
-  }
-  // thread: row rr = tid / 4 of each 64-row piece, columns cq .. cq + 15 of the tile
-  const int rr = tid >> 2, cq = (tid & 3) * 16;
-  const int row0 = q.j0 + 1 + q.c * WY_ROWS;
-  for (int pc = 0; pc < WY_ROWS / 64; pc++) {
-    const int rb = row0 + pc * 64;
-    if (rb >= q.n) break;
-    __syncthreads();  // sY ready; the previous piece's sT no longer read
+  // Y's tile: the chunks' partials summed in chunk order (thread: 8 entries, one load of each per chunk)
+  {
+    const double* P = Ys + B.s_off;
+    double y[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {  // 64 rows x 32 columns of V_p T_p
-      const int e = tid + 256 * i, r2 = e >> 5, cc = e & 31, r = rb + r2;
-      sT[r2][cc] = r < q.n && cc < q.nbp ? VT[r * ld + q.j0 + cc] : 0.0;
-    }
-    __syncthreads();
-    const int r = rb + rr;
-    if (r < q.n) {
-      double* zr = U + (long long)r * q.n + q.ct * 64 + cq;
-      double z[16];
+    for (int i = 0; i < 8; i++) y[i] = 0.0;
+    for (int c = 0; c < q.C; c++) {
 #pragma unroll
-      for (int c = 0; c < 16; c++) z[c] = q.ct * 64 + cq + c < q.m ? zr[c] : 0.0;
-      for (int j = 0; j < q.nbp; j++) {
-        const double v = sT[rr][j];
-#pragma unroll
-        for (int c = 0; c < 16; c++) z[c] -= v * sY[j][cq + c];
+      for (int i = 0; i < 8; i++) {
+        const int e = tid + 256 * i, j = e >> 6, col = q.ct * 64 + (e & 63);
+        if (j < q.nbp && col < q.m) y[i] += P[((long long)c * HT_NB + j) * q.m + col];
       }
+    }
 #pragma unroll
-      for (int c = 0; c < 16; c++)
-        if (q.ct * 64 + cq + c < q.m) zr[c] = z[c];
+    for (int i = 0; i < 8; i++) {
+      const int e = tid + 256 * i;
+      sY[e >> 6][e & 63] = y[i];
     }
   }
+  // wave wv: the chunk's rows 64 wv .. 64 wv + 63 (four 16-row blocks) x the tile's 64 columns (four
+  // blocks).  The accumulators start as Z and take -(V_p T_p)(r, j0 + j) Y(j, c) on the matrix cores,
+  // four panel columns a step: lane l supplies -(V_p T_p)(r + l%16, j0 + j + l/16) and Y(j + l/16, c + l%16)
+  const int wv = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
+  const int rw = q.j0 + 1 + q.c * WY_ROWS + 64 * wv;
+  dbl4 acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = rw + 16 * bi + 4 * r + lk, col = q.ct * 64 + 16 * bj + li;
+        acc[bi][bj][r] = row < q.n && col < q.m ? U[(long long)row * q.n + col] : 0.0;
+      }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < HT_NB / 4; s++) {
+    const int j = 4 * s + lk;
+    double a[4], y[4];
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++) {
+      const int row = rw + 16 * bi + li;
+      a[bi] = row < q.n && j < q.nbp ? -VT[row * ld + q.j0 + j] : 0.0;
+    }
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++) y[bj] = sY[j][16 * bj + li];
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], y[bj], acc[bi][bj], 0, 0, 0);
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = rw + 16 * bi + 4 * r + lk, col = q.ct * 64 + 16 * bj + li;
+        if (row < q.n && col < q.m) U[(long long)row * q.n + col] = acc[bi][bj][r];
+      }
 }
 
 // ----------------------------------------------------------------------------- launchers
