@@ -13,8 +13,8 @@
 //                   column, the lower triangle only)
 //      k_ht_update  one workgroup per 64-row tile-row: reduces those partials to w, stores v and w,
 //                   and forms the next column with the panel corrections and its norm
-//      k_ht_syr2k   the panel's rank-2k update of the trailing lower triangle on the FP64 matrix
-//                   cores; the tiles of the next column also extract it
+//      k_ht_syr2k   the panel's rank-2k update of the trailing lower triangle; the tiles of the
+//                   next column also extract it
 //    Every scalar a later launch needs (the reflector of column k from the norm partials) is
 //    recomputed by each workgroup from the same partials in the same order, so all agree bit for bit
 //    and no launch waits on a single-workgroup reduction.
@@ -466,11 +466,6 @@ __global__ void __launch_bounds__(320) k_ht_update(const HtBlock* __restrict__ b
 
 // Rank-2k update of the trailing lower triangle (rows and columns >= s = k + 1) at the end of a panel:
 // A -= V W^T + W V^T over the panel's columns; the tiles of tile-column s / HT_TB then extract column s.
-// One workgroup per 64 x 64 tile (I, J) on the FP64 matrix cores: A_IJ -= X Y^T with X = [V_I W_I],
-// Y = [W_J V_J] (2 nbu columns, four a step).  Wave wv: rows 32 (wv / 2) .., columns 32 (wv % 2) ..
-// (2 x 2 blocks of 16 x 16); the accumulators start as the tile's entries and take -X Y^T.  Lane l
-// supplies X(r + l%16, kk + l/16) and Y(c + l%16, kk + l/16) straight from memory; entry (i, j) of a
-// block lands in lane 16 (i%4) + j, register i/4.
 __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl, int nblk, const HtLaunch L, int k,
                                                   double* __restrict__ Aa, const double* __restrict__ Va,
                                                   double* __restrict__ Wa) {
@@ -478,7 +473,7 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
   if (!ht_find<2>(bl, nblk, L, k, blockIdx.x, b, t)) return;
   const HtBlock B = bl[b];
   const int n = B.n, T = B.T, s = k + 1, t0 = s / HT_TB;
-  const int j0 = (k / HT_NB) * HT_NB, nbu = k - j0 + 1, K2 = 2 * nbu;
+  const int j0 = (k / HT_NB) * HT_NB, nbu = k - j0 + 1;
   const HtWs w = ht_ws(Wa + B.w_off, n, T);
   double* A = Aa + B.p_off;
   const double* V = Va + B.p_off;
@@ -486,67 +481,82 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
   int Ip, Jp;
   ht_tile_of(t, Ip, Jp);
   const int I = t0 + Ip, J = t0 + Jp;
+  // thread: rows 4 tr .. 4 tr + 3, columns 4 tc .. 4 tc + 3 of the tile
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  __shared__ double VI[16][HT_TB + 1], WI[16][HT_TB + 1], VJ[16][HT_TB + 1], WJ[16][HT_TB + 1];
+  __shared__ double sqp[16];
   const bool diag = I == J;
-  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, li = l & 15, lk = l >> 4;
-  const int rb = I * HT_TB + (wv >> 1) * 32, cb = J * HT_TB + (wv & 1) * 32;
-  __shared__ double sqp[4];
-  dbl4 acc[2][2];
+  // the tile's entries are requested before the product, so their loads overlap it
+  double av[4][4], acc[4][4];
 #pragma unroll
-  for (int bi = 0; bi < 2; bi++)
+  for (int i = 0; i < 4; i++) {
+    const int r = I * HT_TB + 4 * tr + i;
 #pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int row = rb + 16 * bi + 4 * r + lk, col = cb + 16 * bj + li;
-        const bool ok = row < n && col < n && row >= s && col >= s && (!diag || col <= row);
-        acc[bi][bj][r] = ok ? A[row * ld + col] : 0.0;
-      }
-  int xr[2], yc[2];
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    xr[i] = rb + 16 * i + li;
-    yc[i] = cb + 16 * i + li;
-  }
-#pragma unroll
-  for (int s4 = 0; s4 < 2 * HT_NB / 4; s4++) {
-    const int kk = 4 * s4 + lk;
-    if (4 * s4 >= K2) break;
-    double x[2], y[2];
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      const bool okx = xr[i] < n && kk < K2, oky = yc[i] < n && kk < K2;
-      x[i] = !okx ? 0.0 : (kk < nbu ? -V[xr[i] * ld + j0 + kk] : -w.W[(long long)xr[i] * HT_NB + kk - nbu]);
-      y[i] = !oky ? 0.0 : (kk < nbu ? w.W[(long long)yc[i] * HT_NB + kk] : V[yc[i] * ld + j0 + kk - nbu]);
+    for (int j = 0; j < 4; j++) {
+      const int c = J * HT_TB + 4 * tc + j;
+      const bool ok = r < n && c < n && r >= s && c >= s && (!diag || c <= r);
+      av[i][j] = ok ? A[r * ld + c] : 0.0;
+      acc[i][j] = 0.0;
     }
+  }
+  for (int jb = 0; jb < nbu; jb += 16) {
+    // stage 16 panel columns of V and W for the tile's rows (I) and columns (J): sixteen lanes read
+    // 16 consecutive entries of one row
+    for (int e = tid; e < 16 * HT_TB; e += 256) {
+      const int jj = e & 15, l = e >> 4, j = jb + jj;
+      const int ri = I * HT_TB + l, rj = J * HT_TB + l;
+      const bool okj = j < nbu;
+      VI[jj][l] = okj && ri < n ? V[ri * ld + j0 + j] : 0.0;
+      WI[jj][l] = okj && ri < n ? w.W[(long long)ri * HT_NB + j] : 0.0;
+      VJ[jj][l] = okj && rj < n ? V[rj * ld + j0 + j] : 0.0;
+      WJ[jj][l] = okj && rj < n ? w.W[(long long)rj * HT_NB + j] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int jj = 0; jj < 16; jj++) {
+      double vi[4], wi[4], vj[4], wj[4];
 #pragma unroll
-    for (int bi = 0; bi < 2; bi++)
+      for (int q = 0; q < 4; q++) {
+        vi[q] = VI[jj][4 * tr + q];
+        wi[q] = WI[jj][4 * tr + q];
+        vj[q] = VJ[jj][4 * tc + q];
+        wj[q] = WJ[jj][4 * tc + q];
+      }
 #pragma unroll
-      for (int bj = 0; bj < 2; bj++) acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[bi], y[bj], acc[bi][bj], 0, 0, 0);
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] += vi[i] * wj[j] + wi[i] * vj[j];
+    }
+    __syncthreads();
   }
   double sq = 0.0;
+  const int cs = s - J * HT_TB;  // column s inside this tile (when J == t0)
 #pragma unroll
-  for (int bi = 0; bi < 2; bi++)
+  for (int i = 0; i < 4; i++) {
+    const int r = I * HT_TB + 4 * tr + i;
 #pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int row = rb + 16 * bi + 4 * r + lk, col = cb + 16 * bj + li;
-        if (row < n && col < n && row >= s && col >= s && (!diag || col <= row)) {
-          const double na = acc[bi][bj][r];
-          A[row * ld + col] = na;
-          if (col == s) {
-            if (row == s) w.d[s] = na;
-            else w.u[(long long)(s & 1) * n + row] = na;
-            if (row >= s + 2) sq += na * na;
-          }
+    for (int j = 0; j < 4; j++) {
+      const int c = J * HT_TB + 4 * tc + j;
+      if (r < n && c < n && r >= s && c >= s && (!diag || c <= r)) {
+        const double na = av[i][j] - acc[i][j];
+        A[r * ld + c] = na;
+        if (c == s) {
+          if (r == s) w.d[s] = na;
+          else w.u[(long long)(s & 1) * n + r] = na;
+          if (r >= s + 2) sq += na * na;
         }
       }
+    }
+  }
   if (J == t0) {
-    // the tile-row's norm partial: wave sums, then the four waves in order
-    sq = ht_wave_sum(sq);
-    if (l == 0) sqp[wv] = sq;
+    // the threads holding column s: tc == cs / 4 (one per 4 rows); the tile-row's norm partial
+    if (tc == (cs >> 2)) sqp[tr] = sq;
     __syncthreads();
-    if (tid == 0) w.ss[(long long)(s & 1) * T + I] = (sqp[0] + sqp[1]) + (sqp[2] + sqp[3]);
+    if (tid == 0) {
+      double a = 0.0;
+      for (int q = 0; q < 16; q++) a += sqp[q];
+      w.ss[(long long)(s & 1) * T + I] = a;
+    }
   }
 }
 
