@@ -619,13 +619,17 @@ __global__ void __launch_bounds__(256) k_ht_stebz(const HtBlock* __restrict__ bl
   for (int it = 0; it < 200; it++) {
     const double mid = 0.5 * (a + c);
     if (c - a <= 2.0 * eps * fmax(fabs(a), fabs(c)) + atol || mid <= a || mid >= c) break;
-    // Sturm count: eigenvalues below mid
+    // Sturm count: eigenvalues below mid.  e_{i-1}^2 / q by the hardware reciprocal and one Newton
+    // step (|q| >= pivmin keeps it finite, and e^2 / pivmin below the overflow threshold), which
+    // shortens the dependent chain of each step to three fused operations
     int cnt = 0;
     double q = d[0] - mid;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
     for (int i = 1; i < n; i++) {
-      q = (d[i] - mid) - e2[i - 1] / q;
+      double r = __builtin_amdgcn_rcp(q);
+      r = fma(r, fma(-q, r, 1.0), r);
+      q = fma(-e2[i - 1], r, d[i] - mid);
       if (fabs(q) < pivmin) q = -pivmin;
       cnt += q < 0.0;
     }
