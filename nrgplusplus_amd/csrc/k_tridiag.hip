@@ -651,56 +651,111 @@ __device__ __forceinline__ double ht_start(unsigned p, unsigned i) {
 
 // Rows per register tile of the column loops below: each tile's loads are issued together, so a
 // lane waits for memory once per tile rather than once per row of its serial recurrences.
-constexpr int HT_RT = 16;
+constexpr int HT_RT = 8;
 
+// row i0 .. i0 + HT_RT - 1 of the forward elimination: the right-hand side of the next row and T's entries
+__device__ __forceinline__ void ht_fwd_load(const double* __restrict__ d, const double* __restrict__ e, int n,
+                                            const double* __restrict__ U, int p, int i0, double* xn, double* sb,
+                                            double* dn, double* en) {
+  // rows past the end read the last row instead (every load is issued, with no branch around it);
+  // their values are never used
+#pragma unroll
+  for (int j = 0; j < HT_RT; j++) {
+    const int i = min(i0 + j, n - 2);
+    xn[j] = U[(long long)(i + 1) * n + p];
+    sb[j] = e[i];
+    dn[j] = d[i + 1];
+    en[j] = e[i + 1];  // e[n - 1] is 0
+  }
+}
+// rows i1, i1 - 1, .. of the back substitution: the eliminated right-hand side and the factors
+__device__ __forceinline__ void ht_bwd_load(const double* __restrict__ U, int n, int p, int i1,
+                                            const double* __restrict__ f0, const double* __restrict__ f1,
+                                            const double* __restrict__ f2, long long mp, int slot, double* uu,
+                                            double* a0, double* a1, double* a2) {
+#pragma unroll
+  for (int j = 0; j < HT_RT; j++) {
+    const int i = max(i1 - j, 0);
+    const long long o = (long long)i * mp + slot;
+    uu[j] = U[(long long)i * n + p];
+    a0[j] = f0[o];
+    a1[j] = f1[o];
+    a2[j] = f2[o];
+  }
+}
 // One inverse-iteration step for eigenvalue lp: the iterate in column p of U (stride n) is replaced by
 // (T - lp I)^-1 times it.  T - lp I = P L U with partial pivoting; the factorisation and the
 // elimination of the iterate share one pass (the L factor is not kept), the upper factor's three
 // diagonals go to the scratch f0 (as the pivot's reciprocal), f1, f2 at entry i * mp + slot, then one
 // backward pass.  A pivot smaller than piv0 in magnitude is moved out to +-piv0 (dlagtf/dlagts'
 // perturbation), which bounds the growth of one step by about 1 / piv0.
+// Each pass requests the next HT_RT rows before it stores the current ones: a wait on vector memory
+// also waits for every store issued before the awaited load, so the loads are not held behind them.
+// the forward elimination over rows i0 .. i0 + HT_RT - 1 (cd, cu: the pivot row's diagonal and
+// superdiagonal, xi: its right-hand side)
+__device__ __forceinline__ void ht_fwd_rows(int n, double lp, double piv0, double* __restrict__ U, int p,
+                                            double* __restrict__ f0, double* __restrict__ f1,
+                                            double* __restrict__ f2, long long mp, int slot, int i0,
+                                            const double* xn, const double* sb, const double* dn, const double* en,
+                                            double& cd, double& cu, double& xi) {
+#pragma unroll
+  for (int j = 0; j < HT_RT; j++) {
+    const int i = i0 + j;
+    if (i >= n - 1) continue;
+    const double sub = sb[j], nd = dn[j] - lp, nu = en[j];
+    const long long o = (long long)i * mp + slot;
+    if (fabs(cd) >= fabs(sub)) {
+      if (fabs(cd) < piv0) cd = cd < 0.0 ? -piv0 : piv0;
+      const double r = 1.0 / cd, ml = sub * r;
+      f0[o] = r;
+      f1[o] = cu;
+      f2[o] = 0.0;
+      U[(long long)i * n + p] = xi;
+      xi = xn[j] - ml * xi;
+      cd = nd - ml * cu;
+      cu = nu;
+    } else {  // rows i and i + 1 exchanged
+      const double r = 1.0 / sub, ml = cd * r;
+      f0[o] = r;
+      f1[o] = nd;
+      f2[o] = nu;
+      U[(long long)i * n + p] = xn[j];
+      xi = xi - ml * xn[j];
+      cd = cu - ml * nd;
+      cu = -ml * nu;
+    }
+  }
+}
+// the back substitution over rows i1, i1 - 1, .. i1 - HT_RT + 1 (x1, x2: the solution's next two rows)
+__device__ __forceinline__ void ht_bwd_rows(int n, double* __restrict__ U, int p, int i1, const double* uu,
+                                            const double* a0, const double* a1, const double* a2, double& x1,
+                                            double& x2) {
+#pragma unroll
+  for (int j = 0; j < HT_RT; j++) {
+    const int i = i1 - j;
+    if (i < 0) continue;
+    const double v = (uu[j] - a1[j] * x1 - a2[j] * x2) * a0[j];
+    U[(long long)i * n + p] = v;
+    x2 = x1;
+    x1 = v;
+  }
+}
 __device__ void ht_inverse_step(const double* __restrict__ d, const double* __restrict__ e, int n, double lp,
                                 double piv0, double* __restrict__ U, int p, double* __restrict__ f0,
                                 double* __restrict__ f1, double* __restrict__ f2, long long mp, int slot) {
   double cd = d[0] - lp, cu = n > 1 ? e[0] : 0.0;
   double xi = U[p];
-  for (int i0 = 0; i0 < n - 1; i0 += HT_RT) {
-    double xn[HT_RT], sb[HT_RT], dn[HT_RT], en[HT_RT];
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++) {
-      const int i = i0 + j;
-      const bool ok = i < n - 1;
-      xn[j] = ok ? U[(long long)(i + 1) * n + p] : 0.0;
-      sb[j] = ok ? e[i] : 0.0;
-      dn[j] = ok ? d[i + 1] : 0.0;
-      en[j] = i + 1 < n - 1 ? e[i + 1] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++) {
-      const int i = i0 + j;
-      if (i >= n - 1) continue;
-      const double sub = sb[j], nd = dn[j] - lp, nu = en[j];
-      const long long o = (long long)i * mp + slot;
-      if (fabs(cd) >= fabs(sub)) {
-        if (fabs(cd) < piv0) cd = cd < 0.0 ? -piv0 : piv0;
-        const double r = 1.0 / cd, ml = sub * r;
-        f0[o] = r;
-        f1[o] = cu;
-        f2[o] = 0.0;
-        U[(long long)i * n + p] = xi;
-        xi = xn[j] - ml * xi;
-        cd = nd - ml * cu;
-        cu = nu;
-      } else {  // rows i and i + 1 exchanged
-        const double r = 1.0 / sub, ml = cd * r;
-        f0[o] = r;
-        f1[o] = nd;
-        f2[o] = nu;
-        U[(long long)i * n + p] = xn[j];
-        xi = xi - ml * xn[j];
-        cd = cu - ml * nd;
-        cu = -ml * nu;
-      }
+  if (n >= 2) {
+    // two register tiles in turn: each is refilled while the other one's rows are eliminated
+    double xa[HT_RT], sa[HT_RT], da[HT_RT], ea[HT_RT], xb[HT_RT], sb[HT_RT], db[HT_RT], eb[HT_RT];
+    ht_fwd_load(d, e, n, U, p, 0, xa, sa, da, ea);
+    for (int i0 = 0; i0 < n - 1; i0 += 2 * HT_RT) {
+      ht_fwd_load(d, e, n, U, p, i0 + HT_RT, xb, sb, db, eb);
+      __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the stores below
+      ht_fwd_rows(n, lp, piv0, U, p, f0, f1, f2, mp, slot, i0, xa, sa, da, ea, cd, cu, xi);
+      ht_fwd_load(d, e, n, U, p, i0 + 2 * HT_RT, xa, sa, da, ea);
+      __builtin_amdgcn_sched_barrier(0);
+      ht_fwd_rows(n, lp, piv0, U, p, f0, f1, f2, mp, slot, i0 + HT_RT, xb, sb, db, eb, cd, cu, xi);
     }
   }
   if (fabs(cd) < piv0) cd = cd < 0.0 ? -piv0 : piv0;
@@ -710,27 +765,15 @@ __device__ void ht_inverse_step(const double* __restrict__ d, const double* __re
   f2[ol] = 0.0;
   U[(long long)(n - 1) * n + p] = xi;
   double x1 = 0.0, x2 = 0.0;
-  for (int i1 = n - 1; i1 >= 0; i1 -= HT_RT) {
-    double uu[HT_RT], a0[HT_RT], a1[HT_RT], a2[HT_RT];
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++) {
-      const int i = i1 - j;
-      const bool ok = i >= 0;
-      const long long o = (long long)i * mp + slot;
-      uu[j] = ok ? U[(long long)i * n + p] : 0.0;
-      a0[j] = ok ? f0[o] : 0.0;
-      a1[j] = ok ? f1[o] : 0.0;
-      a2[j] = ok ? f2[o] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++) {
-      const int i = i1 - j;
-      if (i < 0) continue;
-      const double v = (uu[j] - a1[j] * x1 - a2[j] * x2) * a0[j];
-      U[(long long)i * n + p] = v;
-      x2 = x1;
-      x1 = v;
-    }
+  double ua[HT_RT], a0[HT_RT], a1[HT_RT], a2[HT_RT], ub[HT_RT], b0[HT_RT], b1[HT_RT], b2[HT_RT];
+  ht_bwd_load(U, n, p, n - 1, f0, f1, f2, mp, slot, ua, a0, a1, a2);
+  for (int i1 = n - 1; i1 >= 0; i1 -= 2 * HT_RT) {
+    ht_bwd_load(U, n, p, i1 - HT_RT, f0, f1, f2, mp, slot, ub, b0, b1, b2);
+    __builtin_amdgcn_sched_barrier(0);
+    ht_bwd_rows(n, U, p, i1, ua, a0, a1, a2, x1, x2);
+    ht_bwd_load(U, n, p, i1 - 2 * HT_RT, f0, f1, f2, mp, slot, ua, a0, a1, a2);
+    __builtin_amdgcn_sched_barrier(0);
+    ht_bwd_rows(n, U, p, i1 - HT_RT, ub, b0, b1, b2, x1, x2);
   }
 }
 
@@ -865,10 +908,20 @@ __global__ void __launch_bounds__(64) k_ht_stein_step(const HtBlock* __restrict_
   double* f2 = f1 + (long long)n * mp;
   // pivots are kept at least eps ||T|| in magnitude (1 for T = 0, where any basis is one)
   const double piv0 = tnorm > 0.0 ? 2.220446049250313e-16 * tnorm : 1.0;
+  // T in LDS: its entries are the same for every lane, and LDS reads do not wait behind the
+  // elimination's stores to memory
+  extern __shared__ double tlds[];
+  double* td = tlds;
+  double* te = tlds + n;
+  for (int i = lane; i < n; i += 64) {
+    td[i] = w.d[i];
+    te[i] = i < n - 1 ? w.e[i] : 0.0;
+  }
+  __syncthreads();
   for (int p = a + lane; p < bend; p += 64) {
     if (first)
       for (int i = 0; i < n; i++) U[(long long)i * n + p] = ht_start((unsigned)p, (unsigned)i);
-    ht_inverse_step(w.d, w.e, n, L[p], piv0, U, p, f0, f1, f2, mp, slot);
+    ht_inverse_step(td, te, n, L[p], piv0, U, p, f0, f1, f2, mp, slot);
     ht_col_normalise(U, n, p);
   }
 }
@@ -1234,15 +1287,19 @@ void launch_ht_stebz(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
 }
 void launch_ht_stein(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* W, const double* lam, double* U,
                      double* S, double ctol_rel, int* list, int* count, hipStream_t s) {
-  int g = 0;
-  for (const auto& b : bl) g += (b.m + HT_CHUNK - 1) / HT_CHUNK;
+  int g = 0, nmax = 1;
+  for (const auto& b : bl) {
+    g += (b.m + HT_CHUNK - 1) / HT_CHUNK;
+    nmax = std::max(nmax, b.n);
+  }
   if (!g) return;
   const int nb = (int)bl.size();
+  const size_t tl = (size_t)nmax * 2 * sizeof(double);  // d and e of the widest block
   NRGK_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
   NRGK_LAUNCH(k_ht_stein_clusters, dim3(nb), dim3(256), 0, s, d_bl, W, lam, ctol_rel, list, count);
   for (int it = 0; it <= HT_ITERS; it++) {
     if (it < HT_ITERS)
-      NRGK_LAUNCH(k_ht_stein_step, dim3(g), dim3(64), 0, s, d_bl, nb, W, lam, U, S, ctol_rel, it == 0 ? 1 : 0);
+      NRGK_LAUNCH(k_ht_stein_step, dim3(g), dim3(64), tl, s, d_bl, nb, W, lam, U, S, ctol_rel, it == 0 ? 1 : 0);
     // a fixed grid that walks the cluster list (its length is read on the device)
     NRGK_LAUNCH(k_ht_stein_mgs, dim3(1024), dim3(256), 0, s, d_bl, (const int*)list, (const int*)count, U);
   }
@@ -1277,13 +1334,14 @@ void launch_ht_larft(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
   if (g) NRGK_LAUNCH(k_ht_larft, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), V, W, A);
   NRGK_HIP(hipGetLastError());
 }
-// dynamic LDS of the bisection kernel (d and e^2 of the widest block); its static LDS (below 8 KB)
-// comes on top of it
+// dynamic LDS of the bisection and inverse-iteration kernels (d and e^2, or d and e, of the widest
+// block); their static LDS (below 8 KB) comes on top of it
 constexpr int HT_STATIC_LDS = 8192;
 int ht_max_n(int lds_bytes) { return (lds_bytes - HT_STATIC_LDS) / (int)(2 * sizeof(double)); }
 void set_ht_lds_limits(int bytes) {
   const int dyn = bytes - HT_STATIC_LDS;
   NRGK_HIP(hipFuncSetAttribute((const void*)k_ht_stebz, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+  NRGK_HIP(hipFuncSetAttribute((const void*)k_ht_stein_step, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
 }
 
 }  // namespace nrgk

@@ -142,6 +142,8 @@ def test_two_channel_workload_at_full_shape(nk, oracle_mod):
                 Us = g.current_hamiltonQ(s)
                 orth = np.abs(Us.T @ Us - np.eye(n)).max()
                 res = np.abs(H @ Us - Us * eq[s]).max()
+                print("iteration %d sector %d n %d: orthogonality %.2e residual %.2e" % (it, s, n, orth, res),
+                      flush=True)
                 assert orth <= 1e-10 and res <= 1e-10, (it, s, n, orth, res)
         g.update_internal_state()
         kept = g.kept_counts()
