@@ -29,6 +29,8 @@
 //    k_ht_wy_z).
 #include <cmath>
 
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -1256,9 +1258,16 @@ void launch_ht_first(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const 
   NRGK_HIP(hipGetLastError());
 }
 int launch_ht_symv(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, const double* V, double* W,
-                   hipStream_t s) {
+                   hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
   const int g = ht_grid(bl, k, 0);
-  if (g) NRGK_LAUNCH(k_ht_symv, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), ht_launch_table(bl, k, 0), k, A, V, W);
+  if (g && t0) {
+    // the events take the kernel's own start and end, not the enqueueing around it
+    thread_counts().launches++;
+    hipExtLaunchKernelGGL(k_ht_symv, dim3(g), dim3(256), 0, s, t0, t1, 0, d_bl, (int)bl.size(),
+                          ht_launch_table(bl, k, 0), k, A, V, W);
+  } else if (g) {
+    NRGK_LAUNCH(k_ht_symv, dim3(g), dim3(256), 0, s, d_bl, (int)bl.size(), ht_launch_table(bl, k, 0), k, A, V, W);
+  }
   NRGK_HIP(hipGetLastError());
   return g;
 }

@@ -33,8 +33,9 @@ void launch_gemm(const GemmProb* d_probs, int n_probs, int tiles, hipStream_t s)
 // k_tridiag.hip (the wide blocks)
 int ht_grid(const std::vector<HtBlock>& bl, int k, int kind);
 void launch_ht_first(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* A, double* W, hipStream_t s);
+// with events t0, t1 (created, not recorded): the launch records its start and end in them
 int launch_ht_symv(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, const double* V, double* W,
-                   hipStream_t s);
+                   hipStream_t s, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 int launch_ht_update(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, double* V, double* W,
                      hipStream_t s);
 int launch_ht_syr2k(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, double* A, const double* V, double* W,

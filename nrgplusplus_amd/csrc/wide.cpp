@@ -111,9 +111,10 @@ void Chain::wide_begin(const std::vector<int>& sectors) {
       };
       // algorithmic bytes of the symmetric products (the lower triangle of the trailing matrix read
       // once per column) and of the rank-2k updates (that triangle read and written once per panel).
-      // With the profile restricted to "ht_symv", one column in HT_SYMV_SAMPLE has an event pair
-      // around its k_ht_symv launch alone (its duration and algorithmic bytes), so the kernel's own
-      // rate is measured without an event between every pair of launches.
+      // With the profile restricted to "ht_symv", one column in HT_SYMV_SAMPLE has its k_ht_symv
+      // launch timed by an event pair that the launch itself records at the kernel's start and end
+      // (its duration and algorithmic bytes), so the kernel's own rate is measured without an event
+      // between every pair of launches and without the enqueueing around it.
       constexpr int HT_SYMV_SAMPLE = 8;
       const bool symv_only = prof_on && prof_only == "ht_symv";
       const bool fam = want("tridiagonalise");
@@ -133,10 +134,16 @@ void Chain::wide_begin(const std::vector<int>& sectors) {
         by += bsym;
         const bool sample = symv_only && k % HT_SYMV_SAMPLE == 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (sample) mark(e0);
-        const int gs = launch_ht_symv(d, ht, k, A, V, W, s2);
         if (sample) {
-          mark(e1);
+          NRGK_HIP(hipEventCreate(&e0));
+          NRGK_HIP(hipEventCreate(&e1));
+        }
+        const int gs = launch_ht_symv(d, ht, k, A, V, W, s2, e0, e1);
+        if (sample) {
+          if (!gs) {  // nothing launched: the pair is recorded empty
+            NRGK_HIP(hipEventRecord(e0, s2));
+            NRGK_HIP(hipEventRecord(e1, s2));
+          }
           wide_prof.push_back(ProfRun{"ht_symv", e0, e1, gs > 0, 0.0, bsym});
         }
         launches += gs > 0;
