@@ -32,6 +32,7 @@ int prepare_gemm(std::vector<GemmProb>& probs);
 void launch_gemm(const GemmProb* d_probs, int n_probs, int tiles, hipStream_t s);
 // k_tridiag.hip (the wide blocks)
 int ht_grid(const std::vector<HtBlock>& bl, int k, int kind);
+void launch_ht_tile_in(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* H, double* A, hipStream_t s);
 void launch_ht_first(const HtBlock* d_bl, const std::vector<HtBlock>& bl, const double* A, double* W, hipStream_t s);
 // with events t0, t1 (created, not recorded): the launch records its start and end in them
 int launch_ht_symv(const HtBlock* d_bl, const std::vector<HtBlock>& bl, int k, const double* A, const double* V, double* W,

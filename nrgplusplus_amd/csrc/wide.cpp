@@ -42,8 +42,9 @@ static int wide_ld(int n) {
 
 // Tridiagonalisation and eigenvalues of the given sectors on side stream 0, started once the main
 // stream has assembled H.  Writes the eigenvalues into dlam_raw; wide_join() makes the main stream
-// wait for them.  Each block's working copy of H is in d_wide_A, its Householder vectors in d_wide_V
-// (n x ld at the block's p_off, 128-byte aligned), the workspace in d_wide_ws.
+// wait for them.  Each block's working copy of H is in d_wide_A (its lower 64 x 64 tiles one after the
+// other), its Householder vectors in d_wide_V (n x ld at the block's p_off, 128-byte aligned), the
+// workspace in d_wide_ws.
 void Chain::wide_begin(const std::vector<int>& sectors) {
   wide_thread_join();  // a worker of an earlier solve that was not joined (an error path)
   wide_err = nullptr;
@@ -59,7 +60,8 @@ void Chain::wide_begin(const std::vector<int>& sectors) {
     b.a_off = cur_off[s];
     b.ld = wide_ld(b.n);
     b.p_off = po;
-    po += ((long long)b.n * b.ld + 31) / 32 * 32;
+    // the working matrix's lower tiles, or (after the reduction) V_p T_p at row stride ld
+    po += (std::max((long long)b.n * b.ld, (long long)b.T * (b.T + 1) / 2 * 4096) + 31) / 32 * 32;
     b.w_off = ws;
     b.e_off = cur_eoff[s];
     ws += (ht_ws_size(b.n, b.T) + 31) / 32 * 32;
@@ -85,11 +87,8 @@ void Chain::wide_begin(const std::vector<int>& sectors) {
   double* A = wide_A;
   double* V = wide_V;
   double* W = d_wide_ws.as<double>();
-  for (const auto& b : ht) {
-    const size_t row = (size_t)b.n * sizeof(double), pitch = (size_t)b.ld * sizeof(double);
-    NRGK_HIP(hipMemcpy2DAsync(A + b.p_off, pitch, dH.as<double>() + b.a_off, row, row, b.n, hipMemcpyDeviceToDevice, s2));
-    NRGK_HIP(hipMemsetAsync(V + b.p_off, 0, (size_t)b.n * pitch, s2));
-  }
+  launch_ht_tile_in(d, ht, dH.as<double>(), A, s2);
+  for (const auto& b : ht) NRGK_HIP(hipMemsetAsync(V + b.p_off, 0, (size_t)b.n * b.ld * sizeof(double), s2));
   launch_ht_first(d, ht, A, W, s2);
   wide_state = WIDE_EIGENVALUES;
   wide_joined = false;
