@@ -523,7 +523,8 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
   int Ip, Jp;
   ht_tile_of(t, Ip, Jp);
   const int I = t0 + Ip, J = t0 + Jp;
-  // thread: rows 4 tr .. 4 tr + 3, columns 4 tc .. 4 tc + 3 of the tile
+  // thread: rows 4 tr .. 4 tr + 3, columns tc, tc + 16, tc + 32, tc + 48 of the tile (sixteen lanes
+  // read sixteen consecutive entries of a staged row: no LDS bank conflicts)
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
   __shared__ double VI[16][HT_TB + 1], WI[16][HT_TB + 1], VJ[16][HT_TB + 1], WJ[16][HT_TB + 1];
   __shared__ double sqp[16];
@@ -535,7 +536,7 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
     const int r = I * HT_TB + 4 * tr + i;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int c = J * HT_TB + 4 * tc + j;
+      const int c = J * HT_TB + tc + 16 * j;
       const bool ok = r < n && c < n && r >= s && c >= s && (!diag || c <= r);
       av[i][j] = ok ? A[ht_at(r, c)] : 0.0;
       acc[i][j] = 0.0;
@@ -561,8 +562,8 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
       for (int q = 0; q < 4; q++) {
         vi[q] = VI[jj][4 * tr + q];
         wi[q] = WI[jj][4 * tr + q];
-        vj[q] = VJ[jj][4 * tc + q];
-        wj[q] = WJ[jj][4 * tc + q];
+        vj[q] = VJ[jj][tc + 16 * q];
+        wj[q] = WJ[jj][tc + 16 * q];
       }
 #pragma unroll
       for (int i = 0; i < 4; i++)
@@ -578,7 +579,7 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
     const int r = I * HT_TB + 4 * tr + i;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int c = J * HT_TB + 4 * tc + j;
+      const int c = J * HT_TB + tc + 16 * j;
       if (r < n && c < n && r >= s && c >= s && (!diag || c <= r)) {
         const double na = av[i][j] - acc[i][j];
         A[ht_at(r, c)] = na;
@@ -591,8 +592,8 @@ __global__ void __launch_bounds__(256) k_ht_syr2k(const HtBlock* __restrict__ bl
     }
   }
   if (J == t0) {
-    // the threads holding column s: tc == cs / 4 (one per 4 rows); the tile-row's norm partial
-    if (tc == (cs >> 2)) sqp[tr] = sq;
+    // the threads holding column s: tc == cs % 16 (one per 4 rows); the tile-row's norm partial
+    if (tc == (cs & 15)) sqp[tr] = sq;
     __syncthreads();
     if (tid == 0) {
       double a = 0.0;
