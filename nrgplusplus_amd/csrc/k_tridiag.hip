@@ -867,25 +867,37 @@ __device__ void ht_col_axpy(double* __restrict__ U, int n, int p, int q, double 
 // column p normalised, scaled by its largest entry first so that the squares of an iterate of any
 // size stay finite
 __device__ void ht_col_normalise(double* __restrict__ U, int n, int p) {
-  double mx = 0.0;
-  for (int i0 = 0; i0 < n; i0 += HT_RT) {
-    double a[HT_RT];
+  // one read-only pass forms the largest magnitude and the plain sum of squares (sixteen rows a
+  // batch); only when that sum over- or underflows does a second pass form it from the scaled
+  // entries; then the column is scaled
+  constexpr int RB = 2 * HT_RT;
+  double mx = 0.0, s2 = 0.0;
+  for (int i0 = 0; i0 < n; i0 += RB) {
+    double a[RB];
 #pragma unroll
-    for (int j = 0; j < HT_RT; j++) a[j] = i0 + j < n ? U[(long long)(i0 + j) * n + p] : 0.0;
+    for (int j = 0; j < RB; j++) a[j] = i0 + j < n ? U[(long long)(i0 + j) * n + p] : 0.0;
 #pragma unroll
-    for (int j = 0; j < HT_RT; j++) mx = fmax(mx, fabs(a[j]));
+    for (int j = 0; j < RB; j++) {
+      mx = fmax(mx, fabs(a[j]));
+      s2 += a[j] * a[j];
+    }
   }
   if (mx == 0.0) return;
-  const double sc = 1.0 / mx;
-  double nr = 0.0;
-  for (int i0 = 0; i0 < n; i0 += HT_RT) {
-    double a[HT_RT];
+  double inv;
+  if (s2 > 1e-280 && s2 < 1e280) {
+    inv = 1.0 / sqrt(s2);
+  } else {
+    const double sc = 1.0 / mx;
+    double nr = 0.0;
+    for (int i0 = 0; i0 < n; i0 += RB) {
+      double a[RB];
 #pragma unroll
-    for (int j = 0; j < HT_RT; j++) a[j] = i0 + j < n ? U[(long long)(i0 + j) * n + p] * sc : 0.0;
+      for (int j = 0; j < RB; j++) a[j] = i0 + j < n ? U[(long long)(i0 + j) * n + p] * sc : 0.0;
 #pragma unroll
-    for (int j = 0; j < HT_RT; j++) nr += a[j] * a[j];
+      for (int j = 0; j < RB; j++) nr += a[j] * a[j];
+    }
+    inv = sc / sqrt(nr);
   }
-  const double inv = sc / sqrt(nr);
   for (int i0 = 0; i0 < n; i0 += HT_RT) {
     double a[HT_RT];
 #pragma unroll
