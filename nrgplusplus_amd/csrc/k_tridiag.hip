@@ -7,10 +7,11 @@
 //    y = A v is formed from the panel-start matrix and corrected by the panel's V and W, and the
 //    panel ends with one rank-2k update A -= V W^T + W V^T.  All wide blocks of an iteration advance
 //    column by column in the same launches:
-//      k_ht_symv    one workgroup per 64 x 64 tile of the trailing lower triangle: the tile's row
-//                   sums (A v) and column sums (A^T v over the strict lower part), v^T A v and the
-//                   panel's V^T v, W^T v shares -- the HBM-bound bulk (each element read once per
-//                   column, the lower triangle only)
+//      k_ht_symv    one workgroup per 64 x 64 tile of the trailing lower triangle (the working
+//                   matrix holds its lower tiles contiguously, see ht_at): the tile's row sums (A v)
+//                   and column sums (A^T v over the strict lower part), v^T A v and the panel's
+//                   V^T v, W^T v shares -- the HBM-bound bulk (each element read once per column,
+//                   the lower triangle only)
 //      k_ht_update  one workgroup per 64-row tile-row: reduces those partials to w, stores v and w,
 //                   and forms the next column with the panel corrections and its norm
 //      k_ht_syr2k   the panel's rank-2k update of the trailing lower triangle; the tiles of the
@@ -21,9 +22,10 @@
 // 2. Eigenvalues of T by bisection on Sturm counts (one thread per eigenvalue, T in LDS), ascending
 //    as dsyevd returns them.
 // 3. Eigenvectors of T by inverse iteration with partial-pivoting LU of T - lambda I (one lane per
-//    eigenvector, all iterated at once; the members of a cluster of eigenvalues closer than ctol
-//    made orthonormal afterwards by modified Gram-Schmidt across the wavefront), for the columns
-//    that are needed: all of them, or only the kept ones.
+//    eigenvector, all iterated at once; after every step the members of each cluster of eigenvalues
+//    closer than ctol are made orthonormal by modified Gram-Schmidt, one workgroup per cluster from
+//    a cluster list built on the device), for the columns that are needed: all of them, or only the
+//    kept ones.
 // 4. Back-transformation U = Q Z in WY form, Q_p = I - V_p T_p V_p^T per panel (k_ht_larft forms
 //    V_p T_p): per panel Y = V_p^T Z and Z -= (V_p T_p) Y on the FP64 matrix cores (k_ht_wy_y,
 //    k_ht_wy_z).
