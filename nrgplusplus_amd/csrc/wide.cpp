@@ -229,7 +229,7 @@ void Chain::wide_vectors(bool all) {
   prof_begin("tridiag_vectors");
   launch_ht_stein(d, bl, W, dlam_raw.as<double>(), U, d_wide_s.as<double>(), ctol, d_wide_q.as<int>() + 64,
                   d_wide_q.as<int>(), st);
-  prof_end("tridiag_vectors", 8, 0.0, 0.0);  // the cluster list, three steps, four orthogonalisations
+  prof_end("tridiag_vectors", 6, 0.0, 0.0);  // the cluster list, two steps, three orthogonalisations
   // V_p T_p over each block's working matrix (no longer needed), then per panel, last first:
   // Y = V_p^T Z, Z -= (V_p T_p) Y, row-parallel (k_ht_wy_y, k_ht_wy_z)
   launch_ht_larft(d, bl, wide_V, W, wide_A, st);
