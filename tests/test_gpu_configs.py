@@ -5,7 +5,8 @@
   * the SIAM FDM spectrum workload (U = 0.3, eps = -0.15, Gamma = 0.01, Lambda = 2, 2000 bins) at
     K = 1024 against the oracle;
   * the two-channel workload at its real shape (Lambda = 3, K = 4096: blocks up to ~4200 rows, the
-    wide-block branch of the eigensolver), every block's eigenvalues against LAPACK on the same H.
+    wide-block branch of the eigensolver), every block's eigenvalues against LAPACK on the same H, and
+    over its whole 50-site flow.
 
 Tolerances as stated in tests/test_gpu_chain.py; the sum rule to 1e-10."""
 import os
@@ -153,3 +154,44 @@ def test_two_channel_workload_at_full_shape(nk, oracle_mod):
             assert np.array_equal(kept, o.kept_counts()), it
         _say("two-channel K=4096 iteration %d: %d states, widest block %d, %.1f s" %
              (it, int(np.sum(dims)), int(np.max(dims)), time.time() - t0))
+
+
+def test_two_channel_workload_whole_flow(nk):
+    """The two-channel workload over all 50 Wilson sites (the benchmark times iterations 3-10 of it):
+    every iteration's eigenvalues finite and its kept counts equal to the truncation rule applied to
+    its own eigenvalues; at iterations 15, 30 and 49 (near the fixed point, where the clusters of close
+    eigenvalues are largest) each block's eigenvalues within 1e-10 of LAPACK on the same H and the
+    widest block's eigenvectors orthonormal with residual within 1e-10."""
+    eps, U, L, K, N = -0.2, 0.4, 3.0, 4096, 50
+    V = np.sqrt(2 * 0.01 / np.pi)
+    g = nk.NrgCore(nk.Site.spinhalf(eps, U).duplicate_ops(), nk.Site.two_channel(), K)
+    t0 = time.time()
+    for it in range(N):
+        t = V if it == 0 else nk.hopping(it - 1, L)
+        r = 1.0 if it <= 1 else np.sqrt(L)
+        check = it in (15, 30, N - 1)
+        g.keep_hamiltonians(check)
+        g.add_bath_site([t] * 4, r)
+        _, dims = g.sectors()
+        eq = g.eigenvaluesQ
+        assert all(np.all(np.isfinite(e)) for e in eq), it
+        if check:
+            widest = int(np.argmax(dims))
+            worst = 0.0
+            for s, n in enumerate(dims):
+                if n == 0:
+                    continue
+                H = g.block_h(s)
+                worst = max(worst, float(np.abs(np.linalg.eigvalsh(H) - eq[s]).max()))
+                if s == widest:
+                    Us = g.current_hamiltonQ(s)
+                    orth = np.abs(Us.T @ Us - np.eye(n)).max()
+                    res = np.abs(H @ Us - Us * eq[s]).max()
+                    assert orth <= 1e-10 and res <= 1e-10, (it, s, n, orth, res)
+            _say("two-channel K=4096 iteration %d: widest block %d, eigenvalue error %.2e, orthogonality %.2e, "
+                 "residual %.2e" % (it, int(np.max(dims)), worst, orth, res))
+            assert worst <= 1e-10, (it, worst)
+        expect = _truncate([np.asarray(e, float) for e in eq], K)
+        g.update_internal_state()
+        assert np.array_equal(g.kept_counts(), expect), it
+    _say("two-channel K=4096, %d iterations: %.1f s" % (N, time.time() - t0))
