@@ -837,37 +837,6 @@ __device__ __forceinline__ int ht_cluster_start_at(const double* L, int n, int x
   return x;
 }
 
-// column p of U (stride n) dotted with column q, summed in row order
-__device__ double ht_col_dot(const double* __restrict__ U, int n, int p, int q) {
-  double s = 0.0;
-  for (int i0 = 0; i0 < n; i0 += HT_RT) {
-    double a[HT_RT], b[HT_RT];
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++) {
-      const int i = i0 + j;
-      a[j] = i < n ? U[(long long)i * n + p] : 0.0;
-      b[j] = i < n ? U[(long long)i * n + q] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++) s += a[j] * b[j];
-  }
-  return s;
-}
-// column p += c * column q
-__device__ void ht_col_axpy(double* __restrict__ U, int n, int p, int q, double c) {
-  for (int i0 = 0; i0 < n; i0 += HT_RT) {
-    double a[HT_RT], b[HT_RT];
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++) {
-      const int i = i0 + j;
-      a[j] = i < n ? U[(long long)i * n + p] : 0.0;
-      b[j] = i < n ? U[(long long)i * n + q] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < HT_RT; j++)
-      if (i0 + j < n) U[(long long)(i0 + j) * n + p] = a[j] + c * b[j];
-  }
-}
 // column p normalised, scaled by its largest entry first so that the squares of an iterate of any
 // size stay finite
 __device__ void ht_col_normalise(double* __restrict__ U, int n, int p) {
